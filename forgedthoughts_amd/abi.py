@@ -1,0 +1,111 @@
+"""ctypes / numpy mirror of include/ft_abi.h (the scene IR and render arguments).
+
+Every struct here must keep the exact field order and size of its C counterpart; the
+`test_abi_layout` test compares these sizes with `sizeof` values exported by the built
+library.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+ABI_VERSION = 1
+
+ACCEL_NAIVE, ACCEL_BVH, ACCEL_BRICKS = 0, 1, 2
+
+NODE_KINDS = [
+    "SPHERE", "BOX", "CYLINDER", "TORUS", "EXTRUDE_POLYGON", "CUSTOM", "DOMAIN_MODIFIER",
+    "DISTANCE_POST_MODIFIER", "NOISE", "UNION", "INTERSECT", "SUBTRACT", "UNION_ROUND",
+    "UNION_CHAMFER", "UNION_COLUMNS", "UNION_STAIRS", "UNION_SOFT", "INTERSECT_ROUND",
+    "INTERSECT_CHAMFER", "INTERSECT_COLUMNS", "INTERSECT_STAIRS", "DIFF_ROUND", "DIFF_CHAMFER",
+    "DIFF_COLUMNS", "DIFF_STAIRS", "PIPE", "ENGRAVE", "GROOVE", "TONGUE", "SLICE", "SMOOTH",
+]
+NODE = {name: i for i, name in enumerate(NODE_KINDS)}
+
+MODEL_STANDARD, MODEL_LAMBERT, MODEL_METAL, MODEL_DIELECTRIC = 0, 1, 2, 3
+PATTERN_NONE, PATTERN_CHECKER3D = 0, 1
+LIGHT_POINT, LIGHT_ENV, LIGHT_SPHERE = 0, 1, 2
+
+AOV = {None: -1, "depth": 0, "normal": 1, "material_id": 2, "ior": 3, "transmission": 4,
+       "fresnel": 5, "hit_t": 6}
+
+NODE_DTYPE = np.dtype([
+    ("kind", "<u4"), ("lhs", "<i4"), ("rhs", "<i4"), ("object_id", "<u4"), ("material_id", "<u4"),
+    ("fn_domain", "<i4"), ("fn_distance", "<i4"), ("fn_post", "<i4"),
+    ("center", "<f4", 3), ("rot_deg", "<f4", 3), ("param", "<f4", 6),
+    ("bounds_min", "<f4", 3), ("bounds_max", "<f4", 3),
+])
+assert NODE_DTYPE.itemsize == 104
+
+EXPR_DTYPE = np.dtype([
+    ("op", "<u4"), ("type", "<u4"), ("a", "<i4"), ("b", "<i4"), ("c", "<i4"), ("d", "<i4"),
+    ("value", "<f4"), ("pad", "<f4"),
+])
+assert EXPR_DTYPE.itemsize == 32
+
+FUNCTION_DTYPE = np.dtype([("first", "<u4"), ("count", "<u4"), ("n_params", "<u4"), ("result_type", "<u4")])
+
+MATERIAL_DTYPE = np.dtype([
+    ("model", "<u4"), ("thin_walled", "<u4"), ("has_medium", "<u4"), ("pattern", "<u4"),
+    ("color", "<f4", 3), ("roughness", "<f4"), ("metallic", "<f4"), ("specular", "<f4"),
+    ("specular_weight", "<f4"), ("ior", "<f4"), ("specular_color", "<f4", 3), ("clearcoat", "<f4"),
+    ("clearcoat_roughness", "<f4"), ("transmission", "<f4"), ("emission_strength", "<f4"),
+    ("medium_ior", "<f4"), ("emission_color", "<f4", 3), ("medium_density", "<f4"),
+    ("medium_absorption", "<f4", 3), ("pattern_scale", "<f4"), ("pattern_color_a", "<f4", 3),
+    ("dynamic_material_id", "<i4"), ("pattern_color_b", "<f4", 3), ("dynamic_override_id", "<i4"),
+])
+assert MATERIAL_DTYPE.itemsize == 144
+
+LIGHT_DTYPE = np.dtype([
+    ("kind", "<u4"), ("samples", "<u4"), ("radius", "<f4"), ("pad", "<f4"),
+    ("position", "<f4", 3), ("pad1", "<f4"), ("radiance", "<f4", 3), ("pad2", "<f4"),
+])
+assert LIGHT_DTYPE.itemsize == 48
+
+
+class Camera(C.Structure):
+    _fields_ = [
+        ("present", C.c_uint32), ("has_origin", C.c_uint32), ("has_target", C.c_uint32),
+        ("has_fov", C.c_uint32), ("origin", C.c_float * 3), ("fov_y_degrees", C.c_float),
+        ("target", C.c_float * 3), ("pad", C.c_float),
+    ]
+
+
+class SceneIR(C.Structure):
+    _fields_ = [
+        ("abi_version", C.c_uint32), ("root", C.c_uint32), ("n_nodes", C.c_uint32),
+        ("n_materials", C.c_uint32), ("n_objects", C.c_uint32), ("n_lights", C.c_uint32),
+        ("n_functions", C.c_uint32), ("n_exprs", C.c_uint32), ("env_fn", C.c_int32),
+        ("pad", C.c_uint32),
+        ("nodes", C.c_void_p), ("materials", C.c_void_p), ("object_transforms", C.c_void_p),
+        ("lights", C.c_void_p), ("functions", C.c_void_p), ("exprs", C.c_void_p),
+        ("camera", Camera),
+    ]
+
+
+class RenderOptionsC(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32), ("max_steps", C.c_uint32),
+        ("max_dist", C.c_float), ("epsilon", C.c_float), ("step_scale", C.c_float),
+        ("camera_z", C.c_float), ("fov_y_degrees", C.c_float),
+    ]
+
+
+class RaySettingsC(C.Structure):
+    _fields_ = [("max_depth", C.c_uint32), ("tile_size", C.c_uint32),
+                ("aa_samples", C.c_uint32), ("debug_aov", C.c_int32)]
+
+
+class RenderStatsC(C.Structure):
+    _fields_ = [("primary_rays", C.c_uint64), ("kernel_launches", C.c_uint64),
+                ("last_kernel_ms", C.c_float), ("pad", C.c_float)]
+
+
+STRUCT_SIZES = {
+    "ft_node": NODE_DTYPE.itemsize, "ft_expr": EXPR_DTYPE.itemsize,
+    "ft_function": FUNCTION_DTYPE.itemsize, "ft_material": MATERIAL_DTYPE.itemsize,
+    "ft_light": LIGHT_DTYPE.itemsize, "ft_camera": C.sizeof(Camera),
+    "ft_scene_ir": C.sizeof(SceneIR), "ft_render_options": C.sizeof(RenderOptionsC),
+    "ft_ray_settings": C.sizeof(RaySettingsC), "ft_render_stats": C.sizeof(RenderStatsC),
+}

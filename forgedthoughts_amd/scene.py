@@ -1,0 +1,1019 @@
+"""Scene compilation: Forge values -> flat scene IR (include/ft_abi.h).
+
+Restates the reference's scene compiler, which runs once per render and produces the
+`CompiledScene` the accelerators consume:
+  find_scene_root                renderer.rs:1375-1382
+  compile_scene / compile_sdf    renderer.rs:1384-1742
+  apply_object_modifier_hooks    renderer.rs:1750-1808
+  split_modifier_base            renderer.rs:1810-1948
+  compile_sdf_offset / mirrored  renderer.rs:1973-2325 (repeat_* / mirror_*)
+  primitive_material_id          renderer.rs:2327-2337
+  material_from_object           renderer.rs:6454-6570, material_from_dynamic_def 6634-6682
+  parse_material                 renderer.rs:6390-6437
+  parse_lights / parse_camera    renderer.rs:6263-6388
+  find_environment_name          renderer.rs:6910-6926
+  eval_custom_sdf_bounds_half_extents  renderer.rs:6181-6204
+  sdf_bounds                     renderer.rs:5241-5386 (for modifier nodes only)
+
+Two reference behaviours depend on HashMap iteration order and are therefore not
+deterministic in the reference itself: the order of lights (renderer.rs:6294) and which
+nested material `parse_material` finds first (renderer.rs:6415-6419).  Here lights keep
+source (binding insertion) order and the material search visits `lhs` before `rhs`.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import abi
+from .forge import evaluator as E
+from .forge.evaluator import Obj, Func, is_num, f32
+
+_libm = ctypes.CDLL("libm.so.6")
+_libm.sinf.restype = ctypes.c_float
+_libm.sinf.argtypes = [ctypes.c_float]
+_libm.cosf.restype = ctypes.c_float
+_libm.cosf.argtypes = [ctypes.c_float]
+
+DEG2RAD = f32(np.float32(math.pi) / np.float32(180.0))
+
+
+class RenderError(Exception):
+    """RenderError (renderer.rs:158-168)."""
+
+    def __init__(self, kind: str, message: str):
+        super().__init__(message)
+        self.kind = kind
+
+
+def _sin(x) -> np.float32:
+    return f32(_libm.sinf(float(x)))
+
+
+def _cos(x) -> np.float32:
+    return f32(_libm.cosf(float(x)))
+
+
+# ----------------------------------------------------------- small f32 math
+def _v(x, y, z):
+    return np.array([x, y, z], dtype=np.float32)
+
+
+def _rot_x(v, deg):
+    r = f32(f32(deg) * DEG2RAD)
+    s, c = _sin(r), _cos(r)
+    return _v(v[0], f32(c * v[1]) - f32(s * v[2]), f32(s * v[1]) + f32(c * v[2]))
+
+
+def _rot_y(v, deg):
+    r = f32(f32(deg) * DEG2RAD)
+    s, c = _sin(r), _cos(r)
+    return _v(f32(c * v[0]) + f32(s * v[2]), v[1], f32(-s * v[0]) + f32(c * v[2]))
+
+
+def _rot_z(v, deg):
+    r = f32(f32(deg) * DEG2RAD)
+    s, c = _sin(r), _cos(r)
+    return _v(f32(c * v[0]) - f32(s * v[1]), f32(s * v[0]) + f32(c * v[1]), v[2])
+
+
+def _length(v) -> np.float32:
+    return f32(np.sqrt(f32(f32(f32(v[0] * v[0]) + f32(v[1] * v[1])) + f32(v[2] * v[2]))))
+
+
+# ----------------------------------------------------------- field readers
+def read_number_field(obj: Obj, names) -> Optional[np.float32]:
+    for n in names:
+        v = obj.fields.get(n)
+        if is_num(v):
+            return v
+    return None
+
+
+def read_number_path(obj: Obj, path) -> Optional[np.float32]:
+    cur = obj
+    for seg in path[:-1]:
+        nxt = cur.fields.get(seg)
+        if not isinstance(nxt, Obj):
+            return None
+        cur = nxt
+    return read_number_field(cur, [path[-1]])
+
+
+def _or(*vals):
+    for v in vals:
+        if v is not None:
+            return v
+    return None
+
+
+def read_center(obj: Obj) -> np.ndarray:
+    return _v(
+        _or(read_number_path(obj, ["pos", "x"]), read_number_field(obj, ["x"]), f32(0)),
+        _or(read_number_path(obj, ["pos", "y"]), read_number_field(obj, ["y"]), f32(0)),
+        _or(read_number_path(obj, ["pos", "z"]), read_number_field(obj, ["z"]), f32(0)),
+    )
+
+
+def read_transform(obj: Obj) -> Tuple[np.ndarray, np.ndarray]:
+    rot = _v(
+        _or(read_number_path(obj, ["rot", "x"]), read_number_field(obj, ["rot_x"]), f32(0)),
+        _or(read_number_path(obj, ["rot", "y"]), read_number_field(obj, ["rot_y"]), f32(0)),
+        _or(read_number_path(obj, ["rot", "z"]), read_number_field(obj, ["rot_z"]), f32(0)),
+    )
+    return read_center(obj), rot
+
+
+def read_vec3_field(obj: Obj, name: str) -> Optional[np.ndarray]:
+    v = obj.fields.get(name)
+    if not isinstance(v, Obj):
+        return None
+    x, y, z = (read_number_field(v, [k]) for k in ("x", "y", "z"))
+    if x is None or y is None or z is None:
+        return None
+    return _v(x, y, z)
+
+
+def spectrum_from_value(v) -> Optional[np.ndarray]:
+    if is_num(v):
+        return _v(v, v, v)
+    if isinstance(v, Obj):
+        r = read_number_field(v, ["r", "x"])
+        g = read_number_field(v, ["g", "y"])
+        b = read_number_field(v, ["b", "z"])
+        if r is None or g is None or b is None:
+            return None
+        return _v(r, g, b)
+    return None
+
+
+def read_spectrum_field(obj: Obj, name: str) -> Optional[np.ndarray]:
+    if name not in obj.fields:
+        return None
+    return spectrum_from_value(obj.fields[name])
+
+
+def _clamp(x, lo, hi):
+    x = f32(x)
+    if x < lo:
+        x = f32(lo)
+    if x > hi:
+        x = f32(hi)
+    return x
+
+
+# --------------------------------------------------------------- materials
+@dataclass
+class Material:
+    model: int
+    color: np.ndarray
+    roughness: float
+    metallic: float
+    specular: float
+    specular_weight: float
+    specular_color: np.ndarray
+    ior: float
+    clearcoat: float
+    clearcoat_roughness: float
+    transmission: float
+    thin_walled: bool
+    emission_color: np.ndarray
+    emission_strength: float
+    medium: Optional[Tuple[float, np.ndarray, float]] = None
+    pattern: Optional[Tuple[np.ndarray, np.ndarray, float]] = None
+    dynamic_material_id: int = -1
+    dynamic_override_id: int = -1
+
+    def key(self):
+        def t(a):
+            return tuple(float(x) for x in a)
+        return (self.model, t(self.color), float(self.roughness), float(self.metallic),
+                float(self.specular), float(self.specular_weight), t(self.specular_color),
+                float(self.ior), float(self.clearcoat), float(self.clearcoat_roughness),
+                float(self.transmission), bool(self.thin_walled), t(self.emission_color),
+                float(self.emission_strength),
+                None if self.medium is None else (float(self.medium[0]), t(self.medium[1]), float(self.medium[2])),
+                None if self.pattern is None else (t(self.pattern[0]), t(self.pattern[1]), float(self.pattern[2])),
+                self.dynamic_material_id, self.dynamic_override_id)
+
+
+def _black():
+    return _v(0, 0, 0)
+
+
+def mat_standard(color, roughness, metallic, specular, specular_weight, specular_color, ior,
+                 clearcoat, clearcoat_roughness, transmission, thin_walled, emission_color,
+                 emission_strength, model=abi.MODEL_STANDARD) -> Material:
+    return Material(model, color, f32(roughness), f32(metallic), f32(specular), f32(specular_weight),
+                    specular_color, f32(ior), f32(clearcoat), f32(clearcoat_roughness),
+                    f32(transmission), bool(thin_walled), emission_color, f32(emission_strength))
+
+
+def mat_lambert(color, emission_color, emission_strength) -> Material:
+    """MaterialParams::lambert (materials/mod.rs:124-146)."""
+    return mat_standard(color, 1.0, 0.0, 0.5, 1.0, _v(1, 1, 1), 1.5, 0.0, 0.1, 0.0, False,
+                        emission_color, emission_strength, model=abi.MODEL_LAMBERT)
+
+
+def mat_metal(color, roughness, emission_color, emission_strength) -> Material:
+    """MaterialParams::metal (materials/mod.rs:148-175)."""
+    return mat_standard(color, roughness, 1.0, 1.0, 1.0, _v(1, 1, 1), 1.5, 0.0, 0.1, 0.0, False,
+                        emission_color, emission_strength, model=abi.MODEL_METAL)
+
+
+def mat_dielectric(color, ior, roughness, thin_walled, emission_color, emission_strength) -> Material:
+    """MaterialParams::dielectric (materials/mod.rs:177-205)."""
+    return mat_standard(color, roughness, 0.0, 0.5, 1.0, _v(1, 1, 1), ior, 0.0, 0.1, 1.0, thin_walled,
+                        emission_color, emission_strength, model=abi.MODEL_DIELECTRIC)
+
+
+def default_material() -> Material:
+    """default_material (renderer.rs:7937-7953)."""
+    return mat_standard(_v(0.8, 0.8, 0.8), 0.5, 0.0, 0.5, 1.0, _v(1, 1, 1), 1.5, 0.0, 0.1, 0.0,
+                        False, _black(), 0.0)
+
+
+def _medium_from_value(v):
+    if not isinstance(v, Obj):
+        return None
+    ior = _clamp(_or(read_number_field(v, ["ior"]), f32(1.0)), f32(1.0), f32(3.0))
+    absorption = _or(read_spectrum_field(v, "absorption_color"), _v(1, 1, 1))
+    density = max(_or(read_number_field(v, ["density"]), f32(0.0)), f32(0.0))
+    return (ior, absorption, f32(density))
+
+
+def _pattern_from_value(v):
+    if not isinstance(v, Obj):
+        return None
+    kind = (v.type_name or "").lower()
+    if kind in ("checker3d", "checker"):
+        a = _or(read_spectrum_field(v, "color_a"), read_spectrum_field(v, "color1"),
+                read_spectrum_field(v, "a"), _v(0.15, 0.15, 0.15))
+        b = _or(read_spectrum_field(v, "color_b"), read_spectrum_field(v, "color2"),
+                read_spectrum_field(v, "b"), _v(0.85, 0.85, 0.85))
+        scale = _or(read_number_field(v, ["scale", "frequency", "freq"]), f32(4.0))
+        return (a, b, f32(max(scale, f32(1.0e-4))))
+    return None
+
+
+def _emission(obj):
+    col = _or(read_spectrum_field(obj, "emission_color"), _black())
+    strength = max(_or(read_number_field(obj, ["emission_strength"]), f32(0.0)), f32(0.0))
+    return col, strength
+
+
+def _finish(m: Material, obj: Obj) -> Material:
+    m.medium = _medium_from_value(obj.fields.get("medium"))
+    m.pattern = _pattern_from_value(obj.fields.get("pattern"))
+    return m
+
+
+class SceneCompiler:
+    def __init__(self, state: E.EvalState):
+        self.state = state
+        self.ev = E.Evaluator(state)
+        self.materials: List[Material] = []
+        self._mat_index: Dict[tuple, int] = {}
+        self.object_transforms: List[Tuple[np.ndarray, np.ndarray]] = [(_v(0, 0, 0), _v(0, 0, 0))]
+        self.next_object_id = 1
+        self.functions: List[Tuple[list, int, int]] = []  # (nodes, result_type, n_params)
+        self.dynamic_overrides: List[dict] = []
+        self.sorted_material_names = sorted(state.material_defs.keys())
+
+    # -------------------------------------------------------- materials
+    def intern_material(self, m: Material) -> int:
+        k = m.key()
+        if k not in self._mat_index:
+            self._mat_index[k] = len(self.materials)
+            self.materials.append(m)
+        return self._mat_index[k]
+
+    def material_from_object(self, obj: Obj, with_ctx: bool) -> Material:
+        tn = obj.type_name or ""
+        low = tn.lower()
+        if tn in self.state.material_defs:
+            return self.material_from_dynamic_def(tn, obj, with_ctx)
+        em_c, em_s = _emission(obj)
+        if low == "lambert":
+            color = _or(read_spectrum_field(obj, "color"), read_spectrum_field(obj, "base_color"), _v(0.8, 0.8, 0.8))
+            return _finish(mat_lambert(color, em_c, em_s), obj)
+        if low in ("material", "standard", "openpbr"):
+            color = _or(read_spectrum_field(obj, "color"), read_spectrum_field(obj, "base_color"), _v(0.8, 0.8, 0.8))
+            m = mat_standard(
+                color,
+                _clamp(_or(read_number_field(obj, ["roughness"]), f32(0.5)), 0, 1),
+                _clamp(_or(read_number_field(obj, ["metallic", "metalness"]), f32(0.0)), 0, 1),
+                _clamp(_or(read_number_field(obj, ["specular"]), f32(0.5)), 0, 1),
+                _clamp(_or(read_number_field(obj, ["specular_weight"]), f32(1.0)), 0, 1),
+                _or(read_spectrum_field(obj, "specular_color"), _v(1, 1, 1)),
+                _clamp(_or(read_number_field(obj, ["ior", "specular_ior"]), f32(1.5)), 1, 3),
+                _clamp(_or(read_number_field(obj, ["clearcoat"]), f32(0.0)), 0, 1),
+                _clamp(_or(read_number_field(obj, ["clearcoat_roughness"]), f32(0.1)), 0, 1),
+                _clamp(_or(read_number_field(obj, ["transmission", "transmission_weight"]), f32(0.0)), 0, 1),
+                _or(read_number_field(obj, ["thin_walled"]), f32(0.0)) >= 0.5, em_c, em_s)
+            return _finish(m, obj)
+        if low == "metal":
+            color = _or(read_spectrum_field(obj, "color"), read_spectrum_field(obj, "base_color"), _v(0.9, 0.9, 0.9))
+            rough = _clamp(_or(read_number_field(obj, ["roughness"]), f32(0.1)), 0, 1)
+            return _finish(mat_metal(color, rough, em_c, em_s), obj)
+        if low == "dielectric":
+            color = _or(read_spectrum_field(obj, "color"), read_spectrum_field(obj, "base_color"), _v(1, 1, 1))
+            ior = _clamp(_or(read_number_field(obj, ["ior"]), f32(1.5)), 1, 3)
+            rough = _clamp(_or(read_number_field(obj, ["roughness"]), f32(0.0)), 0, 1)
+            thin = _or(read_number_field(obj, ["thin_walled"]), f32(0.0)) >= 0.5
+            return _finish(mat_dielectric(color, ior, rough, thin, em_c, em_s), obj)
+        # material_from_legacy_object (renderer.rs:6572-6632)
+        color = _or(read_spectrum_field(obj, "base_color"), read_spectrum_field(obj, "color"),
+                    read_spectrum_field(obj, "albedo"), _v(0.8, 0.8, 0.8))
+        em_c = _or(read_spectrum_field(obj, "emission_color"), read_spectrum_field(obj, "emission"), _black())
+        em_s = max(_or(read_number_field(obj, ["emission_strength", "emission_weight"]), f32(0.0)), f32(0.0))
+        m = mat_standard(
+            color,
+            _clamp(_or(read_number_field(obj, ["roughness"]), f32(0.5)), 0, 1),
+            _clamp(_or(read_number_field(obj, ["metallic", "metalness"]), f32(0.0)), 0, 1),
+            _clamp(_or(read_number_field(obj, ["specular"]), f32(0.5)), 0, 1),
+            _clamp(_or(read_number_field(obj, ["specular_weight"]), f32(1.0)), 0, 1),
+            _or(read_spectrum_field(obj, "specular_color"), _v(1, 1, 1)),
+            _clamp(_or(read_number_field(obj, ["ior", "specular_ior"]), f32(1.5)), 1, 3),
+            _clamp(_or(read_number_field(obj, ["clearcoat"]), f32(0.0)), 0, 1),
+            _clamp(_or(read_number_field(obj, ["clearcoat_roughness"]), f32(0.1)), 0, 1),
+            _clamp(_or(read_number_field(obj, ["transmission", "transmission_weight"]), f32(0.0)), 0, 1),
+            _or(read_number_field(obj, ["thin_walled", "thinwall"]), f32(0.0)) >= 0.5, em_c, em_s)
+        return _finish(m, obj)
+
+    def material_from_dynamic_def(self, name: str, overrides: Obj, with_ctx: bool) -> Material:
+        d = self.state.material_defs[name]
+        model = d.model or "Standard"
+        if model in ("Standard", "Layered"):
+            m = default_material()
+        elif model == "Metal":
+            m = mat_metal(_v(0.9, 0.9, 0.9), 0.1, _black(), 0.0)
+        elif model == "Dielectric":
+            m = mat_dielectric(_v(1, 1, 1), 1.5, 0.0, False, _black(), 0.0)
+        else:
+            m = mat_lambert(_v(0.8, 0.8, 0.8), _black(), 0.0)
+        # static properties (apply_dynamic_material_properties, renderer.rs:6684-6771)
+        props = self._material_properties(d, overrides)
+        for pname, value in props.items():
+            if pname == "color":
+                s = spectrum_from_value(value)
+                if s is not None:
+                    m.color = s
+            elif pname in ("roughness", "metallic", "specular", "specular_weight", "clearcoat",
+                           "clearcoat_roughness", "transmission") and is_num(value):
+                setattr(m, pname, _clamp(value, 0, 1))
+            elif pname == "ior" and is_num(value):
+                m.ior = _clamp(value, 1, 3)
+            elif pname == "specular_color":
+                s = spectrum_from_value(value)
+                if s is not None:
+                    m.specular_color = s
+            elif pname == "thin_walled" and is_num(value):
+                m.thin_walled = value >= 0.5
+            elif pname == "emission_color":
+                s = spectrum_from_value(value)
+                if s is not None:
+                    m.emission_color = s
+            elif pname == "emission_strength" and is_num(value):
+                m.emission_strength = max(value, f32(0.0))
+            elif pname == "medium":
+                m.medium = _medium_from_value(value)
+        m.dynamic_material_id = self.sorted_material_names.index(name)
+        if with_ctx and overrides.fields:
+            key = repr(sorted((k, repr(v)) for k, v in overrides.fields.items()))
+            if key not in self.dynamic_overrides:
+                self.dynamic_overrides.append(key)
+            m.dynamic_override_id = self.dynamic_overrides.index(key)
+        if model == "Metal":
+            m.model = abi.MODEL_METAL
+        elif model == "Dielectric":
+            m.model = abi.MODEL_DIELECTRIC
+        else:
+            m.model = abi.MODEL_STANDARD
+        return m
+
+    def _material_properties(self, d, overrides: Obj) -> dict:
+        locals_ = dict(overrides.fields)
+        props = {}
+        for st in d.statements:
+            if st[0] == "let":
+                locals_[st[1]] = overrides.fields[st[1]] if st[1] in overrides.fields else \
+                    self.ev.eval(st[2], E.Scope(dict(locals_), d, overrides))
+            elif st[0] == "prop":
+                props[st[1]] = overrides.fields[st[1]] if st[1] in overrides.fields else \
+                    self.ev.eval(st[2], E.Scope(dict(locals_), d, overrides))
+        return props
+
+    def primitive_material_id(self, obj: Obj, default_mat: Material) -> int:
+        mv = obj.fields.get("material")
+        if isinstance(mv, Obj):
+            return self.intern_material(self.material_from_object(mv, True))
+        return self.intern_material(default_mat)
+
+    # ----------------------------------------------------------- functions
+    def add_function(self, nodes, result_type, n_params) -> int:
+        self.functions.append((nodes, result_type, n_params))
+        return len(self.functions) - 1
+
+    def lower_block_fn(self, block, fn_name, param_types, overrides) -> int:
+        for st in block.statements:
+            if st[0] == "fn" and st[1] == fn_name and len(st[2]) == len(param_types):
+                nodes, rt = self.ev.lower_function(st[2], st[3], param_types, None, block, overrides)
+                return self.add_function(nodes, rt, len(param_types))
+        return -1
+
+    def lower_func_value(self, fn: Func, param_types) -> int:
+        if len(fn.params) != len(param_types):
+            return -1
+        nodes, rt = self.ev.lower_function(fn.params, fn.body, param_types, fn.captures)
+        return self.add_function(nodes, rt, len(param_types))
+
+    # ---------------------------------------------------------- objects
+    def alloc_object(self, transform) -> int:
+        oid = self.next_object_id
+        self.next_object_id += 1
+        while len(self.object_transforms) <= oid:
+            self.object_transforms.append((_v(0, 0, 0), _v(0, 0, 0)))
+        self.object_transforms[oid] = (transform[0].copy(), transform[1].copy())
+        return oid
+
+    def compile_sdf(self, value, default_mat: Material) -> dict:
+        if not isinstance(value, Obj):
+            raise RenderError("ExpectedObject", "expected object value for rendering")
+        obj = value
+        tn = obj.type_name or "anonymous"
+
+        def req(name):
+            if name not in obj.fields:
+                raise RenderError("UnsupportedObjectType", f"unsupported object type '{obj.type_name or ''}' for rendering")
+            return obj.fields[name]
+
+        def rnum(name, default):
+            v = req(name)
+            return v if is_num(v) else f32(default)
+
+        node = None
+        if tn in ("Sphere", "Box", "Cylinder", "Torus", "ExtrudePolygon"):
+            transform = read_transform(obj)
+            if tn == "Sphere":
+                radius = _or(read_number_field(obj, ["radius", "r"]), f32(1.0))
+                shell = min(max(_or(read_number_field(obj, ["shell"]), f32(0.0)), f32(0.0)), radius)
+                params = [radius, shell]
+            elif tn == "Box":
+                size = _or(read_vec3_field(obj, "size"), _v(1, 1, 1))
+                half = (size * f32(0.5)).astype(np.float32)
+                mn = min(half[0], half[1], half[2])
+                rnd = min(max(_or(read_number_field(obj, ["round", "rounding"]), f32(0.0)), f32(0.0)), mn)
+                shell = min(max(_or(read_number_field(obj, ["shell"]), f32(0.0)), f32(0.0)), mn)
+                params = [half[0], half[1], half[2], rnd, shell]
+            elif tn == "Cylinder":
+                radius = _or(read_number_field(obj, ["radius", "r"]), f32(1.0))
+                hh = f32(_or(read_number_field(obj, ["height", "h"]), f32(1.0)) * f32(0.5))
+                mn = min(radius, hh)
+                rnd = min(max(_or(read_number_field(obj, ["round", "rounding"]), f32(0.0)), f32(0.0)), mn)
+                shell = min(max(_or(read_number_field(obj, ["shell"]), f32(0.0)), f32(0.0)), mn)
+                params = [radius, hh, rnd, shell]
+            elif tn == "Torus":
+                params = [_or(read_number_field(obj, ["major_radius", "R"]), f32(1.0)),
+                          _or(read_number_field(obj, ["minor_radius", "r"]), f32(0.25))]
+            else:
+                sides_v = read_number_field(obj, ["sides", "n"])
+                sides = max(int(np.round(sides_v)) if sides_v is not None else 6, 3)
+                radius = _or(read_number_field(obj, ["radius", "r"]), f32(1.0))
+                hh = f32(_or(read_number_field(obj, ["height", "h"]), f32(1.0)) * f32(0.5))
+                mn = min(radius, hh)
+                rnd = min(max(_or(read_number_field(obj, ["round", "rounding"]), f32(0.0)), f32(0.0)), mn)
+                shell = min(max(_or(read_number_field(obj, ["shell"]), f32(0.0)), f32(0.0)), mn)
+                params = [f32(sides), radius, hh, rnd, shell]
+            oid = self.alloc_object(transform)
+            mid = self.primitive_material_id(obj, default_mat)
+            node = dict(kind=abi.NODE["EXTRUDE_POLYGON" if tn == "ExtrudePolygon" else tn.upper()],
+                        transform=transform, param=params, object_id=oid, material_id=mid)
+        elif tn in self.state.sdf_defs:
+            block = self.state.sdf_defs[tn]
+            transform = read_transform(obj)
+            oid = self.alloc_object(transform)
+            mid = self.primitive_material_id(obj, default_mat)
+            half = self.custom_bounds_half_extents(block, obj)
+            node = dict(kind=abi.NODE["CUSTOM"], transform=transform, param=list(half),
+                        object_id=oid, material_id=mid,
+                        fn_domain=self.lower_block_fn(block, "domain", [1], obj),
+                        fn_distance=self.lower_block_fn(block, "distance", [1], obj),
+                        fn_post=self.lower_block_fn(block, "distance_post", [0, 1], obj))
+        elif tn in ("add", "intersect", "sub"):
+            lhs = self.compile_sdf(req("lhs"), default_mat)
+            rhs = self.compile_sdf(req("rhs"), default_mat)
+            kind = {"add": "UNION", "intersect": "INTERSECT", "sub": "SUBTRACT"}[tn]
+            node = dict(kind=abi.NODE[kind], children=[lhs, rhs])
+        elif tn in ("union_round", "union_chamfer", "union_soft", "intersect_round", "intersect_chamfer",
+                    "diff_round", "diff_chamfer", "pipe", "engrave"):
+            lhs = self.compile_sdf(req("lhs"), default_mat)
+            rhs = self.compile_sdf(req("rhs"), default_mat)
+            node = dict(kind=abi.NODE[tn.upper()], children=[lhs, rhs], param=[rnum("r", 0.0)])
+        elif tn in ("union_columns", "union_stairs", "intersect_columns", "intersect_stairs",
+                    "diff_columns", "diff_stairs"):
+            lhs = self.compile_sdf(req("lhs"), default_mat)
+            rhs = self.compile_sdf(req("rhs"), default_mat)
+            node = dict(kind=abi.NODE[tn.upper()], children=[lhs, rhs],
+                        param=[rnum("r", 0.0), rnum("n", 4.0)])
+        elif tn in ("groove", "tongue"):
+            lhs = self.compile_sdf(req("lhs"), default_mat)
+            rhs = self.compile_sdf(req("rhs"), default_mat)
+            node = dict(kind=abi.NODE[tn.upper()], children=[lhs, rhs],
+                        param=[rnum("ra", 0.0), rnum("rb", 0.0)])
+        elif tn in ("mirror_x", "mirror_y", "mirror_z"):
+            base_v = req("base")
+            lhs = self.compile_sdf(base_v, default_mat)
+            axis = "xyz".index(tn[-1])
+            rhs = self.remap(self.compile_sdf(base_v, default_mat), lambda t: _mirror(t, axis))
+            node = dict(kind=abi.NODE["UNION"], children=[lhs, rhs])
+        elif tn in ("repeat_x", "repeat_y", "repeat_z"):
+            base_v = req("base")
+            spacing = rnum("spacing", 0.0)
+            cv = req("count")
+            count = max(int(np.round(max(float(np.round(cv)), 1.0))), 1) if is_num(cv) else 1
+            axis = "xyz".index(tn[-1])
+            start = f32(f32(-0.5) * f32(max(count - 1, 0))) * spacing
+            start = f32(start)
+
+            def offset(delta):
+                return lambda t: _offset(t, axis, delta)
+            root = self.remap(self.compile_sdf(base_v, default_mat), offset(start))
+            for i in range(1, count):
+                part = self.remap(self.compile_sdf(base_v, default_mat), offset(f32(start + f32(f32(i) * spacing))))
+                root = dict(kind=abi.NODE["UNION"], children=[root, part])
+            node = root
+        elif tn in ("slice_x", "slice_y", "slice_z"):
+            base = self.compile_sdf(req("base"), default_mat)
+            mn = req("min")
+            mx = req("max")
+            node = dict(kind=abi.NODE["SLICE"], children=[base],
+                        param=[f32("xyz".index(tn[-1])), mn if is_num(mn) else f32(-np.inf),
+                               mx if is_num(mx) else f32(np.inf)])
+        elif tn == "noise":
+            base = self.compile_sdf(req("base"), default_mat)
+            ov = req("octaves")
+            octaves = int(min(max(np.round(ov), 1.0), 16.0)) if is_num(ov) else 7
+            sv = req("scale")
+            lv = req("lacunarity")
+            scale = max(sv, f32(1e-4)) if is_num(sv) else f32(1.0)
+            lac = max(lv, f32(1e-4)) if is_num(lv) else f32(1.0)
+            base, transform, bounds = self.split_modifier_base(base)
+            node = dict(kind=abi.NODE["NOISE"], children=[base], transform=transform,
+                        param=[f32(octaves), scale, lac], bounds=bounds)
+        elif tn == "smooth":
+            base = self.compile_sdf(req("base"), default_mat)
+            node = dict(kind=abi.NODE["SMOOTH"], children=[base], param=[rnum("k", 0.0)])
+        else:
+            raise RenderError("UnsupportedObjectType", f"unsupported object type '{tn}' for rendering")
+        return self.apply_object_modifier_hooks(obj, node)
+
+    def custom_bounds_half_extents(self, block, obj: Obj):
+        try:
+            v = self.ev.eval_block_zero_arg(block, "bounds", obj)
+        except E.ForgeEvalError:
+            v = None
+        if isinstance(v, Obj):
+            x = _or(read_number_field(v, ["x"]), f32(0))
+            y = _or(read_number_field(v, ["y"]), f32(0))
+            z = _or(read_number_field(v, ["z"]), f32(0))
+            return [max(abs(x), f32(1e-3)), max(abs(y), f32(1e-3)), max(abs(z), f32(1e-3))]
+        if is_num(v):
+            r = max(abs(v), f32(1e-3))
+            return [r, r, r]
+        return [f32(10000.0)] * 3
+
+    def apply_object_modifier_hooks(self, obj: Obj, node: dict) -> dict:
+        dom = obj.fields.get("domain")
+        post = obj.fields.get("distance_post")
+        if "domain" in obj.fields or "distance_post" in obj.fields:
+            base, transform, bounds = self.split_modifier_base(node)
+            node = base
+            if isinstance(dom, Func):
+                node = dict(kind=abi.NODE["DOMAIN_MODIFIER"], children=[node], transform=transform,
+                            bounds=bounds, fn_domain=self.lower_func_value(dom, [1]))
+            if isinstance(post, Func):
+                node = dict(kind=abi.NODE["DISTANCE_POST_MODIFIER"], children=[node], transform=transform,
+                            bounds=bounds, fn_post=self.lower_func_value(post, [0, 1]))
+        return node
+
+    def split_modifier_base(self, node: dict):
+        bounds = sdf_bounds(node)
+        if node["kind"] in (abi.NODE["SPHERE"], abi.NODE["BOX"], abi.NODE["CYLINDER"], abi.NODE["TORUS"],
+                            abi.NODE["EXTRUDE_POLYGON"], abi.NODE["CUSTOM"], abi.NODE["NOISE"]):
+            transform = node.get("transform", (_v(0, 0, 0), _v(0, 0, 0)))
+            base = dict(node)
+            base["transform"] = (_v(0, 0, 0), _v(0, 0, 0))
+            if node["kind"] == abi.NODE["NOISE"]:
+                base["bounds"] = bounds
+            return base, transform, bounds
+        return node, (_v(0, 0, 0), _v(0, 0, 0)), bounds
+
+    def remap(self, node: dict, fn) -> dict:
+        """remap_sdf_node (renderer.rs:2047-2325): new transform and object id per leaf."""
+        k = node["kind"]
+        out = dict(node)
+        if k in (abi.NODE["SPHERE"], abi.NODE["BOX"], abi.NODE["CYLINDER"], abi.NODE["TORUS"],
+                 abi.NODE["EXTRUDE_POLYGON"], abi.NODE["CUSTOM"]):
+            out["transform"] = fn(node["transform"])
+            out["object_id"] = self.alloc_object(out["transform"])
+            return out
+        if k in (abi.NODE["DOMAIN_MODIFIER"], abi.NODE["DISTANCE_POST_MODIFIER"], abi.NODE["NOISE"]):
+            out["transform"] = fn(node["transform"])
+            out["bounds"] = _transformed_modifier_bounds(node["children"][0], out["transform"])
+            out["children"] = [self.remap(node["children"][0], fn)]
+            return out
+        if "children" in node:
+            out["children"] = [self.remap(c, fn) for c in node["children"]]
+        return out
+
+
+def _mirror(t, axis):
+    c, r = t[0].copy(), t[1].copy()
+    if axis == 0:
+        c[0] = -c[0]; r[1] = -r[1]; r[2] = -r[2]
+    elif axis == 1:
+        c[1] = -c[1]; r[0] = -r[0]; r[2] = -r[2]
+    else:
+        c[2] = -c[2]; r[0] = -r[0]; r[1] = -r[1]
+    return (c, r)
+
+
+def _offset(t, axis, delta):
+    c = t[0].copy()
+    c[axis] = f32(c[axis] + f32(delta))
+    return (c, t[1].copy())
+
+
+# ------------------------------------------------------------------ bounds
+def _transform_offset(transform, off):
+    rot = transform[1]
+    return _rot_z(_rot_y(_rot_x(off, rot[0]), rot[1]), rot[2])
+
+
+def _aabb_union(a, b):
+    return (np.minimum(a[0], b[0]).astype(np.float32), np.maximum(a[1], b[1]).astype(np.float32))
+
+
+def _cube(center, r):
+    r = f32(r)
+    return ((center - r).astype(np.float32), (center + r).astype(np.float32))
+
+
+def _transformed_modifier_bounds(base: dict, transform):
+    b = dict(base)
+    if b["kind"] in (abi.NODE["SPHERE"], abi.NODE["BOX"], abi.NODE["CYLINDER"], abi.NODE["TORUS"],
+                     abi.NODE["EXTRUDE_POLYGON"], abi.NODE["CUSTOM"]):
+        b["transform"] = transform
+    return sdf_bounds(b)
+
+
+def sdf_bounds(node: dict):
+    """sdf_bounds (renderer.rs:5241-5386), evaluated in f32."""
+    k = node["kind"]
+    p = node.get("param", [])
+    center = node.get("transform", (_v(0, 0, 0), _v(0, 0, 0)))[0]
+    if k == abi.NODE["SPHERE"]:
+        return _cube(center, p[0])
+    if k == abi.NODE["BOX"]:
+        return _cube(center, _length(_v(p[0], p[1], p[2])))
+    if k in (abi.NODE["CYLINDER"],):
+        return _cube(center, np.sqrt(f32(f32(p[0] * p[0]) + f32(p[1] * p[1]))))
+    if k == abi.NODE["EXTRUDE_POLYGON"]:
+        return _cube(center, np.sqrt(f32(f32(p[1] * p[1]) + f32(p[2] * p[2]))))
+    if k == abi.NODE["TORUS"]:
+        return _cube(center, f32(p[0] + p[1]))
+    if k == abi.NODE["CUSTOM"]:
+        h = p
+        mn = _v(np.inf, np.inf, np.inf)
+        mx = _v(-np.inf, -np.inf, -np.inf)
+        for sx in (-1, 1):
+            for sy in (-1, 1):
+                for sz in (-1, 1):
+                    w = (center + _transform_offset(node["transform"], _v(sx * h[0], sy * h[1], sz * h[2]))).astype(np.float32)
+                    mn = np.minimum(mn, w)
+                    mx = np.maximum(mx, w)
+        return (mn, mx)
+    if k in (abi.NODE["DOMAIN_MODIFIER"], abi.NODE["DISTANCE_POST_MODIFIER"], abi.NODE["NOISE"]):
+        return node["bounds"]
+    ch = node.get("children", [])
+    if k == abi.NODE["SUBTRACT"]:
+        return sdf_bounds(ch[0])
+    if k == abi.NODE["SLICE"]:
+        b = sdf_bounds(ch[0])
+        mn, mx = b[0].copy(), b[1].copy()
+        axis = int(p[0])
+        mn[axis] = max(mn[axis], p[1])
+        mx[axis] = min(mx[axis], p[2])
+        if mx[axis] < mn[axis]:
+            mx[axis] = mn[axis]
+        return (mn, mx)
+    if k == abi.NODE["SMOOTH"]:
+        b = sdf_bounds(ch[0])
+        d = f32(p[0] * f32(0.1))
+        return ((b[0] - d).astype(np.float32), (b[1] + d).astype(np.float32))
+    return _aabb_union(sdf_bounds(ch[0]), sdf_bounds(ch[1]))
+
+
+# ---------------------------------------------------------- scene-level
+def find_scene_root(state: E.EvalState):
+    for key in ("scene", "result", "b", "s"):
+        if key in state.bindings:
+            return state.bindings[key].value
+    return None
+
+
+def _extract_material_kind(comp: SceneCompiler, value) -> Optional[Material]:
+    if not isinstance(value, Obj):
+        return None
+    mv = value.fields.get("material")
+    if isinstance(mv, Obj):
+        return comp.material_from_object(mv, False)
+    keys = list(value.fields.keys())
+    # deterministic visiting order: lhs, rhs, base first, then the rest
+    order = [k for k in ("lhs", "rhs", "base") if k in value.fields] + [k for k in keys if k not in ("lhs", "rhs", "base")]
+    for k in order:
+        found = _extract_material_kind(comp, value.fields[k])
+        if found is not None:
+            return found
+    tn = (value.type_name or "").lower()
+    if tn in ("material", "standard", "openpbr", "lambert", "metal", "dielectric"):
+        return comp.material_from_object(value, False)
+    if value.type_name in comp.state.material_defs:
+        return comp.material_from_dynamic_def(value.type_name, value, False)
+    return None
+
+
+def parse_material(comp: SceneCompiler, root) -> Material:
+    b = comp.state.bindings.get("material")
+    if b is not None:
+        m = _extract_material_kind(comp, b.value)
+        if m is not None:
+            return m
+    m = _extract_material_kind(comp, root)
+    return m if m is not None else default_material()
+
+
+def _read_light_spectrum(obj: Obj, legacy):
+    col = read_spectrum_field(obj, "color")
+    if col is not None:
+        inten = _or(read_number_field(obj, ["intensity"]), f32(1.0))
+        return (col * max(inten, f32(0.0))).astype(np.float32)
+    for f in legacy:
+        s = read_spectrum_field(obj, f)
+        if s is not None:
+            return s
+    return None
+
+
+def parse_lights(state: E.EvalState) -> list:
+    lights = []
+    for b in state.bindings.values():
+        v = b.value
+        if not isinstance(v, Obj) or v.type_name is None:
+            continue
+        if v.type_name == "PointLight":
+            pos = _or(read_vec3_field(v, "position"), read_center(v))
+            rad = _or(_read_light_spectrum(v, ["intensity"]), _v(8, 8, 8))
+            lights.append((abi.LIGHT_POINT, pos, rad, 0.0, 1))
+        elif v.type_name == "EnvLight":
+            rad = _or(_read_light_spectrum(v, ["radiance", "color"]), _v(0.15, 0.15, 0.15))
+            lights.append((abi.LIGHT_ENV, _v(0, 0, 0), rad, 0.0, 1))
+        elif v.type_name == "SphereLight":
+            pos = _or(read_vec3_field(v, "position"), read_center(v))
+            radius = max(_or(read_number_field(v, ["radius", "r"]), f32(0.35)), f32(0.0))
+            rad = _or(_read_light_spectrum(v, ["intensity"]), _v(8, 8, 8))
+            sv = read_number_field(v, ["samples"])
+            samples = int(max(sv, f32(1.0))) if sv is not None else 8
+            lights.append((abi.LIGHT_SPHERE, pos, rad, radius, samples))
+    if not lights:
+        lights.append((abi.LIGHT_POINT, _v(3, 3, 6), _v(6, 6, 6), 0.0, 1))
+        lights.append((abi.LIGHT_ENV, _v(0, 0, 0), _v(0.1, 0.1, 0.1), 0.0, 1))
+    return lights
+
+
+def find_environment_name(state: E.EvalState) -> Optional[str]:
+    for key in ("environment", "env", "sky"):
+        b = state.bindings.get(key)
+        if b is not None and isinstance(b.value, Obj) and b.value.type_name in state.environment_defs:
+            return b.value.type_name
+    if len(state.environment_defs) == 1:
+        return next(iter(state.environment_defs))
+    return None
+
+
+@dataclass
+class CompiledScene:
+    """Owner of the numpy arrays behind one `ft_scene_ir` (keep alive while in use)."""
+    nodes: np.ndarray
+    materials: np.ndarray
+    object_transforms: np.ndarray
+    lights: np.ndarray
+    functions: np.ndarray
+    exprs: np.ndarray
+    env_fn: int
+    camera: dict
+    root: int
+    leaf_count: int = 0
+    _ir: object = None
+
+    def ir(self) -> abi.SceneIR:
+        if self._ir is None:
+            ir = abi.SceneIR()
+            ir.abi_version = abi.ABI_VERSION
+            ir.root = self.root
+            ir.n_nodes = len(self.nodes)
+            ir.n_materials = len(self.materials)
+            ir.n_objects = len(self.object_transforms)
+            ir.n_lights = len(self.lights)
+            ir.n_functions = len(self.functions)
+            ir.n_exprs = len(self.exprs)
+            ir.env_fn = self.env_fn
+            ir.nodes = self.nodes.ctypes.data
+            ir.materials = self.materials.ctypes.data
+            ir.object_transforms = self.object_transforms.ctypes.data
+            ir.lights = self.lights.ctypes.data if len(self.lights) else None
+            ir.functions = self.functions.ctypes.data if len(self.functions) else None
+            ir.exprs = self.exprs.ctypes.data if len(self.exprs) else None
+            cam = self.camera
+            ir.camera.present = 1 if cam["present"] else 0
+            ir.camera.has_origin = 1 if cam.get("origin") is not None else 0
+            ir.camera.has_target = 1 if cam.get("target") is not None else 0
+            ir.camera.has_fov = 1 if cam.get("fov") is not None else 0
+            for i in range(3):
+                if cam.get("origin") is not None:
+                    ir.camera.origin[i] = float(cam["origin"][i])
+                if cam.get("target") is not None:
+                    ir.camera.target[i] = float(cam["target"][i])
+            if cam.get("fov") is not None:
+                ir.camera.fov_y_degrees = float(cam["fov"])
+            self._ir = ir
+        return self._ir
+
+
+def _flatten(root: dict, nodes: list) -> int:
+    kids = [_flatten(c, nodes) for c in root.get("children", [])]
+    rec = np.zeros((), dtype=abi.NODE_DTYPE)
+    rec["kind"] = root["kind"]
+    rec["lhs"] = kids[0] if len(kids) > 0 else -1
+    rec["rhs"] = kids[1] if len(kids) > 1 else -1
+    rec["object_id"] = root.get("object_id", 0)
+    rec["material_id"] = root.get("material_id", 0)
+    rec["fn_domain"] = root.get("fn_domain", -1)
+    rec["fn_distance"] = root.get("fn_distance", -1)
+    rec["fn_post"] = root.get("fn_post", -1)
+    t = root.get("transform")
+    if t is not None:
+        rec["center"] = t[0]
+        rec["rot_deg"] = t[1]
+    prm = root.get("param", [])
+    for i, v in enumerate(prm):
+        rec["param"][i] = v
+    b = root.get("bounds")
+    if b is not None:
+        rec["bounds_min"] = b[0]
+        rec["bounds_max"] = b[1]
+    else:
+        rec["bounds_min"] = np.nan
+        rec["bounds_max"] = np.nan
+    nodes.append(rec)
+    return len(nodes) - 1
+
+
+def _material_record(m: Material) -> np.ndarray:
+    r = np.zeros((), dtype=abi.MATERIAL_DTYPE)
+    r["model"] = m.model
+    r["thin_walled"] = 1 if m.thin_walled else 0
+    r["color"] = m.color
+    for k in ("roughness", "metallic", "specular", "specular_weight", "ior", "clearcoat",
+              "clearcoat_roughness", "transmission", "emission_strength"):
+        r[k] = getattr(m, k)
+    r["specular_color"] = m.specular_color
+    r["emission_color"] = m.emission_color
+    if m.medium is not None:
+        r["has_medium"] = 1
+        r["medium_ior"] = m.medium[0]
+        r["medium_absorption"] = m.medium[1]
+        r["medium_density"] = m.medium[2]
+    else:
+        r["medium_ior"] = 1.0
+        r["medium_absorption"] = (1.0, 1.0, 1.0)
+    if m.pattern is not None:
+        r["pattern"] = abi.PATTERN_CHECKER3D
+        r["pattern_color_a"] = m.pattern[0]
+        r["pattern_color_b"] = m.pattern[1]
+        r["pattern_scale"] = m.pattern[2]
+    r["dynamic_material_id"] = m.dynamic_material_id
+    r["dynamic_override_id"] = m.dynamic_override_id
+    return r
+
+
+def compile_scene(state: E.EvalState) -> CompiledScene:
+    """find_scene_root + parse_material + compile_scene + build_render_setup, flattened."""
+    root_value = find_scene_root(state)
+    if root_value is None:
+        raise RenderError("MissingSceneRoot", "no suitable scene root found (expected binding like scene/result/b/s)")
+    comp = SceneCompiler(state)
+    default_mat = parse_material(comp, root_value)
+    comp.intern_material(default_mat)
+    tree = comp.compile_sdf(root_value, default_mat)
+
+    nodes: list = []
+    root = _flatten(tree, nodes)
+    node_arr = np.array(nodes, dtype=abi.NODE_DTYPE)
+
+    mats = np.array([_material_record(m) for m in comp.materials], dtype=abi.MATERIAL_DTYPE)
+    ot = np.zeros((len(comp.object_transforms), 6), dtype=np.float32)
+    for i, (c, r) in enumerate(comp.object_transforms):
+        ot[i, :3] = c
+        ot[i, 3:] = r
+
+    env_fn = -1
+    env_name = find_environment_name(state)
+    if env_name is not None:
+        env_fn = comp.lower_block_fn(state.environment_defs[env_name], "color", [1], None)
+
+    lights = parse_lights(state)
+    light_arr = np.zeros(len(lights), dtype=abi.LIGHT_DTYPE)
+    for i, (kind, pos, rad, radius, samples) in enumerate(lights):
+        light_arr[i]["kind"] = kind
+        light_arr[i]["position"] = pos
+        light_arr[i]["radiance"] = rad
+        light_arr[i]["radius"] = radius
+        light_arr[i]["samples"] = samples
+
+    fn_arr = np.zeros(len(comp.functions), dtype=abi.FUNCTION_DTYPE)
+    exprs = []
+    for i, (fnodes, rt, npar) in enumerate(comp.functions):
+        fn_arr[i] = (len(exprs), len(fnodes), npar, rt)
+        for (op, ty, a, b, c, d, val) in fnodes:
+            exprs.append((op, ty, a, b, c, d, val, 0.0))
+    expr_arr = np.array(exprs, dtype=abi.EXPR_DTYPE) if exprs else np.zeros(0, dtype=abi.EXPR_DTYPE)
+
+    cam = {"present": False}
+    cb = state.bindings.get("camera")
+    if cb is not None and isinstance(cb.value, Obj):
+        co = cb.value
+        cam = {"present": True,
+               "origin": _or(read_vec3_field(co, "origin"), read_vec3_field(co, "position")),
+               "target": read_vec3_field(co, "target"),
+               "fov": read_number_field(co, ["fov_y", "fov"])}
+
+    leaves = int(np.sum(node_arr["kind"] <= abi.NODE["CUSTOM"]))
+    return CompiledScene(node_arr, mats, ot, light_arr, fn_arr, expr_arr, env_fn, cam, root, leaves)
+
+
+# ---------------------------------------------------------- render settings
+@dataclass
+class SceneRenderSettings:
+    """SceneRenderSettings + extract_scene_render_settings (renderer.rs:65-80, 1316-1373)."""
+    width: Optional[int] = None
+    height: Optional[int] = None
+    max_steps: Optional[int] = None
+    max_dist: Optional[float] = None
+    epsilon: Optional[float] = None
+    step_scale: Optional[float] = None
+    camera_z: Optional[float] = None
+    fov_y_degrees: Optional[float] = None
+    accel: Optional[int] = None
+    trace_spp: Optional[int] = None
+    trace_bounces: Optional[int] = None
+
+
+def _float_to_u32(v):
+    if v is None or not np.isfinite(v) or v < 1.0:
+        return None
+    r = np.round(v)
+    return int(r) if abs(r - v) < 1e-5 else None
+
+
+def extract_scene_render_settings(state: E.EvalState) -> SceneRenderSettings:
+    out = SceneRenderSettings()
+    obj = None
+    for key in ("render", "render_settings", "settings"):
+        b = state.bindings.get(key)
+        if b is not None and isinstance(b.value, Obj):
+            obj = b.value
+            break
+    if obj is None:
+        return out
+    out.width = _float_to_u32(read_number_field(obj, ["width"]))
+    out.height = _float_to_u32(read_number_field(obj, ["height"]))
+    out.max_steps = _float_to_u32(read_number_field(obj, ["max_steps"]))
+    out.max_dist = read_number_field(obj, ["max_dist"])
+    out.epsilon = read_number_field(obj, ["epsilon"])
+    out.step_scale = read_number_field(obj, ["step_scale"])
+    out.camera_z = read_number_field(obj, ["camera_z"])
+    out.fov_y_degrees = read_number_field(obj, ["fov_y", "fov"])
+    acc = obj.fields.get("accel")
+    if isinstance(acc, Obj):
+        out.accel = {"naive": 0, "bvh": 1, "bricks": 2}.get((acc.type_name or "").lower())
+    elif is_num(acc):
+        out.accel = {0: 0, 1: 1, 2: 2}.get(int(acc))
+    out.trace_spp = _float_to_u32(read_number_field(obj, ["spp", "samples", "trace_spp"]))
+    out.trace_bounces = _float_to_u32(read_number_field(obj, ["bounces", "max_bounces", "trace_bounces", "depth"]))
+    return out

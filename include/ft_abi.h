@@ -1,0 +1,364 @@
+/*
+ * ft_abi.h — C ABI of the MI355X sphere-tracing backend (libft_hip.so).
+ *
+ * The reference renderer (ForgedThoughts, Rust) selects its backend through
+ * `AccelMode` and dispatches to three public entry points:
+ *   render_depth_png_with_accel            crates/forgedthoughts/src/renderer.rs:1139-1156
+ *   render_preview_progressive_with_accel  crates/forgedthoughts/src/renderer.rs:1238-1277
+ *   render_ray_progressive_with_accel      crates/forgedthoughts/src/renderer.rs:1201-1236
+ * Those take an `EvalState`, compile it into a `CompiledScene` (renderer.rs:480-488,
+ * `compile_scene` renderer.rs:1384-1401) and build an `Accelerator` (renderer.rs:609-623).
+ * The per-point `Accelerator` trait is far too fine-grained to cross an FFI, so this
+ * boundary sits one level up: the caller hands over the flattened compiled scene
+ * (`ft_scene_ir`, the data of `CompiledScene` + `RenderSetup`, renderer.rs:480-501),
+ * the library builds its device-side acceleration data once (`ft_scene_create`), and
+ * renders whole images (`ft_render_depth`, `ft_render_whitted`).
+ *
+ * Conventions
+ *   - every entry point returns an int status (FT_OK == 0); on failure a message is
+ *     available from ft_last_error() (thread-local), mirroring RenderError
+ *     (renderer.rs:158-168) plus device-side failures;
+ *   - host output buffers are caller-owned; device memory is library-owned;
+ *   - each scene handle owns one HIP stream; a handle is not thread-safe, distinct
+ *     handles are;
+ *   - `ft_render_options` / `ft_ray_settings` mirror RenderOptions (renderer.rs:31-41)
+ *     and RaySettings (renderer.rs:128-134) field for field.
+ */
+#ifndef FT_ABI_H
+#define FT_ABI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FT_ABI_VERSION 1u
+
+/* ---------------------------------------------------------------- status */
+enum {
+  FT_OK = 0,
+  FT_ERR_MISSING_SCENE_ROOT = 1,   /* RenderError::MissingSceneRoot */
+  FT_ERR_UNSUPPORTED_OBJECT = 2,   /* RenderError::UnsupportedObjectType */
+  FT_ERR_EXPECTED_OBJECT = 3,      /* RenderError::ExpectedObject */
+  FT_ERR_IMAGE = 4,                /* RenderError::Image */
+  FT_ERR_INVALID_ARGUMENT = 5,
+  FT_ERR_GPU_UNAVAILABLE = 6,
+  FT_ERR_COMPILE_FAILED = 7,
+  FT_ERR_OUT_OF_MEMORY = 8,
+  FT_ERR_LAUNCH_FAILED = 9,
+  FT_ERR_CALLBACK_ABORT = 10
+};
+
+/* --------------------------------------------------------- accel modes */
+/* AccelMode (renderer.rs:58-63). */
+enum { FT_ACCEL_NAIVE = 0, FT_ACCEL_BVH = 1, FT_ACCEL_BRICKS = 2 };
+
+/* ----------------------------------------------------------- scene IR  */
+/* SdfNode variants (renderer.rs:170-355).  Nodes are stored children-first
+ * (post-order); `lhs`/`rhs` index into the node array. */
+enum {
+  FT_NODE_SPHERE = 0,
+  FT_NODE_BOX = 1,
+  FT_NODE_CYLINDER = 2,
+  FT_NODE_TORUS = 3,
+  FT_NODE_EXTRUDE_POLYGON = 4,
+  FT_NODE_CUSTOM = 5,
+  FT_NODE_DOMAIN_MODIFIER = 6,
+  FT_NODE_DISTANCE_POST_MODIFIER = 7,
+  FT_NODE_NOISE = 8,
+  FT_NODE_UNION = 9,
+  FT_NODE_INTERSECT = 10,
+  FT_NODE_SUBTRACT = 11,
+  FT_NODE_UNION_ROUND = 12,
+  FT_NODE_UNION_CHAMFER = 13,
+  FT_NODE_UNION_COLUMNS = 14,
+  FT_NODE_UNION_STAIRS = 15,
+  FT_NODE_UNION_SOFT = 16,
+  FT_NODE_INTERSECT_ROUND = 17,
+  FT_NODE_INTERSECT_CHAMFER = 18,
+  FT_NODE_INTERSECT_COLUMNS = 19,
+  FT_NODE_INTERSECT_STAIRS = 20,
+  FT_NODE_DIFF_ROUND = 21,
+  FT_NODE_DIFF_CHAMFER = 22,
+  FT_NODE_DIFF_COLUMNS = 23,
+  FT_NODE_DIFF_STAIRS = 24,
+  FT_NODE_PIPE = 25,
+  FT_NODE_ENGRAVE = 26,
+  FT_NODE_GROOVE = 27,
+  FT_NODE_TONGUE = 28,
+  FT_NODE_SLICE = 29,
+  FT_NODE_SMOOTH = 30,
+  FT_NODE_KIND_COUNT = 31
+};
+
+/*
+ * One SdfNode.  `param` layout per kind (values exactly as compile_sdf stores them,
+ * renderer.rs:1414-1742, e.g. Box stores the HALF size):
+ *   SPHERE            radius, shell
+ *   BOX               half_x, half_y, half_z, round, shell
+ *   CYLINDER          radius, half_height, round, shell
+ *   TORUS             major_radius, minor_radius
+ *   EXTRUDE_POLYGON   sides, radius, half_height, round, shell
+ *   CUSTOM            bounds_half_x, bounds_half_y, bounds_half_z
+ *   NOISE             octaves, scale, lacunarity
+ *   *_ROUND/CHAMFER/SOFT, PIPE, ENGRAVE      r
+ *   *_COLUMNS/STAIRS  r, n
+ *   GROOVE, TONGUE    ra, rb
+ *   SLICE             axis (0/1/2), min, max
+ *   SMOOTH            k
+ * DOMAIN_MODIFIER / DISTANCE_POST_MODIFIER / NOISE carry the bounds computed at scene
+ * compile time (renderer.rs:1810-1948) in bounds_min/bounds_max; other kinds ignore them.
+ * CUSTOM uses fn_domain / fn_distance / fn_post (the sdf definition's `domain`,
+ * `distance`, `distance_post`); DOMAIN_MODIFIER uses fn_domain; DISTANCE_POST_MODIFIER
+ * uses fn_post.  Unused function slots are -1.
+ */
+typedef struct ft_node {
+  uint32_t kind;
+  int32_t lhs;
+  int32_t rhs;
+  uint32_t object_id;
+  uint32_t material_id;
+  int32_t fn_domain;
+  int32_t fn_distance;
+  int32_t fn_post;
+  float center[3];
+  float rot_deg[3];
+  float param[6];
+  float bounds_min[3];
+  float bounds_max[3];
+} ft_node;
+
+/* ------------------------------------------------------- expression IR */
+/*
+ * Forge function bodies (ast.rs:126-171 subset; no control flow exists in the
+ * language) lowered to a typed expression DAG.  Helper calls are inlined and captured
+ * `let` values are folded into constants by the front end.  Each function owns the
+ * contiguous range [first, first+count) of `exprs`; operands a/b/c index into that
+ * range (relative to `first`) and always precede the node; the last node is the
+ * result.  `type` is FT_TYPE_SCALAR or FT_TYPE_VEC3.
+ */
+enum { FT_TYPE_SCALAR = 0, FT_TYPE_VEC3 = 1 };
+
+enum {
+  FT_EXPR_CONST = 0,      /* value */
+  FT_EXPR_PARAM = 1,      /* a = parameter index */
+  FT_EXPR_VEC3 = 2,       /* vec3(a, b, c) */
+  FT_EXPR_GET_X = 3,      /* a.x */
+  FT_EXPR_GET_Y = 4,
+  FT_EXPR_GET_Z = 5,
+  FT_EXPR_NEG = 6,
+  FT_EXPR_ADD = 7,        /* binary ops broadcast scalar over vec3 (eval.rs:1228-1256) */
+  FT_EXPR_SUB = 8,
+  FT_EXPR_MUL = 9,
+  FT_EXPR_DIV = 10,
+  FT_EXPR_ABS = 11,       /* unary maps apply per component (eval.rs:4230-4242) */
+  FT_EXPR_FLOOR = 12,
+  FT_EXPR_CEIL = 13,
+  FT_EXPR_FRACT = 14,
+  FT_EXPR_SQRT = 15,
+  FT_EXPR_SIN = 16,
+  FT_EXPR_COS = 17,
+  FT_EXPR_SATURATE = 18,
+  FT_EXPR_MIN = 19,
+  FT_EXPR_MAX = 20,
+  FT_EXPR_POW = 21,
+  FT_EXPR_CLAMP = 22,     /* clamp(x, a, b) with swapped bounds allowed */
+  FT_EXPR_MIX = 23,
+  FT_EXPR_STEP = 24,      /* step(edge, x) */
+  FT_EXPR_SMOOTHSTEP = 25,
+  FT_EXPR_DOT = 26,
+  FT_EXPR_LENGTH = 27,
+  FT_EXPR_NORMALIZE = 28,
+  FT_EXPR_ROTATE_X = 29,  /* rotate_x(v, degrees) */
+  FT_EXPR_ROTATE_Y = 30,
+  FT_EXPR_ROTATE_Z = 31,
+  FT_EXPR_SPHERE_DIST = 32,   /* Sphere.distance(p, r)      eval.rs:1899-1909 */
+  FT_EXPR_BOX_DIST = 33,      /* Box.distance(p, half)      eval.rs:1881-1898 */
+  FT_EXPR_CYLINDER_DIST = 34, /* Cylinder.distance(p, r, h) eval.rs:1910-1929 */
+  FT_EXPR_TORUS_DIST = 35,    /* Torus.distance(p, R, r)    eval.rs:1930-1945 */
+  FT_EXPR_VALUE_NOISE = 36,   /* value_noise_3d(p)          eval.rs:4292-4317 */
+  FT_EXPR_FBM = 37,           /* fbm_3d(p, octaves, scale, lacunarity)  eval.rs:4319-4336 */
+  FT_EXPR_OP_COUNT = 38
+};
+
+typedef struct ft_expr {
+  uint32_t op;
+  uint32_t type;
+  int32_t a;
+  int32_t b;
+  int32_t c;
+  int32_t d;
+  float value;
+  float pad;
+} ft_expr;
+
+typedef struct ft_function {
+  uint32_t first;
+  uint32_t count;
+  uint32_t n_params;
+  uint32_t result_type;
+} ft_function;
+
+/* ---------------------------------------------------------- materials */
+/* Material / MaterialParams (materials/mod.rs:67-87, 214-221). */
+enum { FT_MODEL_STANDARD = 0, FT_MODEL_LAMBERT = 1, FT_MODEL_METAL = 2, FT_MODEL_DIELECTRIC = 3 };
+enum { FT_PATTERN_NONE = 0, FT_PATTERN_CHECKER3D = 1 };
+
+typedef struct ft_material {
+  uint32_t model;
+  uint32_t thin_walled;
+  uint32_t has_medium;
+  uint32_t pattern;
+  float color[3];
+  float roughness;
+  float metallic;
+  float specular;
+  float specular_weight;
+  float ior;
+  float specular_color[3];
+  float clearcoat;
+  float clearcoat_roughness;
+  float transmission;
+  float emission_strength;
+  float medium_ior;
+  float emission_color[3];
+  float medium_density;
+  float medium_absorption[3];
+  float pattern_scale;
+  float pattern_color_a[3];
+  int32_t dynamic_material_id;   /* index into the sorted material-definition names, or -1 */
+  float pattern_color_b[3];
+  int32_t dynamic_override_id;   /* interned override object of a dynamic material, or -1 */
+} ft_material;
+
+/* -------------------------------------------------------------- lights */
+/* PointLight / EnvLight / SphereLight (render_api.rs:171-289), radiance already
+ * multiplied by intensity (renderer.rs:6786-6800). */
+enum { FT_LIGHT_POINT = 0, FT_LIGHT_ENV = 1, FT_LIGHT_SPHERE = 2 };
+
+typedef struct ft_light {
+  uint32_t kind;
+  uint32_t samples;
+  float radius;
+  float pad;
+  float position[3];
+  float pad1;
+  float radiance[3];
+  float pad2;
+} ft_light;
+
+/* `camera` binding as read by parse_camera (renderer.rs:6263-6286); absent fields fall
+ * back to the scene centre and RenderOptions at render time. */
+typedef struct ft_camera {
+  uint32_t present;
+  uint32_t has_origin;
+  uint32_t has_target;
+  uint32_t has_fov;
+  float origin[3];
+  float fov_y_degrees;
+  float target[3];
+  float pad;
+} ft_camera;
+
+typedef struct ft_scene_ir {
+  uint32_t abi_version;
+  uint32_t root;              /* index of the root node (normally n_nodes - 1) */
+  uint32_t n_nodes;
+  uint32_t n_materials;
+  uint32_t n_objects;         /* object_transforms has 6 floats per object id */
+  uint32_t n_lights;
+  uint32_t n_functions;
+  uint32_t n_exprs;
+  int32_t env_fn;             /* environment `color(dir)` function or -1 */
+  uint32_t pad;
+  const ft_node* nodes;
+  const ft_material* materials;
+  const float* object_transforms;
+  const ft_light* lights;
+  const ft_function* functions;
+  const ft_expr* exprs;
+  ft_camera camera;
+} ft_scene_ir;
+
+/* --------------------------------------------------------- render args */
+typedef struct ft_render_options {
+  uint32_t width;
+  uint32_t height;
+  uint32_t max_steps;
+  float max_dist;
+  float epsilon;
+  float step_scale;
+  float camera_z;
+  float fov_y_degrees;
+} ft_render_options;
+
+/* RayDebugAov (renderer.rs:146-155); -1 = none. */
+enum {
+  FT_AOV_NONE = -1,
+  FT_AOV_DEPTH = 0,
+  FT_AOV_NORMAL = 1,
+  FT_AOV_MATERIAL_ID = 2,
+  FT_AOV_IOR = 3,
+  FT_AOV_TRANSMISSION = 4,
+  FT_AOV_FRESNEL = 5,
+  FT_AOV_HIT_T = 6
+};
+
+typedef struct ft_ray_settings {
+  uint32_t max_depth;
+  uint32_t tile_size;
+  uint32_t aa_samples;
+  int32_t debug_aov;
+} ft_ray_settings;
+
+/* ------------------------------------------------------------ entry points */
+typedef struct ft_scene ft_scene;
+
+/* Returns the thread-local message of the last failed call ("" if none). */
+const char* ft_last_error(void);
+
+/* Number of usable GPUs (0 when no device is visible). */
+int ft_device_count(int* count);
+
+/* Builds device-side scene data for `accel_mode` (AccelMode::from_scene,
+ * renderer.rs:1024-1129).  The IR is copied; the caller may free it afterwards. */
+int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out);
+int ft_scene_destroy(ft_scene* scene);
+
+/* Depth preview (render_depth_tile renderer.rs:3931-3971 via render_preview_tiled).
+ * rgb_out: width*height*3 bytes, row-major, host memory.  t_out (nullable): width*height
+ * floats, refined hit distance of the first AA sample (or -1 on a miss). */
+int ft_render_depth(ft_scene* scene, const ft_render_options* options, uint32_t aa_samples,
+                    uint8_t* rgb_out, float* t_out);
+
+/* Whitted trace (render_ray_tile renderer.rs:3874-3929 + renderer/ray.rs).
+ * hdr_out (nullable): width*height*3 floats, AA-averaged radiance before tone mapping. */
+int ft_render_whitted(ft_scene* scene, const ft_render_options* options,
+                      const ft_ray_settings* settings, uint8_t* rgb_out, float* hdr_out);
+
+/* Device-resident variants: outputs are device pointers, the call enqueues on the
+ * scene's stream and returns without synchronising (ft_scene_synchronize waits). */
+int ft_render_depth_device(ft_scene* scene, const ft_render_options* options,
+                           uint32_t aa_samples, void* rgb_dev, void* t_dev);
+int ft_render_whitted_device(ft_scene* scene, const ft_render_options* options,
+                             const ft_ray_settings* settings, void* rgb_dev, void* hdr_dev);
+int ft_scene_synchronize(ft_scene* scene);
+
+/* The HIP stream (hipStream_t) the scene launches on, for callers that time it. */
+int ft_scene_stream(ft_scene* scene, void** stream_out);
+
+/* Per-kernel statistics of the last render on this scene. */
+typedef struct ft_render_stats {
+  uint64_t primary_rays;
+  uint64_t kernel_launches;
+  float last_kernel_ms;   /* hipEvent time of the last render's kernels */
+  float pad;
+} ft_render_stats;
+int ft_scene_stats(ft_scene* scene, ft_render_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FT_ABI_H */

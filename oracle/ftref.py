@@ -1,0 +1,116 @@
+"""ctypes front of the CPU oracle (TEST INFRASTRUCTURE: used by tests/, bench.py's
+CPU-baseline leg and the package smoke check only, always as the checker)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libftref.so")
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+        L.ftref_render_depth.argtypes = [vp, u32, vp, u32, i32, i32, i32, i32, vp, vp]
+        L.ftref_render_whitted.argtypes = [vp, u32, vp, vp, i32, i32, i32, i32, vp, vp]
+        L.ftref_distance_info.argtypes = [vp, u32, i32, vp, i32, vp, vp, vp]
+        L.ftref_lower_bound.argtypes = [vp, u32, i32, vp, i32, vp]
+        L.ftref_eval_function.argtypes = [vp, i32, vp, i32, vp]
+        L.ftref_scene_bounds.argtypes = [vp, vp, vp, vp]
+        L.ftref_depth_eval_count.argtypes = [vp, u32, vp, u32, i32, i32, i32, vp]
+        L.ftref_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(f"ftref error {rc}: {lib().ftref_last_error().decode()}")
+
+
+def render_depth(scene, accel, opts, aa=1, threads=0, precision=0, rows=None):
+    """Returns (rgb[h,w,3] uint8, t[h,w] float32)."""
+    o = opts.to_c()
+    rgb = np.zeros((o.height, o.width, 3), np.uint8)
+    t = np.full((o.height, o.width), np.nan, np.float32)
+    r0, r1 = rows if rows is not None else (0, o.height)
+    ir = scene.ir()
+    _check(lib().ftref_render_depth(C.addressof(ir), int(accel), C.addressof(o), int(aa), threads, precision,
+                                    r0, r1, rgb.ctypes.data, t.ctypes.data))
+    return rgb, t
+
+
+def render_whitted(scene, accel, opts, settings, threads=0, precision=0, rows=None):
+    """Returns (rgb[h,w,3] uint8, hdr[h,w,3] float32)."""
+    o = opts.to_c()
+    s = settings.to_c()
+    rgb = np.zeros((o.height, o.width, 3), np.uint8)
+    hdr = np.full((o.height, o.width, 3), np.nan, np.float32)
+    r0, r1 = rows if rows is not None else (0, o.height)
+    ir = scene.ir()
+    _check(lib().ftref_render_whitted(C.addressof(ir), int(accel), C.addressof(o), C.addressof(s), threads,
+                                      precision, r0, r1, rgb.ctypes.data, hdr.ctypes.data))
+    return rgb, hdr
+
+
+def distance_info(scene, accel, points, precision=0):
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    n = len(p)
+    d = np.zeros(n, np.float32)
+    oid = np.zeros(n, np.uint32)
+    mid = np.zeros(n, np.uint32)
+    ir = scene.ir()
+    _check(lib().ftref_distance_info(C.addressof(ir), int(accel), precision, p.ctypes.data, n, d.ctypes.data,
+                                     oid.ctypes.data, mid.ctypes.data))
+    return d, oid, mid
+
+
+def lower_bound(scene, accel, points, precision=0):
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    lb = np.zeros(len(p), np.float32)
+    ir = scene.ir()
+    _check(lib().ftref_lower_bound(C.addressof(ir), int(accel), precision, p.ctypes.data, len(p), lb.ctypes.data))
+    return lb
+
+
+def eval_function(scene, fn, params):
+    """params: array [n, n_params, 3] (scalars splatted); returns [n, 3]."""
+    prm = np.ascontiguousarray(params, np.float32)
+    n = prm.shape[0]
+    out = np.zeros((n, 3), np.float32)
+    ir = scene.ir()
+    _check(lib().ftref_eval_function(C.addressof(ir), fn, prm.ctypes.data, n, out.ctypes.data))
+    return out
+
+
+def scene_bounds(scene):
+    mn = np.zeros(3, np.float32)
+    mx = np.zeros(3, np.float32)
+    ce = np.zeros(3, np.float32)
+    ir = scene.ir()
+    _check(lib().ftref_scene_bounds(C.addressof(ir), mn.ctypes.data, mx.ctypes.data, ce.ctypes.data))
+    return mn, mx, ce
+
+
+def depth_eval_count(scene, accel, opts, aa=1, threads=0, rows=None):
+    o = opts.to_c()
+    r0, r1 = rows if rows is not None else (0, o.height)
+    n = np.zeros(1, np.uint64)
+    ir = scene.ir()
+    _check(lib().ftref_depth_eval_count(C.addressof(ir), int(accel), C.addressof(o), int(aa), threads, r0, r1,
+                                        n.ctypes.data))
+    return int(n[0])

@@ -99,7 +99,7 @@ class RaySettingsC(C.Structure):
 
 class RenderStatsC(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("kernel_launches", C.c_uint64),
-                ("last_kernel_ms", C.c_float), ("pad", C.c_float)]
+                ("last_kernel_ms", C.c_float), ("pad", C.c_float), ("sdf_flops", C.c_uint64)]
 
 
 STRUCT_SIZES = {

@@ -1,0 +1,481 @@
+// ft_backend.hip — gfx950 kernels and the C ABI of libft_hip.so (include/ft_abi.h).
+//
+// Kernels (one ray per lane):
+//   k_depth          depth preview, one thread per pixel looping over the AA grid
+//                    (render_depth_tile renderer.rs:3931-3971);
+//   k_whitted<N>     one thread per (pixel, AA sample): Whitted trace with an N-entry ray
+//                    stack, writes the sample radiance;
+//   k_resolve        per pixel: sums the samples in AA order, scales by 1/aa and tone maps
+//                    (render_ray_tile renderer.rs:3874-3929, renderer.rs:5221-5239).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "ft_scene_build.h"
+#include "ft_shade.h"
+
+using namespace ftd;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define FT_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(e_ == hipErrorOutOfMemory ? FT_ERR_OUT_OF_MEMORY : FT_ERR_LAUNCH_FAILED,  \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                       \
+  } while (0)
+
+struct CamArgs {
+  float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
+  int width, height, aa, grid;
+};
+
+__device__ __forceinline__ void primary_ray(const CamArgs& c, int x, int y, int s, float3& o, float3& d) {
+  // pixel_sample_offsets (renderer.rs:4838-4854): sample s sits at grid cell (s % g, s / g)
+  const float g = (float)c.grid;
+  const float sx = ((float)(s % c.grid) + 0.5f) / g, sy = ((float)(s / c.grid) + 0.5f) / g;
+  const float px = (((float)x + sx) / (float)c.width) * 2.0f - 1.0f;
+  const float py = 1.0f - (((float)y + sy) / (float)c.height) * 2.0f;
+  const float ndx = px * c.aspect, ndy = py;
+  float3 dir = add(add(mk(c.fx, c.fy, c.fz), mul(mk(c.rx, c.ry, c.rz), ndx * c.tan_fov)),
+                   mul(mk(c.ux, c.uy, c.uz), ndy * c.tan_fov));
+  d = normalize(normalize(dir));
+  o = mk(c.ox, c.oy, c.oz);
+}
+
+__device__ __forceinline__ float depth_preview_value(float t, float max_dist) {  // renderer.rs:5215-5219
+  float dd = clampf(t / fmaxf(max_dist, 1.0e-6f), 0.0f, 1.0f);
+  float s = 1.0f - dd;
+  return s * s;
+}
+
+// Pixels are visited in 8x8 tiles per wavefront (blockDim 64x4: four tiles per block) so
+// that the 64 lanes of a wave march spatially adjacent rays.
+__device__ __forceinline__ bool tile_pixel(int width, int height, int& x, int& y) {
+  const int lane = threadIdx.x;
+  const int tiles_x = (width + 7) / 8;
+  const int tile = blockIdx.x * blockDim.y + threadIdx.y;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  x = tx * 8 + (lane & 7);
+  y = ty * 8 + (lane >> 3);
+  return x < width && y < height;
+}
+
+__global__ __launch_bounds__(256) void k_depth(DevScene S, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
+                                               float* __restrict__ t_out) {
+  int x, y;
+  if (!tile_pixel(cam.width, cam.height, x, y)) return;
+  float sum = 0.0f;
+  float first_t = -1.0f;
+  for (int s = 0; s < cam.aa; ++s) {
+    float3 o, d;
+    primary_ray(cam, x, y, s, o, d);
+    Hit hit;
+    bool h = raymarch(S, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit);
+    sum = sum + (h ? depth_preview_value(hit.t, tp.max_dist) : 0.0f);
+    if (s == 0) first_t = h ? hit.t : -1.0f;
+  }
+  const unsigned char v = to_u8(sum * (1.0f / (float)cam.aa));
+  const size_t pix = (size_t)y * cam.width + x;
+  rgb[pix * 3] = v;
+  rgb[pix * 3 + 1] = v;
+  rgb[pix * 3 + 2] = v;
+  if (t_out) t_out[pix] = first_t;
+}
+
+template <int kRayStack>
+__global__ __launch_bounds__(256) void k_whitted(DevScene S, CamArgs cam, TraceParams tp, int aov,
+                                                 float* __restrict__ samples, int* __restrict__ overflow) {
+  int x, y;
+  if (!tile_pixel(cam.width, cam.height, x, y)) return;
+  const int s = blockIdx.y;
+  float3 o, d;
+  primary_ray(cam, x, y, s, o, d);
+  float3 c = aov >= 0 ? trace_aov(S, tp, o, d, aov) : trace_whitted<kRayStack>(S, tp, o, d, overflow);
+  const size_t idx = ((size_t)s * cam.height + y) * cam.width + x;
+  samples[idx * 3] = c.x;
+  samples[idx * 3 + 1] = c.y;
+  samples[idx * 3 + 2] = c.z;
+}
+
+__global__ __launch_bounds__(256) void k_resolve(CamArgs cam, int aov, const float* __restrict__ samples,
+                                                 unsigned char* __restrict__ rgb, float* __restrict__ hdr) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npix = cam.width * cam.height;
+  if (pix >= npix) return;
+  float3 sum = mk(0, 0, 0);
+  for (int s = 0; s < cam.aa; ++s) {
+    const float* c = samples + ((size_t)s * npix + pix) * 3;
+    sum = add(sum, mk(c[0], c[1], c[2]));
+  }
+  const float3 avg = mul(sum, 1.0f / (float)cam.aa);
+  unsigned char* o = rgb + (size_t)pix * 3;
+  if (aov >= 0) { o[0] = to_u8(avg.x); o[1] = to_u8(avg.y); o[2] = to_u8(avg.z); }
+  else { o[0] = to_u8_reinhard(avg.x); o[1] = to_u8_reinhard(avg.y); o[2] = to_u8_reinhard(avg.z); }
+  if (hdr) { hdr[(size_t)pix * 3] = avg.x; hdr[(size_t)pix * 3 + 1] = avg.y; hdr[(size_t)pix * 3 + 2] = avg.z; }
+}
+
+// Clears the overflow flag with a kernel (a memset node would misbehave under graph capture).
+__global__ void k_clear_flag(int* flag) {
+  if (threadIdx.x == 0) flag[0] = 0;
+}
+
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+};
+
+template <class T>
+int upload(DevBuf& b, const std::vector<T>& v) {
+  size_t bytes = v.size() * sizeof(T);
+  if (bytes == 0) return FT_OK;
+  FT_HIP(hipMalloc(&b.ptr, bytes));
+  b.bytes = bytes;
+  FT_HIP(hipMemcpy(b.ptr, v.data(), bytes, hipMemcpyHostToDevice));
+  return FT_OK;
+}
+
+int ensure(DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes) return FT_OK;
+  if (b.ptr) FT_HIP(hipFree(b.ptr));
+  b.ptr = nullptr;
+  b.bytes = 0;
+  FT_HIP(hipMalloc(&b.ptr, bytes));
+  b.bytes = bytes;
+  return FT_OK;
+}
+
+}  // namespace
+
+struct ft_scene {
+  fth::HostScene host;
+  DevScene dev;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf nodes, prog, lbprog, mats, objects, lights, fns, exprs, bvh, leaves;
+  DevBuf rgb, tbuf, samples, hdr, overflow, flops;
+  ft_render_stats stats{};
+  bool timed = false;
+};
+
+namespace {
+
+int check_options(const ft_render_options* o) {
+  if (!o) return fail(FT_ERR_INVALID_ARGUMENT, "render options are null");
+  if (o->width == 0 || o->height == 0) return fail(FT_ERR_INVALID_ARGUMENT, "image size must be non-zero");
+  if ((uint64_t)o->width * o->height > (1ull << 30)) return fail(FT_ERR_INVALID_ARGUMENT, "image too large");
+  if (!(o->epsilon > 0.0f) || !(o->max_dist > 0.0f)) return fail(FT_ERR_INVALID_ARGUMENT, "epsilon and max_dist must be positive");
+  return FT_OK;
+}
+
+CamArgs make_cam(const ft_scene* s, const ft_render_options& o, uint32_t aa) {
+  fth::CamBasis cb = fth::camera_basis(s->host, o);
+  CamArgs c;
+  c.ox = cb.origin[0]; c.oy = cb.origin[1]; c.oz = cb.origin[2];
+  c.fx = cb.forward[0]; c.fy = cb.forward[1]; c.fz = cb.forward[2];
+  c.rx = cb.right[0]; c.ry = cb.right[1]; c.rz = cb.right[2];
+  c.ux = cb.up[0]; c.uy = cb.up[1]; c.uz = cb.up[2];
+  c.tan_fov = cb.tan_fov;
+  c.aspect = (float)o.width / (float)o.height;
+  c.width = (int)o.width;
+  c.height = (int)o.height;
+  c.aa = (int)(aa < 1 ? 1 : aa);
+  c.grid = 1;
+  while (c.grid * c.grid < c.aa) ++c.grid;  // ceil(sqrt(aa))
+  return c;
+}
+
+TraceParams make_tp(const ft_render_options& o, uint32_t max_depth) {
+  TraceParams tp;
+  tp.max_steps = (int)o.max_steps;
+  tp.max_depth = (int)(max_depth < 1 ? 1 : max_depth);
+  tp.eps = o.epsilon;
+  tp.step_scale = o.step_scale;
+  tp.max_dist = o.max_dist;
+  return tp;
+}
+
+dim3 tile_grid(int width, int height) {
+  int tiles = ((width + 7) / 8) * ((height + 7) / 8);
+  return dim3((tiles + 3) / 4, 1, 1);
+}
+
+__global__ void k_clear_u64(unsigned long long* v) {
+  if (threadIdx.x == 0) v[0] = 0ull;
+}
+
+int clear_flops(ft_scene* s) {
+  s->stats.sdf_flops = 0;
+  if (!s->dev.flops) return FT_OK;
+  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, s->stream, s->dev.flops);
+  FT_HIP(hipGetLastError());
+  return FT_OK;
+}
+
+int depth_impl(ft_scene* s, const ft_render_options* o, uint32_t aa, unsigned char* rgb_dev, float* t_dev) {
+  int rc = check_options(o);
+  if (rc) return rc;
+  if ((rc = clear_flops(s))) return rc;
+  CamArgs cam = make_cam(s, *o, aa);
+  TraceParams tp = make_tp(*o, 1);
+  FT_HIP(hipSetDevice(s->device));
+  FT_HIP(hipEventRecord(s->ev0, s->stream));
+  hipLaunchKernelGGL(k_depth, tile_grid(cam.width, cam.height), dim3(64, 4), 0, s->stream, s->dev, cam, tp, rgb_dev,
+                     t_dev);
+  FT_HIP(hipGetLastError());
+  FT_HIP(hipEventRecord(s->ev1, s->stream));
+  s->timed = true;
+  s->stats.primary_rays = (uint64_t)cam.width * cam.height * cam.aa;
+  s->stats.kernel_launches = 1;
+  return FT_OK;
+}
+
+int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs, unsigned char* rgb_dev,
+                 float* hdr_dev) {
+  int rc = check_options(o);
+  if (rc) return rc;
+  if (!rs) return fail(FT_ERR_INVALID_ARGUMENT, "ray settings are null");
+  if (rs->debug_aov < -1 || rs->debug_aov > FT_AOV_HIT_T) return fail(FT_ERR_INVALID_ARGUMENT, "unknown debug AOV");
+  if ((rc = clear_flops(s))) return rc;
+  CamArgs cam = make_cam(s, *o, rs->aa_samples);
+  if (cam.aa > 65535) return fail(FT_ERR_INVALID_ARGUMENT, "too many AA samples");
+  TraceParams tp = make_tp(*o, rs->max_depth);
+  // ray-stack capacity: every hit pops one entry and pushes at most max_children
+  const long need = 1 + (long)(tp.max_depth - 1) * (long)(s->host.max_children - 1);
+  rc = ensure(s->samples, (size_t)cam.aa * cam.width * cam.height * 3 * sizeof(float));
+  if (rc) return rc;
+  rc = ensure(s->overflow, sizeof(int));
+  if (rc) return rc;
+  FT_HIP(hipSetDevice(s->device));
+  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s->stream, (int*)s->overflow.ptr);
+  FT_HIP(hipGetLastError());
+  FT_HIP(hipEventRecord(s->ev0, s->stream));
+  dim3 grid = tile_grid(cam.width, cam.height);
+  grid.y = (unsigned)cam.aa;
+  float* samples = (float*)s->samples.ptr;
+  int* ovf = (int*)s->overflow.ptr;
+  if (need <= 16)
+    hipLaunchKernelGGL(k_whitted<16>, grid, dim3(64, 4), 0, s->stream, s->dev, cam, tp, rs->debug_aov, samples, ovf);
+  else if (need <= 48)
+    hipLaunchKernelGGL(k_whitted<48>, grid, dim3(64, 4), 0, s->stream, s->dev, cam, tp, rs->debug_aov, samples, ovf);
+  else if (need <= 128)
+    hipLaunchKernelGGL(k_whitted<128>, grid, dim3(64, 4), 0, s->stream, s->dev, cam, tp, rs->debug_aov, samples, ovf);
+  else
+    return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with this scene's materials");
+  FT_HIP(hipGetLastError());
+  int npix = cam.width * cam.height;
+  hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s->stream, cam, rs->debug_aov, samples, rgb_dev,
+                     hdr_dev);
+  FT_HIP(hipGetLastError());
+  FT_HIP(hipEventRecord(s->ev1, s->stream));
+  s->timed = true;
+  s->stats.primary_rays = (uint64_t)npix * cam.aa;
+  s->stats.kernel_launches = 2;
+  return FT_OK;
+}
+
+int finish(ft_scene* s) {
+  FT_HIP(hipStreamSynchronize(s->stream));
+  if (s->dev.flops) {
+    unsigned long long f = 0;
+    FT_HIP(hipMemcpy(&f, s->dev.flops, sizeof(f), hipMemcpyDeviceToHost));
+    s->stats.sdf_flops = f;
+  }
+  if (s->timed) {
+    float ms = 0.0f;
+    FT_HIP(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->stats.last_kernel_ms = ms;
+    s->timed = false;
+  }
+  if (s->overflow.ptr) {
+    int ovf = 0;
+    FT_HIP(hipMemcpy(&ovf, s->overflow.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    if (ovf) return fail(FT_ERR_LAUNCH_FAILED, "ray stack overflow");
+  }
+  return FT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* ft_last_error(void) { return g_err.c_str(); }
+
+int ft_device_count(int* count) {
+  if (!count) return fail(FT_ERR_INVALID_ARGUMENT, "count is null");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  *count = e == hipSuccess ? n : 0;
+  return FT_OK;
+}
+
+int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) {
+  if (!out) return fail(FT_ERR_INVALID_ARGUMENT, "output handle is null");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FT_ERR_GPU_UNAVAILABLE, "no HIP device available");
+  ft_scene* s = new ft_scene();
+  std::string err;
+  if (!fth::build_scene(ir, accel_mode, s->host, err)) {
+    delete s;
+    return fail(err.find("root") != std::string::npos ? FT_ERR_MISSING_SCENE_ROOT : FT_ERR_INVALID_ARGUMENT, err);
+  }
+  int rc = FT_OK;
+  auto cleanup = [&]() { ft_scene_destroy(s); };
+  if (hipGetDevice(&s->device) != hipSuccess) { cleanup(); return fail(FT_ERR_GPU_UNAVAILABLE, "hipGetDevice failed"); }
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+    cleanup();
+    return fail(FT_ERR_GPU_UNAVAILABLE, "stream/event creation failed");
+  }
+  fth::HostScene& h = s->host;
+  if ((rc = upload(s->nodes, h.nodes)) || (rc = upload(s->prog, h.prog)) || (rc = upload(s->lbprog, h.lbprog)) ||
+      (rc = upload(s->mats, h.mats)) || (rc = upload(s->objects, h.objects)) || (rc = upload(s->lights, h.lights)) ||
+      (rc = upload(s->fns, h.fns)) || (rc = upload(s->exprs, h.exprs)) || (rc = upload(s->bvh, h.bvh)) ||
+      (rc = upload(s->leaves, h.leaves))) {
+    std::string e = g_err;
+    cleanup();
+    return fail(rc, e);
+  }
+  DevScene& d = s->dev;
+  std::memset(&d, 0, sizeof(d));
+  d.nodes = (const DNode*)s->nodes.ptr;
+  d.prog = (const unsigned*)s->prog.ptr;
+  d.lbprog = (const unsigned*)s->lbprog.ptr;
+  d.mats = (const DParams*)s->mats.ptr;
+  d.objects = (const float*)s->objects.ptr;
+  d.lights = (const DLight*)s->lights.ptr;
+  d.fns = (const ft_function*)s->fns.ptr;
+  d.exprs = (const ft_expr*)s->exprs.ptr;
+  d.bvh = (const DBvh*)s->bvh.ptr;
+  d.leaves = (const DLeaf*)s->leaves.ptr;
+  d.n_nodes = (int)h.nodes.size();
+  d.root = h.root;
+  d.n_prog = (int)h.prog.size();
+  d.n_lbprog = (int)h.lbprog.size();
+  d.n_mats = (int)h.mats.size();
+  d.n_objects = (int)(h.objects.size() / 9);
+  d.n_lights = (int)h.lights.size();
+  d.env_fn = h.env_fn;
+  d.accel = h.accel;
+  d.n_leaves = (int)h.leaves.size();
+  d.n_bvh = (int)h.bvh.size();
+  for (int k = 0; k < 3; ++k) { d.bmin[k] = h.bmin[k]; d.bmax[k] = h.bmax[k]; d.env_rad[k] = h.env_rad[k]; }
+  *out = s;
+  return FT_OK;
+}
+
+int ft_scene_destroy(ft_scene* s) {
+  if (!s) return FT_OK;
+  (void)hipSetDevice(s->device);
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (DevBuf* b : {&s->nodes, &s->prog, &s->lbprog, &s->mats, &s->objects, &s->lights, &s->fns, &s->exprs, &s->bvh,
+                    &s->leaves, &s->rgb, &s->tbuf, &s->samples, &s->hdr, &s->overflow, &s->flops})
+    if (b->ptr) (void)hipFree(b->ptr);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return FT_OK;
+}
+
+int ft_render_depth_device(ft_scene* s, const ft_render_options* o, uint32_t aa, void* rgb_dev, void* t_dev) {
+  if (!s || !rgb_dev) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  return depth_impl(s, o, aa, (unsigned char*)rgb_dev, (float*)t_dev);
+}
+
+int ft_render_whitted_device(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs, void* rgb_dev,
+                             void* hdr_dev) {
+  if (!s || !rgb_dev) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  return whitted_impl(s, o, rs, (unsigned char*)rgb_dev, (float*)hdr_dev);
+}
+
+int ft_scene_synchronize(ft_scene* s) {
+  if (!s) return fail(FT_ERR_INVALID_ARGUMENT, "scene is null");
+  return finish(s);
+}
+
+int ft_render_depth(ft_scene* s, const ft_render_options* o, uint32_t aa, uint8_t* rgb_out, float* t_out) {
+  if (!s || !rgb_out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  int rc = check_options(o);
+  if (rc) return rc;
+  size_t npix = (size_t)o->width * o->height;
+  if ((rc = ensure(s->rgb, npix * 3))) return rc;
+  if (t_out && (rc = ensure(s->tbuf, npix * sizeof(float)))) return rc;
+  if ((rc = depth_impl(s, o, aa, (unsigned char*)s->rgb.ptr, t_out ? (float*)s->tbuf.ptr : nullptr))) return rc;
+  if ((rc = finish(s))) return rc;
+  FT_HIP(hipMemcpy(rgb_out, s->rgb.ptr, npix * 3, hipMemcpyDeviceToHost));
+  if (t_out) FT_HIP(hipMemcpy(t_out, s->tbuf.ptr, npix * sizeof(float), hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_render_whitted(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs, uint8_t* rgb_out,
+                      float* hdr_out) {
+  if (!s || !rgb_out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  int rc = check_options(o);
+  if (rc) return rc;
+  size_t npix = (size_t)o->width * o->height;
+  if ((rc = ensure(s->rgb, npix * 3))) return rc;
+  if (hdr_out && (rc = ensure(s->hdr, npix * 3 * sizeof(float)))) return rc;
+  if ((rc = whitted_impl(s, o, rs, (unsigned char*)s->rgb.ptr, hdr_out ? (float*)s->hdr.ptr : nullptr))) return rc;
+  if ((rc = finish(s))) return rc;
+  FT_HIP(hipMemcpy(rgb_out, s->rgb.ptr, npix * 3, hipMemcpyDeviceToHost));
+  if (hdr_out) FT_HIP(hipMemcpy(hdr_out, s->hdr.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_scene_stream(ft_scene* s, void** stream_out) {
+  if (!s || !stream_out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  *stream_out = (void*)s->stream;
+  return FT_OK;
+}
+
+int ft_scene_stats(ft_scene* s, ft_render_stats* out) {
+  if (!s || !out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  *out = s->stats;
+  return FT_OK;
+}
+
+int ft_scene_set_flop_counting(ft_scene* s, int enable) {
+  if (!s) return fail(FT_ERR_INVALID_ARGUMENT, "scene is null");
+  if (enable) {
+    int rc = ensure(s->flops, sizeof(unsigned long long));
+    if (rc) return rc;
+    s->dev.flops = (unsigned long long*)s->flops.ptr;
+  } else {
+    s->dev.flops = nullptr;
+  }
+  return FT_OK;
+}
+
+// sizeof() of every ABI struct, so bindings can verify their layout.
+int ft_abi_struct_size(const char* name) {
+  if (!name) return -1;
+  std::string n(name);
+  if (n == "ft_node") return (int)sizeof(ft_node);
+  if (n == "ft_expr") return (int)sizeof(ft_expr);
+  if (n == "ft_function") return (int)sizeof(ft_function);
+  if (n == "ft_material") return (int)sizeof(ft_material);
+  if (n == "ft_light") return (int)sizeof(ft_light);
+  if (n == "ft_camera") return (int)sizeof(ft_camera);
+  if (n == "ft_scene_ir") return (int)sizeof(ft_scene_ir);
+  if (n == "ft_render_options") return (int)sizeof(ft_render_options);
+  if (n == "ft_ray_settings") return (int)sizeof(ft_ray_settings);
+  if (n == "ft_render_stats") return (int)sizeof(ft_render_stats);
+  return -1;
+}
+
+}  // extern "C"

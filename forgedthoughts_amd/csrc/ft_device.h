@@ -1,0 +1,702 @@
+// ft_device.h — device-side scene layout and evaluation for the gfx950 backend.
+//
+// Scene data lives in HBM as flat arrays built once per scene by ft_scene_build.cpp:
+//   DNode     one per SdfNode (reference renderer.rs:170-355), with the per-primitive
+//             rotation pre-resolved to the sin/cos pairs `to_local` (renderer.rs:6216-6240)
+//             would compute on every call (same angles, same libm), and sdf_bounds
+//             (renderer.rs:5241-5386) per node;
+//   prog      the node tree flattened to a post-order instruction list.  Every lane of a
+//             wavefront walks the same list, so control flow over nodes is wave-uniform
+//             and the value stack pointer is uniform (the stack lives in VGPRs);
+//             modifier nodes (domain / distance_post / noise) get a BEGIN instruction
+//             ahead of their subtree that moves the evaluation point;
+//   BVH       accel leaves (maximal non-Union subtrees, renderer.rs:659-670) under a
+//             median-split BVH (renderer.rs:693-732), traversed near-first with the
+//             reference's pruning rule (renderer.rs:743-822).
+// One ray per lane; all arithmetic f32.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ft_abi.h"
+#include "ft_layout.h"
+
+namespace ftd {
+
+struct DevScene {
+  const DNode* __restrict__ nodes;
+  const unsigned* __restrict__ prog;
+  const DParams* __restrict__ mats;
+  const float* __restrict__ objects;     // 9 floats per object: centre, sin/cos z,y,x
+  const DLight* __restrict__ lights;
+  const ft_function* __restrict__ fns;
+  const ft_expr* __restrict__ exprs;
+  const DBvh* __restrict__ bvh;
+  const DLeaf* __restrict__ leaves;
+  const unsigned* __restrict__ lbprog;   // lower-bound program (post-order)
+  int n_nodes, root, n_prog, n_lbprog;
+  int n_mats, n_objects, n_lights, env_fn;
+  int accel, n_leaves, n_bvh, pad;
+  unsigned long long* flops;             // non-null: count evaluated FLOPs (instrumented runs)
+  float bmin[3], bmax[3];                // scene bounds
+  float env_rad[3];                      // sum of EnvLight radiance
+};
+
+// ----------------------------------------------------------------- math
+__device__ __forceinline__ float3 mk(float x, float y, float z) { return make_float3(x, y, z); }
+__device__ __forceinline__ float3 add(float3 a, float3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ float3 sub(float3 a, float3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ float3 mul(float3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float3 cmul(float3 a, float3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ float3 cross(float3 a, float3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float3 normalize(float3 a) {  // Vec3::normalize (renderer.rs:317-323)
+  float l = len(a);
+  if (l <= 1.1920929e-7f) return a;
+  return mul(a, 1.0f / l);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+  x = x < lo ? lo : x;
+  return x > hi ? hi : x;
+}
+__device__ __forceinline__ float comp(float3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+__device__ __forceinline__ float deg2rad(float d) { return d * (3.14159265358979323846f / 180.0f); }
+__device__ __forceinline__ float powi5(float x) { float x2 = x * x; return x * (x2 * x2); }
+__device__ __forceinline__ float rem_euclid(float a, float b) {
+  float r = fmodf(a, b);
+  return r < 0.0f ? r + fabsf(b) : r;
+}
+
+__device__ __forceinline__ float3 rot_x(float3 v, float s, float c) { return mk(v.x, c * v.y - s * v.z, s * v.y + c * v.z); }
+__device__ __forceinline__ float3 rot_y(float3 v, float s, float c) { return mk(c * v.x + s * v.z, v.y, -s * v.x + c * v.z); }
+__device__ __forceinline__ float3 rot_z(float3 v, float s, float c) { return mk(c * v.x - s * v.y, s * v.x + c * v.y, v.z); }
+
+// to_local (renderer.rs:6216-6222)
+__device__ __forceinline__ float3 node_local(const DNode& n, float3 p) {
+  float3 q = mk(p.x - n.cx, p.y - n.cy, p.z - n.cz);
+  if (n.flags & 1) {
+    q = rot_z(q, n.szs, n.szc);
+    q = rot_y(q, n.sys, n.syc);
+    q = rot_x(q, n.sxs, n.sxc);
+  }
+  return q;
+}
+__device__ __forceinline__ float3 object_local(const DevScene& S, unsigned obj, float3 p) {
+  if (obj >= (unsigned)S.n_objects) return p;
+  const float* o = S.objects + 9 * obj;
+  float3 q = mk(p.x - o[0], p.y - o[1], p.z - o[2]);
+  q = rot_z(q, o[3], o[4]);
+  q = rot_y(q, o[5], o[6]);
+  q = rot_x(q, o[7], o[8]);
+  return q;
+}
+
+constexpr unsigned kAabbFlops = kAabbFlopsHost;
+
+// point_aabb_lower_bound (renderer.rs:5388-5411)
+__device__ __forceinline__ float aabb_lb(float3 p, const float* mn, const float* mx) {
+  float dx = p.x < mn[0] ? mn[0] - p.x : (p.x > mx[0] ? p.x - mx[0] : 0.0f);
+  float dy = p.y < mn[1] ? mn[1] - p.y : (p.y > mx[1] ? p.y - mx[1] : 0.0f);
+  float dz = p.z < mn[2] ? mn[2] - p.z : (p.z > mx[2] ? p.z - mx[2] : 0.0f);
+  return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+// --------------------------------------------------------- hg_sdf ops
+// (renderer.rs:6011-6137)
+__device__ __forceinline__ float op_smax(float a, float b, float k) {
+  float h = fmaxf(k - fabsf(a - b), 0.0f);
+  return fmaxf(a, b) + h * h * 0.25f / fmaxf(k, 1.0e-6f);
+}
+__device__ __forceinline__ float op_smin(float a, float b, float k) {
+  float h = fmaxf(k - fabsf(a - b), 0.0f);
+  return fminf(a, b) - h * h * 0.25f / fmaxf(k, 1.0e-6f);
+}
+__device__ __forceinline__ float op_union_round(float a, float b, float r) {
+  r = fmaxf(fabsf(r), 1.0e-6f);
+  float ux = fmaxf(r - a, 0.0f), uy = fmaxf(r - b, 0.0f);
+  return fmaxf(r, fminf(a, b)) - sqrtf(ux * ux + uy * uy);
+}
+__device__ __forceinline__ float op_intersect_round(float a, float b, float r) {
+  r = fmaxf(fabsf(r), 1.0e-6f);
+  float ux = fmaxf(r + a, 0.0f), uy = fmaxf(r + b, 0.0f);
+  return fminf(-r, fmaxf(a, b)) + sqrtf(ux * ux + uy * uy);
+}
+constexpr float kInvSqrt2 = 0.70710678118654752440f;
+__device__ __forceinline__ float op_union_chamfer(float a, float b, float r) {
+  return fminf(fminf(a, b), (a - r + b) * kInvSqrt2);
+}
+__device__ __forceinline__ float op_intersect_chamfer(float a, float b, float r) {
+  return fmaxf(fmaxf(a, b), (a + r + b) * kInvSqrt2);
+}
+__device__ __forceinline__ float op_union_columns(float a, float b, float r, float n) {
+  if (a < r && b < r) {
+    n = fmaxf(roundf(n), 2.0f);
+    const float s2 = 1.41421356237309504880f;
+    float cr = r * s2 / ((n - 1.0f) * 2.0f + s2);
+    float x = a + cr, y = b + cr;
+    int ni = (int)n;
+    if (ni % 2 == 1) x += cr;
+    float rx = (x + y) * kInvSqrt2, ry = (y - x) * kInvSqrt2;
+    x = rx;
+    y = ry;
+    x -= s2 * 0.5f * r;
+    x += -cr * s2;
+    if (ni % 2 == 1) y += cr;
+    float size = fmaxf(cr * 2.0f, 1.0e-6f);
+    y = rem_euclid(y + size * 0.5f, size) - size * 0.5f;
+    return sqrtf(x * x + y * y) - cr;
+  }
+  return fminf(a, b);
+}
+__device__ __forceinline__ float op_union_stairs(float a, float b, float r, float n) {
+  n = fmaxf(roundf(n), 1.0f);
+  float s = r / n, u = b - r;
+  return fminf(fminf(a, b), 0.5f * (u + a + fabsf(rem_euclid(u - a + s, 2.0f * s) - s)));
+}
+__device__ __forceinline__ float op_union_soft(float a, float b, float r) {
+  r = fmaxf(fabsf(r), 1.0e-6f);
+  float e = fmaxf(r - fabsf(a - b), 0.0f);
+  return fminf(a, b) - e * e * 0.25f / r;
+}
+
+// noise modifier (renderer.rs:5957-6009)
+__device__ __forceinline__ float fractf_(float x) { return x - floorf(x); }
+constexpr float kInvPi = 0.31830988618379067154f;
+__device__ __forceinline__ float hash_noise(float3 p) {
+  float qx = fractf_(p.x * kInvPi + 0.11f) * 17.0f;
+  float qy = fractf_(p.y * kInvPi + 0.17f) * 17.0f;
+  float qz = fractf_(p.z * kInvPi + 0.13f) * 17.0f;
+  return fractf_(qx * qy * qz * (qx + qy + qz));
+}
+__device__ inline float sd_base_noise(float3 p) {
+  p = add(p, mk(0.5f, 0.5f, 0.5f));
+  float3 i = mk(floorf(p.x), floorf(p.y), floorf(p.z));
+  float3 f = mk(fractf_(p.x), fractf_(p.y), fractf_(p.z));
+  float d = __builtin_huge_valf();
+  for (int ix = 0; ix <= 1; ++ix)
+    for (int iy = 0; iy <= 1; ++iy)
+      for (int iz = 0; iz <= 1; ++iz) {
+        float3 c = mk((float)ix, (float)iy, (float)iz);
+        float r = hash_noise(add(i, c));
+        r = r * r * 0.7f;
+        d = fminf(d, len(sub(f, c)) - r);
+      }
+  return d;
+}
+__device__ inline float apply_noise(float3 p, float d, int octaves, float scale, float lac) {
+  float3 q = mul(p, fmaxf(scale, 1.0e-4f));
+  float os = fmaxf(lac, 1.0e-4f);
+  float s = 1.0f, t = 0.0f;
+  int n = octaves < 1 ? 1 : octaves;
+  for (int k = 0; k < n; ++k) {
+    float nn = s * sd_base_noise(q);
+    nn = op_smax(nn, d - 0.1f * s, 0.3f * s);
+    d = op_smin(nn, d, 0.3f * s);
+    t += d;
+    q = mul(mk(0.00f * q.x + 1.60f * q.y + 1.20f * q.z, -1.60f * q.x + 0.72f * q.y - 0.96f * q.z,
+               -1.20f * q.x - 0.96f * q.y + 1.28f * q.z), os);
+    s *= 0.415f;
+    q.z += -4.33f * t * s;
+  }
+  return d;
+}
+
+// sd_regular_ngon (renderer.rs:6206-6214)
+__device__ inline float sd_ngon(float3 p, int sides, float radius) {
+  float n = (float)(sides < 3 ? 3 : sides);
+  float an = 3.14159265358979323846f / n;
+  float apothem = radius * cosf(an);
+  float angle = atan2f(p.z, p.x);
+  float wrapped = rem_euclid(angle + an, 2.0f * an) - an;
+  return hypotf(p.x, p.z) * cosf(wrapped) - apothem;
+}
+
+// ------------------------------------------------------ Forge functions
+// Expression DAG interpreter; builtin semantics eval.rs:2223-2690.
+__device__ __forceinline__ float smooth01(float t) {
+  t = clampf(t, 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+__device__ inline float hash_noise3(float3 p) { return hash_noise(p); }  // eval.rs:4286-4291 (same constants)
+__device__ inline float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ inline float value_noise_3d(float3 p) {
+  float3 i = mk(floorf(p.x), floorf(p.y), floorf(p.z));
+  float3 f = mk(fractf_(p.x), fractf_(p.y), fractf_(p.z));
+  float3 u = mk(smooth01(f.x), smooth01(f.y), smooth01(f.z));
+  float n000 = hash_noise3(i), n001 = hash_noise3(mk(i.x, i.y, i.z + 1.0f));
+  float n010 = hash_noise3(mk(i.x, i.y + 1.0f, i.z)), n011 = hash_noise3(mk(i.x, i.y + 1.0f, i.z + 1.0f));
+  float n100 = hash_noise3(mk(i.x + 1.0f, i.y, i.z)), n101 = hash_noise3(mk(i.x + 1.0f, i.y, i.z + 1.0f));
+  float n110 = hash_noise3(mk(i.x + 1.0f, i.y + 1.0f, i.z)), n111 = hash_noise3(mk(i.x + 1.0f, i.y + 1.0f, i.z + 1.0f));
+  float nxy0 = lerpf(lerpf(n000, n100, u.x), lerpf(n010, n110, u.x), u.y);
+  float nxy1 = lerpf(lerpf(n001, n101, u.x), lerpf(n011, n111, u.x), u.y);
+  return lerpf(nxy0, nxy1, u.z) * 2.0f - 1.0f;
+}
+__device__ inline float fbm_3d(float3 p, int octaves, float scale, float lac) {
+  float3 q = mul(p, scale);
+  float amp = 0.5f, sum = 0.0f;
+  float l = fabsf(lac) < 1.1920929e-7f ? 1.0f : lac;
+  int n = octaves < 1 ? 1 : octaves;
+  for (int k = 0; k < n; ++k) {
+    sum += amp * value_noise_3d(q);
+    q = mul(q, l);
+    amp *= 0.55f;
+  }
+  return sum;
+}
+
+__device__ __noinline__ float3 eval_fn(const DevScene& S, int fn, const float3* params) {
+  const ft_function F = S.fns[fn];
+  float3 v[kMaxExpr];
+  const int count = (int)F.count;
+  for (int k = 0; k < count; ++k) {
+    const ft_expr e = S.exprs[F.first + k];
+    float3 a = e.a >= 0 ? v[e.a] : mk(0, 0, 0);
+    float3 b = e.b >= 0 ? v[e.b] : mk(0, 0, 0);
+    float3 c = e.c >= 0 ? v[e.c] : mk(0, 0, 0);
+    float3 r;
+    switch (e.op) {
+      case FT_EXPR_CONST: r = mk(e.value, e.value, e.value); break;
+      case FT_EXPR_PARAM: r = params[e.a]; break;
+      case FT_EXPR_VEC3: r = mk(a.x, b.x, c.x); break;
+      case FT_EXPR_GET_X: r = mk(a.x, a.x, a.x); break;
+      case FT_EXPR_GET_Y: r = mk(a.y, a.y, a.y); break;
+      case FT_EXPR_GET_Z: r = mk(a.z, a.z, a.z); break;
+      case FT_EXPR_NEG: r = mk(-a.x, -a.y, -a.z); break;
+      case FT_EXPR_ADD: r = add(a, b); break;
+      case FT_EXPR_SUB: r = sub(a, b); break;
+      case FT_EXPR_MUL: r = cmul(a, b); break;
+      case FT_EXPR_DIV: r = mk(a.x / b.x, a.y / b.y, a.z / b.z); break;
+      case FT_EXPR_ABS: r = mk(fabsf(a.x), fabsf(a.y), fabsf(a.z)); break;
+      case FT_EXPR_FLOOR: r = mk(floorf(a.x), floorf(a.y), floorf(a.z)); break;
+      case FT_EXPR_CEIL: r = mk(ceilf(a.x), ceilf(a.y), ceilf(a.z)); break;
+      case FT_EXPR_FRACT: r = mk(fractf_(a.x), fractf_(a.y), fractf_(a.z)); break;
+      case FT_EXPR_SQRT: r = mk(sqrtf(a.x), sqrtf(a.y), sqrtf(a.z)); break;
+      case FT_EXPR_SIN: r = mk(sinf(a.x), sinf(a.y), sinf(a.z)); break;
+      case FT_EXPR_COS: r = mk(cosf(a.x), cosf(a.y), cosf(a.z)); break;
+      case FT_EXPR_SATURATE: r = mk(clampf(a.x, 0, 1), clampf(a.y, 0, 1), clampf(a.z, 0, 1)); break;
+      case FT_EXPR_MIN: r = mk(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); break;
+      case FT_EXPR_MAX: r = mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); break;
+      case FT_EXPR_POW: r = mk(powf(a.x, b.x), powf(a.y, b.y), powf(a.z, b.z)); break;
+      case FT_EXPR_CLAMP: {
+        r.x = b.x <= c.x ? clampf(a.x, b.x, c.x) : clampf(a.x, c.x, b.x);
+        r.y = b.y <= c.y ? clampf(a.y, b.y, c.y) : clampf(a.y, c.y, b.y);
+        r.z = b.z <= c.z ? clampf(a.z, b.z, c.z) : clampf(a.z, c.z, b.z);
+        break;
+      }
+      case FT_EXPR_MIX: r = mk(lerpf(a.x, b.x, c.x), lerpf(a.y, b.y, c.y), lerpf(a.z, b.z, c.z)); break;
+      case FT_EXPR_STEP: { float s = b.x < a.x ? 0.0f : 1.0f; r = mk(s, s, s); break; }
+      case FT_EXPR_SMOOTHSTEP: {
+        float span = b.x - a.x, t;
+        if (fabsf(span) < 1.1920929e-7f) t = c.x < a.x ? 0.0f : 1.0f;
+        else t = clampf((c.x - a.x) / span, 0.0f, 1.0f);
+        t = t * t * (3.0f - 2.0f * t);
+        r = mk(t, t, t);
+        break;
+      }
+      case FT_EXPR_DOT: { float s = dot(a, b); r = mk(s, s, s); break; }
+      case FT_EXPR_LENGTH: { float s = len(a); r = mk(s, s, s); break; }
+      case FT_EXPR_NORMALIZE: {
+        float l = len(a);
+        r = l <= 1.1920929e-7f ? mk(0, 0, 0) : mk(a.x / l, a.y / l, a.z / l);
+        break;
+      }
+      case FT_EXPR_ROTATE_X: { float rr = deg2rad(b.x); r = rot_x(a, sinf(rr), cosf(rr)); break; }
+      case FT_EXPR_ROTATE_Y: { float rr = deg2rad(b.x); r = rot_y(a, sinf(rr), cosf(rr)); break; }
+      case FT_EXPR_ROTATE_Z: { float rr = deg2rad(b.x); r = rot_z(a, sinf(rr), cosf(rr)); break; }
+      case FT_EXPR_SPHERE_DIST: { float s = len(a) - b.x; r = mk(s, s, s); break; }
+      case FT_EXPR_BOX_DIST: {
+        float3 q = mk(fabsf(a.x) - b.x, fabsf(a.y) - b.y, fabsf(a.z) - b.z);
+        float s = len(mk(fmaxf(q.x, 0), fmaxf(q.y, 0), fmaxf(q.z, 0))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
+        r = mk(s, s, s);
+        break;
+      }
+      case FT_EXPR_CYLINDER_DIST: {
+        float radial = sqrtf(a.x * a.x + a.z * a.z) - b.x, axial = fabsf(a.y) - c.x;
+        float ox = fmaxf(radial, 0.0f), oy = fmaxf(axial, 0.0f);
+        float s = sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, axial), 0.0f);
+        r = mk(s, s, s);
+        break;
+      }
+      case FT_EXPR_TORUS_DIST: {
+        float qx = sqrtf(a.x * a.x + a.z * a.z) - b.x;
+        float s = sqrtf(qx * qx + a.y * a.y) - c.x;
+        r = mk(s, s, s);
+        break;
+      }
+      case FT_EXPR_VALUE_NOISE: { float s = value_noise_3d(a); r = mk(s, s, s); break; }
+      case FT_EXPR_FBM: {
+        float3 d = e.d >= 0 ? v[e.d] : mk(0, 0, 0);
+        float s = fbm_3d(a, b.x > 0.0f ? (int)b.x : 0, c.x, d.x);
+        r = mk(s, s, s);
+        break;
+      }
+      default: r = mk(0, 0, 0); break;
+    }
+    v[k] = r;
+  }
+  return v[count - 1];
+}
+
+// eval_custom_sdf_distance (renderer.rs:6139-6179)
+__device__ inline float custom_distance(const DevScene& S, const DNode& n, float3 p) {
+  float3 dp = p;
+  if (n.fn_domain >= 0) dp = eval_fn(S, n.fn_domain, &p);
+  float d = n.fn_distance >= 0 ? eval_fn(S, n.fn_distance, &dp).x : 1.0e6f;
+  if (n.fn_post >= 0) {
+    float3 args[2] = {mk(d, d, d), dp};
+    d = eval_fn(S, n.fn_post, args).x;
+  }
+  return d;
+}
+
+__device__ __forceinline__ float shell_op(float d, float shell) { return shell > 0.0f ? fmaxf(d, -(d + shell)) : d; }
+
+// primitive distances (renderer.rs:5545-5663)
+__device__ __forceinline__ float prim_distance(const DNode& n, float3 p) {
+  float3 q = node_local(n, p);
+  switch (n.kind) {
+    case FT_NODE_SPHERE: return shell_op(len(q) - n.p[0], n.p[1]);
+    case FT_NODE_BOX: {
+      float rd = n.p[3];
+      float3 a = mk(fabsf(q.x) - fmaxf(n.p[0] - rd, 0.0f), fabsf(q.y) - fmaxf(n.p[1] - rd, 0.0f),
+                    fabsf(q.z) - fmaxf(n.p[2] - rd, 0.0f));
+      float outside = len(mk(fmaxf(a.x, 0.0f), fmaxf(a.y, 0.0f), fmaxf(a.z, 0.0f)));
+      float inside = fminf(fmaxf(fmaxf(a.x, a.y), a.z), 0.0f);
+      return shell_op(outside + inside - rd, n.p[4]);
+    }
+    case FT_NODE_CYLINDER: {
+      float rd = n.p[2];
+      float dx = sqrtf(q.x * q.x + q.z * q.z) - fmaxf(n.p[0] - rd, 0.0f);
+      float dy = fabsf(q.y) - fmaxf(n.p[1] - rd, 0.0f);
+      float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
+      return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(dx, dy), 0.0f) - rd, n.p[3]);
+    }
+    case FT_NODE_TORUS: {
+      float qx = sqrtf(q.x * q.x + q.z * q.z) - n.p[0];
+      return sqrtf(qx * qx + q.y * q.y) - n.p[1];
+    }
+    default: {  // FT_NODE_EXTRUDE_POLYGON
+      float rd = n.p[3];
+      float radial = sd_ngon(mk(q.x, 0.0f, q.z), (int)n.p[0], fmaxf(n.p[1] - rd, 0.0f));
+      float dy = fabsf(q.y) - fmaxf(n.p[2] - rd, 0.0f);
+      float ox = fmaxf(radial, 0.0f), oy = fmaxf(dy, 0.0f);
+      return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, dy), 0.0f) - rd, n.p[4]);
+    }
+  }
+}
+
+// ------------------------------------------------------ node program
+struct Info {
+  float d, lb;
+  unsigned obj, mat;
+  int leaf;
+};
+
+__device__ __forceinline__ bool is_prim(int k) { return k <= FT_NODE_EXTRUDE_POLYGON; }
+
+// Evaluates the program range [pf, pl] at p: sdf_distance_info + sdf_lower_bound of the
+// subtree root (renderer.rs:5413-5955).  Control flow over instructions is wave-uniform.
+__device__ __noinline__ Info eval_range(const DevScene& S, int pf, int pl, float3 p) {
+  float sd[kMaxStack], slb[kMaxStack];
+  unsigned sobj[kMaxStack], smat[kMaxStack];
+  int sleaf[kMaxStack];
+  float3 pst[kMaxPointStack];
+  int sp = 0, pp = 0;
+  for (int k = pf; k <= pl; ++k) {
+    const unsigned ins = S.prog[k];
+    const DNode& n = S.nodes[ins & ~kBeginFlag];
+    if (ins & kBeginFlag) {
+      pst[pp++] = p;
+      float3 q = node_local(n, p);
+      if (n.kind == FT_NODE_DOMAIN_MODIFIER && n.fn_domain >= 0) q = eval_fn(S, n.fn_domain, &q);
+      p = q;
+      continue;
+    }
+    const int kind = n.kind;
+    if (is_prim(kind)) {
+      float d = prim_distance(n, p);
+      sd[sp] = d; slb[sp] = d; sobj[sp] = n.obj; smat[sp] = n.mat; sleaf[sp] = n.leaf;
+      ++sp;
+      continue;
+    }
+    if (kind == FT_NODE_CUSTOM) {
+      float3 q = node_local(n, p);
+      float hx = n.p[0], hy = n.p[1], hz = n.p[2];
+      float mn[3] = {-hx, -hy, -hz}, mx[3] = {hx, hy, hz};
+      sd[sp] = custom_distance(S, n, q);
+      slb[sp] = aabb_lb(q, mn, mx);
+      sobj[sp] = n.obj; smat[sp] = n.mat; sleaf[sp] = n.leaf;
+      ++sp;
+      continue;
+    }
+    if (kind == FT_NODE_DOMAIN_MODIFIER || kind == FT_NODE_DISTANCE_POST_MODIFIER || kind == FT_NODE_NOISE) {
+      float3 q = p;
+      p = pst[--pp];
+      float d = sd[sp - 1];
+      if (kind == FT_NODE_DISTANCE_POST_MODIFIER && n.fn_post >= 0) {
+        float3 args[2] = {mk(d, d, d), q};
+        d = eval_fn(S, n.fn_post, args).x;
+      } else if (kind == FT_NODE_NOISE) {
+        d = apply_noise(q, d, (int)n.p[0], n.p[1], n.p[2]);
+      }
+      sd[sp - 1] = d;
+      if (n.flags & 2) slb[sp - 1] = aabb_lb(p, n.bmin, n.bmax);
+      if (n.leaf >= 0) sleaf[sp - 1] = n.leaf;
+      continue;
+    }
+    if (kind == FT_NODE_SLICE) {
+      float c = comp(p, (int)n.p[0]);
+      sd[sp - 1] = fmaxf(sd[sp - 1], fmaxf(n.p[1] - c, c - n.p[2]));
+      if (n.flags & 2) slb[sp - 1] = aabb_lb(p, n.bmin, n.bmax);
+      continue;
+    }
+    if (kind == FT_NODE_SMOOTH) {
+      sd[sp - 1] -= n.p[0] * 0.1f;
+      slb[sp - 1] -= n.p[0] * 0.1f;
+      continue;
+    }
+    // binary
+    --sp;
+    const int l = sp - 1, r = sp;
+    const float a = sd[l], b = sd[r];
+    if (kind == FT_NODE_UNION) {
+      // lower-bound short circuit (renderer.rs:5711-5731) evaluated eagerly
+      const float llb = slb[l], rlb = slb[r];
+      bool take_l = llb <= rlb ? (a <= rlb || a <= b) : !(b <= llb) && a <= b;
+      if (!take_l) { sd[l] = b; sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+      slb[l] = fminf(llb, rlb);
+      continue;
+    }
+    if (kind == FT_NODE_INTERSECT) {
+      if (!(a >= b)) { sd[l] = b; sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+      slb[l] = fmaxf(slb[l], slb[r]);
+      continue;
+    }
+    if (kind == FT_NODE_SUBTRACT) {
+      if (!(a >= -b)) { sd[l] = -b; sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+      continue;
+    }
+    const float rr = n.p[0], nn = n.p[1];
+    float d;
+    bool keep_l;
+    switch (kind) {
+      case FT_NODE_UNION_ROUND: d = op_union_round(a, b, rr); keep_l = a <= b; break;
+      case FT_NODE_UNION_CHAMFER: d = op_union_chamfer(a, b, rr); keep_l = a <= b; break;
+      case FT_NODE_UNION_COLUMNS: d = op_union_columns(a, b, rr, nn); keep_l = a <= b; break;
+      case FT_NODE_UNION_STAIRS: d = op_union_stairs(a, b, rr, nn); keep_l = a <= b; break;
+      case FT_NODE_UNION_SOFT: d = op_union_soft(a, b, rr); keep_l = a <= b; break;
+      case FT_NODE_INTERSECT_ROUND: d = op_intersect_round(a, b, rr); keep_l = a >= b; break;
+      case FT_NODE_INTERSECT_CHAMFER: d = op_intersect_chamfer(a, b, rr); keep_l = a >= b; break;
+      case FT_NODE_INTERSECT_COLUMNS: d = -op_union_columns(-a, -b, rr, nn); keep_l = a >= b; break;
+      case FT_NODE_INTERSECT_STAIRS: d = -op_union_stairs(-a, -b, rr, nn); keep_l = a >= b; break;
+      case FT_NODE_DIFF_ROUND: d = op_intersect_round(a, -b, rr); keep_l = a >= -b; break;
+      case FT_NODE_DIFF_CHAMFER: d = op_intersect_chamfer(a, -b, rr); keep_l = a >= -b; break;
+      case FT_NODE_DIFF_COLUMNS: d = -op_union_columns(-a, b, rr, nn); keep_l = a >= -b; break;
+      case FT_NODE_DIFF_STAIRS: d = -op_union_stairs(-a, b, rr, nn); keep_l = a >= -b; break;
+      case FT_NODE_PIPE: d = sqrtf(a * a + b * b) - fabsf(rr); keep_l = true; break;
+      case FT_NODE_ENGRAVE: d = fmaxf(a, (a + fabsf(rr) - fabsf(b)) * kInvSqrt2); keep_l = true; break;
+      case FT_NODE_GROOVE: d = fmaxf(a, fminf(a + fabsf(rr), fabsf(nn) - fabsf(b))); keep_l = true; break;
+      default: d = fminf(a, fmaxf(a - fabsf(rr), fabsf(b) - fabsf(nn))); keep_l = true; break;  // TONGUE
+    }
+    if (!keep_l) { sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+    sd[l] = d;
+    if (n.flags & 2) slb[l] = aabb_lb(p, n.bmin, n.bmax);
+  }
+  if (S.flops) atomicAdd(S.flops, (unsigned long long)S.nodes[S.prog[pl] & ~kBeginFlag].cost);
+  Info out;
+  out.d = sd[0]; out.lb = slb[0]; out.obj = sobj[0]; out.mat = smat[0]; out.leaf = sleaf[0];
+  return out;
+}
+
+__device__ __forceinline__ Info eval_node(const DevScene& S, int node, float3 p) {
+  const DNode& n = S.nodes[node];
+  return eval_range(S, n.prog_first, n.prog_last, p);
+}
+
+// sdf_lower_bound of the root (renderer.rs:5413-5543): post-order over the nodes reachable
+// through Union/Intersect/Subtract/Smooth; other nodes are terminals.
+__device__ __noinline__ float lower_bound_root(const DevScene& S, float3 p) {
+  float st[kMaxStack];
+  int sp = 0;
+  for (int k = 0; k < S.n_lbprog; ++k) {
+    const DNode& n = S.nodes[S.lbprog[k]];
+    switch (n.kind) {
+      case FT_NODE_UNION: --sp; st[sp - 1] = fminf(st[sp - 1], st[sp]); break;
+      case FT_NODE_INTERSECT: --sp; st[sp - 1] = fmaxf(st[sp - 1], st[sp]); break;
+      case FT_NODE_SUBTRACT: --sp; break;
+      case FT_NODE_SMOOTH: st[sp - 1] -= n.p[0] * 0.1f; break;
+      case FT_NODE_CUSTOM: {
+        float3 q = node_local(n, p);
+        float mn[3] = {-n.p[0], -n.p[1], -n.p[2]}, mx[3] = {n.p[0], n.p[1], n.p[2]};
+        st[sp++] = aabb_lb(q, mn, mx);
+        break;
+      }
+      default:
+        st[sp++] = is_prim(n.kind) ? prim_distance(n, p) : aabb_lb(p, n.bmin, n.bmax);
+        break;
+    }
+  }
+  return st[0];
+}
+
+// accel_leaf_distance_info (renderer.rs:629-657)
+__device__ __forceinline__ Info leaf_info(const DevScene& S, int li, float3 p) {
+  const DLeaf& L = S.leaves[li];
+  if (L.is_custom) {
+    float lb = aabb_lb(p, L.bmin, L.bmax);
+    float ex = L.bmax[0] - L.bmin[0], ey = L.bmax[1] - L.bmin[1], ez = L.bmax[2] - L.bmin[2];
+    float far_t = clampf(sqrtf(ex * ex + ey * ey + ez * ez) * 0.1f, 0.05f, 0.5f);
+    if (lb > far_t) {
+      const DNode& n = S.nodes[L.node];
+      Info o;
+      o.d = lb; o.lb = lb; o.obj = n.obj; o.mat = n.mat; o.leaf = li;
+      return o;
+    }
+  }
+  Info o = eval_node(S, L.node, p);
+  o.leaf = li;
+  return o;
+}
+
+// bvh_distance_info (renderer.rs:743-786), iterative near-first
+__device__ __noinline__ Info bvh_distance_info(const DevScene& S, float3 p) {
+  int stack[kMaxBvhStack];
+  int sp = 0;
+  float best = __builtin_huge_valf();
+  Info bi;
+  bool have = false;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const int ni = stack[--sp];
+    const DBvh& nd = S.bvh[ni];
+    if (S.flops) atomicAdd(S.flops, 3ull * kAabbFlops);
+    if (aabb_lb(p, nd.bmin, nd.bmax) > best) continue;
+    if (nd.left < 0) {
+      Info info = leaf_info(S, -nd.left - 1, p);
+      if (info.d < best) best = info.d;
+      if (!have || info.d < bi.d) { bi = info; have = true; }
+      continue;
+    }
+    const DBvh& L = S.bvh[nd.left];
+    const DBvh& R = S.bvh[nd.right];
+    float llb = aabb_lb(p, L.bmin, L.bmax), rlb = aabb_lb(p, R.bmin, R.bmax);
+    int first = nd.left, second = nd.right;
+    if (!(llb <= rlb)) { first = nd.right; second = nd.left; }
+    if (sp + 2 <= kMaxBvhStack) {
+      stack[sp++] = second;
+      stack[sp++] = first;
+    }
+  }
+  if (!have) {
+    bi = eval_node(S, S.root, p);
+  }
+  return bi;
+}
+
+__device__ __noinline__ float bvh_lower_bound(const DevScene& S, float3 p) {
+  int stack[kMaxBvhStack];
+  int sp = 0;
+  float best = __builtin_huge_valf();
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const DBvh& nd = S.bvh[stack[--sp]];
+    float lb = aabb_lb(p, nd.bmin, nd.bmax);
+    if (lb > best) continue;
+    if (nd.left < 0) { best = fminf(best, lb); continue; }
+    if (sp + 2 <= kMaxBvhStack) {
+      stack[sp++] = nd.right;
+      stack[sp++] = nd.left;
+    }
+  }
+  return best;
+}
+
+// Accelerator::distance_info / lower_bound (renderer.rs:609-623, 1024-1129).  Bricks uses
+// the BVH traversal: both return the minimum over accel leaves (the grid's shell search
+// visits a superset of the leaves that can beat the running best).
+__device__ __forceinline__ Info scene_info(const DevScene& S, float3 p) {
+  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_info(S, p);
+  return eval_node(S, S.root, p);
+}
+__device__ __forceinline__ float scene_lower_bound(const DevScene& S, float3 p) {
+  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_lower_bound(S, p);
+  return lower_bound_root(S, p);
+}
+
+// ---------------------------------------------------------------- march
+struct Hit {
+  float t;
+  float3 pos, n;
+  bool front;
+  unsigned obj, mat;
+  int leaf;
+};
+
+// ray_aabb_intersection (renderer.rs:4141-4173)
+__device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, const float* mx, float& t0, float& t1) {
+  float tmin = -__builtin_huge_valf(), tmax = __builtin_huge_valf();
+  for (int axis = 0; axis < 3; ++axis) {
+    float oo = comp(o, axis), dd = comp(d, axis);
+    if (fabsf(dd) <= 1.0e-8f) {
+      if (oo < mn[axis] || oo > mx[axis]) return false;
+      continue;
+    }
+    float inv = 1.0f / dd;
+    float a = (mn[axis] - oo) * inv, c = (mx[axis] - oo) * inv;
+    if (a > c) { float t = a; a = c; c = t; }
+    tmin = fmaxf(tmin, a);
+    tmax = fminf(tmax, c);
+    if (tmax < tmin) return false;
+  }
+  t0 = tmin;
+  t1 = tmax;
+  return true;
+}
+
+// raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215)
+__device__ __noinline__ bool raymarch(const DevScene& S, float3 o, float3 d, int max_steps, float eps,
+                                      float step_scale, float min_t, float max_t, Hit& hit) {
+  float entry, exit;
+  if (!ray_aabb(o, d, S.bmin, S.bmax, entry, exit)) return false;
+  max_t = fminf(max_t, exit);
+  float traveled = fmaxf(min_t, fmaxf(entry, 0.0f));
+  float prev = traveled;
+  const float sscale = clampf(step_scale, 0.05f, 1.0f);
+  const float min_step = fmaxf(eps * 0.5f, 1.0e-5f);
+  for (int s = 0; s < max_steps; ++s) {
+    if (traveled > max_t) return false;
+    float dist = scene_info(S, add(o, mul(d, traveled))).d;
+    if (fabsf(dist) < eps) {
+      float lo = prev, hi = traveled;
+      for (int k = 0; k < 10; ++k) {
+        float mid = 0.5f * (lo + hi);
+        if (fabsf(scene_info(S, add(o, mul(d, mid))).d) < eps) hi = mid; else lo = mid;
+      }
+      float3 pos = add(o, mul(d, hi));
+      float e = fmaxf(eps * 4.0f, 1.0e-5f);
+      float dx = scene_info(S, mk(pos.x + e, pos.y, pos.z)).d - scene_info(S, mk(pos.x - e, pos.y, pos.z)).d;
+      float dy = scene_info(S, mk(pos.x, pos.y + e, pos.z)).d - scene_info(S, mk(pos.x, pos.y - e, pos.z)).d;
+      float dz = scene_info(S, mk(pos.x, pos.y, pos.z + e)).d - scene_info(S, mk(pos.x, pos.y, pos.z - e)).d;
+      float3 nrm = mk(dx, dy, dz);
+      nrm = len(nrm) > 1.0e-8f ? normalize(nrm) : mk(0.0f, 1.0f, 0.0f);
+      Info fin = scene_info(S, pos);
+      hit.t = hi;
+      hit.pos = pos;
+      hit.n = nrm;
+      hit.front = dot(nrm, d) < 0.0f;
+      hit.obj = fin.obj;
+      hit.mat = fin.mat;
+      hit.leaf = fin.leaf;
+      return true;
+    }
+    prev = traveled;
+    traveled += fmaxf(fabsf(dist) * sscale, min_step);
+  }
+  return false;
+}
+
+}  // namespace ftd

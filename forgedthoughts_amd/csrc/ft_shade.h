@@ -1,0 +1,659 @@
+// ft_shade.h — per-hit resolution, BSDFs, lights and the Whitted integrator (device).
+//
+// The reference resolves the shading normal and material of a hit by walking the scene
+// tree (renderer.rs:4397-4570, 6926-7651) and traces reflection/refraction recursively
+// (renderer/ray.rs:3-435).  Here:
+//   * the walk starts at the accel leaf that produced the hit distance: above the leaves
+//     the tree is hard Unions only, and the reference walk there follows the side with
+//     the smaller distance, i.e. exactly the leaf that wins the min;
+//   * the walk below is iterative with explicit frames, evaluating child subtrees on
+//     demand as the reference does;
+//   * recursion is replaced by a per-lane stack of pending rays carrying a throughput:
+//     every term of trace_hit_with_material is linear in the child radiance, so the
+//     pixel value is the sum over the ray tree of path weight x local term.
+#pragma once
+
+#include "ft_device.h"
+
+namespace ftd {
+
+enum { TAG_STANDARD = 0, TAG_LAMBERT = 1, TAG_METAL = 2, TAG_DIELECTRIC = 3 };
+
+struct Mat {  // Material (materials/mod.rs:214-221); a is used when !blend
+  int blend, a_model, b_model, pad;
+  float t;
+  DParams a, b;
+};
+
+__device__ __forceinline__ float3 v3(const float* f) { return mk(f[0], f[1], f[2]); }
+__device__ __forceinline__ void set3(float* f, float3 v) { f[0] = v.x; f[1] = v.y; f[2] = v.z; }
+
+// lerp_spectrum (renderer.rs:5206-5213, materials/mod.rs:434-441)
+__device__ __forceinline__ float3 lerp_spec(float3 a, float3 b, float t) {
+  float tt = clampf(t, 0.0f, 1.0f);
+  return mk(a.x * (1.0f - tt) + b.x * tt, a.y * (1.0f - tt) + b.y * tt, a.z * (1.0f - tt) + b.z * tt);
+}
+__device__ __forceinline__ float smoothstepf(float x) {  // smoothstepf(0, 1, x) (renderer.rs:7538-7541)
+  float t = clampf(x / 1.0f, 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+
+// blend_params (renderer.rs:7602-7627)
+__device__ inline void blend_params(DParams& a, const DParams& b, float t) {
+  set3(a.color, lerp_spec(v3(a.color), v3(b.color), t));
+  a.roughness = a.roughness + (b.roughness - a.roughness) * t;
+  a.metallic = a.metallic + (b.metallic - a.metallic) * t;
+  a.specular = a.specular + (b.specular - a.specular) * t;
+  a.specular_weight = a.specular_weight + (b.specular_weight - a.specular_weight) * t;
+  set3(a.specular_color, lerp_spec(v3(a.specular_color), v3(b.specular_color), t));
+  a.ior = a.ior + (b.ior - a.ior) * t;
+  a.clearcoat = a.clearcoat + (b.clearcoat - a.clearcoat) * t;
+  a.clearcoat_roughness = a.clearcoat_roughness + (b.clearcoat_roughness - a.clearcoat_roughness) * t;
+  a.transmission = a.transmission + (b.transmission - a.transmission) * t;
+  a.thin_walled = t < 0.5f ? a.thin_walled : b.thin_walled;
+  set3(a.emission_color, lerp_spec(v3(a.emission_color), v3(b.emission_color), t));
+  a.emission_strength = a.emission_strength + (b.emission_strength - a.emission_strength) * t;
+  if (a.has_medium != 0.0f && b.has_medium != 0.0f) {  // blend_medium (renderer.rs:7543-7566)
+    a.medium_ior = a.medium_ior + (b.medium_ior - a.medium_ior) * t;
+    set3(a.medium_absorption, lerp_spec(v3(a.medium_absorption), v3(b.medium_absorption), t));
+    a.medium_density = a.medium_density + (b.medium_density - a.medium_density) * t;
+  } else if (a.has_medium != 0.0f) {
+    a.has_medium = t < 0.5f ? 1.0f : 0.0f;
+  } else if (b.has_medium != 0.0f && t >= 0.5f) {
+    a.has_medium = 1.0f;
+    a.medium_ior = b.medium_ior;
+    set3(a.medium_absorption, v3(b.medium_absorption));
+    a.medium_density = b.medium_density;
+  }
+  a.pattern = 0.0f;
+}
+
+__device__ __forceinline__ int dom_model(const Mat& m) { return !m.blend ? m.a_model : (m.t < 0.5f ? m.a_model : m.b_model); }
+__device__ __forceinline__ const DParams& dom_params(const Mat& m) { return (!m.blend || m.t < 0.5f) ? m.a : m.b; }
+
+// blend_materials (renderer.rs:7629-7651)
+__device__ inline void blend_materials(Mat& left, const Mat& right, float t) {
+  if (!left.blend && !right.blend && left.a_model == right.a_model) {
+    blend_params(left.a, right.a, t);
+    return;
+  }
+  Mat out;
+  out.blend = 1;
+  out.a_model = dom_model(left);
+  out.a = dom_params(left);
+  out.b_model = dom_model(right);
+  out.b = dom_params(right);
+  out.t = clampf(t, 0.0f, 1.0f);
+  out.pad = 0;
+  left = out;
+}
+
+// resolve_leaf_material, static part (renderer.rs:7164-7536)
+__device__ inline void leaf_material(const DevScene& S, unsigned mat_id, unsigned obj, float3 pos, Mat& m) {
+  unsigned id = mat_id < (unsigned)S.n_mats ? mat_id : 0u;
+  m.blend = 0;
+  m.a = S.mats[id];
+  m.a_model = (int)m.a.model;
+  m.b_model = 0;
+  m.t = 0.0f;
+  m.pad = 0;
+  if (m.a.pattern == (float)FT_PATTERN_CHECKER3D) {  // resolve_pattern_color (renderer.rs:7855-7873)
+    float3 lp = object_local(S, obj, pos);
+    int sx = (int)floorf(lp.x * m.a.pattern_scale), sy = (int)floorf(lp.y * m.a.pattern_scale),
+        sz = (int)floorf(lp.z * m.a.pattern_scale);
+    set3(m.a.color, ((sx + sy + sz) & 1) == 0 ? v3(m.a.pattern_a) : v3(m.a.pattern_b));
+  }
+  if (m.a_model != TAG_LAMBERT) m.a.roughness = clampf(m.a.roughness, 0.0f, 1.0f);
+  if (m.a_model == TAG_STANDARD || m.a_model == TAG_DIELECTRIC) m.a.ior = clampf(m.a.ior, 1.0f, 3.0f);
+  m.a.emission_strength = fmaxf(m.a.emission_strength, 0.0f);
+}
+
+__device__ __forceinline__ int blend_class(int k) {
+  switch (k) {
+    case FT_NODE_UNION_ROUND: case FT_NODE_UNION_CHAMFER: case FT_NODE_UNION_SOFT:
+    case FT_NODE_INTERSECT_ROUND: case FT_NODE_INTERSECT_CHAMFER: case FT_NODE_UNION_COLUMNS:
+    case FT_NODE_UNION_STAIRS: case FT_NODE_INTERSECT_COLUMNS: case FT_NODE_INTERSECT_STAIRS: return 1;
+    case FT_NODE_DIFF_ROUND: case FT_NODE_DIFF_CHAMFER: case FT_NODE_PIPE: case FT_NODE_ENGRAVE:
+    case FT_NODE_DIFF_COLUMNS: case FT_NODE_DIFF_STAIRS: case FT_NODE_GROOVE: case FT_NODE_TONGUE: return 2;
+    default: return 0;
+  }
+}
+__device__ __forceinline__ float blend_radius(const DNode& n) {
+  return (n.kind == FT_NODE_GROOVE || n.kind == FT_NODE_TONGUE) ? fmaxf(n.p[0], n.p[1]) : n.p[0];
+}
+// blend weight t of a smooth node from its children's distances
+__device__ __forceinline__ float blend_t(const DNode& n, float dl, float dr) {
+  float k = fmaxf(fabsf(blend_radius(n)), 1.0e-6f);
+  if (blend_class(n.kind) == 2) return smoothstepf(0.5f + 0.5f * (((-dr) - dl) / k));
+  return smoothstepf(0.5f + 0.5f * ((dl - dr) / k));
+}
+// hard boolean side choice (renderer.rs:4473-4498, 7000-7025)
+__device__ __forceinline__ bool hard_left(int kind, float dl, float dr) {
+  if (kind == FT_NODE_UNION) return dl <= dr;
+  if (kind == FT_NODE_INTERSECT) return dl >= dr;
+  return dl >= -dr;
+}
+__device__ __forceinline__ bool is_hard(int k) { return k == FT_NODE_UNION || k == FT_NODE_INTERSECT || k == FT_NODE_SUBTRACT; }
+
+// estimate_node_normal (renderer.rs:4554-4570)
+__device__ inline float3 node_normal(const DevScene& S, int node, float3 p, float eps) {
+  float e = fmaxf(eps, 1.0e-5f);
+  float d1 = eval_node(S, node, mk(p.x + e, p.y - e, p.z - e)).d;
+  float d2 = eval_node(S, node, mk(p.x - e, p.y - e, p.z + e)).d;
+  float d3 = eval_node(S, node, mk(p.x - e, p.y + e, p.z - e)).d;
+  float d4 = eval_node(S, node, mk(p.x + e, p.y + e, p.z + e)).d;
+  float3 n = add(add(add(mk(d1, -d1, -d1), mk(-d2, -d2, d2)), mk(-d3, d3, -d3)), mk(d4, d4, d4));
+  if (len(n) > 1.0e-6f) return normalize(n);
+  return mk(0.0f, 1.0f, 0.0f);
+}
+
+struct Frame { int node, phase; };
+
+// resolve_surface_normal_from_node (renderer.rs:4462-4552), from `start`
+__device__ __noinline__ float3 walk_normal(const DevScene& S, int start, float3 p, float eps) {
+  Frame fr[kMaxWalk];
+  float3 res[kMaxWalk];
+  int fsp = 0, rsp = 0;
+  fr[fsp++] = {start, 0};
+  while (fsp > 0) {
+    Frame f = fr[--fsp];
+    const DNode& n = S.nodes[f.node];
+    if (n.kind <= FT_NODE_NOISE) { res[rsp++] = node_normal(S, f.node, p, eps); continue; }
+    if (n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) { fr[fsp++] = {n.lhs, 0}; continue; }
+    if (is_hard(n.kind)) {
+      if (f.phase == 1) { res[rsp - 1] = mul(res[rsp - 1], -1.0f); continue; }
+      float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+      if (hard_left(n.kind, dl, dr)) fr[fsp++] = {n.lhs, 0};
+      else {
+        if (n.kind == FT_NODE_SUBTRACT) fr[fsp++] = {f.node, 1};
+        fr[fsp++] = {n.rhs, 0};
+      }
+      continue;
+    }
+    if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
+    if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
+    float3 right = res[--rsp], left = res[--rsp];
+    float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+    if (blend_class(n.kind) == 2) right = mul(right, -1.0f);
+    float t = blend_t(n, dl, dr);
+    float3 nn = add(mul(left, 1.0f - t), mul(right, t));
+    res[rsp++] = len(nn) > 1.0e-6f ? normalize(nn) : (t < 0.5f ? left : right);
+  }
+  return res[0];
+}
+
+// resolve_material_from_node (renderer.rs:6945-7059), from `start`
+__device__ __noinline__ void walk_material(const DevScene& S, int start, float3 p, Mat& out) {
+  Frame fr[kMaxWalk];
+  Mat res[kMaxWalkMats];
+  int fsp = 0, rsp = 0;
+  fr[fsp++] = {start, 0};
+  while (fsp > 0) {
+    Frame f = fr[--fsp];
+    const DNode& n = S.nodes[f.node];
+    if (n.kind <= FT_NODE_CUSTOM) { leaf_material(S, n.mat, n.obj, p, res[rsp++]); continue; }
+    if (n.kind <= FT_NODE_NOISE || n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) {
+      fr[fsp++] = {n.lhs, 0};
+      continue;
+    }
+    if (is_hard(n.kind)) {
+      float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+      fr[fsp++] = {hard_left(n.kind, dl, dr) ? n.lhs : n.rhs, 0};
+      continue;
+    }
+    if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
+    if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
+    --rsp;
+    float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+    blend_materials(res[rsp - 1], res[rsp], blend_t(n, dl, dr));
+  }
+  out = res[0];
+}
+
+// resolve_split_material_from_node (renderer.rs:7061-7120)
+__device__ __noinline__ bool split_material(const DevScene& S, int start, float3 p, Mat& a, Mat& b, float& t) {
+  int node = start;
+  for (;;) {
+    const DNode& n = S.nodes[node];
+    if (is_hard(n.kind)) {
+      float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+      node = hard_left(n.kind, dl, dr) ? n.lhs : n.rhs;
+      continue;
+    }
+    if (n.kind == FT_NODE_SMOOTH || n.kind == FT_NODE_NOISE) { node = n.lhs; continue; }
+    if (blend_class(n.kind) == 0) return false;
+    float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+    walk_material(S, n.lhs, p, a);
+    walk_material(S, n.rhs, p, b);
+    t = blend_t(n, dl, dr);
+    return true;
+  }
+}
+
+// ---------------------------------------------------------------- BSDFs
+constexpr float kPi = 3.14159265358979323846f;
+__device__ __forceinline__ float ggx_d(float ndoth, float alpha) {
+  float a2 = alpha * alpha, n2 = ndoth * ndoth;
+  float denom = fmaxf(n2 * (a2 - 1.0f) + 1.0f, 1.0e-5f);
+  return a2 / (kPi * denom * denom);
+}
+__device__ __forceinline__ float smith_g1(float x, float alpha) {
+  float k = (alpha * alpha) * 0.5f;
+  return x / fmaxf(x * (1.0f - k) + k, 1.0e-5f);
+}
+// metal.rs:7-21
+__device__ inline float3 metal_eval(float3 color, float roughness, float3 n, float3 wi, float3 wo) {
+  float ndotl = fmaxf(dot(n, wi), 0.0f), ndotv = fmaxf(dot(n, wo), 0.0f);
+  if (ndotl <= 0.0f || ndotv <= 0.0f) return mk(0, 0, 0);
+  float3 h = normalize(add(wi, wo));
+  float ndoth = fmaxf(dot(n, h), 0.0f), vdoth = fmaxf(dot(wo, h), 0.0f);
+  float r = clampf(roughness, 0.02f, 1.0f);
+  float alpha = r * r;
+  float d = ggx_d(ndoth, alpha);
+  float g = smith_g1(ndotl, alpha) * smith_g1(ndotv, alpha);
+  float m = powi5(1.0f - clampf(vdoth, 0.0f, 1.0f));
+  float3 fres = add(add(color, mk(m, m, m)), mul(color, -m));
+  return mul(fres, (d * g) / fmaxf(4.0f * ndotl * ndotv, 1.0e-5f));
+}
+__device__ __forceinline__ float f0_from_ior(float ior) {
+  float i = fmaxf(ior, 1.0e-3f);
+  float r = (i - 1.0f) / (i + 1.0f);
+  return r * r;
+}
+// standard.rs:7-45
+__device__ inline float3 standard_eval(const DParams& m, float3 n, float3 wi, float3 wo) {
+  float ndotl = fmaxf(dot(n, wi), 0.0f), ndotv = fmaxf(dot(n, wo), 0.0f);
+  if (ndotl <= 0.0f || ndotv <= 0.0f) return mk(0, 0, 0);
+  float metallic = clampf(m.metallic, 0, 1), transmission = clampf(m.transmission, 0, 1);
+  float clearcoat = clampf(m.clearcoat, 0, 1);
+  float mw = clampf(1.0f - clampf(ndotv, 0, 1), 0, 1);
+  float m2 = mw * mw;
+  float coat_cover = clampf(clearcoat * (0.2f + 0.8f * (m2 * m2 * mw)) * 0.65f, 0.0f, 0.95f);
+  float base_kd = clampf((1.0f - metallic) * (1.0f - transmission), 0, 1) * (1.0f - coat_cover);
+  float3 diffuse = mul(v3(m.color), base_kd / kPi);
+  float f0d = clampf(f0_from_ior(m.ior), 0, 1);
+  float3 base_f0 = cmul(mul(mk(f0d, f0d, f0d), clampf(m.specular, 0, 1) * clampf(m.specular_weight, 0, 1)),
+                        v3(m.specular_color));
+  float3 spec_f0 = add(mul(base_f0, 1.0f - metallic), mul(v3(m.color), metallic));
+  float3 spec = mul(metal_eval(spec_f0, m.roughness, n, wi, wo), 1.0f - coat_cover * 0.85f);
+  float3 cc = mk(0, 0, 0);
+  if (clearcoat > 1.0e-5f)
+    cc = mul(metal_eval(mk(1, 1, 1), clampf(m.clearcoat_roughness, 0.02f, 1.0f), n, wi, wo), clearcoat * 0.18f);
+  return add(add(diffuse, spec), cc);
+}
+__device__ inline float3 model_eval(int model, const DParams& p, float3 n, float3 wi, float3 wo) {
+  switch (model) {
+    case TAG_STANDARD: return standard_eval(p, n, wi, wo);
+    case TAG_LAMBERT: return dot(n, wi) <= 0.0f ? mk(0, 0, 0) : mul(v3(p.color), 1.0f / kPi);
+    case TAG_METAL: return metal_eval(v3(p.color), p.roughness, n, wi, wo);
+    default: return mk(0, 0, 0);
+  }
+}
+__device__ inline float3 mat_eval(const Mat& m, float3 n, float3 wi, float3 wo) {
+  if (!m.blend) return model_eval(m.a_model, m.a, n, wi, wo);
+  return lerp_spec(model_eval(m.a_model, m.a, n, wi, wo), model_eval(m.b_model, m.b, n, wi, wo), m.t);
+}
+__device__ __forceinline__ float3 emission_of(const DParams& p) { return mul(v3(p.emission_color), fmaxf(p.emission_strength, 0.0f)); }
+__device__ inline float3 mat_emission(const Mat& m) {
+  if (!m.blend) return emission_of(m.a);
+  return lerp_spec(emission_of(m.a), emission_of(m.b), m.t);
+}
+
+// --------------------------------------------------------------- lights
+// offset_ray_origin (renderer.rs:5067-5078)
+__device__ __forceinline__ float3 offset_origin(float3 pos, float3 gn, float3 dir, float eps) {
+  float3 n = len(gn) > 1.0e-8f ? normalize(gn) : normalize(dir);
+  float sign = dot(dir, n) >= 0.0f ? 1.0f : -1.0f;
+  float3 no = mul(n, sign * fmaxf(eps * 16.0f, 2.0e-4f));
+  return add(add(pos, no), mul(normalize(dir), fmaxf(eps * 8.0f, 1.0e-4f)));
+}
+
+// shadow_visibility (renderer.rs:4706-4733)
+__device__ __noinline__ float shadow_visibility(const DevScene& S, float3 o, float3 d, float max_t, float eps) {
+  float t = eps * 2.0f, vis = 1.0f;
+  const float lb_gate = fmaxf(eps * 8.0f, 0.02f);
+  for (int k = 0; k < 80; ++k) {
+    if (t >= max_t) return clampf(vis, 0.0f, 1.0f);
+    float3 p = add(o, mul(d, t));
+    float lower = fabsf(scene_lower_bound(S, p));
+    float h = lower > lb_gate ? lower : fabsf(scene_info(S, p).d);
+    if (h < eps * 4.0f) return 0.0f;
+    vis = fminf(vis, clampf(10.0f * h / t, 0.0f, 1.0f));
+    t += fmaxf(h, eps * 2.0f);
+  }
+  return clampf(vis, 0.0f, 1.0f);
+}
+
+// render_api.rs:171-289
+__device__ inline void sample_light(const DLight& L, float3 point, unsigned idx, unsigned count, float3& wi,
+                                    float3& rad, float& dist) {
+  if (L.kind == FT_LIGHT_ENV) {
+    wi = mk(0.0f, 1.0f, 0.0f);
+    rad = v3(L.rad);
+    dist = __builtin_huge_valf();
+    return;
+  }
+  float3 target = v3(L.pos);
+  if (L.kind == FT_LIGHT_SPHERE && L.radius > 1.0e-5f) {
+    float3 normal = normalize(sub(point, target));
+    float3 up = fabsf(normal.y) < 0.99f ? mk(0, 1, 0) : mk(1, 0, 0);
+    float3 tangent = normalize(cross(normal, up));
+    float3 bitangent = normalize(cross(tangent, normal));
+    float u1 = ((float)idx + 0.5f) / (float)(count < 1 ? 1 : count);
+    unsigned bits = (idx >> 16) | (idx << 16);
+    bits = ((bits & 0x55555555u) << 1) | ((bits & 0xAAAAAAAAu) >> 1);
+    bits = ((bits & 0x33333333u) << 2) | ((bits & 0xCCCCCCCCu) >> 2);
+    bits = ((bits & 0x0F0F0F0Fu) << 4) | ((bits & 0xF0F0F0F0u) >> 4);
+    bits = ((bits & 0x00FF00FFu) << 8) | ((bits & 0xFF00FF00u) >> 8);
+    float u2 = (float)bits * 2.3283064e-10f;
+    float sx = 2.0f * u1 - 1.0f, sy = 2.0f * u2 - 1.0f;
+    float dx = 0.0f, dy = 0.0f;
+    if (!(fabsf(sx) <= 1.1920929e-7f && fabsf(sy) <= 1.1920929e-7f)) {
+      float r, th;
+      if (fabsf(sx) > fabsf(sy)) { r = sx; th = 0.78539816339744830962f * (sy / sx); }
+      else { r = sy; th = 1.57079632679489661923f - 0.78539816339744830962f * (sx / sy); }
+      dx = r * cosf(th);
+      dy = r * sinf(th);
+    }
+    target = add(add(target, mul(tangent, dx * L.radius)), mul(bitangent, dy * L.radius));
+  }
+  float3 to = sub(target, point);
+  dist = fmaxf(len(to), 1.0e-4f);
+  wi = mul(to, 1.0f / dist);
+  rad = mul(v3(L.rad), 1.0f / (dist * dist));
+}
+
+// shade_color (renderer.rs:4217-4276)
+__device__ __noinline__ float3 shade_color(const DevScene& S, const Mat& mat, float3 hit_pos, float3 hit_n, bool front,
+                                           float3 nrm, float3 wo, float eps, float max_dist) {
+  if (S.n_lights == 0) return mk(1, 1, 1);
+  float3 n = normalize(nrm);
+  float3 gn = front ? normalize(hit_n) : normalize(mul(hit_n, -1.0f));
+  float3 color = mk(0, 0, 0);
+  for (int li = 0; li < S.n_lights; ++li) {
+    const DLight& L = S.lights[li];
+    unsigned ns = L.kind == FT_LIGHT_SPHERE ? (L.samples < 1 ? 1u : L.samples) : 1u;
+    float inv = 1.0f / (float)ns;
+    for (unsigned si = 0; si < ns; ++si) {
+      float3 wi, rad;
+      float dist;
+      sample_light(L, hit_pos, si, ns, wi, rad, dist);
+      wi = normalize(wi);
+      float ndotl = n.x * wi.x + n.y * wi.y + n.z * wi.z;
+      if (ndotl <= 0.0f) continue;
+      float3 so = offset_origin(hit_pos, gn, wi, eps);
+      float max_t = isfinite(dist) ? fmaxf(dist - eps * 8.0f, 0.0f) : max_dist;
+      if (max_t <= 0.0f) continue;
+      float sh = shadow_visibility(S, so, wi, max_t, eps);
+      if (sh <= 0.0f) continue;
+      float3 f = mat_eval(mat, nrm, wi, wo);
+      color = add(color, mul(cmul(f, rad), ndotl * sh * inv));
+    }
+  }
+  return color;
+}
+
+// --------------------------------------------------------------- Whitted
+struct Medium {
+  float ior, ax, ay, az, density;
+};
+__device__ __forceinline__ float3 transmittance(const Medium& m, float dist) {  // renderer.rs:5109-5125
+  if (m.density <= 0.0f || dist <= 0.0f) return mk(1, 1, 1);
+  float ex = m.density * fmaxf(dist, 0.0f);
+  return mk(powf(clampf(m.ax, 0, 1), ex), powf(clampf(m.ay, 0, 1), ex), powf(clampf(m.az, 0, 1), ex));
+}
+__device__ __forceinline__ float fresnel_dielectric(float cos_i, float eta_i, float eta_t) {  // renderer.rs:5186-5204
+  float ei = fmaxf(eta_i, 1.0e-4f), et = fmaxf(eta_t, 1.0e-4f);
+  if (fabsf(ei - et) < 1.0e-6f) return 0.0f;
+  float ci = clampf(cos_i, 0.0f, 1.0f);
+  float eta = ei / et;
+  float sin2t = eta * eta * (1.0f - ci * ci);
+  if (sin2t >= 1.0f) return 1.0f;
+  float ct = sqrtf(1.0f - sin2t);
+  float rs = ((ei * ci) - (et * ct)) / fmaxf((ei * ci) + (et * ct), 1.0e-6f);
+  float rp = ((et * ci) - (ei * ct)) / fmaxf((et * ci) + (ei * ct), 1.0e-6f);
+  return 0.5f * (rs * rs + rp * rp);
+}
+__device__ __forceinline__ float3 reflect(float3 v, float3 n) { return sub(v, mul(n, 2.0f * dot(v, n))); }
+__device__ __forceinline__ bool refract(float3 incident, float3 normal, float eta, float3& out) {  // renderer.rs:5080-5095
+  float3 i = normalize(incident), n = normalize(normal);
+  float cosi = clampf(dot(i, n), -1.0f, 1.0f);
+  if (cosi > 0.0f) n = mul(n, -1.0f); else cosi = -cosi;
+  float sin2t = eta * eta * (1.0f - cosi * cosi);
+  if (sin2t > 1.0f) return false;
+  float cost = sqrtf(1.0f - sin2t);
+  out = normalize(add(mul(i, eta), mul(n, eta * cosi - cost)));
+  return true;
+}
+
+struct RayEntry {
+  float ox, oy, oz, dx, dy, dz;
+  float tr, tg, tb;
+  float m_ior, m_ax, m_ay, m_az, m_density;
+  int depth;
+  int pad;
+};
+
+struct TraceParams {
+  int max_steps, max_depth;
+  float eps, step_scale, max_dist;
+};
+
+template <int kRayStack>
+struct RayStack {
+  RayEntry e[kRayStack];
+  int sp;
+  int overflow;
+  __device__ void push(float3 o, float3 d, float3 thr, const Medium& m, int depth) {
+    if (sp >= kRayStack) { overflow = 1; return; }
+    RayEntry& r = e[sp++];
+    r.ox = o.x; r.oy = o.y; r.oz = o.z; r.dx = d.x; r.dy = d.y; r.dz = d.z;
+    r.tr = thr.x; r.tg = thr.y; r.tb = thr.z;
+    r.m_ior = m.ior; r.m_ax = m.ax; r.m_ay = m.ay; r.m_az = m.az; r.m_density = m.density;
+    r.depth = depth;
+  }
+};
+
+__device__ __forceinline__ float3 environment(const DevScene& S, float3 dir) {
+  if (S.env_fn >= 0) return eval_fn(S, S.env_fn, &dir);
+  return v3(S.env_rad);
+}
+
+// rough_lobe_directions (ray.rs:383-417)
+template <int kRayStack>
+__device__ inline void push_lobes(RayStack<kRayStack>& st, float3 hp, float3 gn, float3 ideal_dir, float rough,
+                                  int count, float3 thr, const Medium& med, int depth, float eps) {
+  float3 ideal = normalize(ideal_dir);
+  float3 seed = fabsf(ideal.y) < 0.99f ? mk(0, 1, 0) : mk(1, 0, 0);
+  float3 tangent = normalize(cross(ideal, seed));
+  float3 bitangent = normalize(cross(tangent, ideal));
+  float spread = clampf(rough, 0.0f, 0.5f) * 0.35f;
+  int n = (count <= 1 || spread <= 1.0e-4f) ? 1 : (count > 7 ? 7 : count);
+  const float ox[6] = {1.0f, -1.0f, 0.0f, 0.0f, 0.707f, -0.707f};
+  const float oy[6] = {0.0f, 0.0f, 1.0f, -1.0f, 0.707f, 0.707f};
+  float inv = 1.0f / (float)n;
+  float3 w = mul(thr, inv);
+  for (int k = 0; k < n; ++k) {
+    float3 d = k == 0 ? ideal : normalize(add(add(ideal, mul(tangent, ox[k - 1] * spread)), mul(bitangent, oy[k - 1] * spread)));
+    st.push(offset_origin(hp, gn, d, eps * (8.0f + rough * 12.0f)), d, w, med, depth + 1);
+  }
+}
+
+// trace_hit_with_material (ray.rs:41-222) for one material of a hit; returns the local
+// term (already weighted by `thr`) and pushes the children.
+template <int kRayStack>
+__device__ __noinline__ float3 hit_material(const DevScene& S, const TraceParams& tp, const Hit& hit, float3 dir,
+                                            float3 nrm, const Medium& med, int depth, const Mat& mat, float3 thr,
+                                            RayStack<kRayStack>& st) {
+  const float3 hp = hit.pos;
+  const float3 normal = hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f));
+  const float3 wo = normalize(mul(dir, -1.0f));
+  const DParams& pr = dom_params(mat);
+  const float rough = clampf(pr.roughness, 0.0f, 1.0f);
+  float transmission, ior, metallic, clearcoat, spec_strength;
+  bool thin;
+  float3 spec_color, base_color = v3(pr.color);
+  switch (dom_model(mat)) {
+    case TAG_STANDARD:
+      transmission = clampf(pr.transmission, 0, 1); ior = clampf(pr.ior, 1, 3); thin = pr.thin_walled != 0.0f;
+      spec_color = v3(pr.specular_color); metallic = clampf(pr.metallic, 0, 1); clearcoat = clampf(pr.clearcoat, 0, 1);
+      spec_strength = clampf(pr.specular * pr.specular_weight, 0, 1);
+      break;
+    case TAG_LAMBERT:
+      transmission = 0; ior = 1; thin = false; spec_color = mk(0, 0, 0); metallic = 0; clearcoat = 0; spec_strength = 0;
+      break;
+    case TAG_METAL:
+      transmission = 0; ior = 1; thin = false; spec_color = base_color; metallic = 1; clearcoat = 0; spec_strength = 1;
+      break;
+    default:
+      transmission = 1; ior = clampf(pr.ior, 1, 3); thin = pr.thin_walled != 0.0f; spec_color = mk(1, 1, 1);
+      metallic = 0; clearcoat = 0; spec_strength = 1;
+      break;
+  }
+  float eta_i, eta_t, next_ior;
+  if (thin) { eta_i = 1.0f; eta_t = 1.0f; next_ior = med.ior; }
+  else if (hit.front) { eta_i = med.ior; eta_t = ior; next_ior = ior; }
+  else { eta_i = med.ior; eta_t = 1.0f; next_ior = 1.0f; }
+  const float cosv = fabsf(dot(normal, wo));
+  float fresnel;
+  if (transmission > 1.0e-4f) fresnel = fresnel_dielectric(cosv, eta_i, eta_t);
+  else {
+    float f0 = clampf(f0_from_ior(ior), 0.0f, 1.0f);
+    fresnel = clampf(f0 + (1.0f - f0) * powi5(1.0f - clampf(cosv, 0.0f, 1.0f)), 0.0f, 1.0f);
+  }
+  float reflect_weight;
+  if (transmission > 1.0e-4f) reflect_weight = clampf(fresnel, 0.0f, 1.0f);
+  else {
+    float base_reflect = metallic > 0.5f ? 1.0f : clampf(spec_strength, 0.0f, 1.0f);
+    reflect_weight = clampf(base_reflect * fresnel + metallic * 0.8f + clearcoat * 0.2f, 0.0f, 1.0f);
+  }
+  const float local_weight = clampf((1.0f - transmission) * (1.0f - reflect_weight), 0.0f, 1.0f);
+  float3 color = mk(0, 0, 0);
+  if (local_weight != 0.0f)
+    color = mul(shade_color(S, mat, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist), local_weight);
+  color = add(color, mat_emission(mat));
+
+  if (reflect_weight > 1.0e-4f) {
+    float3 tint = lerp_spec(spec_color, base_color, clampf(metallic, 0.0f, 1.0f));
+    float3 w = mul(cmul(thr, tint), reflect_weight);
+    if (transmission > 1.0e-4f && rough > 0.06f) {
+      int n = rough < 0.12f ? 1 : (rough < 0.28f ? 2 : 3);
+      push_lobes(st, hp, normal, normalize(reflect(dir, normal)), rough, n, w, med, depth, tp.eps);
+    } else {
+      float3 rd = normalize(reflect(dir, normal));
+      st.push(offset_origin(hp, normal, rd, tp.eps * (1.0f + rough * 6.0f)), rd, w, med, depth + 1);
+    }
+  }
+  if (transmission > 1.0e-4f) {
+    Medium next = med;
+    if (!thin) {  // transition_medium (renderer.rs:5153-5175)
+      if (!hit.front) next = Medium{1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
+      else if (pr.has_medium != 0.0f)
+        next = Medium{clampf(pr.medium_ior, 1, 3), pr.medium_absorption[0], pr.medium_absorption[1],
+                      pr.medium_absorption[2], fmaxf(pr.medium_density, 0.0f)};
+      else next = Medium{clampf(pr.ior, 1, 3), 1.0f, 1.0f, 1.0f, 0.0f};
+    }
+    next.ior = clampf(next_ior, 1.0f, 3.0f);
+    float3 tt = mk(1.0f - (1.0f - clampf(base_color.x, 0, 1)) * transmission,
+                   1.0f - (1.0f - clampf(base_color.y, 0, 1)) * transmission,
+                   1.0f - (1.0f - clampf(base_color.z, 0, 1)) * transmission);
+    float3 w = mul(cmul(thr, tt), transmission * (1.0f - fresnel));
+    float3 rdir;
+    if (refract(dir, normal, eta_i / eta_t, rdir)) {
+      if (rough > 0.06f) {
+        int n = rough < 0.1f ? 2 : (rough < 0.24f ? 3 : 4);
+        push_lobes(st, hp, normal, normalize(rdir), rough, n, w, next, depth, tp.eps);
+      } else {
+        st.push(offset_origin(hp, normal, rdir, tp.eps * (8.0f + rough * 12.0f)), normalize(rdir), w, next,
+                depth + 1);
+      }
+    }
+  }
+  return cmul(color, thr);
+}
+
+// trace_ray_recursive (ray.rs:3-38) with an explicit stack.  Returns the radiance.
+template <int kRayStack>
+__device__ __noinline__ float3 trace_whitted(const DevScene& S, const TraceParams& tp, float3 o, float3 d, int* overflow) {
+  RayStack<kRayStack> st;
+  st.sp = 0;
+  st.overflow = 0;
+  st.push(o, d, mk(1, 1, 1), Medium{1.0f, 1.0f, 1.0f, 1.0f, 0.0f}, 0);
+  float3 acc = mk(0, 0, 0);
+  while (st.sp > 0) {
+    const RayEntry r = st.e[--st.sp];
+    if (r.depth >= tp.max_depth) continue;
+    const float3 ro = mk(r.ox, r.oy, r.oz), rd = mk(r.dx, r.dy, r.dz);
+    const float3 thr = mk(r.tr, r.tg, r.tb);
+    const Medium med{r.m_ior, r.m_ax, r.m_ay, r.m_az, r.m_density};
+    const float min_t = r.depth == 0 ? 0.0f : tp.eps * 2.0f;
+    Hit hit;
+    if (!raymarch(S, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
+      acc = add(acc, cmul(cmul(environment(S, rd), transmittance(med, tp.max_dist)), thr));
+      continue;
+    }
+    const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
+    // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449)
+    float3 sn = walk_normal(S, start, hit.pos, 1.0e-4f);
+    sn = normalize(hit.front ? sn : mul(sn, -1.0f));
+    const float3 att = cmul(thr, transmittance(med, hit.t));
+    Mat a, b;
+    float t;
+    if (split_material(S, start, hit.pos, a, b, t)) {
+      float tt = clampf(t, 0.0f, 1.0f);
+      if (1.0f - tt != 0.0f) acc = add(acc, hit_material(S, tp, hit, rd, sn, med, r.depth, a, mul(att, 1.0f - tt), st));
+      if (tt != 0.0f) acc = add(acc, hit_material(S, tp, hit, rd, sn, med, r.depth, b, mul(att, tt), st));
+    } else {
+      walk_material(S, start, hit.pos, a);
+      acc = add(acc, hit_material(S, tp, hit, rd, sn, med, r.depth, a, att, st));
+    }
+  }
+  if (st.overflow) *overflow = 1;
+  return acc;
+}
+
+// trace_ray_debug_aov (ray.rs:224-285)
+__device__ __noinline__ float3 trace_aov(const DevScene& S, const TraceParams& tp, float3 o, float3 d, int aov) {
+  Hit hit;
+  if (!raymarch(S, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit)) return mk(0, 0, 0);
+  const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
+  Mat mat;
+  walk_material(S, start, hit.pos, mat);
+  float3 normal = hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f));
+  int model = dom_model(mat);
+  float ior = (model == TAG_DIELECTRIC || model == TAG_STANDARD) ? clampf(dom_params(mat).ior, 1, 3) : 1.0f;
+  switch (aov) {
+    case FT_AOV_DEPTH: { float v = clampf(1.0f - hit.t / tp.max_dist, 0, 1); return mk(v, v, v); }
+    case FT_AOV_NORMAL: return mk(normal.x * 0.5f + 0.5f, normal.y * 0.5f + 0.5f, normal.z * 0.5f + 0.5f);
+    case FT_AOV_MATERIAL_ID: {
+      unsigned x = hit.mat * 0x9E3779B9u + 0x7F4A7C15u;
+      x ^= x >> 16;
+      x *= 0x85EBCA6Bu;
+      x ^= x >> 13;
+      return mk((float)(x & 0xFF) / 255.0f, (float)((x >> 8) & 0xFF) / 255.0f, (float)((x >> 16) & 0xFF) / 255.0f);
+    }
+    case FT_AOV_IOR: { float v = clampf((ior - 1.0f) / 2.0f, 0, 1); return mk(v, v, v); }
+    case FT_AOV_TRANSMISSION: { float v = (model == TAG_DIELECTRIC || model == TAG_STANDARD) ? 1.0f : 0.0f; return mk(v, v, v); }
+    case FT_AOV_FRESNEL: {
+      float3 wo = normalize(mul(d, -1.0f));
+      float f = fresnel_dielectric(fabsf(dot(normal, wo)), 1.0f, ior);
+      return mk(f, f, f);
+    }
+    default: { float v = clampf(hit.t / tp.max_dist, 0, 1); return mk(v, v, v); }
+  }
+}
+
+// spectrum_to_rgb8 / spectrum_to_rgb8_reinhard (renderer.rs:5221-5239)
+__device__ __forceinline__ unsigned char to_u8(float v) {
+  float mapped = fminf(powf(fmaxf(v, 0.0f), 1.0f / 2.2f), 1.0f);
+  float s = mapped * 255.0f;
+  if (!(s > 0.0f)) return 0;
+  if (s >= 255.0f) return 255;
+  return (unsigned char)s;
+}
+__device__ __forceinline__ unsigned char to_u8_reinhard(float v) {
+  float x = fmaxf(v, 0.0f);
+  return to_u8(x / (1.0f + x));
+}
+
+}  // namespace ftd

@@ -1,0 +1,217 @@
+"""Host-side mirror of the reference render API over libft_hip.so.
+
+Reference entry points (crates/forgedthoughts/src/renderer.rs):
+  render_depth_png_with_accel            1139-1156
+  render_preview_progressive_with_accel  1238-1277
+  render_ray_progressive_with_accel      1201-1236
+  render_ray_png_with_accel              1179-1199
+Same names, argument meaning and error behaviour (RenderError kinds); the accelerator
+selected by `AccelMode` runs on the GPU.  There is no CPU fallback: if the HIP library
+or a GPU is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+
+from . import abi
+from .forge.evaluator import EvalState
+from .options import AccelMode, RaySettings, RenderOptions
+from .scene import CompiledScene, RenderError, compile_scene
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libft_hip.so")
+_lib = None
+
+ERROR_KINDS = {1: "MissingSceneRoot", 2: "UnsupportedObjectType", 3: "ExpectedObject", 4: "Image",
+               5: "InvalidArgument", 6: "GpuUnavailable", 7: "CompileFailed", 8: "OutOfMemory",
+               9: "LaunchFailed", 10: "CallbackAbort"}
+
+EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_destroy", "ft_render_depth",
+           "ft_render_whitted", "ft_render_depth_device", "ft_render_whitted_device", "ft_scene_synchronize",
+           "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting"]
+
+
+class BackendUnavailable(RuntimeError):
+    pass
+
+
+def library():
+    """Loads libft_hip.so (built in-tree by forgedthoughts_amd/build.py)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BackendUnavailable(f"{LIB_PATH} is not built; run forgedthoughts_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        vp, u32 = C.c_void_p, C.c_uint32
+        L.ft_last_error.restype = C.c_char_p
+        L.ft_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.ft_scene_create.argtypes = [vp, u32, C.POINTER(vp)]
+        L.ft_scene_destroy.argtypes = [vp]
+        L.ft_render_depth.argtypes = [vp, vp, u32, vp, vp]
+        L.ft_render_whitted.argtypes = [vp, vp, vp, vp, vp]
+        L.ft_render_depth_device.argtypes = [vp, vp, u32, vp, vp]
+        L.ft_render_whitted_device.argtypes = [vp, vp, vp, vp, vp]
+        L.ft_scene_synchronize.argtypes = [vp]
+        L.ft_scene_stream.argtypes = [vp, C.POINTER(vp)]
+        L.ft_scene_stats.argtypes = [vp, vp]
+        L.ft_abi_struct_size.argtypes = [C.c_char_p]
+        L.ft_scene_set_flop_counting.argtypes = [vp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = library().ft_last_error().decode()
+        kind = ERROR_KINDS.get(rc, "Unknown")
+        if rc == 6:
+            raise BackendUnavailable(msg)
+        raise RenderError(kind, msg)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    _check(library().ft_device_count(C.byref(n)))
+    return n.value
+
+
+class GpuScene:
+    """One scene uploaded to the GPU for one accel mode (Accelerator::from_scene)."""
+
+    def __init__(self, scene: CompiledScene, accel: AccelMode = AccelMode.Bvh):
+        self.scene = scene
+        self.accel = AccelMode(accel)
+        self._h = C.c_void_p()
+        ir = scene.ir()
+        _check(library().ft_scene_create(C.addressof(ir), int(self.accel), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            library().ft_scene_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def render_depth(self, options: RenderOptions, aa_samples: int = 1) -> Tuple[np.ndarray, np.ndarray]:
+        o = options.to_c()
+        rgb = np.zeros((o.height, o.width, 3), np.uint8)
+        t = np.zeros((o.height, o.width), np.float32)
+        _check(library().ft_render_depth(self._h, C.addressof(o), int(aa_samples), rgb.ctypes.data, t.ctypes.data))
+        return rgb, t
+
+    def render_whitted(self, options: RenderOptions, settings: RaySettings) -> Tuple[np.ndarray, np.ndarray]:
+        o = options.to_c()
+        s = settings.to_c()
+        rgb = np.zeros((o.height, o.width, 3), np.uint8)
+        hdr = np.zeros((o.height, o.width, 3), np.float32)
+        _check(library().ft_render_whitted(self._h, C.addressof(o), C.addressof(s), rgb.ctypes.data, hdr.ctypes.data))
+        return rgb, hdr
+
+    # device-resident variants (outputs already in HBM; used for throughput measurement)
+    def render_depth_device(self, options: RenderOptions, aa_samples: int, rgb_ptr: int, t_ptr: int = 0):
+        o = options.to_c()
+        _check(library().ft_render_depth_device(self._h, C.addressof(o), int(aa_samples), rgb_ptr, t_ptr or None))
+
+    def render_whitted_device(self, options: RenderOptions, settings: RaySettings, rgb_ptr: int, hdr_ptr: int = 0):
+        o = options.to_c()
+        s = settings.to_c()
+        _check(library().ft_render_whitted_device(self._h, C.addressof(o), C.addressof(s), rgb_ptr, hdr_ptr or None))
+
+    def synchronize(self):
+        _check(library().ft_scene_synchronize(self._h))
+
+    def stream(self) -> int:
+        p = C.c_void_p()
+        _check(library().ft_scene_stream(self._h, C.byref(p)))
+        return p.value or 0
+
+    def set_flop_counting(self, enable: bool):
+        _check(library().ft_scene_set_flop_counting(self._h, 1 if enable else 0))
+
+    def stats(self) -> abi.RenderStatsC:
+        st = abi.RenderStatsC()
+        _check(library().ft_scene_stats(self._h, C.addressof(st)))
+        return st
+
+
+# ---------------------------------------------------------------- reference API
+def _tiles(width: int, height: int, tile: int):
+    tile = max(tile, 1)
+    for ty in range(0, height, tile):
+        for tx in range(0, width, tile):
+            yield tx, ty, min(tile, width - tx), min(tile, height - ty)
+
+
+def render_preview_progressive_with_accel(state: EvalState, options: RenderOptions, accel_mode: AccelMode,
+                                          tile_size: int, aa_samples: int,
+                                          on_tile: Optional[Callable[[dict, np.ndarray], None]] = None) -> np.ndarray:
+    """Depth preview of the whole frame; `on_tile(progress, image)` is called on the caller's
+    thread once per tile of `tile_size` in reference order (renderer.rs:3751-3834).  An
+    exception raised by the callback aborts the render, as an Err does in the reference."""
+    scene = compile_scene(state)
+    with GpuScene(scene, accel_mode) as gs:
+        rgb, _ = gs.render_depth(options, max(int(aa_samples), 1))
+    _progress(rgb, options, tile_size, on_tile)
+    return rgb
+
+
+def render_ray_progressive_with_accel(state: EvalState, options: RenderOptions, accel_mode: AccelMode,
+                                      settings: RaySettings,
+                                      on_tile: Optional[Callable[[dict, np.ndarray], None]] = None) -> np.ndarray:
+    """Whitted trace (renderer.rs:1201-1236, 3635-3724)."""
+    scene = compile_scene(state)
+    with GpuScene(scene, accel_mode) as gs:
+        rgb, _ = gs.render_whitted(options, settings)
+    _progress(rgb, options, settings.tile_size, on_tile)
+    return rgb
+
+
+def _progress(rgb, options, tile_size, on_tile):
+    if on_tile is None:
+        return
+    tiles = list(_tiles(options.width, options.height, tile_size))
+    img = np.zeros_like(rgb)
+    for k, (tx, ty, tw, th) in enumerate(tiles):
+        img[ty:ty + th, tx:tx + tw] = rgb[ty:ty + th, tx:tx + tw]
+        on_tile({"tiles_done": k + 1, "tiles_total": len(tiles)}, img.copy())
+
+
+def render_depth_png_with_accel(state: EvalState, output_path: str, options: RenderOptions,
+                                accel_mode: AccelMode) -> None:
+    """render_depth_png_with_accel (renderer.rs:1139-1156): one tile, 1 spp, saved as PNG."""
+    rgb = render_preview_progressive_with_accel(state, options, accel_mode, max(options.width, options.height), 1)
+    save_png(rgb, output_path)
+
+
+def render_ray_png_with_accel(state: EvalState, output_path: str, options: RenderOptions,
+                              accel_mode: AccelMode, max_depth: int) -> None:
+    settings = RaySettings(max_depth=max(max_depth, 1), tile_size=max(options.width, options.height), aa_samples=1)
+    save_png(render_ray_progressive_with_accel(state, options, accel_mode, settings), output_path)
+
+
+def save_png(rgb: np.ndarray, path: str) -> None:
+    try:
+        from PIL import Image
+    except ImportError as exc:  # pragma: no cover
+        raise RenderError("Image", f"failed to write png: {exc}")
+    try:
+        Image.fromarray(rgb, "RGB").save(path)
+    except Exception as exc:
+        raise RenderError("Image", f"failed to write png: {exc}")

@@ -354,8 +354,17 @@ typedef struct ft_render_stats {
   uint64_t kernel_launches;
   float last_kernel_ms;   /* hipEvent time of the last render's kernels */
   float pad;
+  uint64_t sdf_flops;     /* FLOPs of SDF evaluation counted by the last render (counting on) */
 } ft_render_stats;
 int ft_scene_stats(ft_scene* scene, ft_render_stats* out);
+
+/* Turns FLOP counting on (1) or off (0) for later renders on this scene.  Counting adds
+ * atomics to every SDF evaluation: use it for work accounting, never for timing. */
+int ft_scene_set_flop_counting(ft_scene* scene, int enable);
+
+/* sizeof() of the named ABI struct ("ft_node", "ft_scene_ir", ...) or -1, so bindings
+ * can verify their layout against the library they load. */
+int ft_abi_struct_size(const char* name);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,141 @@
+"""GPU parity: the HIP backend (through the C ABI) against the CPU oracle.
+
+Bars (from the depth/Whitted parity statement of the workload):
+  depth   hit/miss masks identical except <= 0.01% of pixels (grazing rays), |dt| <= 1e-4
+          scene units where both hit, preview bytes within 1 where both agree on the mask;
+  Whitted RGB within 1/255 on >= 99.9% of pixels;
+and, for the reference's own fixtures, the reference test's tolerance against its PNGs
+(max |d| <= 8, mean |d| <= 0.35, render_regression.rs:138-145).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from forgedthoughts_amd import synthetic
+from forgedthoughts_amd.options import AccelMode, RaySettings, RenderOptions
+from forgedthoughts_amd.render import GpuScene
+from oracle import ftref
+
+from helpers import REGRESSION, REGRESSION_CASES, compile_ft, compile_src, diff_metrics, read_png
+
+pytestmark = pytest.mark.gpu
+
+T_TOL = 1.0e-4
+GRAZING_FRAC = 1.0e-4
+
+
+def depth_parity(scene, accel, opts, aa=1, label=""):
+    print(f"[depth parity] {label} accel={int(accel)} {opts.width}x{opts.height} aa={aa}", flush=True)
+    rgb_c, t_c = ftref.render_depth(scene, accel, opts, aa)
+    with GpuScene(scene, accel) as gs:
+        rgb_g, t_g = gs.render_depth(opts, aa)
+    hit_c, hit_g = t_c >= 0, t_g >= 0
+    mism = int((hit_c != hit_g).sum())
+    n = hit_c.size
+    both = hit_c & hit_g
+    dt = np.abs(t_c[both] - t_g[both])
+    assert mism <= max(1, int(GRAZING_FRAC * n)), f"{label}: mask mismatch {mism}/{n}"
+    if both.any():
+        assert float(dt.max()) <= T_TOL, f"{label}: max |dt| {dt.max()}"
+    same = (hit_c == hit_g)
+    db = np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2)
+    assert int(db[same].max()) <= 1, f"{label}: byte diff {db[same].max()}"
+    return rgb_g, t_g, (mism, float(dt.max()) if both.any() else 0.0)
+
+
+def whitted_parity(scene, accel, opts, settings, label=""):
+    print(f"[whitted parity] {label} accel={int(accel)} {opts.width}x{opts.height} spp={settings.aa_samples}",
+          flush=True)
+    rgb_c, hdr_c = ftref.render_whitted(scene, accel, opts, settings)
+    with GpuScene(scene, accel) as gs:
+        rgb_g, hdr_g = gs.render_whitted(opts, settings)
+    d = np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2)
+    frac = float((d <= 1).mean())
+    assert frac >= 0.999, f"{label}: only {frac:.5f} of pixels within 1/255 (max {d.max()})"
+    return rgb_g, frac
+
+
+@pytest.mark.parametrize("case", REGRESSION_CASES, ids=[c[0] for c in REGRESSION_CASES])
+@pytest.mark.parametrize("accel", [AccelMode.Bvh, AccelMode.Naive])
+def test_regression_fixture_parity(case, accel):
+    name, kind, opts, settings = case
+    scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))
+    expected = read_png(os.path.join(REGRESSION, name + ".png"))
+    if kind == "preview":
+        rgb, _, _ = depth_parity(scene, accel, opts, 1, name)
+    else:
+        rgb, _ = whitted_parity(scene, accel, opts, settings, name)
+    mx, mean = diff_metrics(rgb, expected)
+    assert mx <= 8 and mean <= 0.35, f"{name}: vs reference PNG max {mx} mean {mean:.4f}"
+
+
+def test_depth_512_full_size():
+    w = synthetic.depth_512()
+    scene = compile_src(w.source)
+    depth_parity(scene, w.accel, w.options, 1, w.name)
+
+
+def test_depth_aa_and_bricks_modes():
+    w = synthetic.depth_512()
+    scene = compile_src(w.source)
+    opts = RenderOptions(**{**w.options.__dict__, "width": 160, "height": 120})
+    for accel in (AccelMode.Naive, AccelMode.Bvh, AccelMode.Bricks):
+        depth_parity(scene, accel, opts, 4, f"aa4-{accel.name}")
+
+
+def test_whitted_1080p_crop():
+    w = synthetic.whitted_1080p(width=192, height=108, spp=4)
+    scene = compile_src(w.source)
+    whitted_parity(scene, w.accel, w.options, w.settings, "whitted_1080p@192x108")
+
+
+def test_whitted_debug_aovs():
+    name, _, opts, settings = REGRESSION_CASES[2]
+    scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))
+    for aov in ("depth", "normal", "material_id", "ior", "transmission", "fresnel", "hit_t"):
+        s = RaySettings(settings.max_depth, settings.tile_size, 1, aov)
+        whitted_parity(scene, AccelMode.Bvh, opts, s, f"aov-{aov}")
+
+
+GLASS = """
+let camera = Camera { origin: vec3(0.0, 1.2, 5.0), target: vec3(0.0, 0.3, 0.0), fov_y: 35.0 };
+let key = PointLight { position: vec3(3.0, 4.0, 4.0), color: #ffffff, intensity: 90.0 };
+let env = EnvLight { color: #c0d0ff, intensity: 0.3 };
+environment Sky { fn color(dir) { return mix(vec3(0.9, 0.9, 0.95), vec3(0.3, 0.5, 0.9), clamp(dir.y, 0.0, 1.0)); } };
+let floor_mat = Lambert { color: #d8dde6, pattern: Checker3d { scale: 2.0 } };
+let glass = Dielectric { ior: 1.5, roughness: 0.15 };
+let gold = Metal { color: #d4af37, roughness: 0.2 };
+let red = Standard { color: #c03030, roughness: 0.4, clearcoat: 0.5, emission_color: #201010, emission_strength: 0.5 };
+var floor = Box { size: vec3(10.0, 0.5, 10.0), material: floor_mat };
+floor.pos.y = -1.0;
+var a = Sphere { radius: 0.7, material: glass };
+a.pos.x = -0.9;
+var b = Box { size: vec3(0.9, 0.9, 0.9), round: 0.1, material: gold };
+b.pos.x = 0.6;
+b.rot.y = 30.0;
+var c = Torus { major_radius: 0.5, minor_radius: 0.15, material: red };
+c.pos.z = 1.0;
+c.pos.y = -0.5;
+let scene = floor + a.union_round(b, 0.25) + c;
+"""
+
+
+def test_whitted_materials_mix():
+    scene = compile_src(GLASS)
+    opts = RenderOptions(96, 72, 256, 30.0, 5.0e-4, 0.7, 6.0, 35.0)
+    for accel in (AccelMode.Bvh, AccelMode.Naive):
+        whitted_parity(scene, accel, opts, RaySettings(5, 64, 2), f"materials-{accel.name}")
+
+
+def test_error_paths():
+    from forgedthoughts_amd.scene import RenderError
+    scene = compile_src("let scene = Sphere { radius: 1.0 };")
+    with GpuScene(scene) as gs:
+        with pytest.raises(RenderError):
+            gs.render_depth(RenderOptions(width=0, height=4))
+    bad = compile_src("let scene = Sphere { radius: 1.0 };")
+    bad.nodes["kind"][0] = 99
+    bad._ir = None
+    with pytest.raises(RenderError):
+        GpuScene(bad)

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -73,7 +74,7 @@ __device__ __forceinline__ bool tile_pixel(int width, int height, int& x, int& y
   return x < width && y < height;
 }
 
-__global__ __launch_bounds__(256) void k_depth(DevScene S, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_depth(int slot, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
                                                float* __restrict__ t_out) {
   int x, y;
   if (!tile_pixel(cam.width, cam.height, x, y)) return;
@@ -83,7 +84,7 @@ __global__ __launch_bounds__(256) void k_depth(DevScene S, CamArgs cam, TracePar
     float3 o, d;
     primary_ray(cam, x, y, s, o, d);
     Hit hit;
-    bool h = raymarch(S, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit);
+    bool h = raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit);
     sum = sum + (h ? depth_preview_value(hit.t, tp.max_dist) : 0.0f);
     if (s == 0) first_t = h ? hit.t : -1.0f;
   }
@@ -96,14 +97,14 @@ __global__ __launch_bounds__(256) void k_depth(DevScene S, CamArgs cam, TracePar
 }
 
 template <int kRayStack>
-__global__ __launch_bounds__(256) void k_whitted(DevScene S, CamArgs cam, TraceParams tp, int aov,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_whitted(int slot, CamArgs cam, TraceParams tp, int aov,
                                                  float* __restrict__ samples, int* __restrict__ overflow) {
   int x, y;
   if (!tile_pixel(cam.width, cam.height, x, y)) return;
   const int s = blockIdx.y;
   float3 o, d;
   primary_ray(cam, x, y, s, o, d);
-  float3 c = aov >= 0 ? trace_aov(S, tp, o, d, aov) : trace_whitted<kRayStack>(S, tp, o, d, overflow);
+  float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow);
   const size_t idx = ((size_t)s * cam.height + y) * cam.width + x;
   samples[idx * 3] = c.x;
   samples[idx * 3 + 1] = c.y;
@@ -162,6 +163,7 @@ int ensure(DevBuf& b, size_t bytes) {
 struct ft_scene {
   fth::HostScene host;
   DevScene dev;
+  int slot = -1;
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -172,6 +174,28 @@ struct ft_scene {
 };
 
 namespace {
+
+std::mutex g_slot_mu;
+bool g_slot_used[kMaxScenes] = {};
+
+int acquire_slot() {
+  std::lock_guard<std::mutex> lk(g_slot_mu);
+  for (int k = 0; k < kMaxScenes; ++k)
+    if (!g_slot_used[k]) { g_slot_used[k] = true; return k; }
+  return -1;
+}
+void release_slot(int k) {
+  if (k < 0) return;
+  std::lock_guard<std::mutex> lk(g_slot_mu);
+  g_slot_used[k] = false;
+}
+
+// Publishes the scene descriptor to its constant-memory slot (call with no render in flight).
+int publish(ft_scene* s) {
+  FT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_scenes), &s->dev, sizeof(DevScene), (size_t)s->slot * sizeof(DevScene),
+                           hipMemcpyHostToDevice));
+  return FT_OK;
+}
 
 int check_options(const ft_render_options* o) {
   if (!o) return fail(FT_ERR_INVALID_ARGUMENT, "render options are null");
@@ -208,6 +232,9 @@ TraceParams make_tp(const ft_render_options& o, uint32_t max_depth) {
   return tp;
 }
 
+// dynamic LDS of a 256-thread block: three value-stack planes
+size_t stack_lds(const ft_scene* s) { return (size_t)3 * s->dev.stack_depth * 256 * sizeof(float); }
+
 dim3 tile_grid(int width, int height) {
   int tiles = ((width + 7) / 8) * ((height + 7) / 8);
   return dim3((tiles + 3) / 4, 1, 1);
@@ -233,7 +260,7 @@ int depth_impl(ft_scene* s, const ft_render_options* o, uint32_t aa, unsigned ch
   TraceParams tp = make_tp(*o, 1);
   FT_HIP(hipSetDevice(s->device));
   FT_HIP(hipEventRecord(s->ev0, s->stream));
-  hipLaunchKernelGGL(k_depth, tile_grid(cam.width, cam.height), dim3(64, 4), 0, s->stream, s->dev, cam, tp, rgb_dev,
+  hipLaunchKernelGGL(k_depth, tile_grid(cam.width, cam.height), dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rgb_dev,
                      t_dev);
   FT_HIP(hipGetLastError());
   FT_HIP(hipEventRecord(s->ev1, s->stream));
@@ -268,11 +295,11 @@ int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings*
   float* samples = (float*)s->samples.ptr;
   int* ovf = (int*)s->overflow.ptr;
   if (need <= 16)
-    hipLaunchKernelGGL(k_whitted<16>, grid, dim3(64, 4), 0, s->stream, s->dev, cam, tp, rs->debug_aov, samples, ovf);
+    hipLaunchKernelGGL(k_whitted<16>, grid, dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rs->debug_aov, samples, ovf);
   else if (need <= 48)
-    hipLaunchKernelGGL(k_whitted<48>, grid, dim3(64, 4), 0, s->stream, s->dev, cam, tp, rs->debug_aov, samples, ovf);
+    hipLaunchKernelGGL(k_whitted<48>, grid, dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rs->debug_aov, samples, ovf);
   else if (need <= 128)
-    hipLaunchKernelGGL(k_whitted<128>, grid, dim3(64, 4), 0, s->stream, s->dev, cam, tp, rs->debug_aov, samples, ovf);
+    hipLaunchKernelGGL(k_whitted<128>, grid, dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rs->debug_aov, samples, ovf);
   else
     return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with this scene's materials");
   FT_HIP(hipGetLastError());
@@ -373,7 +400,18 @@ int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) 
   d.accel = h.accel;
   d.n_leaves = (int)h.leaves.size();
   d.n_bvh = (int)h.bvh.size();
+  d.stack_depth = std::max(1, h.max_stack);
   for (int k = 0; k < 3; ++k) { d.bmin[k] = h.bmin[k]; d.bmax[k] = h.bmax[k]; d.env_rad[k] = h.env_rad[k]; }
+  s->slot = acquire_slot();
+  if (s->slot < 0) {
+    cleanup();
+    return fail(FT_ERR_OUT_OF_MEMORY, "too many live scenes (limit " + std::to_string(kMaxScenes) + ")");
+  }
+  if ((rc = publish(s))) {
+    std::string e = g_err;
+    cleanup();
+    return fail(rc, e);
+  }
   *out = s;
   return FT_OK;
 }
@@ -388,6 +426,7 @@ int ft_scene_destroy(ft_scene* s) {
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
+  release_slot(s->slot);
   delete s;
   return FT_OK;
 }
@@ -451,6 +490,7 @@ int ft_scene_stats(ft_scene* s, ft_render_stats* out) {
 
 int ft_scene_set_flop_counting(ft_scene* s, int enable) {
   if (!s) return fail(FT_ERR_INVALID_ARGUMENT, "scene is null");
+  FT_HIP(hipStreamSynchronize(s->stream));
   if (enable) {
     int rc = ensure(s->flops, sizeof(unsigned long long));
     if (rc) return rc;
@@ -458,7 +498,7 @@ int ft_scene_set_flop_counting(ft_scene* s, int enable) {
   } else {
     s->dev.flops = nullptr;
   }
-  return FT_OK;
+  return publish(s);
 }
 
 // sizeof() of every ABI struct, so bindings can verify their layout.

@@ -37,11 +37,47 @@ struct DevScene {
   const unsigned* __restrict__ lbprog;   // lower-bound program (post-order)
   int n_nodes, root, n_prog, n_lbprog;
   int n_mats, n_objects, n_lights, env_fn;
-  int accel, n_leaves, n_bvh, pad;
+  int accel, n_leaves, n_bvh, stack_depth;   // stack_depth: LDS value-stack entries per thread
   unsigned long long* flops;             // non-null: count evaluated FLOPs (instrumented runs)
   float bmin[3], bmax[3];                // scene bounds
   float env_rad[3];                      // sum of EnvLight radiance
 };
+
+// Scene descriptors live in constant memory, one slot per scene handle.  Device functions
+// receive the slot index and re-read it with readfirstlane, so every scene field is a
+// scalar (SGPR) load and node programs run with wave-uniform control flow.
+constexpr int kMaxScenes = 64;
+__constant__ DevScene c_scenes[kMaxScenes];
+
+__device__ __forceinline__ const DevScene& scene_of(int slot) {
+  return c_scenes[__builtin_amdgcn_readfirstlane(slot)];
+}
+
+// A wave-uniform pointer re-typed into the constant address space, so loads through it
+// with uniform indices are scalar (s_load) instead of per-lane vector loads.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FT_CONST_AS __attribute__((address_space(4)))
+#else
+#define FT_CONST_AS
+#endif
+template <class T>
+__device__ __forceinline__ const FT_CONST_AS T* uniform_ptr(const T* p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (const FT_CONST_AS T*)(((unsigned long long)hi << 32) | lo);
+}
+// Copies a record out of constant-address-space memory field by field (the implicit copy
+// constructor only binds generic references).
+template <class T>
+__device__ __forceinline__ T load_uniform(const FT_CONST_AS T* p) {
+  T out;
+  const FT_CONST_AS unsigned* src = (const FT_CONST_AS unsigned*)p;
+  unsigned* dst = (unsigned*)&out;
+#pragma unroll
+  for (unsigned k = 0; k < sizeof(T) / 4; ++k) dst[k] = src[k];
+  return out;
+}
 
 // ----------------------------------------------------------------- math
 __device__ __forceinline__ float3 mk(float x, float y, float z) { return make_float3(x, y, z); }
@@ -85,7 +121,8 @@ __device__ __forceinline__ float3 node_local(const DNode& n, float3 p) {
   }
   return q;
 }
-__device__ __forceinline__ float3 object_local(const DevScene& S, unsigned obj, float3 p) {
+__device__ __forceinline__ float3 object_local(int slot, unsigned obj, float3 p) {
+  const DevScene& S = scene_of(slot);
   if (obj >= (unsigned)S.n_objects) return p;
   const float* o = S.objects + 9 * obj;
   float3 q = mk(p.x - o[0], p.y - o[1], p.z - o[2]);
@@ -248,7 +285,8 @@ __device__ inline float fbm_3d(float3 p, int octaves, float scale, float lac) {
   return sum;
 }
 
-__device__ __noinline__ float3 eval_fn(const DevScene& S, int fn, const float3* params) {
+__device__ __noinline__ float3 eval_fn(int slot, int fn, const float3* params) {
+  const DevScene& S = scene_of(slot);
   const ft_function F = S.fns[fn];
   float3 v[kMaxExpr];
   const int count = (int)F.count;
@@ -342,13 +380,14 @@ __device__ __noinline__ float3 eval_fn(const DevScene& S, int fn, const float3* 
 }
 
 // eval_custom_sdf_distance (renderer.rs:6139-6179)
-__device__ inline float custom_distance(const DevScene& S, const DNode& n, float3 p) {
+__device__ inline float custom_distance(int slot, const DNode& n, float3 p) {
+  const DevScene& S = scene_of(slot);
   float3 dp = p;
-  if (n.fn_domain >= 0) dp = eval_fn(S, n.fn_domain, &p);
-  float d = n.fn_distance >= 0 ? eval_fn(S, n.fn_distance, &dp).x : 1.0e6f;
+  if (n.fn_domain >= 0) dp = eval_fn(slot, n.fn_domain, &p);
+  float d = n.fn_distance >= 0 ? eval_fn(slot, n.fn_distance, &dp).x : 1.0e6f;
   if (n.fn_post >= 0) {
     float3 args[2] = {mk(d, d, d), dp};
-    d = eval_fn(S, n.fn_post, args).x;
+    d = eval_fn(slot, n.fn_post, args).x;
   }
   return d;
 }
@@ -398,86 +437,101 @@ struct Info {
 
 __device__ __forceinline__ bool is_prim(int k) { return k <= FT_NODE_EXTRUDE_POLYGON; }
 
+// Value stack of the node program: three LDS planes (distance, lower bound, id of the
+// primitive node that produced the value), one column per thread, so a push/pop is one
+// bank-conflict-free ds access even when lanes evaluate different programs.
+extern __shared__ float ft_lds[];
+
 // Evaluates the program range [pf, pl] at p: sdf_distance_info + sdf_lower_bound of the
-// subtree root (renderer.rs:5413-5955).  Control flow over instructions is wave-uniform.
-__device__ __noinline__ Info eval_range(const DevScene& S, int pf, int pl, float3 p) {
-  float sd[kMaxStack], slb[kMaxStack];
-  unsigned sobj[kMaxStack], smat[kMaxStack];
-  int sleaf[kMaxStack];
+// subtree root (renderer.rs:5413-5955).
+__device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int node, float3 p) {
+  // the bounds and every instruction word are wave-uniform; readfirstlane makes that
+  // visible to the compiler so node fetches are scalar loads
+  const auto nodes = uniform_ptr(S.nodes);
+  const auto prog = uniform_ptr(S.prog);
+  const int pf = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_first);
+  const int pl = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_last);
+  const int nth = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  float* __restrict__ sd = ft_lds + tid;
+  float* __restrict__ slb = ft_lds + S.stack_depth * nth + tid;
+  int* __restrict__ sid = reinterpret_cast<int*>(ft_lds + 2 * S.stack_depth * nth) + tid;
   float3 pst[kMaxPointStack];
   int sp = 0, pp = 0;
   for (int k = pf; k <= pl; ++k) {
-    const unsigned ins = S.prog[k];
-    const DNode& n = S.nodes[ins & ~kBeginFlag];
+    const unsigned ins = prog[k];
+    const int ni = (int)(ins & ~kBeginFlag);
+    const DNode n = load_uniform(nodes + ni);
     if (ins & kBeginFlag) {
       pst[pp++] = p;
       float3 q = node_local(n, p);
-      if (n.kind == FT_NODE_DOMAIN_MODIFIER && n.fn_domain >= 0) q = eval_fn(S, n.fn_domain, &q);
+      if (n.kind == FT_NODE_DOMAIN_MODIFIER && n.fn_domain >= 0) q = eval_fn(slot, n.fn_domain, &q);
       p = q;
       continue;
     }
     const int kind = n.kind;
     if (is_prim(kind)) {
-      float d = prim_distance(n, p);
-      sd[sp] = d; slb[sp] = d; sobj[sp] = n.obj; smat[sp] = n.mat; sleaf[sp] = n.leaf;
+      const float d = prim_distance(n, p);
+      sd[sp * nth] = d;
+      slb[sp * nth] = d;
+      sid[sp * nth] = ni;
       ++sp;
       continue;
     }
     if (kind == FT_NODE_CUSTOM) {
       float3 q = node_local(n, p);
-      float hx = n.p[0], hy = n.p[1], hz = n.p[2];
-      float mn[3] = {-hx, -hy, -hz}, mx[3] = {hx, hy, hz};
-      sd[sp] = custom_distance(S, n, q);
-      slb[sp] = aabb_lb(q, mn, mx);
-      sobj[sp] = n.obj; smat[sp] = n.mat; sleaf[sp] = n.leaf;
+      float mn[3] = {-n.p[0], -n.p[1], -n.p[2]}, mx[3] = {n.p[0], n.p[1], n.p[2]};
+      sd[sp * nth] = custom_distance(slot, n, q);
+      slb[sp * nth] = aabb_lb(q, mn, mx);
+      sid[sp * nth] = ni;
       ++sp;
       continue;
     }
+    const int t = (sp - 1) * nth;
     if (kind == FT_NODE_DOMAIN_MODIFIER || kind == FT_NODE_DISTANCE_POST_MODIFIER || kind == FT_NODE_NOISE) {
       float3 q = p;
       p = pst[--pp];
-      float d = sd[sp - 1];
+      float d = sd[t];
       if (kind == FT_NODE_DISTANCE_POST_MODIFIER && n.fn_post >= 0) {
         float3 args[2] = {mk(d, d, d), q};
-        d = eval_fn(S, n.fn_post, args).x;
+        d = eval_fn(slot, n.fn_post, args).x;
       } else if (kind == FT_NODE_NOISE) {
         d = apply_noise(q, d, (int)n.p[0], n.p[1], n.p[2]);
       }
-      sd[sp - 1] = d;
-      if (n.flags & 2) slb[sp - 1] = aabb_lb(p, n.bmin, n.bmax);
-      if (n.leaf >= 0) sleaf[sp - 1] = n.leaf;
+      sd[t] = d;
+      if (n.flags & 2) slb[t] = aabb_lb(p, n.bmin, n.bmax);
       continue;
     }
     if (kind == FT_NODE_SLICE) {
       float c = comp(p, (int)n.p[0]);
-      sd[sp - 1] = fmaxf(sd[sp - 1], fmaxf(n.p[1] - c, c - n.p[2]));
-      if (n.flags & 2) slb[sp - 1] = aabb_lb(p, n.bmin, n.bmax);
+      sd[t] = fmaxf(sd[t], fmaxf(n.p[1] - c, c - n.p[2]));
+      if (n.flags & 2) slb[t] = aabb_lb(p, n.bmin, n.bmax);
       continue;
     }
     if (kind == FT_NODE_SMOOTH) {
-      sd[sp - 1] -= n.p[0] * 0.1f;
-      slb[sp - 1] -= n.p[0] * 0.1f;
+      sd[t] -= n.p[0] * 0.1f;
+      slb[t] -= n.p[0] * 0.1f;
       continue;
     }
-    // binary
+    // binary: lhs at t - nth, rhs at t
     --sp;
-    const int l = sp - 1, r = sp;
+    const int l = t - nth, r = t;
     const float a = sd[l], b = sd[r];
     if (kind == FT_NODE_UNION) {
       // lower-bound short circuit (renderer.rs:5711-5731) evaluated eagerly
       const float llb = slb[l], rlb = slb[r];
-      bool take_l = llb <= rlb ? (a <= rlb || a <= b) : !(b <= llb) && a <= b;
-      if (!take_l) { sd[l] = b; sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+      const bool take_l = llb <= rlb ? (a <= rlb || a <= b) : (!(b <= llb) && a <= b);
+      if (!take_l) { sd[l] = b; sid[l] = sid[r]; }
       slb[l] = fminf(llb, rlb);
       continue;
     }
     if (kind == FT_NODE_INTERSECT) {
-      if (!(a >= b)) { sd[l] = b; sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+      if (!(a >= b)) { sd[l] = b; sid[l] = sid[r]; }
       slb[l] = fmaxf(slb[l], slb[r]);
       continue;
     }
     if (kind == FT_NODE_SUBTRACT) {
-      if (!(a >= -b)) { sd[l] = -b; sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+      if (!(a >= -b)) { sd[l] = -b; sid[l] = sid[r]; }
       continue;
     }
     const float rr = n.p[0], nn = n.p[1];
@@ -502,49 +556,70 @@ __device__ __noinline__ Info eval_range(const DevScene& S, int pf, int pl, float
       case FT_NODE_GROOVE: d = fmaxf(a, fminf(a + fabsf(rr), fabsf(nn) - fabsf(b))); keep_l = true; break;
       default: d = fminf(a, fmaxf(a - fabsf(rr), fabsf(b) - fabsf(nn))); keep_l = true; break;  // TONGUE
     }
-    if (!keep_l) { sobj[l] = sobj[r]; smat[l] = smat[r]; sleaf[l] = sleaf[r]; }
+    if (!keep_l) sid[l] = sid[r];
     sd[l] = d;
     if (n.flags & 2) slb[l] = aabb_lb(p, n.bmin, n.bmax);
   }
   if (S.flops) atomicAdd(S.flops, (unsigned long long)S.nodes[S.prog[pl] & ~kBeginFlag].cost);
   Info out;
-  out.d = sd[0]; out.lb = slb[0]; out.obj = sobj[0]; out.mat = smat[0]; out.leaf = sleaf[0];
+  out.d = sd[0];
+  out.lb = slb[0];
+  const DNode& w = S.nodes[sid[0]];
+  out.obj = w.obj;
+  out.mat = w.mat;
+  out.leaf = w.leaf;
   return out;
 }
 
-__device__ __forceinline__ Info eval_node(const DevScene& S, int node, float3 p) {
-  const DNode& n = S.nodes[node];
-  return eval_range(S, n.prog_first, n.prog_last, p);
+// Evaluates the subtree of `node` at p for every active lane.  Lanes may ask for different
+// subtrees (BVH leaves, tree walks): the loop serves one distinct subtree per iteration, so
+// each program run is wave-uniform.
+__device__ __noinline__ Info eval_node(int slot, int node, float3 p) {
+  const DevScene& S = scene_of(slot);
+  Info out;
+  bool done = false;
+  while (!done) {
+    const int lead = __builtin_amdgcn_readfirstlane(node);
+    if (node == lead) {
+      out = eval_uniform(S, slot, lead, p);
+      done = true;
+    }
+  }
+  return out;
 }
 
 // sdf_lower_bound of the root (renderer.rs:5413-5543): post-order over the nodes reachable
 // through Union/Intersect/Subtract/Smooth; other nodes are terminals.
-__device__ __noinline__ float lower_bound_root(const DevScene& S, float3 p) {
-  float st[kMaxStack];
+__device__ __noinline__ float lower_bound_root(int slot, float3 p) {
+  const DevScene& S = scene_of(slot);
+  const int nth = blockDim.x * blockDim.y;
+  float* __restrict__ st0 = ft_lds + threadIdx.y * blockDim.x + threadIdx.x;  // value-stack plane 0
+  auto st = [&](int i) -> float& { return st0[i * nth]; };
   int sp = 0;
   for (int k = 0; k < S.n_lbprog; ++k) {
     const DNode& n = S.nodes[S.lbprog[k]];
     switch (n.kind) {
-      case FT_NODE_UNION: --sp; st[sp - 1] = fminf(st[sp - 1], st[sp]); break;
-      case FT_NODE_INTERSECT: --sp; st[sp - 1] = fmaxf(st[sp - 1], st[sp]); break;
+      case FT_NODE_UNION: --sp; st(sp - 1) = fminf(st(sp - 1), st(sp)); break;
+      case FT_NODE_INTERSECT: --sp; st(sp - 1) = fmaxf(st(sp - 1), st(sp)); break;
       case FT_NODE_SUBTRACT: --sp; break;
-      case FT_NODE_SMOOTH: st[sp - 1] -= n.p[0] * 0.1f; break;
+      case FT_NODE_SMOOTH: st(sp - 1) -= n.p[0] * 0.1f; break;
       case FT_NODE_CUSTOM: {
         float3 q = node_local(n, p);
         float mn[3] = {-n.p[0], -n.p[1], -n.p[2]}, mx[3] = {n.p[0], n.p[1], n.p[2]};
-        st[sp++] = aabb_lb(q, mn, mx);
+        st(sp++) = aabb_lb(q, mn, mx);
         break;
       }
       default:
-        st[sp++] = is_prim(n.kind) ? prim_distance(n, p) : aabb_lb(p, n.bmin, n.bmax);
+        st(sp++) = is_prim(n.kind) ? prim_distance(n, p) : aabb_lb(p, n.bmin, n.bmax);
         break;
     }
   }
-  return st[0];
+  return st(0);
 }
 
 // accel_leaf_distance_info (renderer.rs:629-657)
-__device__ __forceinline__ Info leaf_info(const DevScene& S, int li, float3 p) {
+__device__ __forceinline__ Info leaf_info(int slot, int li, float3 p) {
+  const DevScene& S = scene_of(slot);
   const DLeaf& L = S.leaves[li];
   if (L.is_custom) {
     float lb = aabb_lb(p, L.bmin, L.bmax);
@@ -557,13 +632,14 @@ __device__ __forceinline__ Info leaf_info(const DevScene& S, int li, float3 p) {
       return o;
     }
   }
-  Info o = eval_node(S, L.node, p);
+  Info o = eval_node(slot, L.node, p);
   o.leaf = li;
   return o;
 }
 
 // bvh_distance_info (renderer.rs:743-786), iterative near-first
-__device__ __noinline__ Info bvh_distance_info(const DevScene& S, float3 p) {
+__device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
+  const DevScene& S = scene_of(slot);
   int stack[kMaxBvhStack];
   int sp = 0;
   float best = __builtin_huge_valf();
@@ -576,7 +652,7 @@ __device__ __noinline__ Info bvh_distance_info(const DevScene& S, float3 p) {
     if (S.flops) atomicAdd(S.flops, 3ull * kAabbFlops);
     if (aabb_lb(p, nd.bmin, nd.bmax) > best) continue;
     if (nd.left < 0) {
-      Info info = leaf_info(S, -nd.left - 1, p);
+      Info info = leaf_info(slot, -nd.left - 1, p);
       if (info.d < best) best = info.d;
       if (!have || info.d < bi.d) { bi = info; have = true; }
       continue;
@@ -592,12 +668,13 @@ __device__ __noinline__ Info bvh_distance_info(const DevScene& S, float3 p) {
     }
   }
   if (!have) {
-    bi = eval_node(S, S.root, p);
+    bi = eval_node(slot, S.root, p);
   }
   return bi;
 }
 
-__device__ __noinline__ float bvh_lower_bound(const DevScene& S, float3 p) {
+__device__ __noinline__ float bvh_lower_bound(int slot, float3 p) {
+  const DevScene& S = scene_of(slot);
   int stack[kMaxBvhStack];
   int sp = 0;
   float best = __builtin_huge_valf();
@@ -618,13 +695,15 @@ __device__ __noinline__ float bvh_lower_bound(const DevScene& S, float3 p) {
 // Accelerator::distance_info / lower_bound (renderer.rs:609-623, 1024-1129).  Bricks uses
 // the BVH traversal: both return the minimum over accel leaves (the grid's shell search
 // visits a superset of the leaves that can beat the running best).
-__device__ __forceinline__ Info scene_info(const DevScene& S, float3 p) {
-  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_info(S, p);
-  return eval_node(S, S.root, p);
+__device__ __forceinline__ Info scene_info(int slot, float3 p) {
+  const DevScene& S = scene_of(slot);
+  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_info(slot, p);
+  return eval_node(slot, S.root, p);
 }
-__device__ __forceinline__ float scene_lower_bound(const DevScene& S, float3 p) {
-  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_lower_bound(S, p);
-  return lower_bound_root(S, p);
+__device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
+  const DevScene& S = scene_of(slot);
+  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_lower_bound(slot, p);
+  return lower_bound_root(slot, p);
 }
 
 // ---------------------------------------------------------------- march
@@ -658,8 +737,9 @@ __device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, co
 }
 
 // raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215)
-__device__ __noinline__ bool raymarch(const DevScene& S, float3 o, float3 d, int max_steps, float eps,
+__device__ __noinline__ bool raymarch(int slot, float3 o, float3 d, int max_steps, float eps,
                                       float step_scale, float min_t, float max_t, Hit& hit) {
+  const DevScene& S = scene_of(slot);
   float entry, exit;
   if (!ray_aabb(o, d, S.bmin, S.bmax, entry, exit)) return false;
   max_t = fminf(max_t, exit);
@@ -669,21 +749,21 @@ __device__ __noinline__ bool raymarch(const DevScene& S, float3 o, float3 d, int
   const float min_step = fmaxf(eps * 0.5f, 1.0e-5f);
   for (int s = 0; s < max_steps; ++s) {
     if (traveled > max_t) return false;
-    float dist = scene_info(S, add(o, mul(d, traveled))).d;
+    float dist = scene_info(slot, add(o, mul(d, traveled))).d;
     if (fabsf(dist) < eps) {
       float lo = prev, hi = traveled;
       for (int k = 0; k < 10; ++k) {
         float mid = 0.5f * (lo + hi);
-        if (fabsf(scene_info(S, add(o, mul(d, mid))).d) < eps) hi = mid; else lo = mid;
+        if (fabsf(scene_info(slot, add(o, mul(d, mid))).d) < eps) hi = mid; else lo = mid;
       }
       float3 pos = add(o, mul(d, hi));
       float e = fmaxf(eps * 4.0f, 1.0e-5f);
-      float dx = scene_info(S, mk(pos.x + e, pos.y, pos.z)).d - scene_info(S, mk(pos.x - e, pos.y, pos.z)).d;
-      float dy = scene_info(S, mk(pos.x, pos.y + e, pos.z)).d - scene_info(S, mk(pos.x, pos.y - e, pos.z)).d;
-      float dz = scene_info(S, mk(pos.x, pos.y, pos.z + e)).d - scene_info(S, mk(pos.x, pos.y, pos.z - e)).d;
+      float dx = scene_info(slot, mk(pos.x + e, pos.y, pos.z)).d - scene_info(slot, mk(pos.x - e, pos.y, pos.z)).d;
+      float dy = scene_info(slot, mk(pos.x, pos.y + e, pos.z)).d - scene_info(slot, mk(pos.x, pos.y - e, pos.z)).d;
+      float dz = scene_info(slot, mk(pos.x, pos.y, pos.z + e)).d - scene_info(slot, mk(pos.x, pos.y, pos.z - e)).d;
       float3 nrm = mk(dx, dy, dz);
       nrm = len(nrm) > 1.0e-8f ? normalize(nrm) : mk(0.0f, 1.0f, 0.0f);
-      Info fin = scene_info(S, pos);
+      Info fin = scene_info(slot, pos);
       hit.t = hi;
       hit.pos = pos;
       hit.n = nrm;

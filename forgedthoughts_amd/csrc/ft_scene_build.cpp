@@ -346,6 +346,7 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
   };
   emit_lb(S.root);
   if (lmax > ftd::kMaxStack) { err = "scene tree needs a deeper lower-bound stack than supported"; return false; }
+  S.max_stack = std::max(S.max_stack, lmax);
 
   // accel leaves (collect_accel_leaves) and per-node leaf ids
   std::vector<int> leaf_nodes;

@@ -89,7 +89,8 @@ __device__ inline void blend_materials(Mat& left, const Mat& right, float t) {
 }
 
 // resolve_leaf_material, static part (renderer.rs:7164-7536)
-__device__ inline void leaf_material(const DevScene& S, unsigned mat_id, unsigned obj, float3 pos, Mat& m) {
+__device__ inline void leaf_material(int slot, unsigned mat_id, unsigned obj, float3 pos, Mat& m) {
+  const DevScene& S = scene_of(slot);
   unsigned id = mat_id < (unsigned)S.n_mats ? mat_id : 0u;
   m.blend = 0;
   m.a = S.mats[id];
@@ -98,7 +99,7 @@ __device__ inline void leaf_material(const DevScene& S, unsigned mat_id, unsigne
   m.t = 0.0f;
   m.pad = 0;
   if (m.a.pattern == (float)FT_PATTERN_CHECKER3D) {  // resolve_pattern_color (renderer.rs:7855-7873)
-    float3 lp = object_local(S, obj, pos);
+    float3 lp = object_local(slot, obj, pos);
     int sx = (int)floorf(lp.x * m.a.pattern_scale), sy = (int)floorf(lp.y * m.a.pattern_scale),
         sz = (int)floorf(lp.z * m.a.pattern_scale);
     set3(m.a.color, ((sx + sy + sz) & 1) == 0 ? v3(m.a.pattern_a) : v3(m.a.pattern_b));
@@ -136,12 +137,13 @@ __device__ __forceinline__ bool hard_left(int kind, float dl, float dr) {
 __device__ __forceinline__ bool is_hard(int k) { return k == FT_NODE_UNION || k == FT_NODE_INTERSECT || k == FT_NODE_SUBTRACT; }
 
 // estimate_node_normal (renderer.rs:4554-4570)
-__device__ inline float3 node_normal(const DevScene& S, int node, float3 p, float eps) {
+__device__ inline float3 node_normal(int slot, int node, float3 p, float eps) {
+  const DevScene& S = scene_of(slot);
   float e = fmaxf(eps, 1.0e-5f);
-  float d1 = eval_node(S, node, mk(p.x + e, p.y - e, p.z - e)).d;
-  float d2 = eval_node(S, node, mk(p.x - e, p.y - e, p.z + e)).d;
-  float d3 = eval_node(S, node, mk(p.x - e, p.y + e, p.z - e)).d;
-  float d4 = eval_node(S, node, mk(p.x + e, p.y + e, p.z + e)).d;
+  float d1 = eval_node(slot, node, mk(p.x + e, p.y - e, p.z - e)).d;
+  float d2 = eval_node(slot, node, mk(p.x - e, p.y - e, p.z + e)).d;
+  float d3 = eval_node(slot, node, mk(p.x - e, p.y + e, p.z - e)).d;
+  float d4 = eval_node(slot, node, mk(p.x + e, p.y + e, p.z + e)).d;
   float3 n = add(add(add(mk(d1, -d1, -d1), mk(-d2, -d2, d2)), mk(-d3, d3, -d3)), mk(d4, d4, d4));
   if (len(n) > 1.0e-6f) return normalize(n);
   return mk(0.0f, 1.0f, 0.0f);
@@ -150,7 +152,8 @@ __device__ inline float3 node_normal(const DevScene& S, int node, float3 p, floa
 struct Frame { int node, phase; };
 
 // resolve_surface_normal_from_node (renderer.rs:4462-4552), from `start`
-__device__ __noinline__ float3 walk_normal(const DevScene& S, int start, float3 p, float eps) {
+__device__ __noinline__ float3 walk_normal(int slot, int start, float3 p, float eps) {
+  const DevScene& S = scene_of(slot);
   Frame fr[kMaxWalk];
   float3 res[kMaxWalk];
   int fsp = 0, rsp = 0;
@@ -158,11 +161,11 @@ __device__ __noinline__ float3 walk_normal(const DevScene& S, int start, float3 
   while (fsp > 0) {
     Frame f = fr[--fsp];
     const DNode& n = S.nodes[f.node];
-    if (n.kind <= FT_NODE_NOISE) { res[rsp++] = node_normal(S, f.node, p, eps); continue; }
+    if (n.kind <= FT_NODE_NOISE) { res[rsp++] = node_normal(slot, f.node, p, eps); continue; }
     if (n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) { fr[fsp++] = {n.lhs, 0}; continue; }
     if (is_hard(n.kind)) {
       if (f.phase == 1) { res[rsp - 1] = mul(res[rsp - 1], -1.0f); continue; }
-      float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+      float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
       if (hard_left(n.kind, dl, dr)) fr[fsp++] = {n.lhs, 0};
       else {
         if (n.kind == FT_NODE_SUBTRACT) fr[fsp++] = {f.node, 1};
@@ -173,7 +176,7 @@ __device__ __noinline__ float3 walk_normal(const DevScene& S, int start, float3 
     if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
     if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
     float3 right = res[--rsp], left = res[--rsp];
-    float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+    float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
     if (blend_class(n.kind) == 2) right = mul(right, -1.0f);
     float t = blend_t(n, dl, dr);
     float3 nn = add(mul(left, 1.0f - t), mul(right, t));
@@ -183,7 +186,8 @@ __device__ __noinline__ float3 walk_normal(const DevScene& S, int start, float3 
 }
 
 // resolve_material_from_node (renderer.rs:6945-7059), from `start`
-__device__ __noinline__ void walk_material(const DevScene& S, int start, float3 p, Mat& out) {
+__device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& out) {
+  const DevScene& S = scene_of(slot);
   Frame fr[kMaxWalk];
   Mat res[kMaxWalkMats];
   int fsp = 0, rsp = 0;
@@ -191,40 +195,41 @@ __device__ __noinline__ void walk_material(const DevScene& S, int start, float3 
   while (fsp > 0) {
     Frame f = fr[--fsp];
     const DNode& n = S.nodes[f.node];
-    if (n.kind <= FT_NODE_CUSTOM) { leaf_material(S, n.mat, n.obj, p, res[rsp++]); continue; }
+    if (n.kind <= FT_NODE_CUSTOM) { leaf_material(slot, n.mat, n.obj, p, res[rsp++]); continue; }
     if (n.kind <= FT_NODE_NOISE || n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) {
       fr[fsp++] = {n.lhs, 0};
       continue;
     }
     if (is_hard(n.kind)) {
-      float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+      float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
       fr[fsp++] = {hard_left(n.kind, dl, dr) ? n.lhs : n.rhs, 0};
       continue;
     }
     if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
     if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
     --rsp;
-    float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+    float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
     blend_materials(res[rsp - 1], res[rsp], blend_t(n, dl, dr));
   }
   out = res[0];
 }
 
 // resolve_split_material_from_node (renderer.rs:7061-7120)
-__device__ __noinline__ bool split_material(const DevScene& S, int start, float3 p, Mat& a, Mat& b, float& t) {
+__device__ __noinline__ bool split_material(int slot, int start, float3 p, Mat& a, Mat& b, float& t) {
+  const DevScene& S = scene_of(slot);
   int node = start;
   for (;;) {
     const DNode& n = S.nodes[node];
     if (is_hard(n.kind)) {
-      float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
+      float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
       node = hard_left(n.kind, dl, dr) ? n.lhs : n.rhs;
       continue;
     }
     if (n.kind == FT_NODE_SMOOTH || n.kind == FT_NODE_NOISE) { node = n.lhs; continue; }
     if (blend_class(n.kind) == 0) return false;
-    float dl = eval_node(S, n.lhs, p).d, dr = eval_node(S, n.rhs, p).d;
-    walk_material(S, n.lhs, p, a);
-    walk_material(S, n.rhs, p, b);
+    float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+    walk_material(slot, n.lhs, p, a);
+    walk_material(slot, n.rhs, p, b);
     t = blend_t(n, dl, dr);
     return true;
   }
@@ -309,14 +314,15 @@ __device__ __forceinline__ float3 offset_origin(float3 pos, float3 gn, float3 di
 }
 
 // shadow_visibility (renderer.rs:4706-4733)
-__device__ __noinline__ float shadow_visibility(const DevScene& S, float3 o, float3 d, float max_t, float eps) {
+__device__ __noinline__ float shadow_visibility(int slot, float3 o, float3 d, float max_t, float eps) {
+  const DevScene& S = scene_of(slot);
   float t = eps * 2.0f, vis = 1.0f;
   const float lb_gate = fmaxf(eps * 8.0f, 0.02f);
   for (int k = 0; k < 80; ++k) {
     if (t >= max_t) return clampf(vis, 0.0f, 1.0f);
     float3 p = add(o, mul(d, t));
-    float lower = fabsf(scene_lower_bound(S, p));
-    float h = lower > lb_gate ? lower : fabsf(scene_info(S, p).d);
+    float lower = fabsf(scene_lower_bound(slot, p));
+    float h = lower > lb_gate ? lower : fabsf(scene_info(slot, p).d);
     if (h < eps * 4.0f) return 0.0f;
     vis = fminf(vis, clampf(10.0f * h / t, 0.0f, 1.0f));
     t += fmaxf(h, eps * 2.0f);
@@ -364,8 +370,9 @@ __device__ inline void sample_light(const DLight& L, float3 point, unsigned idx,
 }
 
 // shade_color (renderer.rs:4217-4276)
-__device__ __noinline__ float3 shade_color(const DevScene& S, const Mat& mat, float3 hit_pos, float3 hit_n, bool front,
+__device__ __noinline__ float3 shade_color(int slot, const Mat& mat, float3 hit_pos, float3 hit_n, bool front,
                                            float3 nrm, float3 wo, float eps, float max_dist) {
+  const DevScene& S = scene_of(slot);
   if (S.n_lights == 0) return mk(1, 1, 1);
   float3 n = normalize(nrm);
   float3 gn = front ? normalize(hit_n) : normalize(mul(hit_n, -1.0f));
@@ -384,7 +391,7 @@ __device__ __noinline__ float3 shade_color(const DevScene& S, const Mat& mat, fl
       float3 so = offset_origin(hit_pos, gn, wi, eps);
       float max_t = isfinite(dist) ? fmaxf(dist - eps * 8.0f, 0.0f) : max_dist;
       if (max_t <= 0.0f) continue;
-      float sh = shadow_visibility(S, so, wi, max_t, eps);
+      float sh = shadow_visibility(slot, so, wi, max_t, eps);
       if (sh <= 0.0f) continue;
       float3 f = mat_eval(mat, nrm, wi, wo);
       color = add(color, mul(cmul(f, rad), ndotl * sh * inv));
@@ -454,8 +461,9 @@ struct RayStack {
   }
 };
 
-__device__ __forceinline__ float3 environment(const DevScene& S, float3 dir) {
-  if (S.env_fn >= 0) return eval_fn(S, S.env_fn, &dir);
+__device__ __forceinline__ float3 environment(int slot, float3 dir) {
+  const DevScene& S = scene_of(slot);
+  if (S.env_fn >= 0) return eval_fn(slot, S.env_fn, &dir);
   return v3(S.env_rad);
 }
 
@@ -482,9 +490,10 @@ __device__ inline void push_lobes(RayStack<kRayStack>& st, float3 hp, float3 gn,
 // trace_hit_with_material (ray.rs:41-222) for one material of a hit; returns the local
 // term (already weighted by `thr`) and pushes the children.
 template <int kRayStack>
-__device__ __noinline__ float3 hit_material(const DevScene& S, const TraceParams& tp, const Hit& hit, float3 dir,
+__device__ __noinline__ float3 hit_material(int slot, const TraceParams& tp, const Hit& hit, float3 dir,
                                             float3 nrm, const Medium& med, int depth, const Mat& mat, float3 thr,
                                             RayStack<kRayStack>& st) {
+  const DevScene& S = scene_of(slot);
   const float3 hp = hit.pos;
   const float3 normal = hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f));
   const float3 wo = normalize(mul(dir, -1.0f));
@@ -530,7 +539,7 @@ __device__ __noinline__ float3 hit_material(const DevScene& S, const TraceParams
   const float local_weight = clampf((1.0f - transmission) * (1.0f - reflect_weight), 0.0f, 1.0f);
   float3 color = mk(0, 0, 0);
   if (local_weight != 0.0f)
-    color = mul(shade_color(S, mat, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist), local_weight);
+    color = mul(shade_color(slot, mat, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist), local_weight);
   color = add(color, mat_emission(mat));
 
   if (reflect_weight > 1.0e-4f) {
@@ -574,7 +583,8 @@ __device__ __noinline__ float3 hit_material(const DevScene& S, const TraceParams
 
 // trace_ray_recursive (ray.rs:3-38) with an explicit stack.  Returns the radiance.
 template <int kRayStack>
-__device__ __noinline__ float3 trace_whitted(const DevScene& S, const TraceParams& tp, float3 o, float3 d, int* overflow) {
+__device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, float3 o, float3 d, int* overflow) {
+  const DevScene& S = scene_of(slot);
   RayStack<kRayStack> st;
   st.sp = 0;
   st.overflow = 0;
@@ -588,24 +598,24 @@ __device__ __noinline__ float3 trace_whitted(const DevScene& S, const TraceParam
     const Medium med{r.m_ior, r.m_ax, r.m_ay, r.m_az, r.m_density};
     const float min_t = r.depth == 0 ? 0.0f : tp.eps * 2.0f;
     Hit hit;
-    if (!raymarch(S, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
-      acc = add(acc, cmul(cmul(environment(S, rd), transmittance(med, tp.max_dist)), thr));
+    if (!raymarch(slot, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
+      acc = add(acc, cmul(cmul(environment(slot, rd), transmittance(med, tp.max_dist)), thr));
       continue;
     }
     const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
     // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449)
-    float3 sn = walk_normal(S, start, hit.pos, 1.0e-4f);
+    float3 sn = walk_normal(slot, start, hit.pos, 1.0e-4f);
     sn = normalize(hit.front ? sn : mul(sn, -1.0f));
     const float3 att = cmul(thr, transmittance(med, hit.t));
     Mat a, b;
     float t;
-    if (split_material(S, start, hit.pos, a, b, t)) {
+    if (split_material(slot, start, hit.pos, a, b, t)) {
       float tt = clampf(t, 0.0f, 1.0f);
-      if (1.0f - tt != 0.0f) acc = add(acc, hit_material(S, tp, hit, rd, sn, med, r.depth, a, mul(att, 1.0f - tt), st));
-      if (tt != 0.0f) acc = add(acc, hit_material(S, tp, hit, rd, sn, med, r.depth, b, mul(att, tt), st));
+      if (1.0f - tt != 0.0f) acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, a, mul(att, 1.0f - tt), st));
+      if (tt != 0.0f) acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, b, mul(att, tt), st));
     } else {
-      walk_material(S, start, hit.pos, a);
-      acc = add(acc, hit_material(S, tp, hit, rd, sn, med, r.depth, a, att, st));
+      walk_material(slot, start, hit.pos, a);
+      acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, a, att, st));
     }
   }
   if (st.overflow) *overflow = 1;
@@ -613,12 +623,13 @@ __device__ __noinline__ float3 trace_whitted(const DevScene& S, const TraceParam
 }
 
 // trace_ray_debug_aov (ray.rs:224-285)
-__device__ __noinline__ float3 trace_aov(const DevScene& S, const TraceParams& tp, float3 o, float3 d, int aov) {
+__device__ __noinline__ float3 trace_aov(int slot, const TraceParams& tp, float3 o, float3 d, int aov) {
+  const DevScene& S = scene_of(slot);
   Hit hit;
-  if (!raymarch(S, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit)) return mk(0, 0, 0);
+  if (!raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit)) return mk(0, 0, 0);
   const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
   Mat mat;
-  walk_material(S, start, hit.pos, mat);
+  walk_material(slot, start, hit.pos, mat);
   float3 normal = hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f));
   int model = dom_model(mat);
   float ior = (model == TAG_DIELECTRIC || model == TAG_STANDARD) ? clampf(dom_params(mat).ior, 1, 3) : 1.0f;

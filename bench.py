@@ -109,11 +109,12 @@ def main():
     gs.synchronize()
     flops = gs.stats().sdf_flops
     gs.set_flop_counting(False)
+    st = gs.stats()
     avg_kernel_s = kernel_ms / args.steps / 1e3
     achieved = flops / avg_kernel_s / 1e12 if avg_kernel_s > 0 else 0.0
 
     line = {
-        "metric": "Mrays/s (Whitted 1080p)",
+        "metric": "Mrays/s depth+Whitted 1080p",
         "value": round(value, 4),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -127,9 +128,12 @@ def main():
         "data": "synthetic",
         "config": {"workload": w.name, "width": opts.width, "height": opts.height, "spp": spp,
                    "max_depth": w.settings.max_depth if w.settings else None, "accel": w.accel.name.lower(),
-                   "max_steps": opts.max_steps, "epsilon": opts.epsilon, "nodes": int(len(scene.nodes))},
+                   "max_steps": opts.max_steps, "epsilon": opts.epsilon, "nodes": int(len(scene.nodes)),
+                   "specialised": {"scene": bool(st.flags & 1), "functions": bool(st.flags & 2),
+                                   "compile_ms": round(st.compile_ms, 1)}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
                      "kernel_ms": round(kernel_ms / args.steps, 3), "stream_ms": round(stream_ms / args.steps, 3),
                      "sdf_flops_per_launch": int(flops)},
     }

@@ -99,7 +99,11 @@ class RaySettingsC(C.Structure):
 
 class RenderStatsC(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("kernel_launches", C.c_uint64),
-                ("last_kernel_ms", C.c_float), ("pad", C.c_float), ("sdf_flops", C.c_uint64)]
+                ("last_kernel_ms", C.c_float), ("flags", C.c_uint32), ("sdf_flops", C.c_uint64),
+                ("compile_ms", C.c_double)]
+
+
+STATS_EMITTED_SCENE, STATS_EMITTED_FUNCTIONS = 1, 2
 
 
 STRUCT_SIZES = {

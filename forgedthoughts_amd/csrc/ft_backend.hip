@@ -1,23 +1,27 @@
-// ft_backend.hip — gfx950 kernels and the C ABI of libft_hip.so (include/ft_abi.h).
+// ft_backend.hip — the C ABI of libft_hip.so (include/ft_abi.h) and kernel dispatch.
 //
-// Kernels (one ray per lane):
-//   k_depth          depth preview, one thread per pixel looping over the AA grid
-//                    (render_depth_tile renderer.rs:3931-3971);
-//   k_whitted<N>     one thread per (pixel, AA sample): Whitted trace with an N-entry ray
-//                    stack, writes the sample radiance;
-//   k_resolve        per pixel: sums the samples in AA order, scales by 1/aa and tone maps
-//                    (render_ray_tile renderer.rs:3874-3929, renderer.rs:5221-5239).
+// The kernels themselves are in ft_kernels.h.  Each march kernel exists twice:
+//   * precompiled into this library, evaluating the scene through the node-program
+//     interpreter (any scene, no start-up cost);
+//   * runtime-compiled per scene with hiprtc from the source ft_emit.cpp writes, where the
+//     scene distance and the Forge functions are straight-line code with every parameter
+//     a literal — the role Cranelift plays for the reference (jit.rs).
+// ft_scene_create decides which (FT_EMIT: "auto" default, "1" always, "0" never); the
+// specialised code object for a kernel is built on its first launch and cached.
 #include <hip/hip_runtime.h>
 
-#include <cstdio>
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
 #include <vector>
 
+#include "ft_emit.h"
+#include "ft_kernels.h"
+#include "ft_rtc.h"
 #include "ft_scene_build.h"
-#include "ft_shade.h"
 
 using namespace ftd;
 
@@ -37,101 +41,6 @@ int fail(int code, const std::string& msg) {
       return fail(e_ == hipErrorOutOfMemory ? FT_ERR_OUT_OF_MEMORY : FT_ERR_LAUNCH_FAILED,  \
                   std::string(#call) + ": " + hipGetErrorString(e_));                       \
   } while (0)
-
-struct CamArgs {
-  float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
-  int width, height, aa, grid;
-};
-
-__device__ __forceinline__ void primary_ray(const CamArgs& c, int x, int y, int s, float3& o, float3& d) {
-  // pixel_sample_offsets (renderer.rs:4838-4854): sample s sits at grid cell (s % g, s / g)
-  const float g = (float)c.grid;
-  const float sx = ((float)(s % c.grid) + 0.5f) / g, sy = ((float)(s / c.grid) + 0.5f) / g;
-  const float px = (((float)x + sx) / (float)c.width) * 2.0f - 1.0f;
-  const float py = 1.0f - (((float)y + sy) / (float)c.height) * 2.0f;
-  const float ndx = px * c.aspect, ndy = py;
-  float3 dir = add(add(mk(c.fx, c.fy, c.fz), mul(mk(c.rx, c.ry, c.rz), ndx * c.tan_fov)),
-                   mul(mk(c.ux, c.uy, c.uz), ndy * c.tan_fov));
-  d = normalize(normalize(dir));
-  o = mk(c.ox, c.oy, c.oz);
-}
-
-__device__ __forceinline__ float depth_preview_value(float t, float max_dist) {  // renderer.rs:5215-5219
-  float dd = clampf(t / fmaxf(max_dist, 1.0e-6f), 0.0f, 1.0f);
-  float s = 1.0f - dd;
-  return s * s;
-}
-
-// Pixels are visited in 8x8 tiles per wavefront (blockDim 64x4: four tiles per block) so
-// that the 64 lanes of a wave march spatially adjacent rays.
-__device__ __forceinline__ bool tile_pixel(int width, int height, int& x, int& y) {
-  const int lane = threadIdx.x;
-  const int tiles_x = (width + 7) / 8;
-  const int tile = blockIdx.x * blockDim.y + threadIdx.y;
-  const int tx = tile % tiles_x, ty = tile / tiles_x;
-  x = tx * 8 + (lane & 7);
-  y = ty * 8 + (lane >> 3);
-  return x < width && y < height;
-}
-
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_depth(int slot, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
-                                               float* __restrict__ t_out) {
-  int x, y;
-  if (!tile_pixel(cam.width, cam.height, x, y)) return;
-  float sum = 0.0f;
-  float first_t = -1.0f;
-  for (int s = 0; s < cam.aa; ++s) {
-    float3 o, d;
-    primary_ray(cam, x, y, s, o, d);
-    Hit hit;
-    bool h = raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit);
-    sum = sum + (h ? depth_preview_value(hit.t, tp.max_dist) : 0.0f);
-    if (s == 0) first_t = h ? hit.t : -1.0f;
-  }
-  const unsigned char v = to_u8(sum * (1.0f / (float)cam.aa));
-  const size_t pix = (size_t)y * cam.width + x;
-  rgb[pix * 3] = v;
-  rgb[pix * 3 + 1] = v;
-  rgb[pix * 3 + 2] = v;
-  if (t_out) t_out[pix] = first_t;
-}
-
-template <int kRayStack>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_whitted(int slot, CamArgs cam, TraceParams tp, int aov,
-                                                 float* __restrict__ samples, int* __restrict__ overflow) {
-  int x, y;
-  if (!tile_pixel(cam.width, cam.height, x, y)) return;
-  const int s = blockIdx.y;
-  float3 o, d;
-  primary_ray(cam, x, y, s, o, d);
-  float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow);
-  const size_t idx = ((size_t)s * cam.height + y) * cam.width + x;
-  samples[idx * 3] = c.x;
-  samples[idx * 3 + 1] = c.y;
-  samples[idx * 3 + 2] = c.z;
-}
-
-__global__ __launch_bounds__(256) void k_resolve(CamArgs cam, int aov, const float* __restrict__ samples,
-                                                 unsigned char* __restrict__ rgb, float* __restrict__ hdr) {
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
-  const int npix = cam.width * cam.height;
-  if (pix >= npix) return;
-  float3 sum = mk(0, 0, 0);
-  for (int s = 0; s < cam.aa; ++s) {
-    const float* c = samples + ((size_t)s * npix + pix) * 3;
-    sum = add(sum, mk(c[0], c[1], c[2]));
-  }
-  const float3 avg = mul(sum, 1.0f / (float)cam.aa);
-  unsigned char* o = rgb + (size_t)pix * 3;
-  if (aov >= 0) { o[0] = to_u8(avg.x); o[1] = to_u8(avg.y); o[2] = to_u8(avg.z); }
-  else { o[0] = to_u8_reinhard(avg.x); o[1] = to_u8_reinhard(avg.y); o[2] = to_u8_reinhard(avg.z); }
-  if (hdr) { hdr[(size_t)pix * 3] = avg.x; hdr[(size_t)pix * 3 + 1] = avg.y; hdr[(size_t)pix * 3 + 2] = avg.z; }
-}
-
-// Clears the overflow flag with a kernel (a memset node would misbehave under graph capture).
-__global__ void k_clear_flag(int* flag) {
-  if (threadIdx.x == 0) flag[0] = 0;
-}
 
 struct DevBuf {
   void* ptr = nullptr;
@@ -171,6 +80,12 @@ struct ft_scene {
   DevBuf rgb, tbuf, samples, hdr, overflow, flops;
   ft_render_stats stats{};
   bool timed = false;
+  // scene specialisation
+  fth::EmitPlan plan;
+  std::string source;                              // emitted translation unit (empty: interpreter)
+  hipModule_t modules[fth::kRtcKernelCount] = {};  // one code object per kernel, built on first use
+  hipFunction_t functions[fth::kRtcKernelCount] = {};
+  double compile_ms = 0.0;
 };
 
 namespace {
@@ -191,9 +106,59 @@ void release_slot(int k) {
 }
 
 // Publishes the scene descriptor to its constant-memory slot (call with no render in flight).
+int publish_module(ft_scene* s, hipModule_t m) {
+  hipDeviceptr_t table = nullptr;
+  size_t bytes = 0;
+  FT_HIP(hipModuleGetGlobal(&table, &bytes, m, "ft_c_scenes"));
+  if (bytes < (size_t)(s->slot + 1) * sizeof(DevScene)) return fail(FT_ERR_LAUNCH_FAILED, "scene table size mismatch");
+  FT_HIP(hipMemcpyHtoD((hipDeviceptr_t)((char*)table + (size_t)s->slot * sizeof(DevScene)), &s->dev, sizeof(DevScene)));
+  return FT_OK;
+}
+
 int publish(ft_scene* s) {
-  FT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_scenes), &s->dev, sizeof(DevScene), (size_t)s->slot * sizeof(DevScene),
+  FT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ft_c_scenes), &s->dev, sizeof(DevScene), (size_t)s->slot * sizeof(DevScene),
                            hipMemcpyHostToDevice));
+  for (hipModule_t m : s->modules) {
+    if (!m) continue;
+    int rc = publish_module(s, m);
+    if (rc) return rc;
+  }
+  return FT_OK;
+}
+
+const char* const kKernelNames[fth::kRtcKernelCount] = {"ft_k_depth", "ft_k_whitted_16", "ft_k_whitted_48",
+                                                        "ft_k_whitted_128"};
+const void* const kKernelStubs[fth::kRtcKernelCount] = {(const void*)ft_k_depth, (const void*)ft_k_whitted_16,
+                                                        (const void*)ft_k_whitted_48, (const void*)ft_k_whitted_128};
+
+// Builds and loads the scene's specialised code object for march kernel `k` (no-op when the
+// scene runs on the interpreter or the object is loaded).
+int prepare(ft_scene* s, int k) {
+  if (s->source.empty() || s->functions[k]) return FT_OK;
+  std::vector<char> code;
+  std::string err;
+  double ms = 0.0;
+  if (!fth::rtc_compile(s->source, k, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
+  s->compile_ms += ms;
+  FT_HIP(hipStreamSynchronize(s->stream));
+  hipModule_t m = nullptr;
+  FT_HIP(hipModuleLoadData(&m, code.data()));
+  s->modules[k] = m;
+  FT_HIP(hipModuleGetFunction(&s->functions[k], m, kKernelNames[k]));
+  return publish_module(s, m);
+}
+
+// Launches march kernel `k`: the scene's specialised code object when it has one, else the
+// precompiled kernel.  `args` points at each argument.
+int launch(ft_scene* s, int k, dim3 grid, dim3 block, size_t lds, void** args) {
+  int rc = prepare(s, k);
+  if (rc) return rc;
+  if (s->functions[k]) {
+    FT_HIP(hipModuleLaunchKernel(s->functions[k], grid.x, grid.y, grid.z, block.x, block.y, block.z, (unsigned)lds,
+                                 s->stream, args, nullptr));
+    return FT_OK;
+  }
+  FT_HIP(hipLaunchKernel(kKernelStubs[k], grid, block, args, lds, s->stream));
   return FT_OK;
 }
 
@@ -240,14 +205,11 @@ dim3 tile_grid(int width, int height) {
   return dim3((tiles + 3) / 4, 1, 1);
 }
 
-__global__ void k_clear_u64(unsigned long long* v) {
-  if (threadIdx.x == 0) v[0] = 0ull;
-}
 
 int clear_flops(ft_scene* s) {
   s->stats.sdf_flops = 0;
   if (!s->dev.flops) return FT_OK;
-  hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(64), 0, s->stream, s->dev.flops);
+  hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, s->stream, s->dev.flops);
   FT_HIP(hipGetLastError());
   return FT_OK;
 }
@@ -259,10 +221,11 @@ int depth_impl(ft_scene* s, const ft_render_options* o, uint32_t aa, unsigned ch
   CamArgs cam = make_cam(s, *o, aa);
   TraceParams tp = make_tp(*o, 1);
   FT_HIP(hipSetDevice(s->device));
+  int slot = s->slot;
+  void* args[] = {&slot, &cam, &tp, &rgb_dev, &t_dev};
+  if ((rc = prepare(s, fth::kRtcDepth))) return rc;  // compile outside the timed region
   FT_HIP(hipEventRecord(s->ev0, s->stream));
-  hipLaunchKernelGGL(k_depth, tile_grid(cam.width, cam.height), dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rgb_dev,
-                     t_dev);
-  FT_HIP(hipGetLastError());
+  if ((rc = launch(s, fth::kRtcDepth, tile_grid(cam.width, cam.height), dim3(64, 4), stack_lds(s), args))) return rc;
   FT_HIP(hipEventRecord(s->ev1, s->stream));
   s->timed = true;
   s->stats.primary_rays = (uint64_t)cam.width * cam.height * cam.aa;
@@ -287,24 +250,24 @@ int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings*
   rc = ensure(s->overflow, sizeof(int));
   if (rc) return rc;
   FT_HIP(hipSetDevice(s->device));
-  hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(64), 0, s->stream, (int*)s->overflow.ptr);
+  int kernel;
+  if (need <= 16) kernel = fth::kRtcWhitted16;
+  else if (need <= 48) kernel = fth::kRtcWhitted48;
+  else if (need <= 128) kernel = fth::kRtcWhitted128;
+  else return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with this scene's materials");
+  hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, s->stream, (int*)s->overflow.ptr);
   FT_HIP(hipGetLastError());
-  FT_HIP(hipEventRecord(s->ev0, s->stream));
   dim3 grid = tile_grid(cam.width, cam.height);
   grid.y = (unsigned)cam.aa;
   float* samples = (float*)s->samples.ptr;
   int* ovf = (int*)s->overflow.ptr;
-  if (need <= 16)
-    hipLaunchKernelGGL(k_whitted<16>, grid, dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rs->debug_aov, samples, ovf);
-  else if (need <= 48)
-    hipLaunchKernelGGL(k_whitted<48>, grid, dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rs->debug_aov, samples, ovf);
-  else if (need <= 128)
-    hipLaunchKernelGGL(k_whitted<128>, grid, dim3(64, 4), stack_lds(s), s->stream, s->slot, cam, tp, rs->debug_aov, samples, ovf);
-  else
-    return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with this scene's materials");
-  FT_HIP(hipGetLastError());
+  int slot = s->slot, aov = rs->debug_aov;
+  void* args[] = {&slot, &cam, &tp, &aov, &samples, &ovf};
+  if ((rc = prepare(s, kernel))) return rc;  // compile outside the timed region
+  FT_HIP(hipEventRecord(s->ev0, s->stream));
+  if ((rc = launch(s, kernel, grid, dim3(64, 4), stack_lds(s), args))) return rc;
   int npix = cam.width * cam.height;
-  hipLaunchKernelGGL(k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s->stream, cam, rs->debug_aov, samples, rgb_dev,
+  hipLaunchKernelGGL(ft_k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s->stream, cam, rs->debug_aov, samples, rgb_dev,
                      hdr_dev);
   FT_HIP(hipGetLastError());
   FT_HIP(hipEventRecord(s->ev1, s->stream));
@@ -402,6 +365,16 @@ int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) 
   d.n_bvh = (int)h.bvh.size();
   d.stack_depth = std::max(1, h.max_stack);
   for (int k = 0; k < 3; ++k) { d.bmin[k] = h.bmin[k]; d.bmax[k] = h.bmax[k]; d.env_rad[k] = h.env_rad[k]; }
+  // scene specialisation (FT_EMIT: unset/"auto", "1" always, "0" never)
+  {
+    const char* mode = std::getenv("FT_EMIT");
+    const std::string m = mode ? mode : "auto";
+    s->plan = fth::plan_emission(h, ir);
+    bool emit = false;
+    if (m == "1" || m == "always") emit = s->plan.any();
+    else if (m != "0" && m != "never") emit = s->plan.any() && fth::emission_pays(h, ir, s->plan);
+    if (emit) s->source = fth::emit_scene_source(h, ir, s->plan);
+  }
   s->slot = acquire_slot();
   if (s->slot < 0) {
     cleanup();
@@ -423,6 +396,8 @@ int ft_scene_destroy(ft_scene* s) {
   for (DevBuf* b : {&s->nodes, &s->prog, &s->lbprog, &s->mats, &s->objects, &s->lights, &s->fns, &s->exprs, &s->bvh,
                     &s->leaves, &s->rgb, &s->tbuf, &s->samples, &s->hdr, &s->overflow, &s->flops})
     if (b->ptr) (void)hipFree(b->ptr);
+  for (hipModule_t m : s->modules)
+    if (m) (void)hipModuleUnload(m);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -485,6 +460,12 @@ int ft_scene_stream(ft_scene* s, void** stream_out) {
 int ft_scene_stats(ft_scene* s, ft_render_stats* out) {
   if (!s || !out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
   *out = s->stats;
+  out->flags = 0;
+  if (!s->source.empty()) {
+    if (s->plan.scene) out->flags |= FT_STATS_EMITTED_SCENE;
+    if (s->plan.functions) out->flags |= FT_STATS_EMITTED_FUNCTIONS;
+  }
+  out->compile_ms = s->compile_ms;
   return FT_OK;
 }
 
@@ -499,6 +480,35 @@ int ft_scene_set_flop_counting(ft_scene* s, int enable) {
     s->dev.flops = nullptr;
   }
   return publish(s);
+}
+
+int ft_emit_scene_source(const ft_scene_ir* ir, uint32_t accel_mode, char* buf, uint64_t cap, uint64_t* len_out) {
+  fth::HostScene h;
+  std::string err;
+  if (!fth::build_scene(ir, accel_mode, h, err))
+    return fail(err.find("root") != std::string::npos ? FT_ERR_MISSING_SCENE_ROOT : FT_ERR_INVALID_ARGUMENT, err);
+  const std::string src = fth::emit_scene_source(h, ir, fth::plan_emission(h, ir));
+  if (len_out) *len_out = src.size();
+  if (buf && cap > 0) {
+    const size_t n = std::min<size_t>(src.size(), (size_t)cap - 1);
+    std::memcpy(buf, src.data(), n);
+    buf[n] = '\0';
+  }
+  return FT_OK;
+}
+
+int ft_emit_compile(const ft_scene_ir* ir, uint32_t accel_mode, int kernel, double* compile_ms_out) {
+  if (kernel < 0 || kernel >= fth::kRtcKernelCount) return fail(FT_ERR_INVALID_ARGUMENT, "unknown kernel");
+  fth::HostScene h;
+  std::string err;
+  if (!fth::build_scene(ir, accel_mode, h, err))
+    return fail(err.find("root") != std::string::npos ? FT_ERR_MISSING_SCENE_ROOT : FT_ERR_INVALID_ARGUMENT, err);
+  const std::string src = fth::emit_scene_source(h, ir, fth::plan_emission(h, ir));
+  std::vector<char> code;
+  double ms = 0.0;
+  if (!fth::rtc_compile(src, kernel, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
+  if (compile_ms_out) *compile_ms_out = ms;
+  return FT_OK;
 }
 
 // sizeof() of every ABI struct, so bindings can verify their layout.

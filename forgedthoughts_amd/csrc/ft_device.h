@@ -16,10 +16,12 @@
 // One ray per lane; all arithmetic f32.
 #pragma once
 
+#if !defined(__HIPCC_RTC__)
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
-#include "../../include/ft_abi.h"
+#include "ft_abi.h"
 #include "ft_layout.h"
 
 namespace ftd {
@@ -47,10 +49,14 @@ struct DevScene {
 // receive the slot index and re-read it with readfirstlane, so every scene field is a
 // scalar (SGPR) load and node programs run with wave-uniform control flow.
 constexpr int kMaxScenes = 64;
-__constant__ DevScene c_scenes[kMaxScenes];
+}  // namespace ftd
+extern "C" {
+__constant__ ftd::DevScene ft_c_scenes[ftd::kMaxScenes];
+}
+namespace ftd {
 
 __device__ __forceinline__ const DevScene& scene_of(int slot) {
-  return c_scenes[__builtin_amdgcn_readfirstlane(slot)];
+  return ft_c_scenes[__builtin_amdgcn_readfirstlane(slot)];
 }
 
 // A wave-uniform pointer re-typed into the constant address space, so loads through it
@@ -140,6 +146,11 @@ __device__ __forceinline__ float aabb_lb(float3 p, const float* mn, const float*
   float dy = p.y < mn[1] ? mn[1] - p.y : (p.y > mx[1] ? p.y - mx[1] : 0.0f);
   float dz = p.z < mn[2] ? mn[2] - p.z : (p.z > mx[2] ? p.z - mx[2] : 0.0f);
   return sqrtf(dx * dx + dy * dy + dz * dz);
+}
+
+__device__ __forceinline__ float aabb_lb_lit(float3 p, float x0, float y0, float z0, float x1, float y1, float z1) {
+  const float mn[3] = {x0, y0, z0}, mx[3] = {x1, y1, z1};
+  return aabb_lb(p, mn, mx);
 }
 
 // --------------------------------------------------------- hg_sdf ops
@@ -285,7 +296,49 @@ __device__ inline float fbm_3d(float3 p, int octaves, float scale, float lac) {
   return sum;
 }
 
+
+// Forge builtins shared by the expression interpreter below and the emitted functions.
+__device__ __forceinline__ float3 splat3(float x) { return mk(x, x, x); }
+__device__ __forceinline__ float saturatef_(float x) { return clampf(x, 0.0f, 1.0f); }
+__device__ __forceinline__ float fclamp_any(float x, float a, float b) {  // clamp with either bound order
+  return a <= b ? clampf(x, a, b) : clampf(x, b, a);
+}
+__device__ __forceinline__ float smoothstep_builtin(float e0, float e1, float x) {
+  const float span = e1 - e0;
+  float t;
+  if (fabsf(span) < 1.1920929e-7f) t = x < e0 ? 0.0f : 1.0f;
+  else t = clampf((x - e0) / span, 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ float3 normalize_div(float3 a) {
+  const float l = len(a);
+  return l <= 1.1920929e-7f ? mk(0, 0, 0) : mk(a.x / l, a.y / l, a.z / l);
+}
+__device__ __forceinline__ float3 rotate_deg_x(float3 a, float deg) { const float r = deg2rad(deg); return rot_x(a, sinf(r), cosf(r)); }
+__device__ __forceinline__ float3 rotate_deg_y(float3 a, float deg) { const float r = deg2rad(deg); return rot_y(a, sinf(r), cosf(r)); }
+__device__ __forceinline__ float3 rotate_deg_z(float3 a, float deg) { const float r = deg2rad(deg); return rot_z(a, sinf(r), cosf(r)); }
+__device__ __forceinline__ float box_dist_builtin(float3 a, float3 b) {
+  const float3 q = mk(fabsf(a.x) - b.x, fabsf(a.y) - b.y, fabsf(a.z) - b.z);
+  return len(mk(fmaxf(q.x, 0), fmaxf(q.y, 0), fmaxf(q.z, 0))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
+}
+__device__ __forceinline__ float cylinder_dist_builtin(float3 a, float radius, float half_height) {
+  const float radial = sqrtf(a.x * a.x + a.z * a.z) - radius, axial = fabsf(a.y) - half_height;
+  const float ox = fmaxf(radial, 0.0f), oy = fmaxf(axial, 0.0f);
+  return sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, axial), 0.0f);
+}
+__device__ __forceinline__ float torus_dist_builtin(float3 a, float major, float minor) {
+  const float qx = sqrtf(a.x * a.x + a.z * a.z) - major;
+  return sqrtf(qx * qx + a.y * a.y) - minor;
+}
+
+#if defined(FT_EMITTED)
+__device__ float3 ftx_fn(int fn, const float3* params);
+#endif
+
 __device__ __noinline__ float3 eval_fn(int slot, int fn, const float3* params) {
+#if defined(FT_EMITTED) && FT_EMITTED_FUNCTIONS
+  return ftx_fn(fn, params);
+#endif
   const DevScene& S = scene_of(slot);
   const ft_function F = S.fns[fn];
   float3 v[kMaxExpr];
@@ -315,56 +368,24 @@ __device__ __noinline__ float3 eval_fn(int slot, int fn, const float3* params) {
       case FT_EXPR_SQRT: r = mk(sqrtf(a.x), sqrtf(a.y), sqrtf(a.z)); break;
       case FT_EXPR_SIN: r = mk(sinf(a.x), sinf(a.y), sinf(a.z)); break;
       case FT_EXPR_COS: r = mk(cosf(a.x), cosf(a.y), cosf(a.z)); break;
-      case FT_EXPR_SATURATE: r = mk(clampf(a.x, 0, 1), clampf(a.y, 0, 1), clampf(a.z, 0, 1)); break;
+      case FT_EXPR_SATURATE: r = mk(saturatef_(a.x), saturatef_(a.y), saturatef_(a.z)); break;
       case FT_EXPR_MIN: r = mk(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); break;
       case FT_EXPR_MAX: r = mk(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); break;
       case FT_EXPR_POW: r = mk(powf(a.x, b.x), powf(a.y, b.y), powf(a.z, b.z)); break;
-      case FT_EXPR_CLAMP: {
-        r.x = b.x <= c.x ? clampf(a.x, b.x, c.x) : clampf(a.x, c.x, b.x);
-        r.y = b.y <= c.y ? clampf(a.y, b.y, c.y) : clampf(a.y, c.y, b.y);
-        r.z = b.z <= c.z ? clampf(a.z, b.z, c.z) : clampf(a.z, c.z, b.z);
-        break;
-      }
+      case FT_EXPR_CLAMP: r = mk(fclamp_any(a.x, b.x, c.x), fclamp_any(a.y, b.y, c.y), fclamp_any(a.z, b.z, c.z)); break;
       case FT_EXPR_MIX: r = mk(lerpf(a.x, b.x, c.x), lerpf(a.y, b.y, c.y), lerpf(a.z, b.z, c.z)); break;
       case FT_EXPR_STEP: { float s = b.x < a.x ? 0.0f : 1.0f; r = mk(s, s, s); break; }
-      case FT_EXPR_SMOOTHSTEP: {
-        float span = b.x - a.x, t;
-        if (fabsf(span) < 1.1920929e-7f) t = c.x < a.x ? 0.0f : 1.0f;
-        else t = clampf((c.x - a.x) / span, 0.0f, 1.0f);
-        t = t * t * (3.0f - 2.0f * t);
-        r = mk(t, t, t);
-        break;
-      }
+      case FT_EXPR_SMOOTHSTEP: r = splat3(smoothstep_builtin(a.x, b.x, c.x)); break;
       case FT_EXPR_DOT: { float s = dot(a, b); r = mk(s, s, s); break; }
       case FT_EXPR_LENGTH: { float s = len(a); r = mk(s, s, s); break; }
-      case FT_EXPR_NORMALIZE: {
-        float l = len(a);
-        r = l <= 1.1920929e-7f ? mk(0, 0, 0) : mk(a.x / l, a.y / l, a.z / l);
-        break;
-      }
-      case FT_EXPR_ROTATE_X: { float rr = deg2rad(b.x); r = rot_x(a, sinf(rr), cosf(rr)); break; }
-      case FT_EXPR_ROTATE_Y: { float rr = deg2rad(b.x); r = rot_y(a, sinf(rr), cosf(rr)); break; }
-      case FT_EXPR_ROTATE_Z: { float rr = deg2rad(b.x); r = rot_z(a, sinf(rr), cosf(rr)); break; }
+      case FT_EXPR_NORMALIZE: r = normalize_div(a); break;
+      case FT_EXPR_ROTATE_X: r = rotate_deg_x(a, b.x); break;
+      case FT_EXPR_ROTATE_Y: r = rotate_deg_y(a, b.x); break;
+      case FT_EXPR_ROTATE_Z: r = rotate_deg_z(a, b.x); break;
       case FT_EXPR_SPHERE_DIST: { float s = len(a) - b.x; r = mk(s, s, s); break; }
-      case FT_EXPR_BOX_DIST: {
-        float3 q = mk(fabsf(a.x) - b.x, fabsf(a.y) - b.y, fabsf(a.z) - b.z);
-        float s = len(mk(fmaxf(q.x, 0), fmaxf(q.y, 0), fmaxf(q.z, 0))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
-        r = mk(s, s, s);
-        break;
-      }
-      case FT_EXPR_CYLINDER_DIST: {
-        float radial = sqrtf(a.x * a.x + a.z * a.z) - b.x, axial = fabsf(a.y) - c.x;
-        float ox = fmaxf(radial, 0.0f), oy = fmaxf(axial, 0.0f);
-        float s = sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, axial), 0.0f);
-        r = mk(s, s, s);
-        break;
-      }
-      case FT_EXPR_TORUS_DIST: {
-        float qx = sqrtf(a.x * a.x + a.z * a.z) - b.x;
-        float s = sqrtf(qx * qx + a.y * a.y) - c.x;
-        r = mk(s, s, s);
-        break;
-      }
+      case FT_EXPR_BOX_DIST: r = splat3(box_dist_builtin(a, b)); break;
+      case FT_EXPR_CYLINDER_DIST: r = splat3(cylinder_dist_builtin(a, b.x, c.x)); break;
+      case FT_EXPR_TORUS_DIST: r = splat3(torus_dist_builtin(a, b.x, c.x)); break;
       case FT_EXPR_VALUE_NOISE: { float s = value_noise_3d(a); r = mk(s, s, s); break; }
       case FT_EXPR_FBM: {
         float3 d = e.d >= 0 ? v[e.d] : mk(0, 0, 0);
@@ -394,37 +415,40 @@ __device__ inline float custom_distance(int slot, const DNode& n, float3 p) {
 
 __device__ __forceinline__ float shell_op(float d, float shell) { return shell > 0.0f ? fmaxf(d, -(d + shell)) : d; }
 
-// primitive distances (renderer.rs:5545-5663)
+// primitive distances in the local frame (renderer.rs:5545-5663)
+__device__ __forceinline__ float sdf_sphere(float3 q, float radius, float shell) {
+  return shell_op(len(q) - radius, shell);
+}
+__device__ __forceinline__ float sdf_box(float3 q, float hx, float hy, float hz, float rd, float shell) {
+  float3 a = mk(fabsf(q.x) - fmaxf(hx - rd, 0.0f), fabsf(q.y) - fmaxf(hy - rd, 0.0f), fabsf(q.z) - fmaxf(hz - rd, 0.0f));
+  float outside = len(mk(fmaxf(a.x, 0.0f), fmaxf(a.y, 0.0f), fmaxf(a.z, 0.0f)));
+  float inside = fminf(fmaxf(fmaxf(a.x, a.y), a.z), 0.0f);
+  return shell_op(outside + inside - rd, shell);
+}
+__device__ __forceinline__ float sdf_cylinder(float3 q, float radius, float hh, float rd, float shell) {
+  float dx = sqrtf(q.x * q.x + q.z * q.z) - fmaxf(radius - rd, 0.0f);
+  float dy = fabsf(q.y) - fmaxf(hh - rd, 0.0f);
+  float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
+  return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(dx, dy), 0.0f) - rd, shell);
+}
+__device__ __forceinline__ float sdf_torus(float3 q, float major, float minor) {
+  float qx = sqrtf(q.x * q.x + q.z * q.z) - major;
+  return sqrtf(qx * qx + q.y * q.y) - minor;
+}
+__device__ __noinline__ float sdf_ngon(float3 q, int sides, float radius, float hh, float rd, float shell) {
+  float radial = sd_ngon(mk(q.x, 0.0f, q.z), sides, fmaxf(radius - rd, 0.0f));
+  float dy = fabsf(q.y) - fmaxf(hh - rd, 0.0f);
+  float ox = fmaxf(radial, 0.0f), oy = fmaxf(dy, 0.0f);
+  return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, dy), 0.0f) - rd, shell);
+}
 __device__ __forceinline__ float prim_distance(const DNode& n, float3 p) {
   float3 q = node_local(n, p);
   switch (n.kind) {
-    case FT_NODE_SPHERE: return shell_op(len(q) - n.p[0], n.p[1]);
-    case FT_NODE_BOX: {
-      float rd = n.p[3];
-      float3 a = mk(fabsf(q.x) - fmaxf(n.p[0] - rd, 0.0f), fabsf(q.y) - fmaxf(n.p[1] - rd, 0.0f),
-                    fabsf(q.z) - fmaxf(n.p[2] - rd, 0.0f));
-      float outside = len(mk(fmaxf(a.x, 0.0f), fmaxf(a.y, 0.0f), fmaxf(a.z, 0.0f)));
-      float inside = fminf(fmaxf(fmaxf(a.x, a.y), a.z), 0.0f);
-      return shell_op(outside + inside - rd, n.p[4]);
-    }
-    case FT_NODE_CYLINDER: {
-      float rd = n.p[2];
-      float dx = sqrtf(q.x * q.x + q.z * q.z) - fmaxf(n.p[0] - rd, 0.0f);
-      float dy = fabsf(q.y) - fmaxf(n.p[1] - rd, 0.0f);
-      float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
-      return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(dx, dy), 0.0f) - rd, n.p[3]);
-    }
-    case FT_NODE_TORUS: {
-      float qx = sqrtf(q.x * q.x + q.z * q.z) - n.p[0];
-      return sqrtf(qx * qx + q.y * q.y) - n.p[1];
-    }
-    default: {  // FT_NODE_EXTRUDE_POLYGON
-      float rd = n.p[3];
-      float radial = sd_ngon(mk(q.x, 0.0f, q.z), (int)n.p[0], fmaxf(n.p[1] - rd, 0.0f));
-      float dy = fabsf(q.y) - fmaxf(n.p[2] - rd, 0.0f);
-      float ox = fmaxf(radial, 0.0f), oy = fmaxf(dy, 0.0f);
-      return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, dy), 0.0f) - rd, n.p[4]);
-    }
+    case FT_NODE_SPHERE: return sdf_sphere(q, n.p[0], n.p[1]);
+    case FT_NODE_BOX: return sdf_box(q, n.p[0], n.p[1], n.p[2], n.p[3], n.p[4]);
+    case FT_NODE_CYLINDER: return sdf_cylinder(q, n.p[0], n.p[1], n.p[2], n.p[3]);
+    case FT_NODE_TORUS: return sdf_torus(q, n.p[0], n.p[1]);
+    default: return sdf_ngon(q, (int)n.p[0], n.p[1], n.p[2], n.p[3], n.p[4]);
   }
 }
 
@@ -695,12 +719,24 @@ __device__ __noinline__ float bvh_lower_bound(int slot, float3 p) {
 // Accelerator::distance_info / lower_bound (renderer.rs:609-623, 1024-1129).  Bricks uses
 // the BVH traversal: both return the minimum over accel leaves (the grid's shell search
 // visits a superset of the leaves that can beat the running best).
+#if defined(FT_EMITTED)
+// Provided by the scene emitter (runtime-compiled scene-specialised code).
+__device__ __forceinline__ Info ftx_scene_info(const DevScene& S, float3 p);
+__device__ __forceinline__ float ftx_lower_bound(const DevScene& S, float3 p);
+#endif
+
 __device__ __forceinline__ Info scene_info(int slot, float3 p) {
+#if defined(FT_EMITTED) && FT_EMITTED_SCENE
+  return ftx_scene_info(scene_of(slot), p);
+#endif
   const DevScene& S = scene_of(slot);
   if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_info(slot, p);
   return eval_node(slot, S.root, p);
 }
 __device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
+#if defined(FT_EMITTED) && FT_EMITTED_SCENE
+  return ftx_lower_bound(scene_of(slot), p);
+#endif
   const DevScene& S = scene_of(slot);
   if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_lower_bound(slot, p);
   return lower_bound_root(slot, p);
@@ -736,7 +772,10 @@ __device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, co
   return true;
 }
 
-// raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215)
+// raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215).
+// One evaluation site: the march, the 10 bisection steps, the 6 central differences and
+// the final distance_info are phases of a single loop, so a specialised scene function is
+// inlined once and lanes in different phases still share every evaluation.
 __device__ __noinline__ bool raymarch(int slot, float3 o, float3 d, int max_steps, float eps,
                                       float step_scale, float min_t, float max_t, Hit& hit) {
   const DevScene& S = scene_of(slot);
@@ -747,36 +786,65 @@ __device__ __noinline__ bool raymarch(int slot, float3 o, float3 d, int max_step
   float prev = traveled;
   const float sscale = clampf(step_scale, 0.05f, 1.0f);
   const float min_step = fmaxf(eps * 0.5f, 1.0e-5f);
-  for (int s = 0; s < max_steps; ++s) {
-    if (traveled > max_t) return false;
-    float dist = scene_info(slot, add(o, mul(d, traveled))).d;
-    if (fabsf(dist) < eps) {
-      float lo = prev, hi = traveled;
-      for (int k = 0; k < 10; ++k) {
-        float mid = 0.5f * (lo + hi);
-        if (fabsf(scene_info(slot, add(o, mul(d, mid))).d) < eps) hi = mid; else lo = mid;
+  const float e = fmaxf(eps * 4.0f, 1.0e-5f);
+  int phase = 0, k = 0, steps = 0;
+  float lo = 0.0f, hi = 0.0f, mid = 0.0f, dplus = 0.0f;
+  float3 pos = mk(0, 0, 0), grad = mk(0, 0, 0);
+  for (;;) {
+    float3 p;
+    if (phase == 0) {
+      if (steps >= max_steps || traveled > max_t) return false;
+      p = add(o, mul(d, traveled));
+    } else if (phase == 1) {
+      mid = 0.5f * (lo + hi);
+      p = add(o, mul(d, mid));
+    } else if (phase == 2) {
+      const float off = (k & 1) ? -e : e;
+      const int axis = k >> 1;
+      p = mk(pos.x + (axis == 0 ? off : 0.0f), pos.y + (axis == 1 ? off : 0.0f), pos.z + (axis == 2 ? off : 0.0f));
+    } else {
+      p = pos;
+    }
+    const Info inf = scene_info(slot, p);
+    if (phase == 0) {
+      if (fabsf(inf.d) < eps) {
+        phase = 1;
+        k = 0;
+        lo = prev;
+        hi = traveled;
+      } else {
+        prev = traveled;
+        traveled += fmaxf(fabsf(inf.d) * sscale, min_step);
+        ++steps;
       }
-      float3 pos = add(o, mul(d, hi));
-      float e = fmaxf(eps * 4.0f, 1.0e-5f);
-      float dx = scene_info(slot, mk(pos.x + e, pos.y, pos.z)).d - scene_info(slot, mk(pos.x - e, pos.y, pos.z)).d;
-      float dy = scene_info(slot, mk(pos.x, pos.y + e, pos.z)).d - scene_info(slot, mk(pos.x, pos.y - e, pos.z)).d;
-      float dz = scene_info(slot, mk(pos.x, pos.y, pos.z + e)).d - scene_info(slot, mk(pos.x, pos.y, pos.z - e)).d;
-      float3 nrm = mk(dx, dy, dz);
-      nrm = len(nrm) > 1.0e-8f ? normalize(nrm) : mk(0.0f, 1.0f, 0.0f);
-      Info fin = scene_info(slot, pos);
+    } else if (phase == 1) {
+      if (fabsf(inf.d) < eps) hi = mid; else lo = mid;
+      if (++k == 10) {
+        pos = add(o, mul(d, hi));
+        phase = 2;
+        k = 0;
+      }
+    } else if (phase == 2) {
+      if ((k & 1) == 0) {
+        dplus = inf.d;
+      } else {
+        const float g = dplus - inf.d;
+        const int axis = k >> 1;
+        grad = mk(axis == 0 ? g : grad.x, axis == 1 ? g : grad.y, axis == 2 ? g : grad.z);
+      }
+      if (++k == 6) phase = 3;
+    } else {
+      const float3 nrm = len(grad) > 1.0e-8f ? normalize(grad) : mk(0.0f, 1.0f, 0.0f);
       hit.t = hi;
       hit.pos = pos;
       hit.n = nrm;
       hit.front = dot(nrm, d) < 0.0f;
-      hit.obj = fin.obj;
-      hit.mat = fin.mat;
-      hit.leaf = fin.leaf;
+      hit.obj = inf.obj;
+      hit.mat = inf.mat;
+      hit.leaf = inf.leaf;
       return true;
     }
-    prev = traveled;
-    traveled += fmaxf(fabsf(dist) * sscale, min_step);
   }
-  return false;
 }
 
 }  // namespace ftd

@@ -1,0 +1,154 @@
+// ft_kernels.h — the gfx950 kernels (compiled by hipcc into libft_hip.so, and by hiprtc
+// together with scene-specialised code emitted by ft_emit.cpp).
+//
+//   ft_k_depth          depth preview, one thread per pixel looping over the AA grid
+//                       (render_depth_tile renderer.rs:3931-3971);
+//   ft_k_whitted_N      one thread per (pixel, AA sample): Whitted trace with an N-entry ray
+//                       stack, writes the sample radiance;
+//   ft_k_resolve        per pixel: sums the samples in AA order, scales by 1/aa and tone maps
+//                       (render_ray_tile renderer.rs:3874-3929, renderer.rs:5221-5239).
+#pragma once
+
+#include "ft_shade.h"
+
+namespace ftd {
+
+struct CamArgs {
+  float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
+  int width, height, aa, grid;
+};
+
+__device__ __forceinline__ void primary_ray(const CamArgs& c, int x, int y, int s, float3& o, float3& d) {
+  // pixel_sample_offsets (renderer.rs:4838-4854): sample s sits at grid cell (s % g, s / g)
+  const float g = (float)c.grid;
+  const float sx = ((float)(s % c.grid) + 0.5f) / g, sy = ((float)(s / c.grid) + 0.5f) / g;
+  const float px = (((float)x + sx) / (float)c.width) * 2.0f - 1.0f;
+  const float py = 1.0f - (((float)y + sy) / (float)c.height) * 2.0f;
+  const float ndx = px * c.aspect, ndy = py;
+  float3 dir = add(add(mk(c.fx, c.fy, c.fz), mul(mk(c.rx, c.ry, c.rz), ndx * c.tan_fov)),
+                   mul(mk(c.ux, c.uy, c.uz), ndy * c.tan_fov));
+  d = normalize(normalize(dir));
+  o = mk(c.ox, c.oy, c.oz);
+}
+
+__device__ __forceinline__ float depth_preview_value(float t, float max_dist) {  // renderer.rs:5215-5219
+  float dd = clampf(t / fmaxf(max_dist, 1.0e-6f), 0.0f, 1.0f);
+  float s = 1.0f - dd;
+  return s * s;
+}
+
+// Pixels are visited in 8x8 tiles per wavefront (blockDim 64x4: four tiles per block) so
+// that the 64 lanes of a wave march spatially adjacent rays.
+__device__ __forceinline__ bool tile_pixel(int width, int height, int& x, int& y) {
+  const int lane = threadIdx.x;
+  const int tiles_x = (width + 7) / 8;
+  const int tile = blockIdx.x * blockDim.y + threadIdx.y;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  x = tx * 8 + (lane & 7);
+  y = ty * 8 + (lane >> 3);
+  return x < width && y < height;
+}
+
+__device__ __forceinline__ void depth_kernel_body(int slot, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
+                                               float* __restrict__ t_out) {
+  int x, y;
+  if (!tile_pixel(cam.width, cam.height, x, y)) return;
+  float sum = 0.0f;
+  float first_t = -1.0f;
+  for (int s = 0; s < cam.aa; ++s) {
+    float3 o, d;
+    primary_ray(cam, x, y, s, o, d);
+    Hit hit;
+    bool h = raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit);
+    sum = sum + (h ? depth_preview_value(hit.t, tp.max_dist) : 0.0f);
+    if (s == 0) first_t = h ? hit.t : -1.0f;
+  }
+  const unsigned char v = to_u8(sum * (1.0f / (float)cam.aa));
+  const size_t pix = (size_t)y * cam.width + x;
+  rgb[pix * 3] = v;
+  rgb[pix * 3 + 1] = v;
+  rgb[pix * 3 + 2] = v;
+  if (t_out) t_out[pix] = first_t;
+}
+
+template <int kRayStack>
+__device__ __forceinline__ void whitted_kernel_body(int slot, CamArgs cam, TraceParams tp, int aov,
+                                                 float* __restrict__ samples, int* __restrict__ overflow) {
+  int x, y;
+  if (!tile_pixel(cam.width, cam.height, x, y)) return;
+  const int s = blockIdx.y;
+  float3 o, d;
+  primary_ray(cam, x, y, s, o, d);
+  float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow);
+  const size_t idx = ((size_t)s * cam.height + y) * cam.width + x;
+  samples[idx * 3] = c.x;
+  samples[idx * 3 + 1] = c.y;
+  samples[idx * 3 + 2] = c.z;
+}
+
+__device__ __forceinline__ void resolve_kernel_body(CamArgs cam, int aov, const float* __restrict__ samples,
+                                                 unsigned char* __restrict__ rgb, float* __restrict__ hdr) {
+  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+  const int npix = cam.width * cam.height;
+  if (pix >= npix) return;
+  float3 sum = mk(0, 0, 0);
+  for (int s = 0; s < cam.aa; ++s) {
+    const float* c = samples + ((size_t)s * npix + pix) * 3;
+    sum = add(sum, mk(c[0], c[1], c[2]));
+  }
+  const float3 avg = mul(sum, 1.0f / (float)cam.aa);
+  unsigned char* o = rgb + (size_t)pix * 3;
+  if (aov >= 0) { o[0] = to_u8(avg.x); o[1] = to_u8(avg.y); o[2] = to_u8(avg.z); }
+  else { o[0] = to_u8_reinhard(avg.x); o[1] = to_u8_reinhard(avg.y); o[2] = to_u8_reinhard(avg.z); }
+  if (hdr) { hdr[(size_t)pix * 3] = avg.x; hdr[(size_t)pix * 3 + 1] = avg.y; hdr[(size_t)pix * 3 + 2] = avg.z; }
+}
+
+// Clears the overflow flag with a kernel (a memset node would misbehave under graph capture).
+__device__ __forceinline__ void clear_flag_body(int* flag) {
+  if (threadIdx.x == 0) flag[0] = 0;
+}
+
+__device__ __forceinline__ void clear_u64_body(unsigned long long* v) {
+  if (threadIdx.x == 0) v[0] = 0ull;
+}
+
+}  // namespace ftd
+
+#define FT_KERNEL extern "C" __global__ __launch_bounds__(256)
+#define FT_MARCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
+
+// FT_RTC_KERNEL selects the one kernel a runtime-compiled (scene-specialised) code object
+// holds: 0 depth, 1/2/3 Whitted with a 16/48/128-entry ray stack.  The precompiled library
+// holds them all.
+#if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 0
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_depth(int slot, ftd::CamArgs cam, ftd::TraceParams tp, unsigned char* __restrict__ rgb,
+                                             float* __restrict__ t_out) {
+  ftd::depth_kernel_body(slot, cam, tp, rgb, t_out);
+}
+#endif
+#if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 1
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_16(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
+                                                  float* __restrict__ samples, int* __restrict__ overflow) {
+  ftd::whitted_kernel_body<16>(slot, cam, tp, aov, samples, overflow);
+}
+#endif
+#if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 2
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_48(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
+                                                  float* __restrict__ samples, int* __restrict__ overflow) {
+  ftd::whitted_kernel_body<48>(slot, cam, tp, aov, samples, overflow);
+}
+#endif
+#if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 3
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_128(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
+                                                   float* __restrict__ samples, int* __restrict__ overflow) {
+  ftd::whitted_kernel_body<128>(slot, cam, tp, aov, samples, overflow);
+}
+#endif
+#if !defined(FT_RTC_KERNEL)
+FT_KERNEL void ft_k_resolve(ftd::CamArgs cam, int aov, const float* __restrict__ samples, unsigned char* __restrict__ rgb,
+                            float* __restrict__ hdr) {
+  ftd::resolve_kernel_body(cam, aov, samples, rgb, hdr);
+}
+FT_KERNEL void ft_k_clear_flag(int* flag) { ftd::clear_flag_body(flag); }
+FT_KERNEL void ft_k_clear_u64(unsigned long long* v) { ftd::clear_u64_body(v); }
+#endif

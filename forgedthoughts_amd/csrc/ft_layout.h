@@ -2,9 +2,11 @@
 // Plain C++: no HIP headers, so the host-side builder compiles with any C++ compiler.
 #pragma once
 
+#if !defined(__HIPCC_RTC__)
 #include <stdint.h>
+#endif
 
-#include "../../include/ft_abi.h"
+#include "ft_abi.h"
 
 namespace ftd {
 

@@ -31,7 +31,8 @@ ERROR_KINDS = {1: "MissingSceneRoot", 2: "UnsupportedObjectType", 3: "ExpectedOb
 
 EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_destroy", "ft_render_depth",
            "ft_render_whitted", "ft_render_depth_device", "ft_render_whitted_device", "ft_scene_synchronize",
-           "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting"]
+           "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting",
+           "ft_emit_scene_source", "ft_emit_compile"]
 
 
 class BackendUnavailable(RuntimeError):
@@ -59,6 +60,8 @@ def library():
         L.ft_scene_stats.argtypes = [vp, vp]
         L.ft_abi_struct_size.argtypes = [C.c_char_p]
         L.ft_scene_set_flop_counting.argtypes = [vp, C.c_int]
+        L.ft_emit_scene_source.argtypes = [vp, u32, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
+        L.ft_emit_compile.argtypes = [vp, u32, C.c_int, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -149,6 +152,25 @@ class GpuScene:
         st = abi.RenderStatsC()
         _check(library().ft_scene_stats(self._h, C.addressof(st)))
         return st
+
+
+def emit_scene_source(scene: CompiledScene, accel: AccelMode = AccelMode.Bvh) -> str:
+    """The scene-specialised HIP translation unit the backend would compile for `scene`."""
+    ir = scene.ir()
+    n = C.c_uint64(0)
+    _check(library().ft_emit_scene_source(C.addressof(ir), int(accel), None, 0, C.byref(n)))
+    buf = C.create_string_buffer(n.value + 1)
+    _check(library().ft_emit_scene_source(C.addressof(ir), int(accel), buf, n.value + 1, C.byref(n)))
+    return buf.value.decode()
+
+
+def emit_compile(scene: CompiledScene, accel: AccelMode = AccelMode.Bvh, kernel: int = 0) -> float:
+    """Compiles the scene-specialised kernel with hiprtc (no GPU needed); returns compile ms
+    (0 when served from the code-object cache)."""
+    ir = scene.ir()
+    ms = C.c_double(0.0)
+    _check(library().ft_emit_compile(C.addressof(ir), int(accel), int(kernel), C.byref(ms)))
+    return ms.value
 
 
 # ---------------------------------------------------------------- reference API

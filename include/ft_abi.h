@@ -27,7 +27,15 @@
 #ifndef FT_ABI_H
 #define FT_ABI_H
 
+#if defined(__HIPCC_RTC__)
+/* runtime-compiled device code: hiprtc provides the fixed-width integer types */
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::int32_t int32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::uint8_t uint8_t;
+#else
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -353,10 +361,23 @@ typedef struct ft_render_stats {
   uint64_t primary_rays;
   uint64_t kernel_launches;
   float last_kernel_ms;   /* hipEvent time of the last render's kernels */
-  float pad;
+  uint32_t flags;         /* FT_STATS_* */
   uint64_t sdf_flops;     /* FLOPs of SDF evaluation counted by the last render (counting on) */
+  double compile_ms;      /* time spent compiling scene-specialised kernels so far */
 } ft_render_stats;
+#define FT_STATS_EMITTED_SCENE 1u     /* scene distance runs as specialised straight-line code */
+#define FT_STATS_EMITTED_FUNCTIONS 2u /* Forge functions run as specialised code */
 int ft_scene_stats(ft_scene* scene, ft_render_stats* out);
+
+/* Scene specialisation diagnostics (no GPU needed).  ft_emit_scene_source writes the
+ * translation unit the emitter produces for `ir` under `accel_mode` (always emitted, the
+ * FT_EMIT policy aside) into buf (cap bytes, NUL-terminated, *len_out = full length);
+ * ft_emit_compile additionally compiles it with hiprtc for one kernel (0 depth, 1/2/3
+ * Whitted with a 16/48/128-entry ray stack) and reports the compile time.  No counterpart
+ * in the reference: its Cranelift JIT (crates/forgedthoughts/src/jit.rs) has no such hook;
+ * these exist so a host without a GPU can check the emitted code. */
+int ft_emit_scene_source(const ft_scene_ir* ir, uint32_t accel_mode, char* buf, uint64_t cap, uint64_t* len_out);
+int ft_emit_compile(const ft_scene_ir* ir, uint32_t accel_mode, int kernel, double* compile_ms_out);
 
 /* Turns FLOP counting on (1) or off (0) for later renders on this scene.  Counting adds
  * atomics to every SDF evaluation: use it for work accounting, never for timing. */

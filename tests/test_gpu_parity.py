@@ -6,15 +6,18 @@ Bars (from the depth/Whitted parity statement of the workload):
   Whitted RGB within 1/255 on >= 99.9% of pixels;
 and, for the reference's own fixtures, the reference test's tolerance against its PNGs
 (max |d| <= 8, mean |d| <= 0.35, render_regression.rs:138-145).
+
+Scenes run both on the node-program interpreter (FT_EMIT=0) and as scene-specialised
+runtime-compiled code (FT_EMIT=1); both are held to the same bars.
 """
 import os
 
 import numpy as np
 import pytest
 
-from forgedthoughts_amd import synthetic
+from forgedthoughts_amd import abi, synthetic
 from forgedthoughts_amd.options import AccelMode, RaySettings, RenderOptions
-from forgedthoughts_amd.render import GpuScene
+from forgedthoughts_amd.render import GpuScene, emit_scene_source
 from oracle import ftref
 
 from helpers import REGRESSION, REGRESSION_CASES, compile_ft, compile_src, diff_metrics, read_png
@@ -25,11 +28,33 @@ T_TOL = 1.0e-4
 GRAZING_FRAC = 1.0e-4
 
 
-def depth_parity(scene, accel, opts, aa=1, label=""):
-    print(f"[depth parity] {label} accel={int(accel)} {opts.width}x{opts.height} aa={aa}", flush=True)
+EMIT_MODES = ["0", "1"]
+
+
+@pytest.fixture(params=EMIT_MODES, ids=["interp", "emitted"])
+def emit(request, monkeypatch):
+    monkeypatch.setenv("FT_EMIT", request.param)
+    return request.param
+
+
+def check_path(gs, emit):
+    """The scene ran on the path the test asked for."""
+    flags = gs.stats().flags
+    if emit == "0":
+        assert flags == 0, f"interpreter requested, stats flags {flags}"
+    elif emit == "1":
+        src = emit_scene_source(gs.scene, gs.accel)
+        want = (abi.STATS_EMITTED_SCENE if "#define FT_EMITTED_SCENE 1" in src else 0) | \
+               (abi.STATS_EMITTED_FUNCTIONS if "#define FT_EMITTED_FUNCTIONS 1" in src else 0)
+        assert flags == want, f"specialised code requested: flags {flags}, expected {want}"
+
+
+def depth_parity(scene, accel, opts, aa=1, label="", emit=None):
+    print(f"[depth parity] {label} accel={int(accel)} {opts.width}x{opts.height} aa={aa} emit={emit}", flush=True)
     rgb_c, t_c = ftref.render_depth(scene, accel, opts, aa)
     with GpuScene(scene, accel) as gs:
         rgb_g, t_g = gs.render_depth(opts, aa)
+        check_path(gs, emit)
     hit_c, hit_g = t_c >= 0, t_g >= 0
     mism = int((hit_c != hit_g).sum())
     n = hit_c.size
@@ -44,12 +69,13 @@ def depth_parity(scene, accel, opts, aa=1, label=""):
     return rgb_g, t_g, (mism, float(dt.max()) if both.any() else 0.0)
 
 
-def whitted_parity(scene, accel, opts, settings, label=""):
-    print(f"[whitted parity] {label} accel={int(accel)} {opts.width}x{opts.height} spp={settings.aa_samples}",
-          flush=True)
+def whitted_parity(scene, accel, opts, settings, label="", emit=None):
+    print(f"[whitted parity] {label} accel={int(accel)} {opts.width}x{opts.height} spp={settings.aa_samples} "
+          f"emit={emit}", flush=True)
     rgb_c, hdr_c = ftref.render_whitted(scene, accel, opts, settings)
     with GpuScene(scene, accel) as gs:
         rgb_g, hdr_g = gs.render_whitted(opts, settings)
+        check_path(gs, emit)
     d = np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2)
     frac = float((d <= 1).mean())
     assert frac >= 0.999, f"{label}: only {frac:.5f} of pixels within 1/255 (max {d.max()})"
@@ -58,14 +84,14 @@ def whitted_parity(scene, accel, opts, settings, label=""):
 
 @pytest.mark.parametrize("case", REGRESSION_CASES, ids=[c[0] for c in REGRESSION_CASES])
 @pytest.mark.parametrize("accel", [AccelMode.Bvh, AccelMode.Naive])
-def test_regression_fixture_parity(case, accel):
+def test_regression_fixture_parity(case, accel, emit):
     name, kind, opts, settings = case
     scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))
     expected = read_png(os.path.join(REGRESSION, name + ".png"))
     if kind == "preview":
-        rgb, _, _ = depth_parity(scene, accel, opts, 1, name)
+        rgb, _, _ = depth_parity(scene, accel, opts, 1, name, emit)
     else:
-        rgb, _ = whitted_parity(scene, accel, opts, settings, name)
+        rgb, _ = whitted_parity(scene, accel, opts, settings, name, emit)
     if accel == AccelMode.Bvh:
         # the fixtures were rendered with AccelMode::Bvh (render_regression.rs:39-105); the
         # naive accel's lower bound (renderer.rs:1024-1046) shapes soft shadows differently
@@ -73,10 +99,10 @@ def test_regression_fixture_parity(case, accel):
         assert mx <= 8 and mean <= 0.35, f"{name}: vs reference PNG max {mx} mean {mean:.4f}"
 
 
-def test_depth_512_full_size():
+def test_depth_512_full_size(emit):
     w = synthetic.depth_512()
     scene = compile_src(w.source)
-    depth_parity(scene, w.accel, w.options, 1, w.name)
+    depth_parity(scene, w.accel, w.options, 1, w.name, emit)
 
 
 def test_depth_aa_and_bricks_modes():
@@ -87,10 +113,10 @@ def test_depth_aa_and_bricks_modes():
         depth_parity(scene, accel, opts, 4, f"aa4-{accel.name}")
 
 
-def test_whitted_1080p_crop():
+def test_whitted_1080p_crop(emit):
     w = synthetic.whitted_1080p(width=192, height=108, spp=4)
     scene = compile_src(w.source)
-    whitted_parity(scene, w.accel, w.options, w.settings, "whitted_1080p@192x108")
+    whitted_parity(scene, w.accel, w.options, w.settings, "whitted_1080p@192x108", emit)
 
 
 def test_whitted_debug_aovs():
@@ -124,11 +150,11 @@ let scene = floor + a.union_round(b, 0.25) + c;
 """
 
 
-def test_whitted_materials_mix():
+def test_whitted_materials_mix(emit):
     scene = compile_src(GLASS)
     opts = RenderOptions(96, 72, 256, 30.0, 5.0e-4, 0.7, 6.0, 35.0)
     for accel in (AccelMode.Bvh, AccelMode.Naive):
-        whitted_parity(scene, accel, opts, RaySettings(5, 64, 2), f"materials-{accel.name}")
+        whitted_parity(scene, accel, opts, RaySettings(5, 64, 2), f"materials-{accel.name}", emit)
 
 
 def test_error_paths():

@@ -1,0 +1,246 @@
+"""Pins the CPU oracle (oracle/ftref.cpp) before it is trusted as the checker.
+
+What pins what:
+  * the reference's own image fixtures (tests/fixtures/render_regression, copied to
+    tests/golden/render_regression) with the reference test's tolerance
+    (render_regression.rs:138-145) — march, shading, tone mapping end to end;
+  * the reference's known-answer tests: SoftBlob distance/bounds (renderer.rs:9418-9465,
+    lib.rs:1768-1815) and the distance_post hook value (lib.rs:1868-1920);
+  * closed forms: primitive SDFs, hg_sdf identities, the ray/sphere hit distance, the
+    rotation convention of to_local (renderer.rs:6216-6240), Lambert under a point light
+    (lambert.rs:7-13, renderer.rs:4217-4276), the Fresnel and IOR debug AOVs
+    (ray.rs:224-285), the depth and Reinhard byte mappings (renderer.rs:5215-5239);
+  * invariants: identical hit masks over naive/bvh/bricks, thread-count determinism, and
+    the f32 march against the same march in f64.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from forgedthoughts_amd import synthetic
+from forgedthoughts_amd.options import AccelMode, RaySettings, RenderOptions
+from oracle import ftref
+
+from helpers import REGRESSION, REGRESSION_CASES, compile_ft, compile_src, diff_metrics, read_png
+
+F32 = np.float32
+
+
+# ----------------------------------------------------------------- reference fixtures
+@pytest.mark.parametrize("case", REGRESSION_CASES, ids=[c[0] for c in REGRESSION_CASES])
+def test_reference_fixture(case):
+    name, kind, opts, settings = case
+    scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))
+    expected = read_png(os.path.join(REGRESSION, name + ".png"))
+    if kind == "preview":
+        rgb, _ = ftref.render_depth(scene, AccelMode.Bvh, opts, 1)
+        # the depth path reproduces the reference bytes exactly
+        assert diff_metrics(rgb, expected) == (0, 0.0)
+    else:
+        rgb, _ = ftref.render_whitted(scene, AccelMode.Bvh, opts, settings)
+        mx, mean = diff_metrics(rgb, expected)
+        assert mx <= 8 and mean <= 0.35, f"{name}: max {mx} mean {mean:.4f}"
+
+
+# ------------------------------------------------------------------ known answers
+SOFT_BLOB = """
+sdf SoftBlob {
+  let wave_scale = 0.15;
+  fn bounds() { return vec3(1.2, 1.2, 1.0); }
+  fn warp(p) { return vec3(p.x, p.y + sin(p.x * 4.0) * wave_scale, p.z); }
+  fn distance(p) { let q = warp(p); return length(q) - 1.0; }
+};
+let scene = SoftBlob { material: Lambert { color: vec3(0.7, 0.45, 0.3) } };
+"""
+
+
+def test_soft_blob_known_answers():
+    """renderer.rs:9458-9464: centre < -0.9, |surface| < 1e-4, bounds.max.x < 3;
+    lib.rs:1809: distance(0, 1, 0) == 0."""
+    scene = compile_src(SOFT_BLOB)
+    pts = np.array([[0, 0, 0], [0, 1, 0]], np.float32)
+    d, _, _ = ftref.distance_info(scene, AccelMode.Naive, pts)
+    assert d[0] < -0.9
+    assert abs(d[1]) < 1.0e-4
+    assert d[1] == 0.0
+    mn, mx, _ = ftref.scene_bounds(scene)
+    assert mx[0] < 3.0
+    fn = int(scene.nodes["fn_distance"][scene.root])
+    out = ftref.eval_function(scene, fn, np.array([[0, 1, 0]], np.float32))
+    assert out[0, 0] == 0.0
+
+
+TWISTED_SHELL = """
+sdf TwistedShell {
+  let radius = 0.5;
+  let thickness = 0.05;
+  fn domain(p) { return rotate_y(p, p.y * 90.0); }
+  fn distance(p) { return length(p) - radius; }
+  fn distance_post(d, p) { return abs(d) - thickness; }
+};
+let scene = TwistedShell {};
+"""
+
+
+def test_distance_post_known_answer():
+    """lib.rs:1901-1919: distance_post(0.2, (0,0,0)) == 0.15 (in f32)."""
+    scene = compile_src(TWISTED_SHELL)
+    fn = int(scene.nodes["fn_post"][scene.root])
+    out = ftref.eval_function(scene, fn, np.array([[0.2, 0.2, 0.2, 0, 0, 0]], np.float32))
+    assert out[0, 0] == F32(0.15)
+    # domain(1, 1, 0) rotates by 90 degrees about y: (x, z) = (cos, -sin) * 1 -> (0, -1)
+    fd = int(scene.nodes["fn_domain"][scene.root])
+    q = ftref.eval_function(scene, fd, np.array([[1, 1, 0]], np.float32))[0]
+    assert abs(q[0]) < 1e-6 and q[1] == 1.0 and abs(q[2] + 1.0) < 1e-6
+
+
+# ------------------------------------------------------------------ closed forms
+def _points(n, lo, hi, seed):
+    return np.random.default_rng(seed).uniform(lo, hi, size=(n, 3)).astype(np.float32)
+
+
+def test_sphere_and_box_sdf_closed_forms():
+    scene = compile_src("var s = Sphere { radius: 0.75 }; s.pos.x = 1.0; s.pos.y = -0.5;\n"
+                        "let scene = s;")
+    p = _points(2000, -3, 3, 1)
+    d, _, _ = ftref.distance_info(scene, AccelMode.Naive, p)
+    q = p.astype(np.float64) - np.array([1.0, -0.5, 0.0])
+    exact = np.linalg.norm(q, axis=1) - 0.75
+    assert np.abs(d - exact).max() < 2e-6
+    # Box.size is the full size (renderer.rs:1434-1435); IQ box distance
+    scene = compile_src("let scene = Box { size: vec3(1.0, 2.0, 3.0) };")
+    d, _, _ = ftref.distance_info(scene, AccelMode.Naive, p)
+    h = np.array([0.5, 1.0, 1.5])
+    a = np.abs(p.astype(np.float64)) - h
+    exact = np.linalg.norm(np.maximum(a, 0), axis=1) + np.minimum(a.max(axis=1), 0)
+    assert np.abs(d - exact).max() < 2e-6
+
+
+def test_hg_sdf_identities():
+    """Away from the blend band (both distances >= r) the smooth unions are plain min
+    (renderer.rs:6019-6137); Subtract is max(a, -b)."""
+    base = ("var a = Sphere { radius: 1.0 }; a.pos.x = -2.0;\n"
+            "var b = Box { size: vec3(1.0, 1.0, 1.0) }; b.pos.x = 2.0;\n")
+    p = _points(4000, -5, 5, 2)
+    da, _, _ = ftref.distance_info(compile_src(base + "let scene = a;"), AccelMode.Naive, p)
+    db, _, _ = ftref.distance_info(compile_src(base + "let scene = b;"), AccelMode.Naive, p)
+    far = (da >= 0.3) & (db >= 0.3)
+    for op in ("union_round", "union_chamfer", "union_soft"):
+        d, _, _ = ftref.distance_info(compile_src(base + f"let scene = a.{op}(b, 0.3);"), AccelMode.Naive, p)
+        np.testing.assert_array_equal(d[far & (np.abs(da - db) >= 0.3 * math.sqrt(2))],
+                                      np.minimum(da, db)[far & (np.abs(da - db) >= 0.3 * math.sqrt(2))])
+    d, _, _ = ftref.distance_info(compile_src(base + "let scene = a + b;"), AccelMode.Naive, p)
+    np.testing.assert_array_equal(d, np.minimum(da, db))
+    d, _, _ = ftref.distance_info(compile_src(base + "let scene = a - b;"), AccelMode.Naive, p)
+    np.testing.assert_array_equal(d, np.maximum(da, -db))
+    d, _, _ = ftref.distance_info(compile_src(base + "let scene = a & b;"), AccelMode.Naive, p)
+    np.testing.assert_array_equal(d, np.maximum(da, db))
+
+
+def test_rotation_convention():
+    """to_local applies Rz(-z), Ry(-y), Rx(-x) (renderer.rs:6216-6240), so a shape with
+    rot.y = 30 has the local +x edge centre of a triangle prism at (cos30, 0, -sin30)."""
+    scene = compile_src("var t = ExtrudePolygon { sides: 3, radius: 1.0, height: 10.0 }; t.rot.y = 30.0;\n"
+                        "let scene = t;")
+    s = 2.0
+    ang = math.radians(30.0)
+    p = np.array([[s * math.cos(ang), 0.0, -s * math.sin(ang)],
+                  [s * math.cos(ang), 0.0, s * math.sin(ang)]], np.float32)
+    d, _, _ = ftref.distance_info(scene, AccelMode.Naive, p)
+    apothem = 1.0 * math.cos(math.pi / 3)
+    assert abs(d[0] - (s - apothem)) < 1e-5             # edge centre
+    assert abs(d[1] - (s * math.cos(math.radians(60)) - apothem)) < 1e-5  # toward a vertex
+    # rot.x = 90 swaps the y and z extents of a box
+    a = compile_src("var b = Box { size: vec3(1.0, 2.0, 3.0) }; b.rot.x = 90.0; let scene = b;")
+    b = compile_src("let scene = Box { size: vec3(1.0, 3.0, 2.0) };")
+    pts = _points(500, -3, 3, 3)
+    da, _, _ = ftref.distance_info(a, AccelMode.Naive, pts)
+    db, _, _ = ftref.distance_info(b, AccelMode.Naive, pts)
+    assert np.abs(da - db).max() < 1e-5
+
+
+def _axis_camera(extra, width=9, height=9, fov=20.0):
+    return ("let camera = Camera { origin: vec3(0.0, 0.0, 5.0), target: vec3(0.0, 0.0, 0.0), "
+            f"fov_y: {fov} }};\n" + extra)
+
+
+def test_ray_sphere_hit_distance():
+    """Centre pixel of an odd-sized image looks down the axis: t = 5 - r - z_c."""
+    for r, zc in ((1.0, 0.0), (0.5, 1.25), (2.0, -1.0)):
+        src = _axis_camera(f"var s = Sphere {{ radius: {r} }}; s.pos.z = {zc}; let scene = s;")
+        scene = compile_src(src)
+        opts = RenderOptions(9, 9, 256, 40.0, 1.0e-4, 0.7, 6.0, 20.0)
+        rgb, t = ftref.render_depth(scene, AccelMode.Naive, opts, 1)
+        exact = 5.0 - r - zc
+        assert abs(t[4, 4] - exact) < 1e-4
+        # depth byte (renderer.rs:5215-5227): trunc(255 * min(((1 - clamp(t/far))^2)^(1/2.2), 1))
+        v = (1.0 - min(max(t[4, 4] / 40.0, 0.0), 1.0)) ** 2
+        assert rgb[4, 4, 0] == int(255.0 * min(v ** (1 / 2.2), 1.0))
+
+
+def test_lambert_point_light_closed_form():
+    """A Lambert wall facing the camera lit by a point light on the axis: radiance =
+    color/pi * intensity / d^2 (lambert.rs:7-13, 1/d^2 falloff render_api.rs:171-289),
+    unshadowed; then Reinhard + gamma for the byte (renderer.rs:5229-5239)."""
+    src = _axis_camera("let key = PointLight { position: vec3(0.0, 0.0, 3.0), color: #ffffff, intensity: 9.0 };\n"
+                       "let wall_mat = Lambert { color: #808080 };\n"
+                       "var wall = Box { size: vec3(20.0, 20.0, 1.0), material: wall_mat }; wall.pos.z = -0.5;\n"
+                       "let scene = wall;")
+    scene = compile_src(src)
+    opts = RenderOptions(9, 9, 256, 40.0, 1.0e-4, 0.7, 6.0, 20.0)
+    rgb, hdr = ftref.render_whitted(scene, AccelMode.Bvh, opts, RaySettings(1, 9, 1))
+    expect = (128.0 / 255.0) / math.pi * 9.0 / 9.0
+    assert abs(hdr[4, 4, 0] - expect) / expect < 2e-4
+    x = float(hdr[4, 4, 0])
+    assert rgb[4, 4, 0] == int(255.0 * min((x / (1.0 + x)) ** (1 / 2.2), 1.0))
+
+
+def test_fresnel_and_ior_aovs():
+    """Debug AOVs at normal incidence on a dielectric (ray.rs:252-277):
+    fresnel = ((n-1)/(n+1))^2, ior AOV = (n-1)/2."""
+    src = _axis_camera("let glass = Dielectric { ior: 1.5 };\n"
+                       "var wall = Box { size: vec3(20.0, 20.0, 1.0), material: glass }; wall.pos.z = -0.5;\n"
+                       "let scene = wall;")
+    scene = compile_src(src)
+    opts = RenderOptions(9, 9, 256, 40.0, 1.0e-4, 0.7, 6.0, 20.0)
+    _, hdr = ftref.render_whitted(scene, AccelMode.Bvh, opts, RaySettings(2, 9, 1, "fresnel"))
+    assert abs(hdr[4, 4, 0] - 0.04) < 1e-5
+    _, hdr = ftref.render_whitted(scene, AccelMode.Bvh, opts, RaySettings(2, 9, 1, "ior"))
+    assert abs(hdr[4, 4, 0] - 0.25) < 1e-6
+
+
+# ------------------------------------------------------------------- invariants
+def test_accel_modes_agree_on_hits():
+    w = synthetic.depth_512()
+    scene = compile_src(w.source)
+    opts = RenderOptions(**{**w.options.__dict__, "width": 96, "height": 96})
+    ts = [ftref.render_depth(scene, a, opts, 1)[1] for a in (AccelMode.Naive, AccelMode.Bvh, AccelMode.Bricks)]
+    for t in ts[1:]:
+        np.testing.assert_array_equal(t >= 0, ts[0] >= 0)
+        both = (t >= 0) & (ts[0] >= 0)
+        assert np.abs(t[both] - ts[0][both]).max() <= 1e-4
+
+
+def test_thread_count_determinism():
+    w = synthetic.whitted_1080p(width=64, height=36, spp=1)
+    scene = compile_src(w.source)
+    a, ha = ftref.render_whitted(scene, w.accel, w.options, w.settings, threads=1)
+    b, hb = ftref.render_whitted(scene, w.accel, w.options, w.settings, threads=8)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(ha, hb)
+
+
+def test_f32_march_against_f64():
+    """The error of the f32 march (the reference's arithmetic) against the same algorithm
+    in f64 on the 512x512 depth scene at 128x128: hit masks agree except grazing pixels,
+    and hit distances agree to the march tolerance."""
+    w = synthetic.depth_512()
+    scene = compile_src(w.source)
+    opts = RenderOptions(**{**w.options.__dict__, "width": 128, "height": 128})
+    _, t32 = ftref.render_depth(scene, w.accel, opts, 1, precision=0)
+    _, t64 = ftref.render_depth(scene, w.accel, opts, 1, precision=1)
+    assert int(((t32 >= 0) != (t64 >= 0)).sum()) <= 2
+    both = (t32 >= 0) & (t64 >= 0)
+    assert np.abs(t32[both] - t64[both]).max() < 2 * w.options.epsilon

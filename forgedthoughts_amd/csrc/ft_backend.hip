@@ -239,6 +239,9 @@ int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings*
   if (rc) return rc;
   if (!rs) return fail(FT_ERR_INVALID_ARGUMENT, "ray settings are null");
   if (rs->debug_aov < -1 || rs->debug_aov > FT_AOV_HIT_T) return fail(FT_ERR_INVALID_ARGUMENT, "unknown debug AOV");
+  if (s->host.dynamic_material >= 0)
+    return fail(FT_ERR_UNSUPPORTED_OBJECT, "material " + std::to_string(s->host.dynamic_material) +
+                                               " has per-hit hook functions, which this backend does not evaluate");
   if ((rc = clear_flops(s))) return rc;
   CamArgs cam = make_cam(s, *o, rs->aa_samples);
   if (cam.aa > 65535) return fail(FT_ERR_INVALID_ARGUMENT, "too many AA samples");

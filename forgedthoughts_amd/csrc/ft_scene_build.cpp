@@ -449,6 +449,7 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     p.medium_density = a.medium_density; p.pattern = (float)a.pattern; p.pattern_scale = a.pattern_scale;
     p.model = (float)a.model;
     S.mats.push_back(p);
+    if (a.dynamic_material_id >= 0 && S.dynamic_material < 0) S.dynamic_material = (int)m;
     bool transmissive = a.model == FT_MODEL_DIELECTRIC || (a.model == FT_MODEL_STANDARD && a.transmission > 0.0f);
     any_transmissive = any_transmissive || transmissive;
     if (transmissive && a.roughness > 0.06f) any_rough_transmissive = true;

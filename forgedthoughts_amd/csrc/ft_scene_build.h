@@ -33,6 +33,7 @@ struct HostScene {
   int max_fn_exprs = 0;
   int max_children = 2;     // rays a single hit can spawn (both split materials together)
   int max_bvh_depth = 0;
+  int dynamic_material = -1;  // first material with per-hit hook functions, or -1
 };
 
 // Validates `ir` and builds the device layout.  Returns false with `err` set on failure.

@@ -33,6 +33,12 @@ from . import abi
 from .forge import evaluator as E
 from .forge.evaluator import Obj, Func, is_num, f32
 
+# material-definition functions the reference evaluates per hit (resolve_leaf_material
+# renderer.rs:7164-7536, eval_bsdf hooks renderer.rs:4952-5030)
+MATERIAL_HOOKS = ("color", "roughness", "metallic", "specular", "ior", "thin_walled", "medium", "subsurface",
+                  "emission", "emission_color", "emission_strength", "bump", "normal", "eval", "pdf", "sample",
+                  "transmission", "clearcoat", "clearcoat_roughness", "specular_color")
+
 _libm = ctypes.CDLL("libm.so.6")
 _libm.sinf.restype = ctypes.c_float
 _libm.sinf.argtypes = [ctypes.c_float]
@@ -282,6 +288,7 @@ class SceneCompiler:
         self.next_object_id = 1
         self.functions: List[Tuple[list, int, int]] = []  # (nodes, result_type, n_params)
         self.dynamic_overrides: List[dict] = []
+        self.dynamic_hooks: List[str] = []
         self.sorted_material_names = sorted(state.material_defs.keys())
 
     # -------------------------------------------------------- materials
@@ -382,8 +389,14 @@ class SceneCompiler:
                 m.emission_strength = max(value, f32(0.0))
             elif pname == "medium":
                 m.medium = _medium_from_value(value)
-        m.dynamic_material_id = self.sorted_material_names.index(name)
-        if with_ctx and overrides.fields:
+        # per-hit hooks (functions of the shading context, resolve_leaf_material
+        # renderer.rs:7164-7536); a material without them is fully static
+        hooks = [st[1] for st in d.statements if st[0] == "fn" and
+                 (st[1] in MATERIAL_HOOKS or (len(st[2]) >= 1 and st[2][0] == "ctx"))]
+        if hooks:
+            m.dynamic_material_id = self.sorted_material_names.index(name)
+            self.dynamic_hooks.extend(f"{name}.{h}" for h in hooks if f"{name}.{h}" not in self.dynamic_hooks)
+        if hooks and with_ctx and overrides.fields:
             key = repr(sorted((k, repr(v)) for k, v in overrides.fields.items()))
             if key not in self.dynamic_overrides:
                 self.dynamic_overrides.append(key)
@@ -819,6 +832,7 @@ class CompiledScene:
     camera: dict
     root: int
     leaf_count: int = 0
+    dynamic_hooks: list = field(default_factory=list)  # "Material.hook" evaluated per hit
     _ir: object = None
 
     def ir(self) -> abi.SceneIR:
@@ -964,7 +978,8 @@ def compile_scene(state: E.EvalState) -> CompiledScene:
                "fov": read_number_field(co, ["fov_y", "fov"])}
 
     leaves = int(np.sum(node_arr["kind"] <= abi.NODE["CUSTOM"]))
-    return CompiledScene(node_arr, mats, ot, light_arr, fn_arr, expr_arr, env_fn, cam, root, leaves)
+    return CompiledScene(node_arr, mats, ot, light_arr, fn_arr, expr_arr, env_fn, cam, root, leaves,
+                         dynamic_hooks=list(comp.dynamic_hooks))
 
 
 # ---------------------------------------------------------- render settings

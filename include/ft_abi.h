@@ -235,7 +235,9 @@ typedef struct ft_material {
   float medium_absorption[3];
   float pattern_scale;
   float pattern_color_a[3];
-  int32_t dynamic_material_id;   /* index into the sorted material-definition names, or -1 */
+  int32_t dynamic_material_id;   /* index into the sorted material-definition names when the
+                                    definition has per-hit hook functions, else -1; Whitted
+                                    renders of such scenes fail with FT_ERR_UNSUPPORTED_OBJECT */
   float pattern_color_b[3];
   int32_t dynamic_override_id;   /* interned override object of a dynamic material, or -1 */
 } ft_material;

@@ -9,7 +9,7 @@ from forgedthoughts_amd.forge.evaluator import ForgeEvalError, load_scene_source
 from forgedthoughts_amd.forge.syntax import ForgeSyntaxError, parse_program
 from forgedthoughts_amd.scene import RenderError, compile_scene, extract_scene_render_settings
 
-from helpers import REGRESSION, REGRESSION_CASES, compile_ft
+from helpers import REGRESSION, REGRESSION_CASES, compile_ft, compile_src
 
 
 @pytest.mark.parametrize("name", [c[0] for c in REGRESSION_CASES])
@@ -94,3 +94,14 @@ def test_repeat_and_mirror():
     prims = cs.nodes[cs.nodes["kind"] == abi.NODE["SPHERE"]]
     xs = sorted(set(float(x) for x in prims["center"][:, 0]))
     assert xs == pytest.approx([-1.0, 0.0, 1.0])
+
+
+def test_per_hit_material_hooks_are_recorded():
+    """Only material definitions with per-hit hook functions are marked dynamic
+    (resolve_leaf_material renderer.rs:7164-7536); helper functions are not hooks."""
+    s = compile_ft(os.path.join(REGRESSION, "preview_material_hook.ft"))
+    assert s.dynamic_hooks == ["StripeLambert.color"]
+    assert sorted(s.materials["dynamic_material_id"].tolist()) == [-1, 0]
+    s = compile_src("material Plain { model: Lambert; let c = #ff0000; color = c; };\n"
+                    "let scene = Sphere { radius: 1.0, material: Plain {} };")
+    assert s.dynamic_hooks == [] and (s.materials["dynamic_material_id"] == -1).all()

@@ -168,3 +168,16 @@ def test_error_paths():
     bad._ir = None
     with pytest.raises(RenderError):
         GpuScene(bad)
+
+
+def test_per_hit_material_hooks_fail_loudly():
+    """Whitted renders of scenes whose materials evaluate per-hit hooks are refused (the
+    backend does not run them); the depth preview of the same scene is unaffected."""
+    from forgedthoughts_amd.scene import RenderError
+    name, _, opts, _ = REGRESSION_CASES[3]
+    scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))
+    with GpuScene(scene, AccelMode.Bvh) as gs:
+        gs.render_depth(opts, 1)
+        with pytest.raises(RenderError) as e:
+            gs.render_whitted(opts, RaySettings(2, 96, 1))
+        assert e.value.kind == "UnsupportedObjectType"

@@ -45,6 +45,107 @@ def cpu_baseline(workload, scene, seconds: float):
                       f"reference renderer, {threads} threads"}
 
 
+def cpu_baseline_batch(w, scenes, seconds: float):
+    """Oracle time per scene on a bounded sample: a few views spread over the primitive-count
+    range, a few rows of each (depth + Whitted), scaled to whole scenes."""
+    import numpy as np
+
+    from oracle import ftref
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import with_camera
+    threads = os.cpu_count() or 1
+    opts, rs = w.options, w.settings
+    by_cost = sorted(range(len(w.views)), key=lambda v: w.counts[w.views[v][0]])
+    picks = [by_cost[int((k + 0.5) * len(by_cost) / 8)] for k in range(8)]
+    rows_per_view = 2
+    spent, per_view = 0.0, []
+    for v in picks:
+        sc, o, t, fov = w.views[v]
+        ref = with_camera(scenes[sc], o, t, fov)
+        step = max(1, opts.height // rows_per_view)
+        t0 = time.perf_counter()
+        for r in range(step // 2, opts.height, step):
+            ftref.render_depth(ref, w.accel, opts, 1, threads=threads, rows=(r, r + 1))
+            ftref.render_whitted(ref, w.accel, opts, rs, threads=threads, rows=(r, r + 1))
+        dt = time.perf_counter() - t0
+        spent += dt
+        per_view.append(dt / rows_per_view * opts.height)
+        if spent > seconds:
+            break
+    view_s = float(np.mean(per_view))
+    per_scene = view_s * len(w.views) / len(scenes)
+    return {"value": 1.0 / per_scene, "unit": "scenes/s", "cores": threads, "kind": "port",
+            "sample": f"{len(per_view)} views spread over the primitive-count range, {rows_per_view} rows each "
+                      f"(depth + Whitted), {spent:.1f} s, scaled to whole views; C++ restatement of the "
+                      f"reference renderer, {threads} threads"}
+
+
+def bench_batch(args, w):
+    import torch
+
+    from forgedthoughts_amd import abi
+    from forgedthoughts_amd.forge.evaluator import load_scene_source
+    from forgedthoughts_amd.render import GpuBatch
+    from forgedthoughts_amd.scene import compile_scene
+
+    t0 = time.perf_counter()
+    scenes = [compile_scene(load_scene_source(src)) for src in w.sources]
+    print(f"[bench] {len(scenes)} scenes compiled in {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    opts, rs = w.options, w.settings
+    n = len(w.views)
+    per = opts.width * opts.height * 3
+    dev = torch.device("cuda:0")
+    dd = torch.empty(n * per, dtype=torch.uint8, device=dev)
+    dw = torch.empty(n * per, dtype=torch.uint8, device=dev)
+    gb = GpuBatch(scenes, w.accel)
+    views = gb.views(w.views)
+    both = abi.BATCH_DEPTH | abi.BATCH_WHITTED
+
+    def run(flags, steps):
+        for _ in range(args.warmup):
+            gb.render_device(views, opts, rs, flags, dd.data_ptr(), dw.data_ptr())
+            gb.synchronize()
+        ms, t1 = 0.0, time.perf_counter()
+        for _ in range(steps):
+            gb.render_device(views, opts, rs, flags, dd.data_ptr(), dw.data_ptr())
+            gb.synchronize()
+            ms += gb.stats().last_kernel_ms
+        return (time.perf_counter() - t1) / steps, ms / steps
+
+    wall_b, kms_b = run(both | abi.BATCH_GRAPH, args.steps)
+    print(f"[bench] batched: {wall_b * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+    wall_s, kms_s = run(both | abi.BATCH_SEPARATE, max(1, args.steps // 2))
+    print(f"[bench] separate: {wall_s * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
+    gb.set_flop_counting(True)
+    gb.render_device(views, opts, rs, both, dd.data_ptr(), dw.data_ptr())
+    gb.synchronize()
+    flops = gb.stats().sdf_flops
+    gb.set_flop_counting(False)
+    achieved = flops / (kms_b / 1e3) / 1e12
+    line = {
+        "metric": "batched scenes/s",
+        "value": round(len(scenes) / wall_b, 4),
+        "unit": "scenes/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(wall_b * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w.name, "scenes": len(scenes), "views": n, "width": opts.width,
+                   "height": opts.height, "prims": [min(w.counts), max(w.counts), sum(w.counts)],
+                   "max_depth": rs.max_depth, "images": "depth + whitted per view", "launch": "batched, hip graph"},
+        "separate_launches": {"value": round(len(scenes) / wall_s, 4), "unit": "scenes/s",
+                              "ms_per_step": round(wall_s * 1e3, 3), "kernel_launches": 1 + 3 * n,
+                              "speedup_of_batched": round(wall_s / wall_b, 3)},
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2, "kernel_ms": round(kms_b, 3),
+                     "sdf_flops_per_launch": int(flops)},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_batch(w, scenes, args.cpu_seconds)
+    gb.close()
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +168,8 @@ def main():
     from forgedthoughts_amd.scene import compile_scene
 
     w = synthetic.WORKLOADS[args.workload]()
+    if isinstance(w, synthetic.BatchWorkload):
+        return bench_batch(args, w)
     scene = compile_scene(load_scene_source(w.source))
     opts = w.options
     dev = torch.device("cuda:0")

@@ -106,10 +106,18 @@ class RenderStatsC(C.Structure):
 STATS_EMITTED_SCENE, STATS_EMITTED_FUNCTIONS = 1, 2
 
 
+class View(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("fov_y_degrees", C.c_float), ("origin", C.c_float * 3),
+                ("target", C.c_float * 3)]
+
+
+BATCH_DEPTH, BATCH_WHITTED, BATCH_SEPARATE, BATCH_GRAPH = 1, 2, 4, 8
+
 STRUCT_SIZES = {
     "ft_node": NODE_DTYPE.itemsize, "ft_expr": EXPR_DTYPE.itemsize,
     "ft_function": FUNCTION_DTYPE.itemsize, "ft_material": MATERIAL_DTYPE.itemsize,
     "ft_light": LIGHT_DTYPE.itemsize, "ft_camera": C.sizeof(Camera),
     "ft_scene_ir": C.sizeof(SceneIR), "ft_render_options": C.sizeof(RenderOptionsC),
     "ft_ray_settings": C.sizeof(RaySettingsC), "ft_render_stats": C.sizeof(RenderStatsC),
+    "ft_view": C.sizeof(View),
 }

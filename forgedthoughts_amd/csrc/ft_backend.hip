@@ -303,6 +303,8 @@ int finish(ft_scene* s) {
 
 }  // namespace
 
+static int create_scene(const ft_scene_ir* ir, uint32_t accel_mode, bool batch_member, ft_scene** out);
+
 extern "C" {
 
 const char* ft_last_error(void) { return g_err.c_str(); }
@@ -316,6 +318,14 @@ int ft_device_count(int* count) {
 }
 
 int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) {
+  return create_scene(ir, accel_mode, false, out);
+}
+
+}  // extern "C"
+
+// Builds and uploads one scene.  Members of a batch share the batch's stream and always
+// run on the interpreter.
+static int create_scene(const ft_scene_ir* ir, uint32_t accel_mode, bool batch_member, ft_scene** out) {
   if (!out) return fail(FT_ERR_INVALID_ARGUMENT, "output handle is null");
   *out = nullptr;
   int ndev = 0;
@@ -329,8 +339,8 @@ int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) 
   int rc = FT_OK;
   auto cleanup = [&]() { ft_scene_destroy(s); };
   if (hipGetDevice(&s->device) != hipSuccess) { cleanup(); return fail(FT_ERR_GPU_UNAVAILABLE, "hipGetDevice failed"); }
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+  if (!batch_member && (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+                        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)) {
     cleanup();
     return fail(FT_ERR_GPU_UNAVAILABLE, "stream/event creation failed");
   }
@@ -376,7 +386,7 @@ int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) 
     bool emit = false;
     if (m == "1" || m == "always") emit = s->plan.any();
     else if (m != "0" && m != "never") emit = s->plan.any() && fth::emission_pays(h, ir, s->plan);
-    if (emit) s->source = fth::emit_scene_source(h, ir, s->plan);
+    if (emit && !batch_member) s->source = fth::emit_scene_source(h, ir, s->plan);
   }
   s->slot = acquire_slot();
   if (s->slot < 0) {
@@ -391,6 +401,8 @@ int ft_scene_create(const ft_scene_ir* ir, uint32_t accel_mode, ft_scene** out) 
   *out = s;
   return FT_OK;
 }
+
+extern "C" {
 
 int ft_scene_destroy(ft_scene* s) {
   if (!s) return FT_OK;
@@ -528,7 +540,363 @@ int ft_abi_struct_size(const char* name) {
   if (n == "ft_render_options") return (int)sizeof(ft_render_options);
   if (n == "ft_ray_settings") return (int)sizeof(ft_ray_settings);
   if (n == "ft_render_stats") return (int)sizeof(ft_render_stats);
+  if (n == "ft_view") return (int)sizeof(ft_view);
   return -1;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------- batches
+struct ft_batch {
+  std::vector<ft_scene*> scenes;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  DevBuf order, slots, cams, samples, overflow, flops;
+  bool counting = false;
+  std::vector<int> h_order, h_slots;
+  std::vector<CamArgs> h_cams;
+  int stack_depth = 1;
+  long max_children = 2;
+  ft_render_stats stats{};
+  bool timed = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t graph_key = 0;
+};
+
+namespace {
+
+uint64_t mix_bytes(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = (const unsigned char*)p;
+  for (size_t k = 0; k < n; ++k) {
+    h ^= b[k];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+void drop_graph(ft_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key = 0;
+}
+
+int ray_stack_kernel(long need) {
+  if (need <= 16) return 16;
+  if (need <= 48) return 48;
+  if (need <= 128) return 128;
+  return 0;
+}
+
+// Enqueues the launches of one batch render on the batch's stream (captured or not).
+int batch_enqueue(ft_batch* b, uint32_t n_views, const TraceParams& tp, int aov, const CamArgs& c0, uint32_t flags,
+                  unsigned char* depth_rgb, unsigned char* whitted_rgb, int& launches) {
+  const int tpv = ((c0.width + 7) / 8) * ((c0.height + 7) / 8);
+  const size_t npix = (size_t)c0.width * c0.height;
+  const size_t lds = (size_t)3 * b->stack_depth * 256 * sizeof(float);
+  float* samples = (float*)b->samples.ptr;
+  int* ovf = (int*)b->overflow.ptr;
+  launches = 0;
+  if (flags & FT_BATCH_WHITTED) {
+    hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, b->stream, ovf);
+    ++launches;
+  }
+  if (flags & FT_BATCH_SEPARATE) {
+    // the single-scene path, once per view
+    for (uint32_t r = 0; r < n_views; ++r) {
+      const int v = b->h_order[r];
+      ft_scene* s = b->scenes[b->h_slots[v]];
+      const CamArgs cam = b->h_cams[v];
+      const size_t slds = stack_lds(s);
+      if (flags & FT_BATCH_DEPTH) {
+        hipLaunchKernelGGL(ft_k_depth, tile_grid(cam.width, cam.height), dim3(64, 4), slds, b->stream, s->slot, cam, tp,
+                           depth_rgb + (size_t)v * npix * 3, (float*)nullptr);
+        ++launches;
+      }
+      if (flags & FT_BATCH_WHITTED) {
+        const long need = 1 + (long)(tp.max_depth - 1) * (long)(s->host.max_children - 1);
+        dim3 grid = tile_grid(cam.width, cam.height);
+        grid.y = (unsigned)cam.aa;
+        float* vs = samples + (size_t)v * cam.aa * npix * 3;
+        switch (ray_stack_kernel(need)) {
+          case 16: hipLaunchKernelGGL(ft_k_whitted_16, grid, dim3(64, 4), slds, b->stream, s->slot, cam, tp, aov, vs, ovf); break;
+          case 48: hipLaunchKernelGGL(ft_k_whitted_48, grid, dim3(64, 4), slds, b->stream, s->slot, cam, tp, aov, vs, ovf); break;
+          default: hipLaunchKernelGGL(ft_k_whitted_128, grid, dim3(64, 4), slds, b->stream, s->slot, cam, tp, aov, vs, ovf); break;
+        }
+        hipLaunchKernelGGL(ft_k_resolve, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, b->stream, cam, aov, vs,
+                           whitted_rgb + (size_t)v * npix * 3, (float*)nullptr);
+        launches += 2;
+      }
+    }
+    FT_HIP(hipGetLastError());
+    return FT_OK;
+  }
+  BatchArgs ba;
+  ba.order = (const int*)b->order.ptr;
+  ba.slots = (const int*)b->slots.ptr;
+  ba.cams = (const CamArgs*)b->cams.ptr;
+  ba.n_views = (int)n_views;
+  ba.tiles_per_view = tpv;
+  const unsigned waves = (unsigned)((size_t)n_views * tpv);
+  const dim3 grid((waves + 3) / 4);
+  if (flags & FT_BATCH_DEPTH) {
+    hipLaunchKernelGGL(ft_k_batch_depth, grid, dim3(64, 4), lds, b->stream, ba, tp, depth_rgb);
+    ++launches;
+  }
+  if (flags & FT_BATCH_WHITTED) {
+    dim3 wg = grid;
+    wg.y = (unsigned)c0.aa;
+    switch (ray_stack_kernel(1 + (long)(tp.max_depth - 1) * (b->max_children - 1))) {
+      case 16: hipLaunchKernelGGL(ft_k_batch_whitted_16, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf); break;
+      case 48: hipLaunchKernelGGL(ft_k_batch_whitted_48, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf); break;
+      default: hipLaunchKernelGGL(ft_k_batch_whitted_128, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf); break;
+    }
+    hipLaunchKernelGGL(ft_k_resolve, dim3((unsigned)((npix + 255) / 256), n_views), dim3(256), 0, b->stream, c0, aov,
+                       samples, whitted_rgb, (float*)nullptr);
+    launches += 2;
+  }
+  FT_HIP(hipGetLastError());
+  return FT_OK;
+}
+
+int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const ft_render_options* o,
+                      const ft_ray_settings* rs, uint32_t flags, unsigned char* depth_rgb, unsigned char* whitted_rgb) {
+  int rc = check_options(o);
+  if (rc) return rc;
+  if (!views || n_views == 0) return fail(FT_ERR_INVALID_ARGUMENT, "no views");
+  if (!(flags & (FT_BATCH_DEPTH | FT_BATCH_WHITTED))) return fail(FT_ERR_INVALID_ARGUMENT, "nothing to render");
+  if ((flags & FT_BATCH_DEPTH) && !depth_rgb) return fail(FT_ERR_INVALID_ARGUMENT, "depth output is null");
+  if ((flags & FT_BATCH_WHITTED) && !whitted_rgb) return fail(FT_ERR_INVALID_ARGUMENT, "whitted output is null");
+  if ((flags & FT_BATCH_WHITTED) && !rs) return fail(FT_ERR_INVALID_ARGUMENT, "ray settings are null");
+  const int aov = rs ? rs->debug_aov : -1;
+  if (aov < -1 || aov > FT_AOV_HIT_T) return fail(FT_ERR_INVALID_ARGUMENT, "unknown debug AOV");
+  const uint32_t aa = (flags & FT_BATCH_WHITTED) ? rs->aa_samples : 1;
+  const int tpv = (((int)o->width + 7) / 8) * (((int)o->height + 7) / 8);
+  if ((uint64_t)n_views * tpv > (1ull << 31) / 64) return fail(FT_ERR_INVALID_ARGUMENT, "batch too large");
+  TraceParams tp = make_tp(*o, rs ? rs->max_depth : 1);
+  if ((flags & FT_BATCH_WHITTED) && !ray_stack_kernel(1 + (long)(tp.max_depth - 1) * (b->max_children - 1)))
+    return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with these scenes' materials");
+  // per-view cameras, slots and the cost order
+  std::vector<CamArgs> cams(n_views);
+  std::vector<int> slots(n_views), order(n_views);
+  std::vector<float> cost(n_views);
+  for (uint32_t v = 0; v < n_views; ++v) {
+    if (views[v].scene >= b->scenes.size()) return fail(FT_ERR_INVALID_ARGUMENT, "view names an unknown scene");
+    ft_scene* s = b->scenes[views[v].scene];
+    if ((flags & FT_BATCH_WHITTED) && s->host.dynamic_material >= 0)
+      return fail(FT_ERR_UNSUPPORTED_OBJECT, "a scene has materials with per-hit hook functions");
+    fth::CamBasis cb = fth::camera_basis(views[v].origin, views[v].target, views[v].fov_y_degrees);
+    CamArgs& c = cams[v];
+    c.ox = cb.origin[0]; c.oy = cb.origin[1]; c.oz = cb.origin[2];
+    c.fx = cb.forward[0]; c.fy = cb.forward[1]; c.fz = cb.forward[2];
+    c.rx = cb.right[0]; c.ry = cb.right[1]; c.rz = cb.right[2];
+    c.ux = cb.up[0]; c.uy = cb.up[1]; c.uz = cb.up[2];
+    c.tan_fov = cb.tan_fov;
+    c.aspect = (float)o->width / (float)o->height;
+    c.width = (int)o->width;
+    c.height = (int)o->height;
+    c.aa = (int)(aa < 1 ? 1 : aa);
+    c.grid = 1;
+    while (c.grid * c.grid < c.aa) ++c.grid;
+    slots[v] = (int)views[v].scene;  // host side: scene index; device side: constant slot
+    order[v] = (int)v;
+    cost[v] = s->host.nodes[s->host.root].cost;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return cost[a] > cost[c]; });
+  FT_HIP(hipSetDevice(b->device));
+  const size_t npix = (size_t)o->width * o->height;
+  if (flags & FT_BATCH_WHITTED) {
+    if ((rc = ensure(b->samples, (size_t)n_views * cams[0].aa * npix * 3 * sizeof(float)))) return rc;
+    if ((rc = ensure(b->overflow, sizeof(int)))) return rc;
+  }
+  // device tables (uploaded when they change; a captured graph reads them in place)
+  std::vector<int> dev_slots(n_views);
+  for (uint32_t v = 0; v < n_views; ++v) dev_slots[v] = b->scenes[slots[v]]->slot;
+  const bool tables_changed = order != b->h_order || slots != b->h_slots || cams.size() != b->h_cams.size() ||
+                              std::memcmp(cams.data(), b->h_cams.data(), cams.size() * sizeof(CamArgs)) != 0;
+  if (tables_changed) {
+    FT_HIP(hipStreamSynchronize(b->stream));
+    if ((rc = ensure(b->order, n_views * sizeof(int))) || (rc = ensure(b->slots, n_views * sizeof(int))) ||
+        (rc = ensure(b->cams, n_views * sizeof(CamArgs))))
+      return rc;
+    FT_HIP(hipMemcpy(b->order.ptr, order.data(), n_views * sizeof(int), hipMemcpyHostToDevice));
+    FT_HIP(hipMemcpy(b->slots.ptr, dev_slots.data(), n_views * sizeof(int), hipMemcpyHostToDevice));
+    FT_HIP(hipMemcpy(b->cams.ptr, cams.data(), n_views * sizeof(CamArgs), hipMemcpyHostToDevice));
+    b->h_order = order;
+    b->h_slots = slots;
+    b->h_cams = cams;
+  }
+  int launches = 0;
+  if (b->counting) {
+    hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, b->stream, (unsigned long long*)b->flops.ptr);
+    FT_HIP(hipGetLastError());
+  }
+  FT_HIP(hipEventRecord(b->ev0, b->stream));
+  if (flags & FT_BATCH_GRAPH) {
+    uint64_t key = 1469598103934665603ull;
+    key = mix_bytes(key, views, n_views * sizeof(ft_view));
+    key = mix_bytes(key, o, sizeof(*o));
+    if (rs) key = mix_bytes(key, rs, sizeof(*rs));
+    key = mix_bytes(key, &flags, sizeof(flags));
+    key = mix_bytes(key, &depth_rgb, sizeof(depth_rgb));
+    key = mix_bytes(key, &whitted_rgb, sizeof(whitted_rgb));
+    key = mix_bytes(key, &b->samples.ptr, sizeof(void*));
+    key = mix_bytes(key, &b->order.ptr, sizeof(void*));
+    if (!b->exec || key != b->graph_key || tables_changed) {
+      drop_graph(b);
+      FT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      rc = batch_enqueue(b, n_views, tp, aov, cams[0], flags & ~FT_BATCH_GRAPH, depth_rgb, whitted_rgb, launches);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(b->stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      FT_HIP(e);
+      b->graph = g;
+      FT_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+      b->graph_key = key;
+    } else {
+      launches = (int)b->stats.kernel_launches;  // same sequence as the captured call
+    }
+    FT_HIP(hipGraphLaunch(b->exec, b->stream));
+  } else {
+    if ((rc = batch_enqueue(b, n_views, tp, aov, cams[0], flags, depth_rgb, whitted_rgb, launches))) return rc;
+  }
+  FT_HIP(hipEventRecord(b->ev1, b->stream));
+  b->timed = true;
+  b->stats.primary_rays = (uint64_t)n_views * npix *
+                          (((flags & FT_BATCH_DEPTH) ? 1u : 0u) + ((flags & FT_BATCH_WHITTED) ? (uint64_t)cams[0].aa : 0u));
+  b->stats.kernel_launches = (uint64_t)launches;
+  return FT_OK;
+}
+
+int batch_finish(ft_batch* b) {
+  FT_HIP(hipStreamSynchronize(b->stream));
+  if (b->counting) {
+    unsigned long long f = 0;
+    FT_HIP(hipMemcpy(&f, b->flops.ptr, sizeof(f), hipMemcpyDeviceToHost));
+    b->stats.sdf_flops = f;
+  }
+  if (b->timed) {
+    float ms = 0.0f;
+    FT_HIP(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->stats.last_kernel_ms = ms;
+    b->timed = false;
+  }
+  if (b->overflow.ptr) {
+    int ovf = 0;
+    FT_HIP(hipMemcpy(&ovf, b->overflow.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    if (ovf) return fail(FT_ERR_LAUNCH_FAILED, "ray stack overflow");
+  }
+  return FT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ft_batch_create(const ft_scene_ir* const* scenes, uint32_t n_scenes, uint32_t accel_mode, ft_batch** out) {
+  if (!out) return fail(FT_ERR_INVALID_ARGUMENT, "output handle is null");
+  *out = nullptr;
+  if (!scenes || n_scenes == 0) return fail(FT_ERR_INVALID_ARGUMENT, "no scenes");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FT_ERR_GPU_UNAVAILABLE, "no HIP device available");
+  ft_batch* b = new ft_batch();
+  if (hipGetDevice(&b->device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) {
+    ft_batch_destroy(b);
+    return fail(FT_ERR_GPU_UNAVAILABLE, "stream/event creation failed");
+  }
+  for (uint32_t k = 0; k < n_scenes; ++k) {
+    ft_scene* s = nullptr;
+    int rc = create_scene(scenes[k], accel_mode, true, &s);
+    if (rc) {
+      std::string e = "scene " + std::to_string(k) + ": " + g_err;
+      ft_batch_destroy(b);
+      return fail(rc, e);
+    }
+    b->scenes.push_back(s);
+    b->stack_depth = std::max(b->stack_depth, s->dev.stack_depth);
+    b->max_children = std::max<long>(b->max_children, s->host.max_children);
+  }
+  *out = b;
+  return FT_OK;
+}
+
+int ft_batch_destroy(ft_batch* b) {
+  if (!b) return FT_OK;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  for (ft_scene* s : b->scenes) ft_scene_destroy(s);
+  for (ft_scene* s : b->scenes) s->dev.flops = nullptr;  // the counter belongs to the batch
+  for (DevBuf* d : {&b->order, &b->slots, &b->cams, &b->samples, &b->overflow, &b->flops})
+    if (d->ptr) (void)hipFree(d->ptr);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return FT_OK;
+}
+
+int ft_batch_render_device(ft_batch* b, const ft_view* views, uint32_t n_views, const ft_render_options* o,
+                           const ft_ray_settings* rs, uint32_t flags, void* depth_rgb_dev, void* whitted_rgb_dev) {
+  if (!b) return fail(FT_ERR_INVALID_ARGUMENT, "batch is null");
+  return batch_render_impl(b, views, n_views, o, rs, flags, (unsigned char*)depth_rgb_dev, (unsigned char*)whitted_rgb_dev);
+}
+
+int ft_batch_render(ft_batch* b, const ft_view* views, uint32_t n_views, const ft_render_options* o,
+                    const ft_ray_settings* rs, uint32_t flags, uint8_t* depth_rgb_out, uint8_t* whitted_rgb_out) {
+  if (!b || !o) return fail(FT_ERR_INVALID_ARGUMENT, "batch or options are null");
+  const size_t bytes = (size_t)n_views * o->width * o->height * 3;
+  void* dd = nullptr;
+  void* dw = nullptr;
+  if ((flags & FT_BATCH_DEPTH) && depth_rgb_out) FT_HIP(hipMalloc(&dd, bytes));
+  if ((flags & FT_BATCH_WHITTED) && whitted_rgb_out) {
+    hipError_t e = hipMalloc(&dw, bytes);
+    if (e != hipSuccess) {
+      if (dd) (void)hipFree(dd);
+      return fail(FT_ERR_OUT_OF_MEMORY, "hipMalloc failed");
+    }
+  }
+  int rc = batch_render_impl(b, views, n_views, o, rs, flags & ~FT_BATCH_GRAPH, (unsigned char*)dd, (unsigned char*)dw);
+  if (!rc) rc = batch_finish(b);
+  if (!rc && dd && hipMemcpy(depth_rgb_out, dd, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(FT_ERR_LAUNCH_FAILED, "copy of the depth images failed");
+  if (!rc && dw && hipMemcpy(whitted_rgb_out, dw, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+    rc = fail(FT_ERR_LAUNCH_FAILED, "copy of the Whitted images failed");
+  if (dd) (void)hipFree(dd);
+  if (dw) (void)hipFree(dw);
+  return rc;
+}
+
+int ft_batch_synchronize(ft_batch* b) {
+  if (!b) return fail(FT_ERR_INVALID_ARGUMENT, "batch is null");
+  return batch_finish(b);
+}
+
+int ft_batch_stats(ft_batch* b, ft_render_stats* out) {
+  if (!b || !out) return fail(FT_ERR_INVALID_ARGUMENT, "batch or output is null");
+  *out = b->stats;
+  return FT_OK;
+}
+
+int ft_batch_set_flop_counting(ft_batch* b, int enable) {
+  if (!b) return fail(FT_ERR_INVALID_ARGUMENT, "batch is null");
+  FT_HIP(hipStreamSynchronize(b->stream));
+  drop_graph(b);  // a captured graph would not see the new scene descriptors' counter
+  int rc = FT_OK;
+  if (enable && (rc = ensure(b->flops, sizeof(unsigned long long)))) return rc;
+  for (ft_scene* s : b->scenes) {
+    s->dev.flops = enable ? (unsigned long long*)b->flops.ptr : nullptr;
+    if ((rc = publish(s))) return rc;
+  }
+  b->counting = enable != 0;
+  return FT_OK;
 }
 
 }  // extern "C"

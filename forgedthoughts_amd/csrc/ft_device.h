@@ -48,7 +48,7 @@ struct DevScene {
 // Scene descriptors live in constant memory, one slot per scene handle.  Device functions
 // receive the slot index and re-read it with readfirstlane, so every scene field is a
 // scalar (SGPR) load and node programs run with wave-uniform control flow.
-constexpr int kMaxScenes = 64;
+constexpr int kMaxScenes = 1024;
 }  // namespace ftd
 extern "C" {
 __constant__ ftd::DevScene ft_c_scenes[ftd::kMaxScenes];

@@ -39,20 +39,20 @@ __device__ __forceinline__ float depth_preview_value(float t, float max_dist) { 
 
 // Pixels are visited in 8x8 tiles per wavefront (blockDim 64x4: four tiles per block) so
 // that the 64 lanes of a wave march spatially adjacent rays.
-__device__ __forceinline__ bool tile_pixel(int width, int height, int& x, int& y) {
+__device__ __forceinline__ bool tile_xy(int width, int height, int tile, int& x, int& y) {
   const int lane = threadIdx.x;
   const int tiles_x = (width + 7) / 8;
-  const int tile = blockIdx.x * blockDim.y + threadIdx.y;
   const int tx = tile % tiles_x, ty = tile / tiles_x;
   x = tx * 8 + (lane & 7);
   y = ty * 8 + (lane >> 3);
   return x < width && y < height;
 }
+__device__ __forceinline__ int wave_index() { return blockIdx.x * blockDim.y + threadIdx.y; }
 
-__device__ __forceinline__ void depth_kernel_body(int slot, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
-                                               float* __restrict__ t_out) {
+__device__ __forceinline__ void depth_tile(int slot, const CamArgs& cam, const TraceParams& tp, int tile,
+                                           unsigned char* __restrict__ rgb, float* __restrict__ t_out) {
   int x, y;
-  if (!tile_pixel(cam.width, cam.height, x, y)) return;
+  if (!tile_xy(cam.width, cam.height, tile, x, y)) return;
   float sum = 0.0f;
   float first_t = -1.0f;
   for (int s = 0; s < cam.aa; ++s) {
@@ -72,11 +72,10 @@ __device__ __forceinline__ void depth_kernel_body(int slot, CamArgs cam, TracePa
 }
 
 template <int kRayStack>
-__device__ __forceinline__ void whitted_kernel_body(int slot, CamArgs cam, TraceParams tp, int aov,
-                                                 float* __restrict__ samples, int* __restrict__ overflow) {
+__device__ __forceinline__ void whitted_tile(int slot, const CamArgs& cam, const TraceParams& tp, int aov, int tile,
+                                             int s, float* __restrict__ samples, int* __restrict__ overflow) {
   int x, y;
-  if (!tile_pixel(cam.width, cam.height, x, y)) return;
-  const int s = blockIdx.y;
+  if (!tile_xy(cam.width, cam.height, tile, x, y)) return;
   float3 o, d;
   primary_ray(cam, x, y, s, o, d);
   float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow);
@@ -86,11 +85,66 @@ __device__ __forceinline__ void whitted_kernel_body(int slot, CamArgs cam, Trace
   samples[idx * 3 + 2] = c.z;
 }
 
+__device__ __forceinline__ void depth_kernel_body(int slot, CamArgs cam, TraceParams tp, unsigned char* __restrict__ rgb,
+                                                  float* __restrict__ t_out) {
+  depth_tile(slot, cam, tp, wave_index(), rgb, t_out);
+}
+
+template <int kRayStack>
+__device__ __forceinline__ void whitted_kernel_body(int slot, CamArgs cam, TraceParams tp, int aov,
+                                                    float* __restrict__ samples, int* __restrict__ overflow) {
+  whitted_tile<kRayStack>(slot, cam, tp, aov, wave_index(), blockIdx.y, samples, overflow);
+}
+
+// Batches: many (scene, view) images in one launch.  Wave w renders tile w % tiles_per_view
+// of the view ranked w / tiles_per_view in `order` (views sorted by expected cost, most
+// expensive first, so the long tiles start early and the tail of the launch is short).
+// Every view has the batch's image size and sample count; its camera is cams[view].
+struct BatchArgs {
+  const int* order;     // rank -> view
+  const int* slots;     // view -> scene slot
+  const CamArgs* cams;  // view -> camera
+  int n_views, tiles_per_view;
+};
+
+__device__ __forceinline__ bool batch_item(const BatchArgs& b, int& view, int& tile, int& slot, CamArgs& cam) {
+  const int w = __builtin_amdgcn_readfirstlane(wave_index());
+  const int rank = w / b.tiles_per_view;
+  if (rank >= b.n_views) return false;
+  view = __builtin_amdgcn_readfirstlane(uniform_ptr(b.order)[rank]);
+  tile = w - rank * b.tiles_per_view;
+  slot = __builtin_amdgcn_readfirstlane(uniform_ptr(b.slots)[view]);
+  cam = load_uniform(uniform_ptr(b.cams) + view);
+  return true;
+}
+
+__device__ __forceinline__ void batch_depth_body(BatchArgs b, TraceParams tp, unsigned char* __restrict__ rgb) {
+  int view, tile, slot;
+  CamArgs cam;
+  if (!batch_item(b, view, tile, slot, cam)) return;
+  depth_tile(slot, cam, tp, tile, rgb + (size_t)view * cam.width * cam.height * 3, nullptr);
+}
+
+template <int kRayStack>
+__device__ __forceinline__ void batch_whitted_body(BatchArgs b, TraceParams tp, int aov, float* __restrict__ samples,
+                                                   int* __restrict__ overflow) {
+  int view, tile, slot;
+  CamArgs cam;
+  if (!batch_item(b, view, tile, slot, cam)) return;
+  const size_t per_view = (size_t)cam.aa * cam.width * cam.height * 3;
+  whitted_tile<kRayStack>(slot, cam, tp, aov, tile, blockIdx.y, samples + (size_t)view * per_view, overflow);
+}
+
+// One thread per pixel; blockIdx.y selects the image of a batch (samples and outputs of
+// image k start at k times the per-image size).
 __device__ __forceinline__ void resolve_kernel_body(CamArgs cam, int aov, const float* __restrict__ samples,
                                                  unsigned char* __restrict__ rgb, float* __restrict__ hdr) {
   const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   const int npix = cam.width * cam.height;
   if (pix >= npix) return;
+  samples += (size_t)blockIdx.y * cam.aa * npix * 3;
+  rgb += (size_t)blockIdx.y * npix * 3;
+  if (hdr) hdr += (size_t)blockIdx.y * npix * 3;
   float3 sum = mk(0, 0, 0);
   for (int s = 0; s < cam.aa; ++s) {
     const float* c = samples + ((size_t)s * npix + pix) * 3;
@@ -145,6 +199,21 @@ FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_128(int slot, ftd::CamArgs cam, f
 }
 #endif
 #if !defined(FT_RTC_KERNEL)
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_depth(ftd::BatchArgs b, ftd::TraceParams tp, unsigned char* __restrict__ rgb) {
+  ftd::batch_depth_body(b, tp, rgb);
+}
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_16(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
+                                                        float* __restrict__ samples, int* __restrict__ overflow) {
+  ftd::batch_whitted_body<16>(b, tp, aov, samples, overflow);
+}
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_48(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
+                                                        float* __restrict__ samples, int* __restrict__ overflow) {
+  ftd::batch_whitted_body<48>(b, tp, aov, samples, overflow);
+}
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_128(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
+                                                         float* __restrict__ samples, int* __restrict__ overflow) {
+  ftd::batch_whitted_body<128>(b, tp, aov, samples, overflow);
+}
 FT_KERNEL void ft_k_resolve(ftd::CamArgs cam, int aov, const float* __restrict__ samples, unsigned char* __restrict__ rgb,
                             float* __restrict__ hdr) {
   ftd::resolve_kernel_body(cam, aov, samples, rgb, hdr);

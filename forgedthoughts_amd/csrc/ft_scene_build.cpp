@@ -497,6 +497,10 @@ CamBasis camera_basis(const HostScene& s, const ft_render_options& o) {
     target[k] = (c.present && c.has_target) ? c.target[k] : s.center[k];
   }
   fov = (c.present && c.has_fov) ? c.fov_y_degrees : o.fov_y_degrees;
+  return camera_basis(origin, target, fov);
+}
+
+CamBasis camera_basis(const float origin[3], const float target[3], float fov) {
   auto normalize = [](float v[3]) {
     float l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
     if (l <= 1.1920929e-7f) return;

@@ -46,5 +46,7 @@ struct CamBasis {
   float tan_fov;
 };
 CamBasis camera_basis(const HostScene& s, const ft_render_options& o);
+// The same for an explicit camera (a Camera binding with origin, target and fov_y).
+CamBasis camera_basis(const float origin[3], const float target[3], float fov_y_degrees);
 
 }  // namespace fth

@@ -32,7 +32,9 @@ ERROR_KINDS = {1: "MissingSceneRoot", 2: "UnsupportedObjectType", 3: "ExpectedOb
 EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_destroy", "ft_render_depth",
            "ft_render_whitted", "ft_render_depth_device", "ft_render_whitted_device", "ft_scene_synchronize",
            "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting",
-           "ft_emit_scene_source", "ft_emit_compile"]
+           "ft_emit_scene_source", "ft_emit_compile", "ft_batch_create", "ft_batch_destroy",
+           "ft_batch_render_device", "ft_batch_render", "ft_batch_synchronize", "ft_batch_stats",
+           "ft_batch_set_flop_counting"]
 
 
 class BackendUnavailable(RuntimeError):
@@ -62,6 +64,13 @@ def library():
         L.ft_scene_set_flop_counting.argtypes = [vp, C.c_int]
         L.ft_emit_scene_source.argtypes = [vp, u32, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.ft_emit_compile.argtypes = [vp, u32, C.c_int, C.POINTER(C.c_double)]
+        L.ft_batch_create.argtypes = [C.POINTER(vp), u32, u32, C.POINTER(vp)]
+        L.ft_batch_destroy.argtypes = [vp]
+        L.ft_batch_render_device.argtypes = [vp, vp, u32, vp, vp, u32, vp, vp]
+        L.ft_batch_render.argtypes = [vp, vp, u32, vp, vp, u32, vp, vp]
+        L.ft_batch_synchronize.argtypes = [vp]
+        L.ft_batch_stats.argtypes = [vp, vp]
+        L.ft_batch_set_flop_counting.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -151,6 +160,78 @@ class GpuScene:
     def stats(self) -> abi.RenderStatsC:
         st = abi.RenderStatsC()
         _check(library().ft_scene_stats(self._h, C.addressof(st)))
+        return st
+
+
+class GpuBatch:
+    """Many scenes rendered from many views per call (ft_batch_*): one launch covers every
+    (scene, view) image; FT_BATCH_SEPARATE issues the single-scene launches instead."""
+
+    def __init__(self, scenes, accel: AccelMode = AccelMode.Bvh):
+        self.scenes = list(scenes)
+        self._irs = [s.ir() for s in self.scenes]
+        arr = (C.c_void_p * len(self._irs))(*[C.addressof(ir) for ir in self._irs])
+        self._h = C.c_void_p()
+        _check(library().ft_batch_create(arr, len(self._irs), int(accel), C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            library().ft_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @staticmethod
+    def views(items) -> "C.Array":
+        """items: (scene index, origin xyz, target xyz, fov_y degrees)."""
+        arr = (abi.View * len(items))()
+        for k, (sc, o, t, fov) in enumerate(items):
+            arr[k].scene = sc
+            arr[k].fov_y_degrees = fov
+            arr[k].origin[:] = [float(x) for x in o]
+            arr[k].target[:] = [float(x) for x in t]
+        return arr
+
+    def render(self, views, options: RenderOptions, settings: Optional[RaySettings], flags: int):
+        """Host outputs: (depth[n,h,w,3] or None, whitted[n,h,w,3] or None)."""
+        o = options.to_c()
+        s = settings.to_c() if settings is not None else None
+        n = len(views)
+        depth = np.zeros((n, o.height, o.width, 3), np.uint8) if flags & abi.BATCH_DEPTH else None
+        whit = np.zeros((n, o.height, o.width, 3), np.uint8) if flags & abi.BATCH_WHITTED else None
+        _check(library().ft_batch_render(self._h, C.addressof(views), n, C.addressof(o),
+                                         C.addressof(s) if s is not None else None, int(flags),
+                                         depth.ctypes.data if depth is not None else None,
+                                         whit.ctypes.data if whit is not None else None))
+        return depth, whit
+
+    def render_device(self, views, options: RenderOptions, settings: Optional[RaySettings], flags: int,
+                      depth_ptr: int = 0, whitted_ptr: int = 0):
+        o = options.to_c()
+        s = settings.to_c() if settings is not None else None
+        _check(library().ft_batch_render_device(self._h, C.addressof(views), len(views), C.addressof(o),
+                                                C.addressof(s) if s is not None else None, int(flags),
+                                                depth_ptr or None, whitted_ptr or None))
+
+    def synchronize(self):
+        _check(library().ft_batch_synchronize(self._h))
+
+    def set_flop_counting(self, enable: bool):
+        _check(library().ft_batch_set_flop_counting(self._h, 1 if enable else 0))
+
+    def stats(self) -> abi.RenderStatsC:
+        st = abi.RenderStatsC()
+        _check(library().ft_batch_stats(self._h, C.addressof(st)))
         return st
 
 

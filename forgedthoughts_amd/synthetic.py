@@ -230,4 +230,49 @@ def log_uniform_counts(n: int, lo: int, hi: int, seed: int) -> List[int]:
     return [int(math.floor(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(n)]
 
 
-WORKLOADS = {"depth_512": depth_512, "whitted_1080p": whitted_1080p, "bricks_4096": bricks_4096}
+LIGHTS_BATCH = ("let key = PointLight { position: vec3(6.0, 9.0, -8.0), color: #fff2d8, intensity: 180.0 };\n"
+                "let skylight = EnvLight { color: #cfe0ff, intensity: 0.25 };\n")
+
+
+@dataclass
+class BatchWorkload:
+    name: str
+    sources: List[str]
+    views: List[tuple]          # (scene index, origin, target, fov_y)
+    options: RenderOptions
+    settings: RaySettings
+    accel: AccelMode
+    counts: List[int]
+
+
+def random_views(rng: SplitMix64, n_scenes: int, per_scene: int, radius: float = 16.0) -> List[tuple]:
+    """Cameras on a sphere around the scene cube [-5, 5]^3, looking near its centre."""
+    views = []
+    for sc in range(n_scenes):
+        for _ in range(per_scene):
+            z = rng.uniform(-0.3, 0.9)          # mostly from above the horizon
+            phi = rng.uniform(0.0, 2.0 * math.pi)
+            r = math.sqrt(max(1.0 - z * z, 0.0))
+            origin = (radius * r * math.cos(phi), radius * z, radius * r * math.sin(phi))
+            target = (rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0), rng.uniform(-1.0, 1.0))
+            views.append((sc, origin, target, 45.0))
+    return views
+
+
+def batch_256(n_scenes: int = 256, views_per_scene: int = 4, size: int = 256, lo: int = 8, hi: int = 2048,
+              seed: int = 4) -> BatchWorkload:
+    """Batched persistent-launch workload: n_scenes seeded scenes with primitive counts
+    log-uniform in [lo, hi] (mixed union/diff/intersect x round/chamfer/stairs), each seen
+    from views_per_scene random views at size x size: depth preview plus a max_depth-1
+    Whitted trace (primary hit + shadowed direct light) with all-Lambert materials."""
+    counts = log_uniform_counts(n_scenes, lo, hi, seed)
+    sources = [LIGHTS_BATCH + mixed_scene(n, seed * 100003 + k) for k, n in enumerate(counts)]
+    views = random_views(SplitMix64(seed + 1), n_scenes, views_per_scene)
+    opts = RenderOptions(width=size, height=size, max_steps=128, max_dist=40.0, epsilon=1.0e-3,
+                         step_scale=0.7, camera_z=6.0, fov_y_degrees=45.0)
+    settings = RaySettings(max_depth=1, tile_size=64, aa_samples=1)
+    return BatchWorkload("batch_256", sources, views, opts, settings, AccelMode.Bvh, counts)
+
+
+WORKLOADS = {"depth_512": depth_512, "whitted_1080p": whitted_1080p, "bricks_4096": bricks_4096,
+             "batch_256": batch_256}

@@ -371,6 +371,45 @@ typedef struct ft_render_stats {
 #define FT_STATS_EMITTED_FUNCTIONS 2u /* Forge functions run as specialised code */
 int ft_scene_stats(ft_scene* scene, ft_render_stats* out);
 
+/* ------------------------------------------------------------------ batches
+ * Many scenes, each seen from several views, rendered together.  No counterpart in the
+ * reference (it renders one scene per call, renderer.rs:1139-1277); a batch is what a
+ * caller of those entry points would otherwise issue as one call per (scene, view).
+ * Every view of a call has the options' image size and the settings' sample count; its
+ * camera replaces the scene's `camera` binding (parse_camera renderer.rs:6263-6286).
+ * Scenes of a batch run on the node-program interpreter (no per-scene compilation). */
+typedef struct ft_batch ft_batch;
+
+typedef struct ft_view {
+  uint32_t scene;            /* index into the batch's scenes */
+  float fov_y_degrees;
+  float origin[3];
+  float target[3];
+} ft_view;
+
+#define FT_BATCH_DEPTH 1u     /* depth preview per view (render_preview_progressive_with_accel) */
+#define FT_BATCH_WHITTED 2u   /* Whitted trace per view (render_ray_progressive_with_accel) */
+#define FT_BATCH_SEPARATE 4u  /* issue the per-view launches of the single-scene path instead of
+                                 the batched launch (for comparison) */
+#define FT_BATCH_GRAPH 8u     /* capture the launch sequence in a HIP graph; a later call with the
+                                 same views, options, flags and outputs replays it */
+
+int ft_batch_create(const ft_scene_ir* const* scenes, uint32_t n_scenes, uint32_t accel_mode, ft_batch** out);
+int ft_batch_destroy(ft_batch* batch);
+/* Outputs are device pointers: n_views images of width*height*3 bytes each, view order;
+ * either may be null when its flag is not set.  Enqueues and returns. */
+int ft_batch_render_device(ft_batch* batch, const ft_view* views, uint32_t n_views,
+                           const ft_render_options* options, const ft_ray_settings* settings,
+                           uint32_t flags, void* depth_rgb_dev, void* whitted_rgb_dev);
+/* The same into host buffers (synchronous). */
+int ft_batch_render(ft_batch* batch, const ft_view* views, uint32_t n_views,
+                    const ft_render_options* options, const ft_ray_settings* settings,
+                    uint32_t flags, uint8_t* depth_rgb_out, uint8_t* whitted_rgb_out);
+int ft_batch_synchronize(ft_batch* batch);
+int ft_batch_stats(ft_batch* batch, ft_render_stats* out);
+/* FLOP counting over every scene of the batch (see ft_scene_set_flop_counting). */
+int ft_batch_set_flop_counting(ft_batch* batch, int enable);
+
 /* Scene specialisation diagnostics (no GPU needed).  ft_emit_scene_source writes the
  * translation unit the emitter produces for `ir` under `accel_mode` (always emitted, the
  * FT_EMIT policy aside) into buf (cap bytes, NUL-terminated, *len_out = full length);

@@ -41,3 +41,14 @@ def diff_metrics(a, b):
     """max |d| and mean |d| over all channels (render_regression.rs:196-219)."""
     d = np.abs(a.astype(np.int32) - b.astype(np.int32))
     return int(d.max()), float(d.mean())
+
+
+def with_camera(scene, origin, target, fov):
+    """The scene seen through an explicit camera (what a batch view means)."""
+    import copy
+    import numpy as np
+    c = copy.copy(scene)
+    c.camera = {"present": True, "origin": np.asarray(origin, np.float32),
+                "target": np.asarray(target, np.float32), "fov": np.float32(fov)}
+    c._ir = None
+    return c

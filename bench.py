@@ -146,6 +146,77 @@ def bench_batch(args, w):
     print(json.dumps(line), flush=True)
 
 
+def bench_adjoint(args, w):
+    import numpy as np
+
+    from forgedthoughts_amd.diff import DiffRenderer
+
+    dr = DiffRenderer(w.n_prims, w.views, w.options, w.light_dir, w.normal_step)
+    dr.set_params(w.params_true)
+    tt, st = dr.forward()
+    dr.set_params(w.params_init)
+    dr.set_target(tt, st)
+    warm = max(args.warmup, 1)
+    dr.fit(warm, w.lr)                      # warm-up steps (part of the same fit)
+    steps = w.steps - warm
+    t0 = time.perf_counter()
+    losses = dr.fit(steps, w.lr)
+    wall = time.perf_counter() - t0
+    kernel_ms = dr.stats().last_kernel_ms
+    p = dr.get_params()
+    err0 = float(np.abs(np.float32(w.params_init) - np.float32(w.params_true)).mean())
+    err1 = float(np.abs(p - np.float32(w.params_true)).mean())
+    npix = len(w.views) * w.options.width * w.options.height
+    line = {
+        "metric": "adjoint grad-steps/s",
+        "value": round(steps / wall, 4),
+        "unit": "grad-steps/s",
+        "n_gpus": 1, "steps": steps, "warmup": warm,
+        "ms_per_step": round(wall / steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (fp64 reduction)",
+        "data": "synthetic",
+        "config": {"workload": w.name, "prims": w.n_prims, "params": 5 * w.n_prims - 1, "views": len(w.views),
+                   "width": w.options.width, "height": w.options.height, "optimizer": f"adam lr {w.lr}",
+                   "step": "forward + L2 loss + adjoint + fp64 reduction + adam, on the device"},
+        "fit": {"loss_first": float(losses[0]), "loss_last": float(losses[-1]),
+                "mean_abs_param_error": [round(err0, 5), round(err1, 5)]},
+        "kernel_ms_per_step": round(kernel_ms / steps, 3),
+        "pixels_per_step": npix,
+        "roofline": None,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_adjoint(w, args.cpu_seconds)
+    dr.close()
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline_adjoint(w, seconds: float):
+    """The oracle's f32 forward of every view at a reduced resolution, scaled by the pixel
+    count to a full step: an upper bound on CPU grad-steps/s (a CPU gradient step costs
+    more than one forward)."""
+    import copy
+
+    from oracle import ftref
+    threads = os.cpu_count() or 1
+    size = 128
+    o = copy.copy(w.options)
+    o.width = o.height = size
+    t0 = time.perf_counter()
+    n_done = 0
+    for v in w.views:
+        ftref.diff_forward(w.n_prims, w.params_true, [v], o, w.light_dir, w.normal_step, precision=0,
+                           threads=threads)
+        n_done += 1
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    scale = (w.options.width * w.options.height) / (size * size) * len(w.views) / n_done
+    return {"value": 1.0 / (dt * scale), "unit": "grad-steps/s", "cores": threads, "kind": "port",
+            "sample": f"forward only (a lower bound on the cost of a CPU gradient step): {n_done} views at "
+                      f"{size}x{size} in {dt:.1f} s, scaled by pixel count to {len(w.views)} views at "
+                      f"{w.options.width}x{w.options.height}; {threads} threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -170,6 +241,8 @@ def main():
     w = synthetic.WORKLOADS[args.workload]()
     if isinstance(w, synthetic.BatchWorkload):
         return bench_batch(args, w)
+    if isinstance(w, synthetic.AdjointWorkload):
+        return bench_adjoint(args, w)
     scene = compile_scene(load_scene_source(w.source))
     opts = w.options
     dev = torch.device("cuda:0")

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "ft_diff.h"
 #include "ft_emit.h"
 #include "ft_kernels.h"
 #include "ft_rtc.h"
@@ -896,6 +897,363 @@ int ft_batch_set_flop_counting(ft_batch* b, int enable) {
     if ((rc = publish(s))) return rc;
   }
   b->counting = enable != 0;
+  return FT_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ differentiable rendering
+struct ft_diff {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  uint32_t n_prims = 0, n_params = 0, n_views = 0, n_nodes = 0;
+  ft_render_options opts{};
+  float light[3] = {0, 1, 0};
+  float normal_step = 1.0e-2f;
+  std::vector<int> h_nodes;  // 4 ints per node
+  DevBuf nodes, th, bounds, cams, t, s, tt, st, gt, gs, partials, red_a, red_b, grad, m, v, lpart, losses;
+  bool have_target = false, have_forward = false;
+  int adam_step = 0;
+  ft_render_stats stats{};
+};
+
+namespace {
+
+void diff_tree(int lo, int hi, std::vector<int>& nodes, int& joins) {
+  if (hi - lo == 1) {
+    nodes.insert(nodes.end(), {lo, -1, -1, -1});
+    return;
+  }
+  const int mid = lo + (hi - lo) / 2;
+  diff_tree(lo, mid, nodes, joins);
+  const int l = (int)nodes.size() / 4 - 1;
+  diff_tree(mid, hi, nodes, joins);
+  const int r = (int)nodes.size() / 4 - 1;
+  nodes.insert(nodes.end(), {-1, joins++, l, r});
+}
+
+void diff_bounds_host(const float* th, uint32_t n, uint32_t n_params, float out[6]) {
+  float kmax = 0.0f;
+  for (uint32_t j = 4 * n; j < n_params; ++j) kmax = std::fmax(kmax, std::fabs(th[j]));
+  const float mg = kmax + 1.0e-3f;
+  for (int k = 0; k < 3; ++k) { out[k] = __builtin_huge_valf(); out[3 + k] = -__builtin_huge_valf(); }
+  for (uint32_t i = 0; i < n; ++i) {
+    const float r = std::fabs(th[4 * i + 3]) + mg;
+    for (int k = 0; k < 3; ++k) {
+      out[k] = std::fmin(out[k], th[4 * i + k] - r);
+      out[3 + k] = std::fmax(out[3 + k], th[4 * i + k] + r);
+    }
+  }
+}
+
+const int kDiffThreads = 128;  // 64 x 2 per block
+
+DiffArgs diff_args(ft_diff* d) {
+  DiffArgs a;
+  a.nodes = (const int4*)d->nodes.ptr;
+  a.th = (const float*)d->th.ptr;
+  a.bounds = (const float*)d->bounds.ptr;
+  a.cams = (const CamArgs*)d->cams.ptr;
+  a.n_nodes = (int)d->n_nodes;
+  a.n_prims = (int)d->n_prims;
+  a.n_params = (int)d->n_params;
+  a.n_views = (int)d->n_views;
+  a.tiles_per_view = (((int)d->opts.width + 7) / 8) * (((int)d->opts.height + 7) / 8);
+  a.max_steps = (int)d->opts.max_steps;
+  a.eps = d->opts.epsilon;
+  a.step_scale = d->opts.step_scale;
+  a.max_dist = d->opts.max_dist;
+  a.normal_step = d->normal_step;
+  a.lx = d->light[0];
+  a.ly = d->light[1];
+  a.lz = d->light[2];
+  return a;
+}
+
+size_t diff_lds(const ft_diff* d) {
+  return (size_t)2 * ((d->n_params + 1) & ~1u) * sizeof(float) + (size_t)d->n_nodes * kDiffThreads * sizeof(float);
+}
+unsigned diff_blocks(const ft_diff* d) {
+  const size_t waves = (size_t)d->n_views * ((((int)d->opts.width + 7) / 8) * (((int)d->opts.height + 7) / 8));
+  return (unsigned)((waves + 1) / 2);
+}
+size_t diff_pixels(const ft_diff* d) { return (size_t)d->n_views * d->opts.width * d->opts.height; }
+
+int diff_forward_enqueue(ft_diff* d) {
+  hipLaunchKernelGGL(ft_k_diff_forward, dim3(diff_blocks(d)), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
+                     (float*)d->t.ptr, (float*)d->s.ptr);
+  FT_HIP(hipGetLastError());
+  d->have_forward = true;
+  return FT_OK;
+}
+
+// Sums `rows` x cols doubles at `in` over rows into d->grad-like `out` (deterministic order).
+int diff_reduce(ft_diff* d, const double* in, int rows, int cols, double* out) {
+  const int group = 256;
+  const double* src = in;
+  double* bufs[2] = {(double*)d->red_a.ptr, (double*)d->red_b.ptr};
+  int which = 0;
+  while (rows > 1) {
+    const int groups = (rows + group - 1) / group;
+    double* dst = groups == 1 ? out : bufs[which];
+    hipLaunchKernelGGL(ft_k_diff_reduce, dim3(groups, (cols + 255) / 256), dim3(256), 0, d->stream, src, rows, cols,
+                       group, dst);
+    FT_HIP(hipGetLastError());
+    src = dst;
+    which ^= 1;
+    rows = groups;
+  }
+  if (src != out) FT_HIP(hipMemcpyAsync(out, src, cols * sizeof(double), hipMemcpyDeviceToDevice, d->stream));
+  return FT_OK;
+}
+
+int diff_backward_enqueue(ft_diff* d, const float* gt, const float* gs) {
+  const unsigned blocks = diff_blocks(d);
+  hipLaunchKernelGGL(ft_k_diff_adjoint, dim3(blocks), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
+                     (const float*)d->t.ptr, gt, gs, (double*)d->partials.ptr);
+  FT_HIP(hipGetLastError());
+  return diff_reduce(d, (const double*)d->partials.ptr, (int)blocks, (int)d->n_params, (double*)d->grad.ptr);
+}
+
+int diff_loss_enqueue(ft_diff* d, float wt, float ws, double* loss_dst) {
+  const size_t npix = diff_pixels(d);
+  const unsigned blocks = (unsigned)((npix + 255) / 256);
+  hipLaunchKernelGGL(ft_k_diff_loss, dim3(blocks), dim3(256), 0, d->stream, (int)d->opts.width, (int)d->opts.height,
+                     (int)d->n_views, (const float*)d->t.ptr, (const float*)d->s.ptr, (const float*)d->tt.ptr,
+                     (const float*)d->st.ptr, wt, ws, (float*)d->gt.ptr, (float*)d->gs.ptr, (double*)d->lpart.ptr);
+  FT_HIP(hipGetLastError());
+  return diff_reduce(d, (const double*)d->lpart.ptr, (int)blocks, 1, loss_dst);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ft_diff_create(uint32_t n_prims, const ft_view* views, uint32_t n_views, const ft_render_options* o,
+                   const float light_dir[3], float normal_step, ft_diff** out) {
+  if (!out) return fail(FT_ERR_INVALID_ARGUMENT, "output handle is null");
+  *out = nullptr;
+  int rc = check_options(o);
+  if (rc) return rc;
+  if (n_prims < 1 || n_prims > 48) return fail(FT_ERR_INVALID_ARGUMENT, "n_prims must be in [1, 48]");
+  if (!views || n_views == 0) return fail(FT_ERR_INVALID_ARGUMENT, "no views");
+  if (!light_dir) return fail(FT_ERR_INVALID_ARGUMENT, "light direction is null");
+  if (!(normal_step > 0.0f)) return fail(FT_ERR_INVALID_ARGUMENT, "normal_step must be positive");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(FT_ERR_GPU_UNAVAILABLE, "no HIP device available");
+  ft_diff* d = new ft_diff();
+  d->n_prims = n_prims;
+  d->n_params = 5 * n_prims - 1;
+  d->n_views = n_views;
+  d->opts = *o;
+  d->normal_step = normal_step;
+  const float ll = std::sqrt(light_dir[0] * light_dir[0] + light_dir[1] * light_dir[1] + light_dir[2] * light_dir[2]);
+  for (int k = 0; k < 3; ++k) d->light[k] = ll > 0.0f ? light_dir[k] / ll : (k == 1 ? 1.0f : 0.0f);
+  int joins = 0;
+  diff_tree(0, (int)n_prims, d->h_nodes, joins);
+  d->n_nodes = (uint32_t)(d->h_nodes.size() / 4);
+  auto bail = [&](int code) {
+    std::string e = g_err;
+    ft_diff_destroy(d);
+    return fail(code, e);
+  };
+  if (hipGetDevice(&d->device) != hipSuccess || hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess) {
+    g_err = "stream/event creation failed";
+    return bail(FT_ERR_GPU_UNAVAILABLE);
+  }
+  std::vector<CamArgs> cams(n_views);
+  for (uint32_t v = 0; v < n_views; ++v) {
+    fth::CamBasis cb = fth::camera_basis(views[v].origin, views[v].target, views[v].fov_y_degrees);
+    CamArgs& c = cams[v];
+    c.ox = cb.origin[0]; c.oy = cb.origin[1]; c.oz = cb.origin[2];
+    c.fx = cb.forward[0]; c.fy = cb.forward[1]; c.fz = cb.forward[2];
+    c.rx = cb.right[0]; c.ry = cb.right[1]; c.rz = cb.right[2];
+    c.ux = cb.up[0]; c.uy = cb.up[1]; c.uz = cb.up[2];
+    c.tan_fov = cb.tan_fov;
+    c.aspect = (float)o->width / (float)o->height;
+    c.width = (int)o->width;
+    c.height = (int)o->height;
+    c.aa = 1;
+    c.grid = 1;
+  }
+  const size_t npix = diff_pixels(d);
+  const unsigned blocks = diff_blocks(d);
+  const size_t lblocks = (npix + 255) / 256;
+  const size_t red_rows = std::max<size_t>((blocks + 255) / 256, (lblocks + 255) / 256) + 1;
+  std::vector<float> zeros(d->n_params, 0.0f);
+  if ((rc = upload(d->nodes, d->h_nodes)) || (rc = upload(d->cams, cams)) ||
+      (rc = ensure(d->th, d->n_params * sizeof(float))) || (rc = ensure(d->bounds, 6 * sizeof(float))) ||
+      (rc = ensure(d->t, npix * sizeof(float))) || (rc = ensure(d->s, npix * sizeof(float))) ||
+      (rc = ensure(d->tt, npix * sizeof(float))) || (rc = ensure(d->st, npix * sizeof(float))) ||
+      (rc = ensure(d->gt, npix * sizeof(float))) || (rc = ensure(d->gs, npix * sizeof(float))) ||
+      (rc = ensure(d->partials, (size_t)blocks * d->n_params * sizeof(double))) ||
+      (rc = ensure(d->red_a, red_rows * std::max<size_t>(d->n_params, 1) * sizeof(double))) ||
+      (rc = ensure(d->red_b, red_rows * std::max<size_t>(d->n_params, 1) * sizeof(double))) ||
+      (rc = ensure(d->grad, d->n_params * sizeof(double))) || (rc = ensure(d->m, d->n_params * sizeof(double))) ||
+      (rc = ensure(d->v, d->n_params * sizeof(double))) || (rc = ensure(d->lpart, lblocks * sizeof(double))) ||
+      (rc = ensure(d->losses, sizeof(double))))
+    return bail(rc);
+  if (diff_lds(d) > 64 * 1024) {
+    g_err = "scene too large for the adjoint's LDS budget";
+    return bail(FT_ERR_INVALID_ARGUMENT);
+  }
+  *out = d;
+  return FT_OK;
+}
+
+int ft_diff_destroy(ft_diff* d) {
+  if (!d) return FT_OK;
+  (void)hipSetDevice(d->device);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  for (DevBuf* b : {&d->nodes, &d->th, &d->bounds, &d->cams, &d->t, &d->s, &d->tt, &d->st, &d->gt, &d->gs,
+                    &d->partials, &d->red_a, &d->red_b, &d->grad, &d->m, &d->v, &d->lpart, &d->losses})
+    if (b->ptr) (void)hipFree(b->ptr);
+  if (d->ev0) (void)hipEventDestroy(d->ev0);
+  if (d->ev1) (void)hipEventDestroy(d->ev1);
+  if (d->stream) (void)hipStreamDestroy(d->stream);
+  delete d;
+  return FT_OK;
+}
+
+int ft_diff_param_count(ft_diff* d, uint32_t* n) {
+  if (!d || !n) return fail(FT_ERR_INVALID_ARGUMENT, "diff or output is null");
+  *n = d->n_params;
+  return FT_OK;
+}
+
+int ft_diff_set_params(ft_diff* d, const float* params) {
+  if (!d || !params) return fail(FT_ERR_INVALID_ARGUMENT, "diff or params are null");
+  float b[6];
+  diff_bounds_host(params, d->n_prims, d->n_params, b);
+  FT_HIP(hipStreamSynchronize(d->stream));
+  FT_HIP(hipMemcpy(d->th.ptr, params, d->n_params * sizeof(float), hipMemcpyHostToDevice));
+  FT_HIP(hipMemcpy(d->bounds.ptr, b, sizeof(b), hipMemcpyHostToDevice));
+  d->have_forward = false;
+  return FT_OK;
+}
+
+int ft_diff_get_params(ft_diff* d, float* params) {
+  if (!d || !params) return fail(FT_ERR_INVALID_ARGUMENT, "diff or params are null");
+  FT_HIP(hipStreamSynchronize(d->stream));
+  FT_HIP(hipMemcpy(params, d->th.ptr, d->n_params * sizeof(float), hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_diff_forward_device(ft_diff* d, void* t_dev, void* s_dev) {
+  if (!d) return fail(FT_ERR_INVALID_ARGUMENT, "diff is null");
+  FT_HIP(hipSetDevice(d->device));
+  FT_HIP(hipEventRecord(d->ev0, d->stream));
+  int rc = diff_forward_enqueue(d);
+  if (rc) return rc;
+  FT_HIP(hipEventRecord(d->ev1, d->stream));
+  const size_t bytes = diff_pixels(d) * sizeof(float);
+  if (t_dev) FT_HIP(hipMemcpyAsync(t_dev, d->t.ptr, bytes, hipMemcpyDeviceToDevice, d->stream));
+  if (s_dev) FT_HIP(hipMemcpyAsync(s_dev, d->s.ptr, bytes, hipMemcpyDeviceToDevice, d->stream));
+  d->stats.primary_rays = diff_pixels(d);
+  d->stats.kernel_launches = 1;
+  return FT_OK;
+}
+
+int ft_diff_synchronize(ft_diff* d) {
+  if (!d) return fail(FT_ERR_INVALID_ARGUMENT, "diff is null");
+  FT_HIP(hipStreamSynchronize(d->stream));
+  return FT_OK;
+}
+
+int ft_diff_forward(ft_diff* d, float* t_out, float* s_out) {
+  if (!d) return fail(FT_ERR_INVALID_ARGUMENT, "diff is null");
+  int rc = ft_diff_forward_device(d, nullptr, nullptr);
+  if (rc) return rc;
+  FT_HIP(hipStreamSynchronize(d->stream));
+  float ms = 0.0f;
+  FT_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  d->stats.last_kernel_ms = ms;
+  const size_t bytes = diff_pixels(d) * sizeof(float);
+  if (t_out) FT_HIP(hipMemcpy(t_out, d->t.ptr, bytes, hipMemcpyDeviceToHost));
+  if (s_out) FT_HIP(hipMemcpy(s_out, d->s.ptr, bytes, hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_diff_backward_device(ft_diff* d, const void* gt, const void* gs, double* grad_out) {
+  if (!d || !grad_out) return fail(FT_ERR_INVALID_ARGUMENT, "diff or output is null");
+  if (!d->have_forward) return fail(FT_ERR_INVALID_ARGUMENT, "backward needs a forward at the current parameters");
+  FT_HIP(hipSetDevice(d->device));
+  FT_HIP(hipEventRecord(d->ev0, d->stream));
+  int rc = diff_backward_enqueue(d, (const float*)gt, (const float*)gs);
+  if (rc) return rc;
+  FT_HIP(hipEventRecord(d->ev1, d->stream));
+  FT_HIP(hipStreamSynchronize(d->stream));
+  float ms = 0.0f;
+  FT_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  d->stats.last_kernel_ms = ms;
+  FT_HIP(hipMemcpy(grad_out, d->grad.ptr, d->n_params * sizeof(double), hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_diff_set_target(ft_diff* d, const float* t_target, const float* s_target) {
+  if (!d || !t_target || !s_target) return fail(FT_ERR_INVALID_ARGUMENT, "diff or targets are null");
+  const size_t bytes = diff_pixels(d) * sizeof(float);
+  FT_HIP(hipStreamSynchronize(d->stream));
+  FT_HIP(hipMemcpy(d->tt.ptr, t_target, bytes, hipMemcpyHostToDevice));
+  FT_HIP(hipMemcpy(d->st.ptr, s_target, bytes, hipMemcpyHostToDevice));
+  d->have_target = true;
+  return FT_OK;
+}
+
+int ft_diff_loss_grad(ft_diff* d, float wt, float ws, double* loss_out, double* grad_out) {
+  if (!d) return fail(FT_ERR_INVALID_ARGUMENT, "diff is null");
+  if (!d->have_target) return fail(FT_ERR_INVALID_ARGUMENT, "no target images");
+  FT_HIP(hipSetDevice(d->device));
+  int rc;
+  if ((rc = diff_forward_enqueue(d)) || (rc = diff_loss_enqueue(d, wt, ws, (double*)d->losses.ptr)) ||
+      (rc = diff_backward_enqueue(d, (const float*)d->gt.ptr, (const float*)d->gs.ptr)))
+    return rc;
+  FT_HIP(hipStreamSynchronize(d->stream));
+  if (loss_out) FT_HIP(hipMemcpy(loss_out, d->losses.ptr, sizeof(double), hipMemcpyDeviceToHost));
+  if (grad_out) FT_HIP(hipMemcpy(grad_out, d->grad.ptr, d->n_params * sizeof(double), hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_diff_fit(ft_diff* d, uint32_t steps, float lr, float wt, float ws, double* losses_out) {
+  if (!d) return fail(FT_ERR_INVALID_ARGUMENT, "diff is null");
+  if (!d->have_target) return fail(FT_ERR_INVALID_ARGUMENT, "no target images");
+  if (steps == 0) return FT_OK;
+  FT_HIP(hipSetDevice(d->device));
+  int rc = ensure(d->losses, steps * sizeof(double));
+  if (rc) return rc;
+  FT_HIP(hipStreamSynchronize(d->stream));
+  if (d->adam_step == 0) {
+    FT_HIP(hipMemset(d->m.ptr, 0, d->n_params * sizeof(double)));
+    FT_HIP(hipMemset(d->v.ptr, 0, d->n_params * sizeof(double)));
+  }
+  FT_HIP(hipEventRecord(d->ev0, d->stream));
+  for (uint32_t k = 0; k < steps; ++k) {
+    if ((rc = diff_forward_enqueue(d)) || (rc = diff_loss_enqueue(d, wt, ws, (double*)d->losses.ptr + k)) ||
+        (rc = diff_backward_enqueue(d, (const float*)d->gt.ptr, (const float*)d->gs.ptr)))
+      return rc;
+    ++d->adam_step;
+    hipLaunchKernelGGL(ft_k_diff_adam, dim3(1), dim3(256), 0, d->stream, (float*)d->th.ptr, (const double*)d->grad.ptr,
+                       (double*)d->m.ptr, (double*)d->v.ptr, (int)d->n_params, (int)d->n_prims, d->adam_step, lr, 0.9f,
+                       0.999f, 1.0e-8f, (float*)d->bounds.ptr);
+    FT_HIP(hipGetLastError());
+  }
+  FT_HIP(hipEventRecord(d->ev1, d->stream));
+  FT_HIP(hipStreamSynchronize(d->stream));
+  d->have_forward = false;
+  float ms = 0.0f;
+  FT_HIP(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+  d->stats.last_kernel_ms = ms;
+  d->stats.kernel_launches = (uint64_t)steps * 7;
+  d->stats.primary_rays = (uint64_t)steps * diff_pixels(d);
+  if (losses_out) FT_HIP(hipMemcpy(losses_out, d->losses.ptr, steps * sizeof(double), hipMemcpyDeviceToHost));
+  return FT_OK;
+}
+
+int ft_diff_stats(ft_diff* d, ft_render_stats* out) {
+  if (!d || !out) return fail(FT_ERR_INVALID_ARGUMENT, "diff or output is null");
+  *out = d->stats;
   return FT_OK;
 }
 

@@ -85,37 +85,10 @@ __device__ __forceinline__ T load_uniform(const FT_CONST_AS T* p) {
   return out;
 }
 
-// ----------------------------------------------------------------- math
-__device__ __forceinline__ float3 mk(float x, float y, float z) { return make_float3(x, y, z); }
-__device__ __forceinline__ float3 add(float3 a, float3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
-__device__ __forceinline__ float3 sub(float3 a, float3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
-__device__ __forceinline__ float3 mul(float3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
-__device__ __forceinline__ float3 cmul(float3 a, float3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
-__device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
-__device__ __forceinline__ float3 cross(float3 a, float3 b) {
-  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float3 normalize(float3 a) {  // Vec3::normalize (renderer.rs:317-323)
-  float l = len(a);
-  if (l <= 1.1920929e-7f) return a;
-  return mul(a, 1.0f / l);
-}
-__device__ __forceinline__ float clampf(float x, float lo, float hi) {
-  x = x < lo ? lo : x;
-  return x > hi ? hi : x;
-}
-__device__ __forceinline__ float comp(float3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
-__device__ __forceinline__ float deg2rad(float d) { return d * (3.14159265358979323846f / 180.0f); }
-__device__ __forceinline__ float powi5(float x) { float x2 = x * x; return x * (x2 * x2); }
-__device__ __forceinline__ float rem_euclid(float a, float b) {
-  float r = fmodf(a, b);
-  return r < 0.0f ? r + fabsf(b) : r;
-}
+}  // namespace ftd
+#include "ft_math.h"
+namespace ftd {
 
-__device__ __forceinline__ float3 rot_x(float3 v, float s, float c) { return mk(v.x, c * v.y - s * v.z, s * v.y + c * v.z); }
-__device__ __forceinline__ float3 rot_y(float3 v, float s, float c) { return mk(c * v.x + s * v.z, v.y, -s * v.x + c * v.z); }
-__device__ __forceinline__ float3 rot_z(float3 v, float s, float c) { return mk(c * v.x - s * v.y, s * v.x + c * v.y, v.z); }
 
 // to_local (renderer.rs:6216-6222)
 __device__ __forceinline__ float3 node_local(const DNode& n, float3 p) {
@@ -136,199 +109,6 @@ __device__ __forceinline__ float3 object_local(int slot, unsigned obj, float3 p)
   q = rot_y(q, o[5], o[6]);
   q = rot_x(q, o[7], o[8]);
   return q;
-}
-
-constexpr unsigned kAabbFlops = kAabbFlopsHost;
-
-// point_aabb_lower_bound (renderer.rs:5388-5411)
-__device__ __forceinline__ float aabb_lb(float3 p, const float* mn, const float* mx) {
-  float dx = p.x < mn[0] ? mn[0] - p.x : (p.x > mx[0] ? p.x - mx[0] : 0.0f);
-  float dy = p.y < mn[1] ? mn[1] - p.y : (p.y > mx[1] ? p.y - mx[1] : 0.0f);
-  float dz = p.z < mn[2] ? mn[2] - p.z : (p.z > mx[2] ? p.z - mx[2] : 0.0f);
-  return sqrtf(dx * dx + dy * dy + dz * dz);
-}
-
-__device__ __forceinline__ float aabb_lb_lit(float3 p, float x0, float y0, float z0, float x1, float y1, float z1) {
-  const float mn[3] = {x0, y0, z0}, mx[3] = {x1, y1, z1};
-  return aabb_lb(p, mn, mx);
-}
-
-// --------------------------------------------------------- hg_sdf ops
-// (renderer.rs:6011-6137)
-__device__ __forceinline__ float op_smax(float a, float b, float k) {
-  float h = fmaxf(k - fabsf(a - b), 0.0f);
-  return fmaxf(a, b) + h * h * 0.25f / fmaxf(k, 1.0e-6f);
-}
-__device__ __forceinline__ float op_smin(float a, float b, float k) {
-  float h = fmaxf(k - fabsf(a - b), 0.0f);
-  return fminf(a, b) - h * h * 0.25f / fmaxf(k, 1.0e-6f);
-}
-__device__ __forceinline__ float op_union_round(float a, float b, float r) {
-  r = fmaxf(fabsf(r), 1.0e-6f);
-  float ux = fmaxf(r - a, 0.0f), uy = fmaxf(r - b, 0.0f);
-  return fmaxf(r, fminf(a, b)) - sqrtf(ux * ux + uy * uy);
-}
-__device__ __forceinline__ float op_intersect_round(float a, float b, float r) {
-  r = fmaxf(fabsf(r), 1.0e-6f);
-  float ux = fmaxf(r + a, 0.0f), uy = fmaxf(r + b, 0.0f);
-  return fminf(-r, fmaxf(a, b)) + sqrtf(ux * ux + uy * uy);
-}
-constexpr float kInvSqrt2 = 0.70710678118654752440f;
-__device__ __forceinline__ float op_union_chamfer(float a, float b, float r) {
-  return fminf(fminf(a, b), (a - r + b) * kInvSqrt2);
-}
-__device__ __forceinline__ float op_intersect_chamfer(float a, float b, float r) {
-  return fmaxf(fmaxf(a, b), (a + r + b) * kInvSqrt2);
-}
-__device__ __forceinline__ float op_union_columns(float a, float b, float r, float n) {
-  if (a < r && b < r) {
-    n = fmaxf(roundf(n), 2.0f);
-    const float s2 = 1.41421356237309504880f;
-    float cr = r * s2 / ((n - 1.0f) * 2.0f + s2);
-    float x = a + cr, y = b + cr;
-    int ni = (int)n;
-    if (ni % 2 == 1) x += cr;
-    float rx = (x + y) * kInvSqrt2, ry = (y - x) * kInvSqrt2;
-    x = rx;
-    y = ry;
-    x -= s2 * 0.5f * r;
-    x += -cr * s2;
-    if (ni % 2 == 1) y += cr;
-    float size = fmaxf(cr * 2.0f, 1.0e-6f);
-    y = rem_euclid(y + size * 0.5f, size) - size * 0.5f;
-    return sqrtf(x * x + y * y) - cr;
-  }
-  return fminf(a, b);
-}
-__device__ __forceinline__ float op_union_stairs(float a, float b, float r, float n) {
-  n = fmaxf(roundf(n), 1.0f);
-  float s = r / n, u = b - r;
-  return fminf(fminf(a, b), 0.5f * (u + a + fabsf(rem_euclid(u - a + s, 2.0f * s) - s)));
-}
-__device__ __forceinline__ float op_union_soft(float a, float b, float r) {
-  r = fmaxf(fabsf(r), 1.0e-6f);
-  float e = fmaxf(r - fabsf(a - b), 0.0f);
-  return fminf(a, b) - e * e * 0.25f / r;
-}
-
-// noise modifier (renderer.rs:5957-6009)
-__device__ __forceinline__ float fractf_(float x) { return x - floorf(x); }
-constexpr float kInvPi = 0.31830988618379067154f;
-__device__ __forceinline__ float hash_noise(float3 p) {
-  float qx = fractf_(p.x * kInvPi + 0.11f) * 17.0f;
-  float qy = fractf_(p.y * kInvPi + 0.17f) * 17.0f;
-  float qz = fractf_(p.z * kInvPi + 0.13f) * 17.0f;
-  return fractf_(qx * qy * qz * (qx + qy + qz));
-}
-__device__ inline float sd_base_noise(float3 p) {
-  p = add(p, mk(0.5f, 0.5f, 0.5f));
-  float3 i = mk(floorf(p.x), floorf(p.y), floorf(p.z));
-  float3 f = mk(fractf_(p.x), fractf_(p.y), fractf_(p.z));
-  float d = __builtin_huge_valf();
-  for (int ix = 0; ix <= 1; ++ix)
-    for (int iy = 0; iy <= 1; ++iy)
-      for (int iz = 0; iz <= 1; ++iz) {
-        float3 c = mk((float)ix, (float)iy, (float)iz);
-        float r = hash_noise(add(i, c));
-        r = r * r * 0.7f;
-        d = fminf(d, len(sub(f, c)) - r);
-      }
-  return d;
-}
-__device__ inline float apply_noise(float3 p, float d, int octaves, float scale, float lac) {
-  float3 q = mul(p, fmaxf(scale, 1.0e-4f));
-  float os = fmaxf(lac, 1.0e-4f);
-  float s = 1.0f, t = 0.0f;
-  int n = octaves < 1 ? 1 : octaves;
-  for (int k = 0; k < n; ++k) {
-    float nn = s * sd_base_noise(q);
-    nn = op_smax(nn, d - 0.1f * s, 0.3f * s);
-    d = op_smin(nn, d, 0.3f * s);
-    t += d;
-    q = mul(mk(0.00f * q.x + 1.60f * q.y + 1.20f * q.z, -1.60f * q.x + 0.72f * q.y - 0.96f * q.z,
-               -1.20f * q.x - 0.96f * q.y + 1.28f * q.z), os);
-    s *= 0.415f;
-    q.z += -4.33f * t * s;
-  }
-  return d;
-}
-
-// sd_regular_ngon (renderer.rs:6206-6214)
-__device__ inline float sd_ngon(float3 p, int sides, float radius) {
-  float n = (float)(sides < 3 ? 3 : sides);
-  float an = 3.14159265358979323846f / n;
-  float apothem = radius * cosf(an);
-  float angle = atan2f(p.z, p.x);
-  float wrapped = rem_euclid(angle + an, 2.0f * an) - an;
-  return hypotf(p.x, p.z) * cosf(wrapped) - apothem;
-}
-
-// ------------------------------------------------------ Forge functions
-// Expression DAG interpreter; builtin semantics eval.rs:2223-2690.
-__device__ __forceinline__ float smooth01(float t) {
-  t = clampf(t, 0.0f, 1.0f);
-  return t * t * (3.0f - 2.0f * t);
-}
-__device__ inline float hash_noise3(float3 p) { return hash_noise(p); }  // eval.rs:4286-4291 (same constants)
-__device__ inline float lerpf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
-__device__ inline float value_noise_3d(float3 p) {
-  float3 i = mk(floorf(p.x), floorf(p.y), floorf(p.z));
-  float3 f = mk(fractf_(p.x), fractf_(p.y), fractf_(p.z));
-  float3 u = mk(smooth01(f.x), smooth01(f.y), smooth01(f.z));
-  float n000 = hash_noise3(i), n001 = hash_noise3(mk(i.x, i.y, i.z + 1.0f));
-  float n010 = hash_noise3(mk(i.x, i.y + 1.0f, i.z)), n011 = hash_noise3(mk(i.x, i.y + 1.0f, i.z + 1.0f));
-  float n100 = hash_noise3(mk(i.x + 1.0f, i.y, i.z)), n101 = hash_noise3(mk(i.x + 1.0f, i.y, i.z + 1.0f));
-  float n110 = hash_noise3(mk(i.x + 1.0f, i.y + 1.0f, i.z)), n111 = hash_noise3(mk(i.x + 1.0f, i.y + 1.0f, i.z + 1.0f));
-  float nxy0 = lerpf(lerpf(n000, n100, u.x), lerpf(n010, n110, u.x), u.y);
-  float nxy1 = lerpf(lerpf(n001, n101, u.x), lerpf(n011, n111, u.x), u.y);
-  return lerpf(nxy0, nxy1, u.z) * 2.0f - 1.0f;
-}
-__device__ inline float fbm_3d(float3 p, int octaves, float scale, float lac) {
-  float3 q = mul(p, scale);
-  float amp = 0.5f, sum = 0.0f;
-  float l = fabsf(lac) < 1.1920929e-7f ? 1.0f : lac;
-  int n = octaves < 1 ? 1 : octaves;
-  for (int k = 0; k < n; ++k) {
-    sum += amp * value_noise_3d(q);
-    q = mul(q, l);
-    amp *= 0.55f;
-  }
-  return sum;
-}
-
-
-// Forge builtins shared by the expression interpreter below and the emitted functions.
-__device__ __forceinline__ float3 splat3(float x) { return mk(x, x, x); }
-__device__ __forceinline__ float saturatef_(float x) { return clampf(x, 0.0f, 1.0f); }
-__device__ __forceinline__ float fclamp_any(float x, float a, float b) {  // clamp with either bound order
-  return a <= b ? clampf(x, a, b) : clampf(x, b, a);
-}
-__device__ __forceinline__ float smoothstep_builtin(float e0, float e1, float x) {
-  const float span = e1 - e0;
-  float t;
-  if (fabsf(span) < 1.1920929e-7f) t = x < e0 ? 0.0f : 1.0f;
-  else t = clampf((x - e0) / span, 0.0f, 1.0f);
-  return t * t * (3.0f - 2.0f * t);
-}
-__device__ __forceinline__ float3 normalize_div(float3 a) {
-  const float l = len(a);
-  return l <= 1.1920929e-7f ? mk(0, 0, 0) : mk(a.x / l, a.y / l, a.z / l);
-}
-__device__ __forceinline__ float3 rotate_deg_x(float3 a, float deg) { const float r = deg2rad(deg); return rot_x(a, sinf(r), cosf(r)); }
-__device__ __forceinline__ float3 rotate_deg_y(float3 a, float deg) { const float r = deg2rad(deg); return rot_y(a, sinf(r), cosf(r)); }
-__device__ __forceinline__ float3 rotate_deg_z(float3 a, float deg) { const float r = deg2rad(deg); return rot_z(a, sinf(r), cosf(r)); }
-__device__ __forceinline__ float box_dist_builtin(float3 a, float3 b) {
-  const float3 q = mk(fabsf(a.x) - b.x, fabsf(a.y) - b.y, fabsf(a.z) - b.z);
-  return len(mk(fmaxf(q.x, 0), fmaxf(q.y, 0), fmaxf(q.z, 0))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
-}
-__device__ __forceinline__ float cylinder_dist_builtin(float3 a, float radius, float half_height) {
-  const float radial = sqrtf(a.x * a.x + a.z * a.z) - radius, axial = fabsf(a.y) - half_height;
-  const float ox = fmaxf(radial, 0.0f), oy = fmaxf(axial, 0.0f);
-  return sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, axial), 0.0f);
-}
-__device__ __forceinline__ float torus_dist_builtin(float3 a, float major, float minor) {
-  const float qx = sqrtf(a.x * a.x + a.z * a.z) - major;
-  return sqrtf(qx * qx + a.y * a.y) - minor;
 }
 
 #if defined(FT_EMITTED)
@@ -413,34 +193,6 @@ __device__ inline float custom_distance(int slot, const DNode& n, float3 p) {
   return d;
 }
 
-__device__ __forceinline__ float shell_op(float d, float shell) { return shell > 0.0f ? fmaxf(d, -(d + shell)) : d; }
-
-// primitive distances in the local frame (renderer.rs:5545-5663)
-__device__ __forceinline__ float sdf_sphere(float3 q, float radius, float shell) {
-  return shell_op(len(q) - radius, shell);
-}
-__device__ __forceinline__ float sdf_box(float3 q, float hx, float hy, float hz, float rd, float shell) {
-  float3 a = mk(fabsf(q.x) - fmaxf(hx - rd, 0.0f), fabsf(q.y) - fmaxf(hy - rd, 0.0f), fabsf(q.z) - fmaxf(hz - rd, 0.0f));
-  float outside = len(mk(fmaxf(a.x, 0.0f), fmaxf(a.y, 0.0f), fmaxf(a.z, 0.0f)));
-  float inside = fminf(fmaxf(fmaxf(a.x, a.y), a.z), 0.0f);
-  return shell_op(outside + inside - rd, shell);
-}
-__device__ __forceinline__ float sdf_cylinder(float3 q, float radius, float hh, float rd, float shell) {
-  float dx = sqrtf(q.x * q.x + q.z * q.z) - fmaxf(radius - rd, 0.0f);
-  float dy = fabsf(q.y) - fmaxf(hh - rd, 0.0f);
-  float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
-  return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(dx, dy), 0.0f) - rd, shell);
-}
-__device__ __forceinline__ float sdf_torus(float3 q, float major, float minor) {
-  float qx = sqrtf(q.x * q.x + q.z * q.z) - major;
-  return sqrtf(qx * qx + q.y * q.y) - minor;
-}
-__device__ __noinline__ float sdf_ngon(float3 q, int sides, float radius, float hh, float rd, float shell) {
-  float radial = sd_ngon(mk(q.x, 0.0f, q.z), sides, fmaxf(radius - rd, 0.0f));
-  float dy = fabsf(q.y) - fmaxf(hh - rd, 0.0f);
-  float ox = fmaxf(radial, 0.0f), oy = fmaxf(dy, 0.0f);
-  return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, dy), 0.0f) - rd, shell);
-}
 __device__ __forceinline__ float prim_distance(const DNode& n, float3 p) {
   float3 q = node_local(n, p);
   switch (n.kind) {
@@ -743,108 +495,21 @@ __device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
 }
 
 // ---------------------------------------------------------------- march
-struct Hit {
-  float t;
-  float3 pos, n;
-  bool front;
-  unsigned obj, mat;
-  int leaf;
+// The march itself is ft_march.h; the scene evaluation it samples is scene_info.
+struct SceneEval {
+  int slot;
+  __device__ __forceinline__ Info operator()(float3 p) const { return scene_info(slot, p); }
 };
 
-// ray_aabb_intersection (renderer.rs:4141-4173)
-__device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, const float* mx, float& t0, float& t1) {
-  float tmin = -__builtin_huge_valf(), tmax = __builtin_huge_valf();
-  for (int axis = 0; axis < 3; ++axis) {
-    float oo = comp(o, axis), dd = comp(d, axis);
-    if (fabsf(dd) <= 1.0e-8f) {
-      if (oo < mn[axis] || oo > mx[axis]) return false;
-      continue;
-    }
-    float inv = 1.0f / dd;
-    float a = (mn[axis] - oo) * inv, c = (mx[axis] - oo) * inv;
-    if (a > c) { float t = a; a = c; c = t; }
-    tmin = fmaxf(tmin, a);
-    tmax = fminf(tmax, c);
-    if (tmax < tmin) return false;
-  }
-  t0 = tmin;
-  t1 = tmax;
-  return true;
-}
+}  // namespace ftd
+#include "ft_march.h"
+namespace ftd {
 
-// raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215).
-// One evaluation site: the march, the 10 bisection steps, the 6 central differences and
-// the final distance_info are phases of a single loop, so a specialised scene function is
-// inlined once and lanes in different phases still share every evaluation.
 __device__ __noinline__ bool raymarch(int slot, float3 o, float3 d, int max_steps, float eps,
                                       float step_scale, float min_t, float max_t, Hit& hit) {
   const DevScene& S = scene_of(slot);
-  float entry, exit;
-  if (!ray_aabb(o, d, S.bmin, S.bmax, entry, exit)) return false;
-  max_t = fminf(max_t, exit);
-  float traveled = fmaxf(min_t, fmaxf(entry, 0.0f));
-  float prev = traveled;
-  const float sscale = clampf(step_scale, 0.05f, 1.0f);
-  const float min_step = fmaxf(eps * 0.5f, 1.0e-5f);
-  const float e = fmaxf(eps * 4.0f, 1.0e-5f);
-  int phase = 0, k = 0, steps = 0;
-  float lo = 0.0f, hi = 0.0f, mid = 0.0f, dplus = 0.0f;
-  float3 pos = mk(0, 0, 0), grad = mk(0, 0, 0);
-  for (;;) {
-    float3 p;
-    if (phase == 0) {
-      if (steps >= max_steps || traveled > max_t) return false;
-      p = add(o, mul(d, traveled));
-    } else if (phase == 1) {
-      mid = 0.5f * (lo + hi);
-      p = add(o, mul(d, mid));
-    } else if (phase == 2) {
-      const float off = (k & 1) ? -e : e;
-      const int axis = k >> 1;
-      p = mk(pos.x + (axis == 0 ? off : 0.0f), pos.y + (axis == 1 ? off : 0.0f), pos.z + (axis == 2 ? off : 0.0f));
-    } else {
-      p = pos;
-    }
-    const Info inf = scene_info(slot, p);
-    if (phase == 0) {
-      if (fabsf(inf.d) < eps) {
-        phase = 1;
-        k = 0;
-        lo = prev;
-        hi = traveled;
-      } else {
-        prev = traveled;
-        traveled += fmaxf(fabsf(inf.d) * sscale, min_step);
-        ++steps;
-      }
-    } else if (phase == 1) {
-      if (fabsf(inf.d) < eps) hi = mid; else lo = mid;
-      if (++k == 10) {
-        pos = add(o, mul(d, hi));
-        phase = 2;
-        k = 0;
-      }
-    } else if (phase == 2) {
-      if ((k & 1) == 0) {
-        dplus = inf.d;
-      } else {
-        const float g = dplus - inf.d;
-        const int axis = k >> 1;
-        grad = mk(axis == 0 ? g : grad.x, axis == 1 ? g : grad.y, axis == 2 ? g : grad.z);
-      }
-      if (++k == 6) phase = 3;
-    } else {
-      const float3 nrm = len(grad) > 1.0e-8f ? normalize(grad) : mk(0.0f, 1.0f, 0.0f);
-      hit.t = hi;
-      hit.pos = pos;
-      hit.n = nrm;
-      hit.front = dot(nrm, d) < 0.0f;
-      hit.obj = inf.obj;
-      hit.mat = inf.mat;
-      hit.leaf = inf.leaf;
-      return true;
-    }
-  }
+  const SceneEval ev{slot};
+  return raymarch_core(ev, S.bmin, S.bmax, o, d, max_steps, eps, step_scale, min_t, max_t, hit);
 }
 
 }  // namespace ftd

@@ -1,0 +1,116 @@
+// ft_march.h — sphere tracing of one ray against any distance evaluator (raymarch_hit,
+// refine_hit_distance and estimate_normal of the reference).
+#pragma once
+
+#include "ft_math.h"
+
+namespace ftd {
+
+struct Hit {
+  float t;
+  float3 pos, n;
+  bool front;
+  unsigned obj, mat;
+  int leaf;
+};
+
+// ray_aabb_intersection (renderer.rs:4141-4173)
+__device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, const float* mx, float& t0, float& t1) {
+  float tmin = -__builtin_huge_valf(), tmax = __builtin_huge_valf();
+  for (int axis = 0; axis < 3; ++axis) {
+    float oo = comp(o, axis), dd = comp(d, axis);
+    if (fabsf(dd) <= 1.0e-8f) {
+      if (oo < mn[axis] || oo > mx[axis]) return false;
+      continue;
+    }
+    float inv = 1.0f / dd;
+    float a = (mn[axis] - oo) * inv, c = (mx[axis] - oo) * inv;
+    if (a > c) { float t = a; a = c; c = t; }
+    tmin = fmaxf(tmin, a);
+    tmax = fminf(tmax, c);
+    if (tmax < tmin) return false;
+  }
+  t0 = tmin;
+  t1 = tmax;
+  return true;
+}
+
+// raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215).
+// One evaluation site: the march, the 10 bisection steps, the 6 central differences and
+// the final distance_info are phases of a single loop, so a specialised scene function is
+// inlined once and lanes in different phases still share every evaluation.
+// `ev(p)` returns the scene's distance info at p (fields d, obj, mat, leaf); bmin/bmax
+// are the scene bounds the ray is clipped to.
+template <class Eval>
+__device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin, const float* bmax, float3 o, float3 d,
+                                              int max_steps, float eps, float step_scale, float min_t, float max_t,
+                                              Hit& hit) {
+  float entry, exit;
+  if (!ray_aabb(o, d, bmin, bmax, entry, exit)) return false;
+  max_t = fminf(max_t, exit);
+  float traveled = fmaxf(min_t, fmaxf(entry, 0.0f));
+  float prev = traveled;
+  const float sscale = clampf(step_scale, 0.05f, 1.0f);
+  const float min_step = fmaxf(eps * 0.5f, 1.0e-5f);
+  const float e = fmaxf(eps * 4.0f, 1.0e-5f);
+  int phase = 0, k = 0, steps = 0;
+  float lo = 0.0f, hi = 0.0f, mid = 0.0f, dplus = 0.0f;
+  float3 pos = mk(0, 0, 0), grad = mk(0, 0, 0);
+  for (;;) {
+    float3 p;
+    if (phase == 0) {
+      if (steps >= max_steps || traveled > max_t) return false;
+      p = add(o, mul(d, traveled));
+    } else if (phase == 1) {
+      mid = 0.5f * (lo + hi);
+      p = add(o, mul(d, mid));
+    } else if (phase == 2) {
+      const float off = (k & 1) ? -e : e;
+      const int axis = k >> 1;
+      p = mk(pos.x + (axis == 0 ? off : 0.0f), pos.y + (axis == 1 ? off : 0.0f), pos.z + (axis == 2 ? off : 0.0f));
+    } else {
+      p = pos;
+    }
+    const auto inf = ev(p);
+    if (phase == 0) {
+      if (fabsf(inf.d) < eps) {
+        phase = 1;
+        k = 0;
+        lo = prev;
+        hi = traveled;
+      } else {
+        prev = traveled;
+        traveled += fmaxf(fabsf(inf.d) * sscale, min_step);
+        ++steps;
+      }
+    } else if (phase == 1) {
+      if (fabsf(inf.d) < eps) hi = mid; else lo = mid;
+      if (++k == 10) {
+        pos = add(o, mul(d, hi));
+        phase = 2;
+        k = 0;
+      }
+    } else if (phase == 2) {
+      if ((k & 1) == 0) {
+        dplus = inf.d;
+      } else {
+        const float g = dplus - inf.d;
+        const int axis = k >> 1;
+        grad = mk(axis == 0 ? g : grad.x, axis == 1 ? g : grad.y, axis == 2 ? g : grad.z);
+      }
+      if (++k == 6) phase = 3;
+    } else {
+      const float3 nrm = len(grad) > 1.0e-8f ? normalize(grad) : mk(0.0f, 1.0f, 0.0f);
+      hit.t = hi;
+      hit.pos = pos;
+      hit.n = nrm;
+      hit.front = dot(nrm, d) < 0.0f;
+      hit.obj = inf.obj;
+      hit.mat = inf.mat;
+      hit.leaf = inf.leaf;
+      return true;
+    }
+  }
+}
+
+}  // namespace ftd

@@ -34,7 +34,10 @@ EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_dest
            "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting",
            "ft_emit_scene_source", "ft_emit_compile", "ft_batch_create", "ft_batch_destroy",
            "ft_batch_render_device", "ft_batch_render", "ft_batch_synchronize", "ft_batch_stats",
-           "ft_batch_set_flop_counting"]
+           "ft_batch_set_flop_counting", "ft_diff_create", "ft_diff_destroy", "ft_diff_param_count",
+           "ft_diff_set_params", "ft_diff_get_params", "ft_diff_forward", "ft_diff_forward_device",
+           "ft_diff_synchronize", "ft_diff_backward_device", "ft_diff_set_target", "ft_diff_loss_grad",
+           "ft_diff_fit", "ft_diff_stats"]
 
 
 class BackendUnavailable(RuntimeError):

@@ -274,5 +274,49 @@ def batch_256(n_scenes: int = 256, views_per_scene: int = 4, size: int = 256, lo
     return BatchWorkload("batch_256", sources, views, opts, settings, AccelMode.Bvh, counts)
 
 
+@dataclass
+class AdjointWorkload:
+    name: str
+    n_prims: int
+    params_true: List[float]
+    params_init: List[float]
+    views: List[tuple]
+    options: RenderOptions
+    light_dir: tuple
+    normal_step: float
+    steps: int
+    lr: float
+
+
+def gauss(rng: SplitMix64) -> float:
+    """Standard normal by Box-Muller from two SplitMix64 uniforms."""
+    u1 = max(rng.uniform(), 1.0e-12)
+    u2 = rng.uniform()
+    return math.sqrt(-2.0 * math.log(u1)) * math.cos(2.0 * math.pi * u2)
+
+
+def adjoint_fit(n_prims: int = 32, n_views: int = 16, size: int = 512, seed: int = 5, sigma: float = 0.05,
+                steps: int = 200) -> AdjointWorkload:
+    """Differentiable inverse fit: n_prims spheres (centre U[-2,2]^3, radius U[0.4,0.9]) in a
+    balanced union_round tree (k U[0.1,0.4]); n_views cameras on a sphere of radius 7;
+    target rendered from these parameters, the fit starts from them perturbed by
+    N(0, sigma)."""
+    rng = SplitMix64(seed)
+    th = []
+    for _ in range(n_prims):
+        th += [rng.uniform(-2.0, 2.0), rng.uniform(-2.0, 2.0), rng.uniform(-2.0, 2.0), rng.uniform(0.4, 0.9)]
+    th += [rng.uniform(0.1, 0.4) for _ in range(n_prims - 1)]
+    init = [v + sigma * gauss(rng) for v in th]
+    views = []
+    for k in range(n_views):
+        z = rng.uniform(-0.5, 0.9)
+        phi = rng.uniform(0.0, 2.0 * math.pi)
+        r = math.sqrt(max(1.0 - z * z, 0.0))
+        views.append((0, (7.0 * r * math.cos(phi), 7.0 * z, 7.0 * r * math.sin(phi)), (0.0, 0.0, 0.0), 45.0))
+    opts = RenderOptions(width=size, height=size, max_steps=256, max_dist=30.0, epsilon=1.0e-5, step_scale=0.7,
+                         camera_z=6.0, fov_y_degrees=45.0)
+    return AdjointWorkload("adjoint_fit", n_prims, th, init, views, opts, (0.4, 1.0, 0.3), 1.0e-2, steps, 5.0e-3)
+
+
 WORKLOADS = {"depth_512": depth_512, "whitted_1080p": whitted_1080p, "bricks_4096": bricks_4096,
-             "batch_256": batch_256}
+             "batch_256": batch_256, "adjoint_fit": adjoint_fit}

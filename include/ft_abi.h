@@ -410,6 +410,43 @@ int ft_batch_stats(ft_batch* batch, ft_render_stats* out);
 /* FLOP counting over every scene of the batch (see ft_scene_set_flop_counting). */
 int ft_batch_set_flop_counting(ft_batch* batch, int enable);
 
+/* --------------------------------------------------------- differentiable rendering
+ * Gradients of depth and Lambert shade with respect to scene parameters.  No counterpart
+ * in the reference (it renders only); the forward is its march (raymarch_hit
+ * renderer.rs:4091-4139) on a parametric scene.
+ * Scene: n_prims (1..48) spheres joined by a balanced union_round tree (split [lo, mid) + [mid, hi),
+ * mid = lo + (hi - lo) / 2).  Parameters, n_params = 5 * n_prims - 1:
+ *   [cx, cy, cz, r] per sphere, then the blend radius k of each join in post order.
+ * Per view (1 spp, pixel centres) the forward gives t* (the march's hit distance refined
+ * by one Newton step, -1 on a miss) and s = max(0, n . light) with n the central-difference
+ * normal of step normal_step at o + t* d.  Gradients use pixels whose 3x3 neighbourhood
+ * hits (interior pixels) and are summed in FP64. */
+typedef struct ft_diff ft_diff;
+
+int ft_diff_create(uint32_t n_prims, const ft_view* views, uint32_t n_views, const ft_render_options* options,
+                   const float light_dir[3], float normal_step, ft_diff** out);
+int ft_diff_destroy(ft_diff* diff);
+int ft_diff_param_count(ft_diff* diff, uint32_t* n_params);
+int ft_diff_set_params(ft_diff* diff, const float* params);
+int ft_diff_get_params(ft_diff* diff, float* params);
+/* Forward into host images (n_views * width * height floats each), synchronous. */
+int ft_diff_forward(ft_diff* diff, float* t_out, float* s_out);
+/* Forward into device images (either may be null: the result is still kept for backward). */
+int ft_diff_forward_device(ft_diff* diff, void* t_dev, void* s_dev);
+int ft_diff_synchronize(ft_diff* diff);
+/* dL/dparams (FP64, host) for upstream per-pixel gradients dL/dt*, dL/ds (device images;
+ * either may be null) at the parameters and images of the last forward.  Synchronous. */
+int ft_diff_backward_device(ft_diff* diff, const void* grad_t_dev, const void* grad_s_dev, double* grad_out);
+/* Target images (host) for the built-in L2 objective
+ *   L = sum over pixels interior to both hit masks of w_t (t* - T)^2 + w_s (s - S)^2. */
+int ft_diff_set_target(ft_diff* diff, const float* t_target, const float* s_target);
+/* L and dL/dparams at the current parameters (FP64, host). */
+int ft_diff_loss_grad(ft_diff* diff, float w_t, float w_s, double* loss_out, double* grad_out);
+/* `steps` Adam steps on the device (beta1 0.9, beta2 0.999, eps 1e-8); losses_out
+ * (nullable) receives L before each step.  Synchronous. */
+int ft_diff_fit(ft_diff* diff, uint32_t steps, float lr, float w_t, float w_s, double* losses_out);
+int ft_diff_stats(ft_diff* diff, ft_render_stats* out);
+
 /* Scene specialisation diagnostics (no GPU needed).  ft_emit_scene_source writes the
  * translation unit the emitter produces for `ir` under `accel_mode` (always emitted, the
  * FT_EMIT policy aside) into buf (cap bytes, NUL-terminated, *len_out = full length);

@@ -2020,3 +2020,191 @@ int ftref_scene_bounds(const ft_scene_ir* ir, float* min3, float* max3, float* c
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------- differentiable sphere tree
+// The workload of the ft_diff_* entry points, which have no counterpart in the reference:
+// n spheres joined by a balanced tree of union_round (op_union_round R:6029-6034), split
+// [lo, mid) + [mid, hi) with mid = lo + (hi - lo) / 2, joins numbered in post order.
+// Parameters: centre xyz and radius per sphere, then the blend radius k per join.
+// Per pixel: march (raymarch_hit R:4091-4139 with its bisection), one Newton step on the
+// hit distance, t* = t - f(x)/(grad f(x) . d), then the Lambert shade max(0, n . l) with n
+// from central differences of step `normal_step` at x* = o + t* d.
+namespace {
+
+struct DiffNode {
+  int leaf;      // sphere index, or -1
+  int join;      // join index, or -1
+  int lhs, rhs;  // node indices of the children of a join
+};
+
+void diff_build(int lo, int hi, std::vector<DiffNode>& out, int& joins) {
+  if (hi - lo == 1) {
+    out.push_back({lo, -1, -1, -1});
+    return;
+  }
+  const int mid = lo + (hi - lo) / 2;
+  diff_build(lo, mid, out, joins);
+  const int l = (int)out.size() - 1;
+  diff_build(mid, hi, out, joins);
+  const int r = (int)out.size() - 1;
+  out.push_back({-1, joins++, l, r});
+}
+
+template <class T> struct DiffTree {
+  std::vector<DiffNode> nodes;
+  std::vector<T> th;  // parameters
+  int n = 0;
+  Aabb<T> bounds;
+
+  V3<T> centre(int i) const { return V3<T>(th[4 * i], th[4 * i + 1], th[4 * i + 2]); }
+  T radius(int i) const { return th[4 * i + 3]; }
+  T k(int j) const { return th[4 * n + j]; }
+
+  T eval(V3<T> p, std::vector<T>& v) const {
+    for (size_t a = 0; a < nodes.size(); ++a) {
+      const DiffNode& d = nodes[a];
+      if (d.leaf >= 0) v[a] = (p - centre(d.leaf)).length() - radius(d.leaf);
+      else v[a] = op_union_round(v[d.lhs], v[d.rhs], k(d.join));
+    }
+    return v.back();
+  }
+  T value(V3<T> p) const {
+    std::vector<T> v(nodes.size());
+    return eval(p, v);
+  }
+  // value and analytic gradient with respect to p (reverse mode through the tree)
+  T value_grad(V3<T> p, V3<T>& g) const {
+    std::vector<T> v(nodes.size()), adj(nodes.size(), T(0));
+    const T f = eval(p, v);
+    adj.back() = T(1);
+    g = V3<T>(0, 0, 0);
+    for (size_t a = nodes.size(); a-- > 0;) {
+      const DiffNode& d = nodes[a];
+      if (d.leaf >= 0) {
+        V3<T> q = p - centre(d.leaf);
+        T l = q.length();
+        if (l > T(0)) g = g + q * (adj[a] / l);
+        continue;
+      }
+      const T av = v[d.lhs], bv = v[d.rhs];
+      const T r = rmax(std::fabs(k(d.join)), T(1.0e-6));
+      const T ux = rmax(r - av, T(0)), uy = rmax(r - bv, T(0));
+      const T m = rmin(av, bv);
+      const T L = std::sqrt(ux * ux + uy * uy);
+      T da = (m > r && av <= bv) ? T(1) : T(0), db = (m > r && !(av <= bv)) ? T(1) : T(0);
+      if (L > T(0)) {
+        da = da + ux / L;
+        db = db + uy / L;
+      }
+      adj[d.lhs] = adj[d.lhs] + adj[a] * da;
+      adj[d.rhs] = adj[d.rhs] + adj[a] * db;
+    }
+    return f;
+  }
+};
+
+template <class T> DiffTree<T> make_diff_tree(uint32_t n, const double* params) {
+  DiffTree<T> D;
+  D.n = (int)n;
+  int joins = 0;
+  diff_build(0, (int)n, D.nodes, joins);
+  D.th.resize(4 * n + (n - 1));
+  for (size_t a = 0; a < D.th.size(); ++a) D.th[a] = (T)params[a];
+  T kmax = T(0);
+  for (int j = 0; j < joins; ++j) kmax = rmax(kmax, std::fabs(D.k(j)));
+  const T m = kmax + T(1.0e-3);
+  for (int i = 0; i < (int)n; ++i) {
+    const V3<T> c = D.centre(i);
+    const T r = std::fabs(D.radius(i)) + m;
+    Aabb<T> b{c - V3<T>::splat(r), c + V3<T>::splat(r)};
+    D.bounds = i == 0 ? b : aabb_union(D.bounds, b);
+  }
+  return D;
+}
+
+// The march of raymarch_hit (R:4091-4139 + bisection R:4180-4200) against the tree.
+template <class T> bool diff_march(const DiffTree<T>& D, V3<T> o, V3<T> d, const Opts<T>& op, T& t_hit) {
+  T entry, exit;
+  if (!ray_aabb(o, d, D.bounds, entry, exit)) return false;
+  const T max_t = rmin(op.max_dist, exit);
+  T traveled = rmax(T(0), rmax(entry, T(0)));
+  T prev = traveled;
+  for (uint32_t s = 0; s < op.max_steps; ++s) {
+    if (traveled > max_t) return false;
+    const T dist = D.value(o + d * traveled);
+    if (std::fabs(dist) < op.epsilon) {
+      T lo = prev, hi = traveled;
+      for (int k = 0; k < 10; ++k) {
+        const T mid = T(0.5) * (lo + hi);
+        if (std::fabs(D.value(o + d * mid)) < op.epsilon) hi = mid; else lo = mid;
+      }
+      t_hit = hi;
+      return true;
+    }
+    prev = traveled;
+    traveled += rmax(std::fabs(dist) * rclamp(op.step_scale, T(0.05), T(1)), rmax(op.epsilon * T(0.5), T(1.0e-5)));
+  }
+  return false;
+}
+
+template <class T>
+int diff_forward_t(uint32_t n, const double* params, const ft_view* views, uint32_t n_views, const ft_render_options* o,
+                   const float light[3], float normal_step, int32_t threads, double* t_out, double* s_out) {
+  const DiffTree<T> D = make_diff_tree<T>(n, params);
+  const Opts<T> op = opts_from<T>(o);
+  const T aspect = (T)op.width / (T)op.height;
+  const V3<T> l = V3<T>(light[0], light[1], light[2]).normalize();
+  const T e = (T)normal_step;
+  const size_t npix = (size_t)op.width * op.height;
+  for (uint32_t v = 0; v < n_views; ++v) {
+    Cam<T> cam;
+    cam.up = V3<T>(0, 1, 0);
+    cam.origin = V3<T>(views[v].origin[0], views[v].origin[1], views[v].origin[2]);
+    cam.target = V3<T>(views[v].target[0], views[v].target[1], views[v].target[2]);
+    cam.fov = (T)views[v].fov_y_degrees;
+    parallel_rows(0, (int32_t)op.height, threads, [&](int32_t y) {
+      for (uint32_t x = 0; x < op.width; ++x) {
+        const size_t idx = (size_t)v * npix + (size_t)y * op.width + x;
+        const T px = (((T)x + T(0.5)) / (T)op.width) * T(2) - T(1);
+        const T py = T(1) - (((T)y + T(0.5)) / (T)op.height) * T(2);
+        V3<T> ro, rd;
+        cam.gen(px * aspect, py, ro, rd);
+        rd = rd.normalize();
+        T th;
+        if (!diff_march(D, ro, rd, op, th)) {
+          t_out[idx] = -1.0;
+          s_out[idx] = 0.0;
+          continue;
+        }
+        V3<T> g;
+        const T f = D.value_grad(ro + rd * th, g);
+        const T gd = g.dot(rd);
+        const T ts = std::fabs(gd) > T(1.0e-6) ? th - f / gd : th;
+        const V3<T> xs = ro + rd * ts;
+        const V3<T> G(D.value(V3<T>(xs.x + e, xs.y, xs.z)) - D.value(V3<T>(xs.x - e, xs.y, xs.z)),
+                      D.value(V3<T>(xs.x, xs.y + e, xs.z)) - D.value(V3<T>(xs.x, xs.y - e, xs.z)),
+                      D.value(V3<T>(xs.x, xs.y, xs.z + e)) - D.value(V3<T>(xs.x, xs.y, xs.z - e)));
+        const T gl = G.length();
+        const V3<T> nrm = gl > T(1.0e-12) ? G * (T(1) / gl) : V3<T>(0, 1, 0);
+        t_out[idx] = (double)ts;
+        s_out[idx] = (double)rmax(nrm.dot(l), T(0));
+      }
+    });
+  }
+  return FT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftref_diff_forward(uint32_t n_prims, const double* params, const ft_view* views, uint32_t n_views,
+                       const ft_render_options* opts, const float light_dir[3], float normal_step, int32_t threads,
+                       int32_t precision, double* t_out, double* s_out) {
+  if (n_prims < 1 || !params || !views || !opts || !light_dir || !t_out || !s_out) return FT_ERR_INVALID_ARGUMENT;
+  if (precision == 1)
+    return diff_forward_t<double>(n_prims, params, views, n_views, opts, light_dir, normal_step, threads, t_out, s_out);
+  return diff_forward_t<float>(n_prims, params, views, n_views, opts, light_dir, normal_step, threads, t_out, s_out);
+}
+
+}  // extern "C"

@@ -54,6 +54,15 @@ int ftref_depth_eval_count(const ft_scene_ir* ir, uint32_t accel, const ft_rende
                            uint32_t aa, int32_t threads, int32_t row0, int32_t row1,
                            uint64_t* evals);
 
+/* Differentiable sphere-tree workload (no reference counterpart; restates the forward of
+ * the ft_diff_* entry points): n_prims spheres in a balanced union_round tree, params =
+ * 4 * n_prims (centre xyz, radius) + n_prims - 1 (blend radius per join, post order).
+ * Per view (1 spp, pixel centres): t_out = Newton-refined hit distance (-1 on a miss),
+ * s_out = max(0, n . light) with n from central differences of step normal_step. */
+int ftref_diff_forward(uint32_t n_prims, const double* params, const ft_view* views, uint32_t n_views,
+                       const ft_render_options* opts, const float light_dir[3], float normal_step, int32_t threads,
+                       int32_t precision, double* t_out, double* s_out);
+
 const char* ftref_last_error(void);
 
 #ifdef __cplusplus

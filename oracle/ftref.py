@@ -32,6 +32,8 @@ def lib():
         L.ftref_eval_function.argtypes = [vp, i32, vp, i32, vp]
         L.ftref_scene_bounds.argtypes = [vp, vp, vp, vp]
         L.ftref_depth_eval_count.argtypes = [vp, u32, vp, u32, i32, i32, i32, vp]
+        L.ftref_diff_forward.argtypes = [C.c_uint32, vp, vp, C.c_uint32, vp, C.POINTER(C.c_float), C.c_float, i32,
+                                         i32, vp, vp]
         L.ftref_last_error.restype = C.c_char_p
         _lib = L
     return _lib
@@ -114,3 +116,23 @@ def depth_eval_count(scene, accel, opts, aa=1, threads=0, rows=None):
     _check(lib().ftref_depth_eval_count(C.addressof(ir), int(accel), C.addressof(o), int(aa), threads, r0, r1,
                                         n.ctypes.data))
     return int(n[0])
+
+
+def diff_forward(n_prims, params, views, opts, light_dir=(0.4, 1.0, 0.3), normal_step=1.0e-2, precision=1,
+                 threads=0):
+    """Forward of the differentiable sphere tree: (t*, s) float64 images [views, H, W]."""
+    import forgedthoughts_amd.abi as abi
+    o = opts.to_c()
+    arr = (abi.View * len(views))()
+    for k, (sc, org, tgt, fov) in enumerate(views):
+        arr[k].scene = sc
+        arr[k].fov_y_degrees = fov
+        arr[k].origin[:] = [float(x) for x in org]
+        arr[k].target[:] = [float(x) for x in tgt]
+    p = np.ascontiguousarray(params, np.float64)
+    t = np.zeros((len(views), o.height, o.width), np.float64)
+    s = np.zeros_like(t)
+    light = (C.c_float * 3)(*[float(x) for x in light_dir])
+    _check(lib().ftref_diff_forward(n_prims, p.ctypes.data, C.addressof(arr), len(views), C.addressof(o), light,
+                                    float(normal_step), threads, precision, t.ctypes.data, s.ctypes.data))
+    return t, s

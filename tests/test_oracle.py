@@ -11,7 +11,11 @@ What pins what:
     (lambert.rs:7-13, renderer.rs:4217-4276), the Fresnel and IOR debug AOVs
     (ray.rs:224-285), the depth and Reinhard byte mappings (renderer.rs:5215-5239);
   * invariants: identical hit masks over naive/bvh/bricks, thread-count determinism, and
-    the f32 march against the same march in f64.
+    the f32 march against the same march in f64;
+  * the differentiable sphere-tree forward (no reference counterpart): the analytic
+    ray/sphere root for one sphere, f32 against f64, and smoothness of the f64 forward in
+    the parameters (central differences of two step sizes agree), which is what makes its
+    finite differences the gradient reference.
 """
 import math
 import os
@@ -244,3 +248,47 @@ def test_f32_march_against_f64():
     assert int(((t32 >= 0) != (t64 >= 0)).sum()) <= 2
     both = (t32 >= 0) & (t64 >= 0)
     assert np.abs(t32[both] - t64[both]).max() < 2 * w.options.epsilon
+
+
+# ------------------------------------------------- differentiable sphere-tree forward
+def _sphere_ray_t(o, d, c, r):
+    oc = np.asarray(o, np.float64) - np.asarray(c, np.float64)
+    b = float(np.dot(oc, d))
+    disc = b * b - (float(np.dot(oc, oc)) - r * r)
+    return -b - math.sqrt(disc) if disc >= 0 else None
+
+
+def test_diff_forward_single_sphere_closed_form():
+    """One sphere: t* is the analytic ray/sphere root; the centre pixel looks along -z."""
+    views = [(0, (0.0, 0.0, 6.0), (0.0, 0.0, 0.0), 30.0)]
+    opts = RenderOptions(9, 9, 256, 30.0, 1.0e-5, 0.7, 6.0, 30.0)
+    for c, r in (((0.0, 0.0, 0.0), 1.0), ((0.0, 0.0, 0.5), 0.7), ((0.0, 0.0, -1.0), 1.5)):
+        t, s = ftref.diff_forward(1, list(c) + [r], views, opts, (0.0, 0.0, 1.0), 1.0e-2, precision=1)
+        exact = _sphere_ray_t((0, 0, 6), (0, 0, -1), c, r)
+        assert abs(t[0, 4, 4] - exact) < 1e-9
+        assert abs(s[0, 4, 4] - 1.0) < 1e-6   # normal (0,0,1) facing the light
+
+
+def test_diff_forward_f32_against_f64_and_smoothness():
+    w = synthetic.adjoint_fit(n_prims=8, n_views=2, size=48)
+    t64, s64 = ftref.diff_forward(w.n_prims, w.params_true, w.views, w.options, w.light_dir, w.normal_step, precision=1)
+    t32, s32 = ftref.diff_forward(w.n_prims, w.params_true, w.views, w.options, w.light_dir, w.normal_step, precision=0)
+    assert int(((t64 >= 0) != (t32 >= 0)).sum()) <= 2
+    m = (t64 >= 0) & (t32 >= 0)
+    assert np.abs(t64 - t32)[m].max() < 1e-4 and np.abs(s64 - s32)[m].max() < 1e-3
+    # the f64 forward is smooth in the parameters on interior pixels, so central
+    # differences of step 1e-4 and 1e-5 agree: it can serve as the gradient reference
+    from scipy.ndimage import binary_erosion
+    interior = binary_erosion(t64 >= 0, structure=np.ones((1, 3, 3)))
+    p = np.array(w.params_true, np.float64)
+
+    def loss(q):
+        t, s = ftref.diff_forward(w.n_prims, q, w.views, w.options, w.light_dir, w.normal_step, precision=1)
+        return float(((t * interior) ** 2).sum() + ((s * interior) ** 2).sum())
+
+    for j in (0, 3, 4 * w.n_prims):
+        e = np.zeros_like(p)
+        e[j] = 1.0
+        g4 = (loss(p + 1e-4 * e) - loss(p - 1e-4 * e)) / 2e-4
+        g5 = (loss(p + 1e-5 * e) - loss(p - 1e-5 * e)) / 2e-5
+        assert abs(g4 - g5) <= 1e-4 * max(abs(g4), 1.0), (j, g4, g5)

@@ -279,6 +279,20 @@ def main():
     rays_per_step = opts.width * opts.height * spp
     value = rays_per_step * args.steps / wall / 1e6
 
+    # the depth preview of the same scene and image size (1 spp), timed the same way
+    depth = None
+    if w.kind == "whitted":
+        for _ in range(args.warmup):
+            gs.render_depth_device(opts, 1, rgb.data_ptr())
+            gs.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.steps):
+            gs.render_depth_device(opts, 1, rgb.data_ptr())
+            gs.synchronize()
+        dwall = (time.perf_counter() - t1) / args.steps
+        depth = {"value": round(opts.width * opts.height / dwall / 1e6, 4), "unit": "Mrays/s",
+                 "ms_per_step": round(dwall * 1e3, 3), "spp": 1}
+
     # work accounting: one extra, untimed frame with FLOP counting on
     gs.set_flop_counting(True)
     step()
@@ -313,6 +327,8 @@ def main():
                      "kernel_ms": round(kernel_ms / args.steps, 3), "stream_ms": round(stream_ms / args.steps, 3),
                      "sdf_flops_per_launch": int(flops)},
     }
+    if depth is not None:
+        line["depth_preview"] = depth
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, scene, args.cpu_seconds)
     gs.close()

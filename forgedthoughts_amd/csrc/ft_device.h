@@ -408,7 +408,20 @@ __device__ __forceinline__ Info leaf_info(int slot, int li, float3 p) {
       return o;
     }
   }
-  Info o = eval_node(slot, L.node, p);
+  const int node = L.node;
+  if (is_prim(S.nodes[node].kind)) {
+    // a single-primitive leaf: evaluated per lane, no program walk
+    const DNode n = S.nodes[node];
+    Info o;
+    o.d = prim_distance(n, p);
+    o.lb = o.d;
+    o.obj = n.obj;
+    o.mat = n.mat;
+    o.leaf = li;
+    if (S.flops) atomicAdd(S.flops, (unsigned long long)n.cost);
+    return o;
+  }
+  Info o = eval_node(slot, node, p);
   o.leaf = li;
   return o;
 }

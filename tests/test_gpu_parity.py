@@ -119,6 +119,16 @@ def test_whitted_1080p_crop(emit):
     whitted_parity(scene, w.accel, w.options, w.settings, "whitted_1080p@192x108", emit)
 
 
+def test_bricks_4096_scene():
+    """4096 primitives + a twisted custom SDF under a hard-union tree (4097 accel leaves):
+    depth in every accel mode and Whitted at 1 spp, against the oracle, at 96x54."""
+    w = synthetic.bricks_4096(width=96, height=54)
+    scene = compile_src(w.source)
+    for accel in (AccelMode.Bvh, AccelMode.Bricks):
+        depth_parity(scene, accel, w.options, 1, f"bricks_4096-depth-{accel.name}")
+    whitted_parity(scene, AccelMode.Bvh, w.options, w.settings, "bricks_4096-whitted")
+
+
 def test_whitted_debug_aovs():
     name, _, opts, settings = REGRESSION_CASES[2]
     scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))

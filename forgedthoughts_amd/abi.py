@@ -100,7 +100,8 @@ class RaySettingsC(C.Structure):
 class RenderStatsC(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("kernel_launches", C.c_uint64),
                 ("last_kernel_ms", C.c_float), ("flags", C.c_uint32), ("sdf_flops", C.c_uint64),
-                ("compile_ms", C.c_double)]
+                ("compile_ms", C.c_double), ("atlas_samples", C.c_uint64), ("atlas_bytes", C.c_uint64),
+                ("atlas_bricks", C.c_uint64)]
 
 
 STATS_EMITTED_SCENE, STATS_EMITTED_FUNCTIONS = 1, 2

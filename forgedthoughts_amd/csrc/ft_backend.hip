@@ -79,6 +79,7 @@ struct ft_scene {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   DevBuf nodes, prog, lbprog, mats, objects, lights, fns, exprs, bvh, leaves;
   DevBuf rgb, tbuf, samples, hdr, overflow, flops;
+  DevBuf atlas_cell_d, atlas_cells, atlas_brick_cell, atlas_bricks, atlas_counter;
   ft_render_stats stats{};
   bool timed = false;
   // scene specialisation
@@ -210,7 +211,7 @@ dim3 tile_grid(int width, int height) {
 int clear_flops(ft_scene* s) {
   s->stats.sdf_flops = 0;
   if (!s->dev.flops) return FT_OK;
-  hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, s->stream, s->dev.flops);
+  hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, s->stream, s->dev.flops, 2);
   FT_HIP(hipGetLastError());
   return FT_OK;
 }
@@ -284,9 +285,10 @@ int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings*
 int finish(ft_scene* s) {
   FT_HIP(hipStreamSynchronize(s->stream));
   if (s->dev.flops) {
-    unsigned long long f = 0;
-    FT_HIP(hipMemcpy(&f, s->dev.flops, sizeof(f), hipMemcpyDeviceToHost));
-    s->stats.sdf_flops = f;
+    unsigned long long f[2] = {0, 0};
+    FT_HIP(hipMemcpy(f, s->dev.flops, sizeof(f), hipMemcpyDeviceToHost));
+    s->stats.sdf_flops = f[0];
+    s->stats.atlas_samples = f[1];
   }
   if (s->timed) {
     float ms = 0.0f;
@@ -410,7 +412,8 @@ int ft_scene_destroy(ft_scene* s) {
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
   for (DevBuf* b : {&s->nodes, &s->prog, &s->lbprog, &s->mats, &s->objects, &s->lights, &s->fns, &s->exprs, &s->bvh,
-                    &s->leaves, &s->rgb, &s->tbuf, &s->samples, &s->hdr, &s->overflow, &s->flops})
+                    &s->leaves, &s->rgb, &s->tbuf, &s->samples, &s->hdr, &s->overflow, &s->flops, &s->atlas_cell_d,
+                    &s->atlas_cells, &s->atlas_brick_cell, &s->atlas_bricks, &s->atlas_counter})
     if (b->ptr) (void)hipFree(b->ptr);
   for (hipModule_t m : s->modules)
     if (m) (void)hipModuleUnload(m);
@@ -485,15 +488,98 @@ int ft_scene_stats(ft_scene* s, ft_render_stats* out) {
   return FT_OK;
 }
 
+int ft_scene_build_atlas(ft_scene* s, float voxel, uint32_t bits, uint32_t max_bricks) {
+  if (!s) return fail(FT_ERR_INVALID_ARGUMENT, "scene is null");
+  FT_HIP(hipSetDevice(s->device));
+  FT_HIP(hipStreamSynchronize(s->stream));
+  if (bits == 0) {
+    s->dev.atlas_on = 0;
+    s->stats.atlas_bytes = 0;
+    s->stats.atlas_bricks = 0;
+    return publish(s);
+  }
+  if (bits != 16 && bits != 32) return fail(FT_ERR_INVALID_ARGUMENT, "atlas bits must be 16 or 32");
+  if (!(voxel > 0.0f)) return fail(FT_ERR_INVALID_ARGUMENT, "voxel size must be positive");
+  if (max_bricks == 0) max_bricks = 1u << 20;
+  AtlasGrid g;
+  const float cell = 8.0f * voxel;
+  g.ox = s->host.bmin[0] - voxel;
+  g.oy = s->host.bmin[1] - voxel;
+  g.oz = s->host.bmin[2] - voxel;
+  g.voxel = voxel;
+  g.nx = (int)std::ceil((s->host.bmax[0] - s->host.bmin[0] + 2.0f * voxel) / cell);
+  g.ny = (int)std::ceil((s->host.bmax[1] - s->host.bmin[1] + 2.0f * voxel) / cell);
+  g.nz = (int)std::ceil((s->host.bmax[2] - s->host.bmin[2] + 2.0f * voxel) / cell);
+  if (!(g.nx > 0 && g.ny > 0 && g.nz > 0) || (double)g.nx * g.ny * g.nz > (double)(1 << 26))
+    return fail(FT_ERR_INVALID_ARGUMENT, "atlas grid too large for this voxel size");
+  g.band = 1.5f * 0.8660254f * cell + 2.0f * voxel;
+  // sample indices of the brick pass stay below 2^31
+  g.max_bricks = (int)std::min<uint32_t>(max_bricks, (uint32_t)(((1u << 31) - 1) / kBrickSamples));
+  const size_t n_cells = (size_t)g.nx * g.ny * g.nz;
+  int rc;
+  if ((rc = ensure(s->atlas_cell_d, n_cells * sizeof(float))) || (rc = ensure(s->atlas_cells, n_cells * sizeof(int))) ||
+      (rc = ensure(s->atlas_brick_cell, (size_t)g.max_bricks * sizeof(int))) ||
+      (rc = ensure(s->atlas_counter, sizeof(unsigned))))
+    return rc;
+  // the build evaluates the exact scene: no coarse steps, no counting
+  s->dev.atlas_on = 0;
+  unsigned long long* saved_flops = s->dev.flops;
+  unsigned long long* saved_samples = s->dev.atlas_samples;
+  s->dev.flops = nullptr;
+  s->dev.atlas_samples = nullptr;
+  if ((rc = publish(s))) return rc;
+  hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, s->stream, (int*)s->atlas_counter.ptr);
+  const unsigned cell_blocks = (unsigned)((n_cells + 255) / 256);
+  hipLaunchKernelGGL(ft_k_atlas_cells, dim3(cell_blocks), dim3(64, 4), stack_lds(s), s->stream, s->slot, g,
+                     (float*)s->atlas_cell_d.ptr, (int*)s->atlas_cells.ptr, (int*)s->atlas_brick_cell.ptr,
+                     (unsigned*)s->atlas_counter.ptr);
+  FT_HIP(hipGetLastError());
+  unsigned count = 0;
+  FT_HIP(hipStreamSynchronize(s->stream));
+  FT_HIP(hipMemcpy(&count, s->atlas_counter.ptr, sizeof(count), hipMemcpyDeviceToHost));
+  const int n_bricks = (int)std::min<unsigned>(count, (unsigned)g.max_bricks);
+  const size_t sample_bytes = bits == 16 ? 2 : 4;
+  if (n_bricks > 0) {
+    if ((rc = ensure(s->atlas_bricks, (size_t)n_bricks * kBrickSamples * sample_bytes))) return rc;
+    const size_t n_samples = (size_t)n_bricks * kBrickSamples;
+    hipLaunchKernelGGL(ft_k_atlas_bricks, dim3((unsigned)((n_samples + 255) / 256)), dim3(64, 4), stack_lds(s),
+                       s->stream, s->slot, g, (const int*)s->atlas_brick_cell.ptr, n_bricks, bits == 16 ? 1 : 0,
+                       s->atlas_bricks.ptr);
+    FT_HIP(hipGetLastError());
+    FT_HIP(hipStreamSynchronize(s->stream));
+  }
+  DevScene& d = s->dev;
+  d.atlas_cells = (const int*)s->atlas_cells.ptr;
+  d.atlas_cell_d = (const float*)s->atlas_cell_d.ptr;
+  d.atlas_bricks = s->atlas_bricks.ptr;
+  d.atlas_origin[0] = g.ox;
+  d.atlas_origin[1] = g.oy;
+  d.atlas_origin[2] = g.oz;
+  d.atlas_voxel = voxel;
+  d.atlas_dims[0] = g.nx;
+  d.atlas_dims[1] = g.ny;
+  d.atlas_dims[2] = g.nz;
+  d.atlas_fp16 = bits == 16 ? 1 : 0;
+  d.atlas_switch = 2.0f * voxel;
+  d.atlas_on = 1;
+  d.flops = saved_flops;
+  d.atlas_samples = saved_samples;
+  s->stats.atlas_bricks = (uint64_t)n_bricks;
+  s->stats.atlas_bytes = (uint64_t)n_cells * (sizeof(float) + sizeof(int)) + (uint64_t)n_bricks * kBrickSamples * sample_bytes;
+  return publish(s);
+}
+
 int ft_scene_set_flop_counting(ft_scene* s, int enable) {
   if (!s) return fail(FT_ERR_INVALID_ARGUMENT, "scene is null");
   FT_HIP(hipStreamSynchronize(s->stream));
   if (enable) {
-    int rc = ensure(s->flops, sizeof(unsigned long long));
+    int rc = ensure(s->flops, 2 * sizeof(unsigned long long));
     if (rc) return rc;
     s->dev.flops = (unsigned long long*)s->flops.ptr;
+    s->dev.atlas_samples = s->dev.flops + 1;
   } else {
     s->dev.flops = nullptr;
+    s->dev.atlas_samples = nullptr;
   }
   return publish(s);
 }
@@ -732,7 +818,7 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
   }
   int launches = 0;
   if (b->counting) {
-    hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, b->stream, (unsigned long long*)b->flops.ptr);
+    hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, b->stream, (unsigned long long*)b->flops.ptr, 2);
     FT_HIP(hipGetLastError());
   }
   FT_HIP(hipEventRecord(b->ev0, b->stream));
@@ -891,9 +977,10 @@ int ft_batch_set_flop_counting(ft_batch* b, int enable) {
   FT_HIP(hipStreamSynchronize(b->stream));
   drop_graph(b);  // a captured graph would not see the new scene descriptors' counter
   int rc = FT_OK;
-  if (enable && (rc = ensure(b->flops, sizeof(unsigned long long)))) return rc;
+  if (enable && (rc = ensure(b->flops, 2 * sizeof(unsigned long long)))) return rc;
   for (ft_scene* s : b->scenes) {
     s->dev.flops = enable ? (unsigned long long*)b->flops.ptr : nullptr;
+    s->dev.atlas_samples = enable ? (unsigned long long*)b->flops.ptr + 1 : nullptr;
     if ((rc = publish(s))) return rc;
   }
   b->counting = enable != 0;

@@ -43,6 +43,17 @@ struct DevScene {
   unsigned long long* flops;             // non-null: count evaluated FLOPs (instrumented runs)
   float bmin[3], bmax[3];                // scene bounds
   float env_rad[3];                      // sum of EnvLight radiance
+  // sampled distance atlas (optional): a grid of cells of 8^3 voxels; cells near the
+  // surface own a brick of 9^3 distance samples (corners of the voxels), the others keep
+  // the exact distance at their centre
+  const int* __restrict__ atlas_cells;   // per cell: brick index, -1 (far: use atlas_cell_d), -2 (exact)
+  const float* __restrict__ atlas_cell_d;
+  const void* __restrict__ atlas_bricks; // 729 samples per brick, FP16 or FP32
+  unsigned long long* atlas_samples;     // non-null: count coarse samples (instrumented runs)
+  float atlas_origin[3], atlas_voxel;
+  int atlas_dims[3], atlas_fp16;
+  int atlas_on;
+  float atlas_switch;                    // exact evaluation below this distance (2 voxels)
 };
 
 // Scene descriptors live in constant memory, one slot per scene handle.  Device functions
@@ -518,11 +529,56 @@ struct SceneEval {
 #include "ft_march.h"
 namespace ftd {
 
+constexpr int kBrickSide = 9;                 // samples per brick edge (8 voxels)
+constexpr int kBrickSamples = kBrickSide * kBrickSide * kBrickSide;
+
+// Approximate |distance| from the atlas, or -1 where the lane must evaluate exactly.
+__device__ __forceinline__ float atlas_distance(const DevScene& S, float3 p) {
+  const float cell = 8.0f * S.atlas_voxel;
+  const float qx = (p.x - S.atlas_origin[0]) / cell, qy = (p.y - S.atlas_origin[1]) / cell,
+              qz = (p.z - S.atlas_origin[2]) / cell;
+  const int cx = (int)floorf(qx), cy = (int)floorf(qy), cz = (int)floorf(qz);
+  if (cx < 0 || cy < 0 || cz < 0 || cx >= S.atlas_dims[0] || cy >= S.atlas_dims[1] || cz >= S.atlas_dims[2])
+    return -1.0f;
+  const size_t idx = ((size_t)cz * S.atlas_dims[1] + cy) * S.atlas_dims[0] + cx;
+  if (S.atlas_samples) atomicAdd(S.atlas_samples, 1ull);
+  const int b = S.atlas_cells[idx];
+  if (b < 0) return b == -1 ? fabsf(S.atlas_cell_d[idx]) - 0.8660254f * cell : -1.0f;
+  // trilinear interpolation inside the brick
+  const float lx = (qx - (float)cx) * 8.0f, ly = (qy - (float)cy) * 8.0f, lz = (qz - (float)cz) * 8.0f;
+  const int ix = min(max((int)lx, 0), 7), iy = min(max((int)ly, 0), 7), iz = min(max((int)lz, 0), 7);
+  const float fx = lx - (float)ix, fy = ly - (float)iy, fz = lz - (float)iz;
+  const size_t base = (size_t)b * kBrickSamples + ((size_t)iz * kBrickSide + iy) * kBrickSide + ix;
+  float c[8];
+  if (S.atlas_fp16) {
+    const _Float16* h = (const _Float16*)S.atlas_bricks + base;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = (float)h[((k >> 2) & 1) * 81 + ((k >> 1) & 1) * 9 + (k & 1)];
+  } else {
+    const float* f = (const float*)S.atlas_bricks + base;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) c[k] = f[((k >> 2) & 1) * 81 + ((k >> 1) & 1) * 9 + (k & 1)];
+  }
+  const float x00 = c[0] + (c[1] - c[0]) * fx, x10 = c[2] + (c[3] - c[2]) * fx;
+  const float x01 = c[4] + (c[5] - c[4]) * fx, x11 = c[6] + (c[7] - c[6]) * fx;
+  const float y0 = x00 + (x10 - x00) * fy, y1 = x01 + (x11 - x01) * fy;
+  return fabsf(y0 + (y1 - y0) * fz);
+}
+
+struct AtlasCoarse {
+  int slot;
+  bool on;
+  float switch_dist;
+  __device__ __forceinline__ bool enabled() const { return on; }
+  __device__ __forceinline__ float operator()(float3 p) const { return atlas_distance(scene_of(slot), p); }
+};
+
 __device__ __noinline__ bool raymarch(int slot, float3 o, float3 d, int max_steps, float eps,
                                       float step_scale, float min_t, float max_t, Hit& hit) {
   const DevScene& S = scene_of(slot);
   const SceneEval ev{slot};
-  return raymarch_core(ev, S.bmin, S.bmax, o, d, max_steps, eps, step_scale, min_t, max_t, hit);
+  const AtlasCoarse coarse{slot, S.atlas_on != 0, S.atlas_switch};
+  return raymarch_core(ev, S.bmin, S.bmax, o, d, max_steps, eps, step_scale, min_t, max_t, hit, coarse);
 }
 
 }  // namespace ftd

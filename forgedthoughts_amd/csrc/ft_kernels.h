@@ -135,6 +135,65 @@ __device__ __forceinline__ void batch_whitted_body(BatchArgs b, TraceParams tp, 
   whitted_tile<kRayStack>(slot, cam, tp, aov, tile, blockIdx.y, samples + (size_t)view * per_view, overflow);
 }
 
+// Distance atlas construction (exact scene distances sampled on a grid).
+struct AtlasGrid {
+  float ox, oy, oz, voxel;
+  int nx, ny, nz;
+  float band;      // cells whose centre distance is within this get a brick
+  int max_bricks;
+};
+
+__device__ __forceinline__ long linear_thread() {
+  return ((long)blockIdx.x * blockDim.y + threadIdx.y) * blockDim.x + threadIdx.x;
+}
+
+// Pass 1, one thread per cell: exact distance at the centre; near-surface cells take a
+// brick slot (a vector atomic on a global counter), or -2 when the slots ran out.
+__device__ __forceinline__ void atlas_cells_body(int slot, AtlasGrid g, float* __restrict__ cell_d,
+                                                 int* __restrict__ cells, int* __restrict__ brick_cell,
+                                                 unsigned* __restrict__ counter) {
+  const long li = linear_thread();
+  const int n = g.nx * g.ny * g.nz;
+  const int i = li < n ? (int)li : n;
+  const int ci = i < n ? i : n - 1;
+  const int cx = ci % g.nx, cy = (ci / g.nx) % g.ny, cz = ci / (g.nx * g.ny);
+  const float cell = 8.0f * g.voxel;
+  const float3 p = mk(g.ox + ((float)cx + 0.5f) * cell, g.oy + ((float)cy + 0.5f) * cell, g.oz + ((float)cz + 0.5f) * cell);
+  const float d = scene_info(slot, p).d;  // every lane evaluates (wave-uniform programs)
+  if (i >= n) return;
+  cell_d[i] = d;
+  int b = -1;
+  if (fabsf(d) <= g.band) {
+    const unsigned k = atomicAdd(counter, 1u);
+    if ((int)k < g.max_bricks) {
+      b = (int)k;
+      brick_cell[k] = i;
+    } else {
+      b = -2;
+    }
+  }
+  cells[i] = b;
+}
+
+// Pass 2, one thread per brick sample: exact distance at the voxel corner.
+__device__ __forceinline__ void atlas_bricks_body(int slot, AtlasGrid g, const int* __restrict__ brick_cell,
+                                                  int n_bricks, int fp16, void* __restrict__ bricks) {
+  const long i = linear_thread();
+  const long n = (long)n_bricks * kBrickSamples;
+  const long si = i < n ? i : n - 1;
+  const int b = (int)(si / kBrickSamples), s = (int)(si % kBrickSamples);
+  const int c = brick_cell[b];
+  const int cx = c % g.nx, cy = (c / g.nx) % g.ny, cz = c / (g.nx * g.ny);
+  const int sx = s % kBrickSide, sy = (s / kBrickSide) % kBrickSide, sz = s / (kBrickSide * kBrickSide);
+  const float cell = 8.0f * g.voxel;
+  const float3 p = mk(g.ox + (float)cx * cell + (float)sx * g.voxel, g.oy + (float)cy * cell + (float)sy * g.voxel,
+                      g.oz + (float)cz * cell + (float)sz * g.voxel);
+  const float d = scene_info(slot, p).d;
+  if (i >= n) return;
+  if (fp16) ((_Float16*)bricks)[i] = (_Float16)d;
+  else ((float*)bricks)[i] = d;
+}
+
 // One thread per pixel; blockIdx.y selects the image of a batch (samples and outputs of
 // image k start at k times the per-image size).
 __device__ __forceinline__ void resolve_kernel_body(CamArgs cam, int aov, const float* __restrict__ samples,
@@ -162,8 +221,8 @@ __device__ __forceinline__ void clear_flag_body(int* flag) {
   if (threadIdx.x == 0) flag[0] = 0;
 }
 
-__device__ __forceinline__ void clear_u64_body(unsigned long long* v) {
-  if (threadIdx.x == 0) v[0] = 0ull;
+__device__ __forceinline__ void clear_u64_body(unsigned long long* v, int n) {
+  if ((int)threadIdx.x < n) v[threadIdx.x] = 0ull;
 }
 
 }  // namespace ftd
@@ -218,6 +277,14 @@ FT_KERNEL void ft_k_resolve(ftd::CamArgs cam, int aov, const float* __restrict__
                             float* __restrict__ hdr) {
   ftd::resolve_kernel_body(cam, aov, samples, rgb, hdr);
 }
+FT_KERNEL void ft_k_atlas_cells(int slot, ftd::AtlasGrid g, float* __restrict__ cell_d, int* __restrict__ cells,
+                                int* __restrict__ brick_cell, unsigned* __restrict__ counter) {
+  ftd::atlas_cells_body(slot, g, cell_d, cells, brick_cell, counter);
+}
+FT_KERNEL void ft_k_atlas_bricks(int slot, ftd::AtlasGrid g, const int* __restrict__ brick_cell, int n_bricks, int fp16,
+                                 void* __restrict__ bricks) {
+  ftd::atlas_bricks_body(slot, g, brick_cell, n_bricks, fp16, bricks);
+}
 FT_KERNEL void ft_k_clear_flag(int* flag) { ftd::clear_flag_body(flag); }
-FT_KERNEL void ft_k_clear_u64(unsigned long long* v) { ftd::clear_u64_body(v); }
+FT_KERNEL void ft_k_clear_u64(unsigned long long* v, int n) { ftd::clear_u64_body(v, n); }
 #endif

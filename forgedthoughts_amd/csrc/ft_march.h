@@ -39,12 +39,22 @@ __device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, co
 // One evaluation site: the march, the 10 bisection steps, the 6 central differences and
 // the final distance_info are phases of a single loop, so a specialised scene function is
 // inlined once and lanes in different phases still share every evaluation.
+// No coarse distance: every march step evaluates the scene.
+struct NoCoarse {
+  __device__ __forceinline__ bool enabled() const { return false; }
+  __device__ __forceinline__ float operator()(float3) const { return -1.0f; }
+  float switch_dist = 0.0f;
+};
+
 // `ev(p)` returns the scene's distance info at p (fields d, obj, mat, leaf); bmin/bmax
-// are the scene bounds the ray is clipped to.
-template <class Eval>
+// are the scene bounds the ray is clipped to.  `coarse(p)`, when enabled, returns an
+// approximate distance (or a negative value where it has none): while it exceeds
+// coarse.switch_dist the march steps on it without evaluating the scene; from the first
+// step below, the lane marches, refines and shades on exact evaluations only.
+template <class Eval, class Coarse = NoCoarse>
 __device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin, const float* bmax, float3 o, float3 d,
                                               int max_steps, float eps, float step_scale, float min_t, float max_t,
-                                              Hit& hit) {
+                                              Hit& hit, const Coarse& coarse = Coarse()) {
   float entry, exit;
   if (!ray_aabb(o, d, bmin, bmax, entry, exit)) return false;
   max_t = fminf(max_t, exit);
@@ -56,11 +66,22 @@ __device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin,
   int phase = 0, k = 0, steps = 0;
   float lo = 0.0f, hi = 0.0f, mid = 0.0f, dplus = 0.0f;
   float3 pos = mk(0, 0, 0), grad = mk(0, 0, 0);
+  bool exact = !coarse.enabled();
   for (;;) {
     float3 p;
     if (phase == 0) {
       if (steps >= max_steps || traveled > max_t) return false;
       p = add(o, mul(d, traveled));
+      if (!exact) {
+        const float dc = coarse(p);
+        if (dc > coarse.switch_dist) {
+          prev = traveled;
+          traveled += fmaxf(dc * sscale, min_step);
+          ++steps;
+          continue;
+        }
+        exact = true;
+      }
     } else if (phase == 1) {
       mid = 0.5f * (lo + hi);
       p = add(o, mul(d, mid));

@@ -37,7 +37,7 @@ EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_dest
            "ft_batch_set_flop_counting", "ft_diff_create", "ft_diff_destroy", "ft_diff_param_count",
            "ft_diff_set_params", "ft_diff_get_params", "ft_diff_forward", "ft_diff_forward_device",
            "ft_diff_synchronize", "ft_diff_backward_device", "ft_diff_set_target", "ft_diff_loss_grad",
-           "ft_diff_fit", "ft_diff_stats"]
+           "ft_diff_fit", "ft_diff_stats", "ft_scene_build_atlas"]
 
 
 class BackendUnavailable(RuntimeError):
@@ -65,6 +65,7 @@ def library():
         L.ft_scene_stats.argtypes = [vp, vp]
         L.ft_abi_struct_size.argtypes = [C.c_char_p]
         L.ft_scene_set_flop_counting.argtypes = [vp, C.c_int]
+        L.ft_scene_build_atlas.argtypes = [vp, C.c_float, u32, u32]
         L.ft_emit_scene_source.argtypes = [vp, u32, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.ft_emit_compile.argtypes = [vp, u32, C.c_int, C.POINTER(C.c_double)]
         L.ft_batch_create.argtypes = [C.POINTER(vp), u32, u32, C.POINTER(vp)]
@@ -159,6 +160,10 @@ class GpuScene:
 
     def set_flop_counting(self, enable: bool):
         _check(library().ft_scene_set_flop_counting(self._h, 1 if enable else 0))
+
+    def build_atlas(self, voxel: float = 0.02, bits: int = 16, max_bricks: int = 0):
+        """Distance atlas for coarse march steps (bits 16 or 32; 0 removes it)."""
+        _check(library().ft_scene_build_atlas(self._h, float(voxel), int(bits), int(max_bricks)))
 
     def stats(self) -> abi.RenderStatsC:
         st = abi.RenderStatsC()

@@ -366,6 +366,9 @@ typedef struct ft_render_stats {
   uint32_t flags;         /* FT_STATS_* */
   uint64_t sdf_flops;     /* FLOPs of SDF evaluation counted by the last render (counting on) */
   double compile_ms;      /* time spent compiling scene-specialised kernels so far */
+  uint64_t atlas_samples; /* distance-atlas samples taken by the last render (counting on) */
+  uint64_t atlas_bytes;   /* device bytes of the scene's distance atlas (0: none) */
+  uint64_t atlas_bricks;  /* bricks in the atlas */
 } ft_render_stats;
 #define FT_STATS_EMITTED_SCENE 1u     /* scene distance runs as specialised straight-line code */
 #define FT_STATS_EMITTED_FUNCTIONS 2u /* Forge functions run as specialised code */
@@ -456,6 +459,18 @@ int ft_diff_stats(ft_diff* diff, ft_render_stats* out);
  * these exist so a host without a GPU can check the emitted code. */
 int ft_emit_scene_source(const ft_scene_ir* ir, uint32_t accel_mode, char* buf, uint64_t cap, uint64_t* len_out);
 int ft_emit_compile(const ft_scene_ir* ir, uint32_t accel_mode, int kernel, double* compile_ms_out);
+
+/* Distance atlas for coarse march steps (opt-in; no counterpart in the reference, whose
+ * Bricks accelerator is a grid of leaf lists, renderer.rs:824-954).  Builds, on the GPU,
+ * a grid of cells of 8^3 voxels of size `voxel` over the scene bounds; cells whose centre
+ * lies within 1.5 half-diagonals + 2 voxels of the surface hold a 9^3 brick of exact
+ * distances stored in `bits` = 16 (FP16) or 32 (FP32); the others keep their centre
+ * distance.  Later renders on the scene step on the atlas while it reads more than two
+ * voxels and evaluate the exact scene from there on (refinement, normal and shading are
+ * always exact).  Hits are the reference's surface points but the step sequence differs,
+ * so t agrees to about 2 * epsilon rather than bit for bit.  bits = 0 removes the atlas.
+ * max_bricks bounds the memory (0: 1 << 20). */
+int ft_scene_build_atlas(ft_scene* scene, float voxel, uint32_t bits, uint32_t max_bricks);
 
 /* Turns FLOP counting on (1) or off (0) for later renders on this scene.  Counting adds
  * atomics to every SDF evaluation: use it for work accounting, never for timing. */

@@ -129,6 +129,31 @@ def test_bricks_4096_scene():
     whitted_parity(scene, AccelMode.Bvh, w.options, w.settings, "bricks_4096-whitted")
 
 
+@pytest.mark.parametrize("bits", [16, 32])
+def test_distance_atlas_march(bits):
+    """Coarse steps on the FP16/FP32 distance atlas, exact evaluation within two voxels of
+    the surface: the same surface points as the exact march, t within 3 epsilon."""
+    w = synthetic.bricks_4096(width=160, height=90)
+    scene = compile_src(w.source)
+    opts = w.options
+    print(f"[atlas] bits={bits} {opts.width}x{opts.height}", flush=True)
+    _, t_c = ftref.render_depth(scene, AccelMode.Bvh, opts, 1)
+    with GpuScene(scene, AccelMode.Bricks) as gs:
+        gs.build_atlas(0.02, bits)
+        st = gs.stats()
+        assert st.atlas_bricks > 0 and st.atlas_bytes > 0
+        _, t_g = gs.render_depth(opts, 1)
+        rgb_w, _ = gs.render_whitted(opts, w.settings)
+    hit_c, hit_g = t_c >= 0, t_g >= 0
+    assert (hit_c != hit_g).mean() <= 2e-3, f"mask mismatch {(hit_c != hit_g).sum()}"
+    both = hit_c & hit_g
+    dt = np.abs(t_c[both] - t_g[both])
+    assert np.quantile(dt, 0.995) <= 3 * opts.epsilon, f"dt q99.5 {np.quantile(dt, 0.995)}"
+    rgb_c, _ = ftref.render_whitted(scene, AccelMode.Bvh, opts, w.settings)
+    d = np.abs(rgb_c.astype(int) - rgb_w.astype(int)).max(axis=2)
+    assert float((d <= 2).mean()) >= 0.99, f"whitted: {(d > 2).mean():.4f} of pixels differ by more than 2"
+
+
 def test_whitted_debug_aovs():
     name, _, opts, settings = REGRESSION_CASES[2]
     scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))

@@ -50,6 +50,13 @@ def library():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise BackendUnavailable(f"{LIB_PATH} is not built; run forgedthoughts_amd/build.py")
+        # One HIP runtime per process: when PyTorch is present its runtime library must be
+        # the one loaded first (ours then binds to it by soname); loaded the other way round,
+        # PyTorch finds no GPU.  Device buffers handed over from torch need the same runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         vp, u32 = C.c_void_p, C.c_uint32
         L.ft_last_error.restype = C.c_char_p

@@ -19,11 +19,15 @@ from oracle import ftref
 pytestmark = pytest.mark.gpu
 
 
-def interior(t):
-    from scipy.ndimage import binary_erosion
+def interior(t, max_jump=0.1):
+    """Pixels whose 3x3 neighbourhood hits the surface with no depth discontinuity: away
+    from silhouettes and occlusion edges, where t is a smooth function of the parameters."""
+    from scipy.ndimage import binary_erosion, maximum_filter, minimum_filter
     hit = t >= 0
     out = binary_erosion(hit, structure=np.ones((1, 3, 3)), border_value=0)
-    return out
+    tt = np.where(hit, t, 0.0)
+    jump = maximum_filter(tt, size=(1, 3, 3)) - minimum_filter(tt, size=(1, 3, 3))
+    return out & (jump < max_jump)
 
 
 @pytest.fixture(scope="module")

@@ -87,6 +87,8 @@ struct ft_scene {
   std::string source;                              // emitted translation unit (empty: interpreter)
   hipModule_t modules[fth::kRtcKernelCount] = {};  // one code object per kernel, built on first use
   hipFunction_t functions[fth::kRtcKernelCount] = {};
+  hipModule_t counted_modules[fth::kRtcKernelCount] = {};  // *_counted variants (FLOP counting on)
+  hipFunction_t counted_functions[fth::kRtcKernelCount] = {};
   double compile_ms = 0.0;
 };
 
@@ -125,6 +127,11 @@ int publish(ft_scene* s) {
     int rc = publish_module(s, m);
     if (rc) return rc;
   }
+  for (hipModule_t m : s->counted_modules) {
+    if (!m) continue;
+    int rc = publish_module(s, m);
+    if (rc) return rc;
+  }
   return FT_OK;
 }
 
@@ -134,30 +141,42 @@ const void* const kKernelStubs[fth::kRtcKernelCount] = {(const void*)ft_k_depth,
                                                         (const void*)ft_k_whitted_48, (const void*)ft_k_whitted_128};
 
 // Builds and loads the scene's specialised code object for march kernel `k` (no-op when the
-// scene runs on the interpreter or the object is loaded).
+// scene runs on the interpreter or the object is loaded).  With FLOP counting on, the
+// *_counted variant is used instead; it is built for every scene (from the interpreter
+// alone when the scene is not specialised).
 int prepare(ft_scene* s, int k) {
-  if (s->source.empty() || s->functions[k]) return FT_OK;
+  const bool counted = s->dev.flops != nullptr;
+  if (counted ? s->counted_functions[k] != nullptr : (s->source.empty() || s->functions[k] != nullptr)) return FT_OK;
+  const std::string src = s->source.empty() ? std::string("#include \"ft_kernels.h\"\n") : s->source;
   std::vector<char> code;
   std::string err;
   double ms = 0.0;
-  if (!fth::rtc_compile(s->source, k, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
+  if (!fth::rtc_compile(src, k, counted, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
   s->compile_ms += ms;
   FT_HIP(hipStreamSynchronize(s->stream));
   hipModule_t m = nullptr;
   FT_HIP(hipModuleLoadData(&m, code.data()));
-  s->modules[k] = m;
-  FT_HIP(hipModuleGetFunction(&s->functions[k], m, kKernelNames[k]));
+  const std::string name = std::string(kKernelNames[k]) + (counted ? "_counted" : "");
+  if (counted) {
+    s->counted_modules[k] = m;
+    FT_HIP(hipModuleGetFunction(&s->counted_functions[k], m, name.c_str()));
+  } else {
+    s->modules[k] = m;
+    FT_HIP(hipModuleGetFunction(&s->functions[k], m, name.c_str()));
+  }
   return publish_module(s, m);
 }
 
-// Launches march kernel `k`: the scene's specialised code object when it has one, else the
-// precompiled kernel.  `args` points at each argument.
+// Launches march kernel `k`: the FLOP-counting variant when counting is on, else the
+// scene's specialised code object when it has one, else the precompiled kernel.  `args`
+// points at each argument.
 int launch(ft_scene* s, int k, dim3 grid, dim3 block, size_t lds, void** args) {
   int rc = prepare(s, k);
   if (rc) return rc;
-  if (s->functions[k]) {
-    FT_HIP(hipModuleLaunchKernel(s->functions[k], grid.x, grid.y, grid.z, block.x, block.y, block.z, (unsigned)lds,
-                                 s->stream, args, nullptr));
+  hipFunction_t f = s->dev.flops ? s->counted_functions[k] : s->functions[k];
+  if (f) {
+    FT_HIP(hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y, block.z, (unsigned)lds, s->stream, args,
+                                 nullptr));
     return FT_OK;
   }
   FT_HIP(hipLaunchKernel(kKernelStubs[k], grid, block, args, lds, s->stream));
@@ -417,6 +436,8 @@ int ft_scene_destroy(ft_scene* s) {
     if (b->ptr) (void)hipFree(b->ptr);
   for (hipModule_t m : s->modules)
     if (m) (void)hipModuleUnload(m);
+  for (hipModule_t m : s->counted_modules)
+    if (m) (void)hipModuleUnload(m);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -608,7 +629,7 @@ int ft_emit_compile(const ft_scene_ir* ir, uint32_t accel_mode, int kernel, doub
   const std::string src = fth::emit_scene_source(h, ir, fth::plan_emission(h, ir));
   std::vector<char> code;
   double ms = 0.0;
-  if (!fth::rtc_compile(src, kernel, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
+  if (!fth::rtc_compile(src, kernel, false, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
   if (compile_ms_out) *compile_ms_out = ms;
   return FT_OK;
 }

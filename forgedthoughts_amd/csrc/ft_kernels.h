@@ -232,27 +232,33 @@ __device__ __forceinline__ void clear_u64_body(unsigned long long* v, int n) {
 
 // FT_RTC_KERNEL selects the one kernel a runtime-compiled (scene-specialised) code object
 // holds: 0 depth, 1/2/3 Whitted with a 16/48/128-entry ray stack.  The precompiled library
-// holds them all.
+// holds them all.  FT_COUNTED names the kernel *_counted: the FLOP-counting launches of
+// instrumented runs then show up apart from the timed ones in a kernel trace.
+#if defined(FT_COUNTED)
+#define FT_KNAME(n) n##_counted
+#else
+#define FT_KNAME(n) n
+#endif
 #if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 0
-FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_depth(int slot, ftd::CamArgs cam, ftd::TraceParams tp, unsigned char* __restrict__ rgb,
+FT_KERNEL FT_MARCH_OCCUPANCY void FT_KNAME(ft_k_depth)(int slot, ftd::CamArgs cam, ftd::TraceParams tp, unsigned char* __restrict__ rgb,
                                              float* __restrict__ t_out) {
   ftd::depth_kernel_body(slot, cam, tp, rgb, t_out);
 }
 #endif
 #if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 1
-FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_16(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
+FT_KERNEL FT_MARCH_OCCUPANCY void FT_KNAME(ft_k_whitted_16)(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
                                                   float* __restrict__ samples, int* __restrict__ overflow) {
   ftd::whitted_kernel_body<16>(slot, cam, tp, aov, samples, overflow);
 }
 #endif
 #if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 2
-FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_48(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
+FT_KERNEL FT_MARCH_OCCUPANCY void FT_KNAME(ft_k_whitted_48)(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
                                                   float* __restrict__ samples, int* __restrict__ overflow) {
   ftd::whitted_kernel_body<48>(slot, cam, tp, aov, samples, overflow);
 }
 #endif
 #if !defined(FT_RTC_KERNEL) || FT_RTC_KERNEL == 3
-FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_whitted_128(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
+FT_KERNEL FT_MARCH_OCCUPANCY void FT_KNAME(ft_k_whitted_128)(int slot, ftd::CamArgs cam, ftd::TraceParams tp, int aov,
                                                    float* __restrict__ samples, int* __restrict__ overflow) {
   ftd::whitted_kernel_body<128>(slot, cam, tp, aov, samples, overflow);
 }

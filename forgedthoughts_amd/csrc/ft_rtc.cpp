@@ -53,10 +53,12 @@ const std::vector<std::string>& options() {
 
 }  // namespace
 
-bool rtc_compile(const std::string& source, int kernel, std::vector<char>& code, double& compile_ms, std::string& err) {
+bool rtc_compile(const std::string& source, int kernel, bool counted, std::vector<char>& code, double& compile_ms,
+                 std::string& err) {
   compile_ms = 0.0;
   std::vector<std::string> opts = options();
   opts.push_back("-DFT_RTC_KERNEL=" + std::to_string(kernel));
+  if (counted) opts.push_back("-DFT_COUNTED=1");
   std::string key_text = source;
   for (const auto& o : opts) key_text += "\n" + o;
   for (int i = 0; i < kRtcHeaderCount; ++i) key_text += kRtcHeaderSources[i];

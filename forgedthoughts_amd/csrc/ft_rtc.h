@@ -11,6 +11,8 @@ enum RtcKernel { kRtcDepth = 0, kRtcWhitted16 = 1, kRtcWhitted48 = 2, kRtcWhitte
 
 // Compiles `source` for gfx950 with only kernel `kernel` instantiated.  Returns the code
 // object in `code`; `compile_ms` is 0 when it came from a cache.
-bool rtc_compile(const std::string& source, int kernel, std::vector<char>& code, double& compile_ms, std::string& err);
+// `counted` builds the *_counted variant (FT_COUNTED).
+bool rtc_compile(const std::string& source, int kernel, bool counted, std::vector<char>& code, double& compile_ms,
+                 std::string& err);
 
 }  // namespace fth

@@ -228,3 +228,21 @@ def test_per_hit_material_hooks_fail_loudly():
         with pytest.raises(RenderError) as e:
             gs.render_whitted(opts, RaySettings(2, 96, 1))
         assert e.value.kind == "UnsupportedObjectType"
+
+
+@pytest.mark.parametrize("emit", ["0", "1"], ids=["interp", "emitted"])
+def test_flop_counting_run_matches(emit, monkeypatch):
+    """The instrumented (FLOP-counting) kernels render the same image as the timed ones."""
+    monkeypatch.setenv("FT_EMIT", emit)
+    w = synthetic.whitted_1080p(width=96, height=54, spp=1)
+    scene = compile_src(w.source)
+    with GpuScene(scene, w.accel) as gs:
+        a, _ = gs.render_whitted(w.options, w.settings)
+        gs.set_flop_counting(True)
+        b, _ = gs.render_whitted(w.options, w.settings)
+        flops = gs.stats().sdf_flops
+        gs.set_flop_counting(False)
+        c, _ = gs.render_whitted(w.options, w.settings)
+    assert flops > 0
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a, c)

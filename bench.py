@@ -225,6 +225,7 @@ def main():
     ap.add_argument("--workload", default="whitted_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--scenes", type=int, default=0, help="batch workloads: number of scenes (default: the workload's)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
@@ -238,7 +239,7 @@ def main():
     from forgedthoughts_amd.render import GpuScene
     from forgedthoughts_amd.scene import compile_scene
 
-    w = synthetic.WORKLOADS[args.workload]()
+    w = synthetic.WORKLOADS[args.workload](n_scenes=args.scenes) if args.scenes else synthetic.WORKLOADS[args.workload]()
     if isinstance(w, synthetic.BatchWorkload):
         return bench_batch(args, w)
     if isinstance(w, synthetic.AdjointWorkload):

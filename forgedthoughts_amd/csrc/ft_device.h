@@ -231,7 +231,9 @@ extern __shared__ float ft_lds[];
 
 // Evaluates the program range [pf, pl] at p: sdf_distance_info + sdf_lower_bound of the
 // subtree root (renderer.rs:5413-5955).
-__device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int node, float3 p) {
+// `base`: first value-stack entry this evaluation may use (entries below belong to a
+// caller that is itself walking a node program).
+__device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int node, float3 p, int base = 0) {
   // the bounds and every instruction word are wave-uniform; readfirstlane makes that
   // visible to the compiler so node fetches are scalar loads
   const auto nodes = uniform_ptr(S.nodes);
@@ -240,9 +242,9 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
   const int pl = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_last);
   const int nth = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-  float* __restrict__ sd = ft_lds + tid;
-  float* __restrict__ slb = ft_lds + S.stack_depth * nth + tid;
-  int* __restrict__ sid = reinterpret_cast<int*>(ft_lds + 2 * S.stack_depth * nth) + tid;
+  float* __restrict__ sd = ft_lds + base * nth + tid;
+  float* __restrict__ slb = ft_lds + (S.stack_depth + base) * nth + tid;
+  int* __restrict__ sid = reinterpret_cast<int*>(ft_lds + (2 * S.stack_depth + base) * nth) + tid;
   float3 pst[kMaxPointStack];
   int sp = 0, pp = 0;
   for (int k = pf; k <= pl; ++k) {
@@ -358,20 +360,29 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
   return out;
 }
 
+// Runs f(key) once per distinct value of `key` among the active lanes, each time for the
+// lanes holding that value (a waterfall).  The leader's value is read with a lane index
+// from the ballot, so the compiler cannot mistake `key` for a uniform value.
+template <class F>
+__device__ __forceinline__ void for_each_uniform(int key, F&& f) {
+  unsigned long long pending = __builtin_amdgcn_ballot_w64(true);
+  while (pending) {
+    const int leader = (int)__builtin_ctzll(pending);
+    const int lead = __builtin_amdgcn_readlane(key, leader);
+    const bool mine = key == lead;
+    pending &= ~__builtin_amdgcn_ballot_w64(mine);
+    if (mine) f(lead);
+  }
+}
+
 // Evaluates the subtree of `node` at p for every active lane.  Lanes may ask for different
 // subtrees (BVH leaves, tree walks): the loop serves one distinct subtree per iteration, so
 // each program run is wave-uniform.
-__device__ __noinline__ Info eval_node(int slot, int node, float3 p) {
+__device__ __noinline__ Info eval_node(int slot, int node, float3 p, int base = 0) {
   const DevScene& S = scene_of(slot);
+  const int b = __builtin_amdgcn_readfirstlane(base);
   Info out;
-  bool done = false;
-  while (!done) {
-    const int lead = __builtin_amdgcn_readfirstlane(node);
-    if (node == lead) {
-      out = eval_uniform(S, slot, lead, p);
-      done = true;
-    }
-  }
+  for_each_uniform(node, [&](int lead) { out = eval_uniform(S, slot, lead, p, b); });
   return out;
 }
 

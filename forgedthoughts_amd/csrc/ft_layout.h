@@ -26,7 +26,10 @@ struct DNode {
   int fn_distance, fn_post, prog_first, prog_last;
   float cx, cy, cz, cost;               // cost: FLOPs of one evaluation of this node
   float szs, szc, sys, syc;             // sin/cos of (-rot_z), (-rot_y)
-  float sxs, sxc, pad0, pad1;           // sin/cos of (-rot_x)
+  float sxs, sxc;                       // sin/cos of (-rot_x)
+  int walk_split;                       // shading sweep from this node: smooth node whose children's
+                                        // materials are traced apart, -1 none, -2 use the per-node walks
+  int walk_ok;                          // 1: the shading sweep from this node fits its stacks
   float p[8];
   float bmin[4];
   float bmax[4];

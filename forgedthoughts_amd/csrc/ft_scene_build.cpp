@@ -383,6 +383,29 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     walk_req(n.rhs, frames + 1, mats);
   };
   for (int ln : leaf_nodes) walk_req(ln, 0, 0);
+
+  // shading sweep (one post-order pass that yields the tree-walk normal and materials):
+  // stack depth with modifier / noise subtrees as single terminals, and the split node
+  std::vector<int> sweep_depth(N, 1);
+  for (uint32_t i = 0; i < N; ++i) {  // children precede parents in the IR
+    const ft_node& n = ir->nodes[i];
+    if (n.kind <= FT_NODE_NOISE) sweep_depth[i] = 1;
+    else if (n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) sweep_depth[i] = sweep_depth[n.lhs];
+    else sweep_depth[i] = std::max(sweep_depth[n.lhs], sweep_depth[n.rhs] + 1);
+  }
+  for (uint32_t i = 0; i < N; ++i) {
+    int split = -1, j = (int)i;
+    for (;;) {
+      const uint32_t k = ir->nodes[j].kind;
+      if (is_blend(k)) { split = j; break; }
+      if (k == FT_NODE_SMOOTH) { j = ir->nodes[j].lhs; continue; }
+      if (k == FT_NODE_NOISE || k == FT_NODE_UNION || k == FT_NODE_INTERSECT || k == FT_NODE_SUBTRACT)
+        split = (j == (int)i && k != FT_NODE_NOISE) ? -1 : -2;  // hard nodes below Smooth / Noise: per-node walks
+      break;
+    }
+    S.nodes[i].walk_split = split;
+    S.nodes[i].walk_ok = (sweep_depth[i] <= ftd::kMaxWalkMats && sweep_depth[i] <= S.max_stack) ? 1 : 0;
+  }
   if (S.max_walk_frames > ftd::kMaxWalk || S.max_walk_mats > ftd::kMaxWalkMats) {
     err = "smooth-boolean nesting inside one accel leaf is deeper than supported";
     return false;

@@ -186,7 +186,7 @@ __device__ __noinline__ float3 walk_normal(int slot, int start, float3 p, float 
 }
 
 // resolve_material_from_node (renderer.rs:6945-7059), from `start`
-__device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& out) {
+__device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& out, int base = 0) {
   const DevScene& S = scene_of(slot);
   Frame fr[kMaxWalk];
   Mat res[kMaxWalkMats];
@@ -201,14 +201,14 @@ __device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& o
       continue;
     }
     if (is_hard(n.kind)) {
-      float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+      float dl = eval_node(slot, n.lhs, p, base).d, dr = eval_node(slot, n.rhs, p, base).d;
       fr[fsp++] = {hard_left(n.kind, dl, dr) ? n.lhs : n.rhs, 0};
       continue;
     }
     if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
     if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
     --rsp;
-    float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+    float dl = eval_node(slot, n.lhs, p, base).d, dr = eval_node(slot, n.rhs, p, base).d;
     blend_materials(res[rsp - 1], res[rsp], blend_t(n, dl, dr));
   }
   out = res[0];
@@ -233,6 +233,194 @@ __device__ __noinline__ bool split_material(int slot, int start, float3 p, Mat& 
     t = blend_t(n, dl, dr);
     return true;
   }
+}
+
+// ----------------------------------------------------------- shading sweep
+// walk_normal, walk_material and split_material above descend the tree from the hit leaf
+// and, at every smooth boolean, evaluate both child subtrees again (O(n log n) subtree
+// evaluations per hit, as the reference does).  The sweep below produces the same three
+// results in one post-order pass over the node program: each node's distance and lower
+// bound (exactly as eval_uniform computes them), its walk normal and its walk material are
+// built from its children's.  Hard booleans above the first non-hard node are resolved
+// first per lane (the walks' own side choice); modifier and noise subtrees, terminals of
+// the walks, use the per-node functions.
+
+// tetrahedral normal of a terminal node (estimate_node_normal renderer.rs:4554-4570)
+template <class F>
+__device__ __forceinline__ float3 tetra_normal(F&& f, float3 p, float eps) {
+  const float e = fmaxf(eps, 1.0e-5f);
+  const float d1 = f(mk(p.x + e, p.y - e, p.z - e));
+  const float d2 = f(mk(p.x - e, p.y - e, p.z + e));
+  const float d3 = f(mk(p.x - e, p.y + e, p.z - e));
+  const float d4 = f(mk(p.x + e, p.y + e, p.z + e));
+  const float3 n = add(add(add(mk(d1, -d1, -d1), mk(-d2, -d2, d2)), mk(-d3, d3, -d3)), mk(d4, d4, d4));
+  if (len(n) > 1.0e-6f) return normalize(n);
+  return mk(0.0f, 1.0f, 0.0f);
+}
+
+struct ShadeWalk {
+  float3 normal;
+  Mat a;        // the walk material, or (split) the split node's lhs material
+  Mat b;        // (split) the split node's rhs material
+  float split_t;
+  bool split;
+};
+
+// The sweep of node N (wave-uniform).  Stack entries: distance / lower bound in the LDS
+// value planes, normal and material per lane in registers / private memory.
+__device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot, int N, float3 p, float eps,
+                                                    ShadeWalk& w) {
+  const auto nodes = uniform_ptr(S.nodes);
+  const auto prog = uniform_ptr(S.prog);
+  const int pf = __builtin_amdgcn_readfirstlane(S.nodes[N].prog_first);
+  const int pl = __builtin_amdgcn_readfirstlane(S.nodes[N].prog_last);
+  const int split_node = __builtin_amdgcn_readfirstlane(S.nodes[N].walk_split);
+  const int nth = blockDim.x * blockDim.y;
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+  float* __restrict__ sd = ft_lds + tid;
+  float* __restrict__ slb = ft_lds + S.stack_depth * nth + tid;
+  Mat ms[kMaxWalkMats];
+  float3 ns[kMaxWalkMats];
+  int sp = 0;
+  w.split = false;
+  for (int k = pf; k <= pl; ++k) {
+    const unsigned ins = prog[k];
+    const int ni = (int)(ins & ~kBeginFlag);
+    const DNode n = load_uniform(nodes + ni);
+    if (ins & kBeginFlag) {
+      // modifier / noise subtree: one terminal of the walks
+      const Info v = eval_uniform(S, slot, ni, p, sp);
+      sd[sp * nth] = v.d;
+      slb[sp * nth] = v.lb;
+      ns[sp] = tetra_normal([&](float3 q) { return eval_uniform(S, slot, ni, q, sp).d; }, p, eps);
+      walk_material(slot, ni, p, ms[sp], sp);
+      ++sp;
+      k = n.prog_last;
+      continue;
+    }
+    const int kind = n.kind;
+    if (is_prim(kind)) {
+      if (S.flops) atomicAdd(S.flops, 5ull * (unsigned long long)n.cost);  // value + 4 normal samples
+      const float d = prim_distance(n, p);
+      sd[sp * nth] = d;
+      slb[sp * nth] = d;
+      ns[sp] = tetra_normal([&](float3 q) { return prim_distance(n, q); }, p, eps);
+      leaf_material(slot, n.mat, n.obj, p, ms[sp]);
+      ++sp;
+      continue;
+    }
+    if (kind == FT_NODE_CUSTOM) {
+      const float3 q = node_local(n, p);
+      const float mn[3] = {-n.p[0], -n.p[1], -n.p[2]}, mx[3] = {n.p[0], n.p[1], n.p[2]};
+      sd[sp * nth] = custom_distance(slot, n, q);
+      slb[sp * nth] = aabb_lb(q, mn, mx);
+      ns[sp] = tetra_normal([&](float3 r) { return custom_distance(slot, n, node_local(n, r)); }, p, eps);
+      leaf_material(slot, n.mat, n.obj, p, ms[sp]);
+      ++sp;
+      continue;
+    }
+    const int t = (sp - 1) * nth;
+    if (kind == FT_NODE_SLICE) {
+      const float c = comp(p, (int)n.p[0]);
+      sd[t] = fmaxf(sd[t], fmaxf(n.p[1] - c, c - n.p[2]));
+      if (n.flags & 2) slb[t] = aabb_lb(p, n.bmin, n.bmax);
+      continue;
+    }
+    if (kind == FT_NODE_SMOOTH) {
+      sd[t] -= n.p[0] * 0.1f;
+      slb[t] -= n.p[0] * 0.1f;
+      continue;
+    }
+    // binary: lhs entry l, rhs entry r
+    --sp;
+    const int l = t - nth, r = t;
+    const int il = sp - 1, ir = sp;
+    const float a = sd[l], b = sd[r];
+    if (kind == FT_NODE_UNION || kind == FT_NODE_INTERSECT || kind == FT_NODE_SUBTRACT) {
+      if (kind == FT_NODE_UNION) {
+        const float llb = slb[l], rlb = slb[r];
+        const bool take_l = llb <= rlb ? (a <= rlb || a <= b) : (!(b <= llb) && a <= b);
+        if (!take_l) sd[l] = b;
+        slb[l] = fminf(llb, rlb);
+      } else if (kind == FT_NODE_INTERSECT) {
+        if (!(a >= b)) sd[l] = b;
+        slb[l] = fmaxf(slb[l], slb[r]);
+      } else {
+        if (!(a >= -b)) sd[l] = -b;
+      }
+      // the walks' side choice (renderer.rs:4473-4498, 7000-7025)
+      if (!hard_left(kind, a, b)) {
+        ns[il] = kind == FT_NODE_SUBTRACT ? mul(ns[ir], -1.0f) : ns[ir];
+        ms[il] = ms[ir];
+      }
+      continue;
+    }
+    const float rr = n.p[0], nn = n.p[1];
+    float d;
+    switch (kind) {
+      case FT_NODE_UNION_ROUND: d = op_union_round(a, b, rr); break;
+      case FT_NODE_UNION_CHAMFER: d = op_union_chamfer(a, b, rr); break;
+      case FT_NODE_UNION_COLUMNS: d = op_union_columns(a, b, rr, nn); break;
+      case FT_NODE_UNION_STAIRS: d = op_union_stairs(a, b, rr, nn); break;
+      case FT_NODE_UNION_SOFT: d = op_union_soft(a, b, rr); break;
+      case FT_NODE_INTERSECT_ROUND: d = op_intersect_round(a, b, rr); break;
+      case FT_NODE_INTERSECT_CHAMFER: d = op_intersect_chamfer(a, b, rr); break;
+      case FT_NODE_INTERSECT_COLUMNS: d = -op_union_columns(-a, -b, rr, nn); break;
+      case FT_NODE_INTERSECT_STAIRS: d = -op_union_stairs(-a, -b, rr, nn); break;
+      case FT_NODE_DIFF_ROUND: d = op_intersect_round(a, -b, rr); break;
+      case FT_NODE_DIFF_CHAMFER: d = op_intersect_chamfer(a, -b, rr); break;
+      case FT_NODE_DIFF_COLUMNS: d = -op_union_columns(-a, b, rr, nn); break;
+      case FT_NODE_DIFF_STAIRS: d = -op_union_stairs(-a, b, rr, nn); break;
+      case FT_NODE_PIPE: d = sqrtf(a * a + b * b) - fabsf(rr); break;
+      case FT_NODE_ENGRAVE: d = fmaxf(a, (a + fabsf(rr) - fabsf(b)) * kInvSqrt2); break;
+      case FT_NODE_GROOVE: d = fmaxf(a, fminf(a + fabsf(rr), fabsf(nn) - fabsf(b))); break;
+      default: d = fminf(a, fmaxf(a - fabsf(rr), fabsf(b) - fabsf(nn))); break;  // TONGUE
+    }
+    sd[l] = d;
+    if (n.flags & 2) slb[l] = aabb_lb(p, n.bmin, n.bmax);
+    // blend_surface_normals (renderer.rs:4525-4552)
+    const float tt = blend_t(n, a, b);
+    const float3 left = ns[il];
+    const float3 right = blend_class(kind) == 2 ? mul(ns[ir], -1.0f) : ns[ir];
+    const float3 bn = add(mul(left, 1.0f - tt), mul(right, tt));
+    ns[il] = len(bn) > 1.0e-6f ? normalize(bn) : (tt < 0.5f ? left : right);
+    if (ni == split_node) {  // resolve_split_material_from_node (renderer.rs:7061-7120)
+      w.a = ms[il];
+      w.b = ms[ir];
+      w.split_t = tt;
+      w.split = true;
+    }
+    blend_materials(ms[il], ms[ir], tt);
+  }
+  w.normal = ns[0];
+  if (!w.split) w.a = ms[0];
+}
+
+// The three walks from `start` for one hit: hard booleans first (per lane, the walks'
+// side choice), then the sweep of the node reached, lanes grouped by that node.
+__device__ __noinline__ void shade_walks(int slot, int start, float3 p, float eps, ShadeWalk& w) {
+  const DevScene& S = scene_of(slot);
+  int node = start;
+  float sign = 1.0f;
+  for (;;) {
+    const DNode& n = S.nodes[node];
+    if (!is_hard(n.kind)) break;
+    const float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+    if (hard_left(n.kind, dl, dr)) {
+      node = n.lhs;
+    } else {
+      if (n.kind == FT_NODE_SUBTRACT) sign = -sign;
+      node = n.rhs;
+    }
+  }
+  if (S.nodes[node].walk_split == -2 || !S.nodes[node].walk_ok) {
+    w.normal = walk_normal(slot, start, p, eps);
+    w.split = split_material(slot, start, p, w.a, w.b, w.split_t);
+    if (!w.split) walk_material(slot, start, p, w.a);
+    return;
+  }
+  for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w); });
+  w.normal = mul(w.normal, sign);
 }
 
 // ---------------------------------------------------------------- BSDFs
@@ -603,19 +791,19 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
       continue;
     }
     const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
-    // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449)
-    float3 sn = walk_normal(slot, start, hit.pos, 1.0e-4f);
-    sn = normalize(hit.front ? sn : mul(sn, -1.0f));
+    // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449), and
+    // the hit's material or split materials, from one shading sweep
+    ShadeWalk w;
+    shade_walks(slot, start, hit.pos, 1.0e-4f, w);
+    float3 sn = normalize(hit.front ? w.normal : mul(w.normal, -1.0f));
     const float3 att = cmul(thr, transmittance(med, hit.t));
-    Mat a, b;
-    float t;
-    if (split_material(slot, start, hit.pos, a, b, t)) {
-      float tt = clampf(t, 0.0f, 1.0f);
-      if (1.0f - tt != 0.0f) acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, a, mul(att, 1.0f - tt), st));
-      if (tt != 0.0f) acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, b, mul(att, tt), st));
+    if (w.split) {
+      float tt = clampf(w.split_t, 0.0f, 1.0f);
+      if (1.0f - tt != 0.0f)
+        acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, mul(att, 1.0f - tt), st));
+      if (tt != 0.0f) acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.b, mul(att, tt), st));
     } else {
-      walk_material(slot, start, hit.pos, a);
-      acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, a, att, st));
+      acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, att, st));
     }
   }
   if (st.overflow) *overflow = 1;

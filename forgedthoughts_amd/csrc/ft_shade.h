@@ -88,25 +88,34 @@ __device__ inline void blend_materials(Mat& left, const Mat& right, float t) {
   left = out;
 }
 
-// resolve_leaf_material, static part (renderer.rs:7164-7536)
-__device__ inline void leaf_material(int slot, unsigned mat_id, unsigned obj, float3 pos, Mat& m) {
+// resolve_leaf_material, static part (renderer.rs:7164-7536): the parameter set of a leaf
+// material at `pos`; returns its model
+__device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float3 pos, DParams& a) {
   const DevScene& S = scene_of(slot);
   unsigned id = mat_id < (unsigned)S.n_mats ? mat_id : 0u;
+  a = S.mats[id];
+  const int model = (int)a.model;
+  if (a.pattern == (float)FT_PATTERN_CHECKER3D) {  // resolve_pattern_color (renderer.rs:7855-7873)
+    float3 lp = object_local(slot, obj, pos);
+    int sx = (int)floorf(lp.x * a.pattern_scale), sy = (int)floorf(lp.y * a.pattern_scale),
+        sz = (int)floorf(lp.z * a.pattern_scale);
+    set3(a.color, ((sx + sy + sz) & 1) == 0 ? v3(a.pattern_a) : v3(a.pattern_b));
+  }
+  if (model != TAG_LAMBERT) a.roughness = clampf(a.roughness, 0.0f, 1.0f);
+  if (model == TAG_STANDARD || model == TAG_DIELECTRIC) a.ior = clampf(a.ior, 1.0f, 3.0f);
+  a.emission_strength = fmaxf(a.emission_strength, 0.0f);
+  return model;
+}
+__device__ inline void leaf_material(int slot, unsigned mat_id, unsigned obj, float3 pos, Mat& m) {
   m.blend = 0;
-  m.a = S.mats[id];
-  m.a_model = (int)m.a.model;
+  m.a_model = leaf_params(slot, mat_id, obj, pos, m.a);
   m.b_model = 0;
   m.t = 0.0f;
   m.pad = 0;
-  if (m.a.pattern == (float)FT_PATTERN_CHECKER3D) {  // resolve_pattern_color (renderer.rs:7855-7873)
-    float3 lp = object_local(slot, obj, pos);
-    int sx = (int)floorf(lp.x * m.a.pattern_scale), sy = (int)floorf(lp.y * m.a.pattern_scale),
-        sz = (int)floorf(lp.z * m.a.pattern_scale);
-    set3(m.a.color, ((sx + sy + sz) & 1) == 0 ? v3(m.a.pattern_a) : v3(m.a.pattern_b));
-  }
-  if (m.a_model != TAG_LAMBERT) m.a.roughness = clampf(m.a.roughness, 0.0f, 1.0f);
-  if (m.a_model == TAG_STANDARD || m.a_model == TAG_DIELECTRIC) m.a.ior = clampf(m.a.ior, 1.0f, 3.0f);
-  m.a.emission_strength = fmaxf(m.a.emission_strength, 0.0f);
+}
+__device__ __forceinline__ int leaf_model(const DevScene& S, unsigned mat_id) {
+  unsigned id = mat_id < (unsigned)S.n_mats ? mat_id : 0u;
+  return (int)S.mats[id].model;
 }
 
 __device__ __forceinline__ int blend_class(int k) {
@@ -258,12 +267,95 @@ __device__ __forceinline__ float3 tetra_normal(F&& f, float3 p, float eps) {
   return mk(0.0f, 1.0f, 0.0f);
 }
 
+// Materials of the sweep by reference.  A material value is a single parameter set or a
+// blend of two (blend_materials renderer.rs:7629-7651).  A sweep stack entry keeps the
+// models, the blend weight and two handles: h >= 0 names the primitive or custom node whose
+// leaf material it is (resolved at p when needed), h < 0 the pool entry -1-h holding a
+// parameter set that a blend produced.  A smooth node over two single sets of one model
+// lerps them field by field (blend_params); at t == 0 that is the left set itself and at
+// t == 1 the right one up to rounding (a + (b - a) for b), so only 0 < t < 1 computes
+// parameters.  A two-model blend takes each side's dominant set by handle.  Every pool
+// entry has at most one live reference, so a lerp overwrites the entry of an operand.
+struct MRef {
+  int blend, a_model, b_model;
+  float t;
+  int ha, hb;
+};
+constexpr int kMatPool = 4;
+struct MatPool {
+  DParams e[kMatPool];
+  int n;
+  int overflow;
+};
+__device__ __forceinline__ void ref_params(int slot, const MatPool& pool, int h, float3 p, DParams& out) {
+  if (h < 0) {
+    out = pool.e[-1 - h];
+  } else {
+    const DNode& n = scene_of(slot).nodes[h];
+    leaf_params(slot, n.mat, n.obj, p, out);
+  }
+}
+__device__ inline void ref_to_mat(int slot, const MatPool& pool, const MRef& r, float3 p, Mat& m) {
+  m.blend = r.blend;
+  m.a_model = r.a_model;
+  m.b_model = r.b_model;
+  m.t = r.t;
+  m.pad = 0;
+  ref_params(slot, pool, r.ha, p, m.a);
+  if (r.blend) ref_params(slot, pool, r.hb, p, m.b);
+}
+__device__ __forceinline__ int pool_take(MatPool& pool) {
+  if (pool.n >= kMatPool) { pool.overflow = 1; return -1; }
+  return pool.n++;
+}
+// a walk material (modifier terminal) into the pool
+__device__ inline MRef ref_of_mat(MatPool& pool, const Mat& m) {
+  MRef r{m.blend, m.a_model, m.blend ? m.b_model : 0, m.blend ? m.t : 0.0f, 0, 0};
+  const int ia = pool_take(pool);
+  if (ia < 0) return r;
+  pool.e[ia] = m.a;
+  r.ha = -1 - ia;
+  if (m.blend) {
+    const int ib = pool_take(pool);
+    if (ib < 0) return r;
+    pool.e[ib] = m.b;
+    r.hb = -1 - ib;
+  }
+  return r;
+}
+// blend_materials on references
+__device__ inline void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
+  if (!l.blend && !r.blend && l.a_model == r.a_model) {
+    if (t == 0.0f) return;
+    if (t == 1.0f) { l = r; return; }
+    const int dst = l.ha < 0 ? -1 - l.ha : (r.ha < 0 ? -1 - r.ha : pool_take(pool));
+    if (dst < 0) return;
+    DParams a, b;
+    ref_params(slot, pool, l.ha, p, a);
+    ref_params(slot, pool, r.ha, p, b);
+    blend_params(a, b, t);
+    pool.e[dst] = a;
+    l.ha = -1 - dst;
+    return;
+  }
+  const bool la = !l.blend || l.t < 0.5f, ra = !r.blend || r.t < 0.5f;
+  MRef out;
+  out.blend = 1;
+  out.a_model = la ? l.a_model : l.b_model;
+  out.ha = la ? l.ha : l.hb;
+  out.b_model = ra ? r.a_model : r.b_model;
+  out.hb = ra ? r.ha : r.hb;
+  out.t = clampf(t, 0.0f, 1.0f);
+  l = out;
+}
+
 struct ShadeWalk {
   float3 normal;
   Mat a;        // the walk material, or (split) the split node's lhs material
   Mat b;        // (split) the split node's rhs material
   float split_t;
   bool split;
+  bool mat_overflow;  // the material pool ran out: materials come from the per-node walks
 };
 
 // The sweep of node N (wave-uniform).  Stack entries: distance / lower bound in the LDS
@@ -279,8 +371,11 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   float* __restrict__ sd = ft_lds + tid;
   float* __restrict__ slb = ft_lds + S.stack_depth * nth + tid;
-  Mat ms[kMaxWalkMats];
+  MRef ms[kMaxWalkMats];
   float3 ns[kMaxWalkMats];
+  MatPool pool;
+  pool.n = 0;
+  pool.overflow = 0;
   int sp = 0;
   w.split = false;
   for (int k = pf; k <= pl; ++k) {
@@ -293,7 +388,9 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
       sd[sp * nth] = v.d;
       slb[sp * nth] = v.lb;
       ns[sp] = tetra_normal([&](float3 q) { return eval_uniform(S, slot, ni, q, sp).d; }, p, eps);
-      walk_material(slot, ni, p, ms[sp], sp);
+      Mat tm;
+      walk_material(slot, ni, p, tm, sp);
+      ms[sp] = ref_of_mat(pool, tm);
       ++sp;
       k = n.prog_last;
       continue;
@@ -305,7 +402,7 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
       sd[sp * nth] = d;
       slb[sp * nth] = d;
       ns[sp] = tetra_normal([&](float3 q) { return prim_distance(n, q); }, p, eps);
-      leaf_material(slot, n.mat, n.obj, p, ms[sp]);
+      ms[sp] = MRef{0, leaf_model(S, n.mat), 0, 0.0f, ni, 0};
       ++sp;
       continue;
     }
@@ -315,7 +412,7 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
       sd[sp * nth] = custom_distance(slot, n, q);
       slb[sp * nth] = aabb_lb(q, mn, mx);
       ns[sp] = tetra_normal([&](float3 r) { return custom_distance(slot, n, node_local(n, r)); }, p, eps);
-      leaf_material(slot, n.mat, n.obj, p, ms[sp]);
+      ms[sp] = MRef{0, leaf_model(S, n.mat), 0, 0.0f, ni, 0};
       ++sp;
       continue;
     }
@@ -385,15 +482,16 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
     const float3 bn = add(mul(left, 1.0f - tt), mul(right, tt));
     ns[il] = len(bn) > 1.0e-6f ? normalize(bn) : (tt < 0.5f ? left : right);
     if (ni == split_node) {  // resolve_split_material_from_node (renderer.rs:7061-7120)
-      w.a = ms[il];
-      w.b = ms[ir];
+      ref_to_mat(slot, pool, ms[il], p, w.a);
+      ref_to_mat(slot, pool, ms[ir], p, w.b);
       w.split_t = tt;
       w.split = true;
     }
-    blend_materials(ms[il], ms[ir], tt);
+    blend_refs(slot, pool, ms[il], ms[ir], tt, p);
   }
   w.normal = ns[0];
-  if (!w.split) w.a = ms[0];
+  w.mat_overflow = pool.overflow != 0;
+  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, ms[0], p, w.a);
 }
 
 // The three walks from `start` for one hit: hard booleans first (per lane, the walks'
@@ -421,6 +519,10 @@ __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float ep
   }
   for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w); });
   w.normal = mul(w.normal, sign);
+  if (w.mat_overflow) {
+    w.split = split_material(slot, start, p, w.a, w.b, w.split_t);
+    if (!w.split) walk_material(slot, start, p, w.a);
+  }
 }
 
 // ---------------------------------------------------------------- BSDFs

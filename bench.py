@@ -217,6 +217,35 @@ def cpu_baseline_adjoint(w, seconds: float):
                       f"{w.options.width}x{w.options.height}; {threads} threads"}
 
 
+def copy_calibration(mib: int):
+    import torch
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n, "copy_calibration_bytes_written": n, "launches": 3}),
+          flush=True)
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "whitted_1080p_counters.md")
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` as measured with the FETCH_SIZE / WRITE_SIZE counters
+    (scaled by a plain copy of known size) and recorded in the counters profile, or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                    return int(line.split("=")[-1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +255,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--scenes", type=int, default=0, help="batch workloads: number of scenes (default: the workload's)")
+    ap.add_argument("--atlas", type=int, default=0, choices=(0, 16, 32),
+                    help="single-scene workloads: coarse march steps on an FP16 / FP32 distance atlas")
+    ap.add_argument("--atlas-voxel", type=float, default=0.02)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device-to-device copy of MIB MiB, three times (the reference "
+                         "kernel for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU); "
               "run with --gpus 1", file=sys.stderr)
@@ -249,6 +286,8 @@ def main():
     dev = torch.device("cuda:0")
     rgb = torch.empty(opts.height * opts.width * 3, dtype=torch.uint8, device=dev)
     gs = GpuScene(scene, w.accel)
+    if args.atlas:
+        gs.build_atlas(args.atlas_voxel, args.atlas)
 
     def step():
         if w.kind == "whitted":
@@ -323,13 +362,24 @@ def main():
                    "specialised": {"scene": bool(st.flags & 1), "functions": bool(st.flags & 2),
                                    "compile_ms": round(st.compile_ms, 1)}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": measured_traffic(f"ft_k_{w.kind}_16" if w.kind == "whitted" else "ft_k_depth"),
                      "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
                      "kernel_ms": round(kernel_ms / args.steps, 3), "stream_ms": round(stream_ms / args.steps, 3),
                      "sdf_flops_per_launch": int(flops)},
     }
     if depth is not None:
         line["depth_preview"] = depth
+    if args.atlas:
+        # coarse steps read 8 brick samples and one cell entry each (no cache credit)
+        per_sample = 8 * (2 if args.atlas == 16 else 4) + 4
+        atlas_bytes = st.atlas_samples * per_sample
+        gbs = atlas_bytes / avg_kernel_s / 1e9 if avg_kernel_s > 0 else 0.0
+        line["config"]["atlas"] = {"bits": args.atlas, "voxel": args.atlas_voxel, "bricks": int(st.atlas_bricks),
+                                   "bytes": int(st.atlas_bytes)}
+        line["atlas_roofline"] = {"bound": "hbm", "achieved": round(gbs, 2), "peak": 8000.0, "unit": "GB/s",
+                                  "frac": round(gbs / 8000.0, 5), "coarse_samples": int(st.atlas_samples),
+                                  "bytes_per_sample": per_sample}
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(w, scene, args.cpu_seconds)
     gs.close()

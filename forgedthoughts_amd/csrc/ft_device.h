@@ -29,6 +29,7 @@ namespace ftd {
 struct DevScene {
   const DNode* __restrict__ nodes;
   const unsigned* __restrict__ prog;
+  const DOp* __restrict__ ops;           // prog as records (plus one padding record)
   const DParams* __restrict__ mats;
   const float* __restrict__ objects;     // 9 floats per object: centre, sin/cos z,y,x
   const DLight* __restrict__ lights;
@@ -224,6 +225,50 @@ struct Info {
 
 __device__ __forceinline__ bool is_prim(int k) { return k <= FT_NODE_EXTRUDE_POLYGON; }
 
+// primitive distance from a program record (prim_distance on the record's copy of the node)
+__device__ __forceinline__ float op_prim_distance(const DOp& o, int kind, float3 p) {
+  float3 q = mk(p.x - o.c[0], p.y - o.c[1], p.z - o.c[2]);
+  if (o.code & kOpRotated) {
+    q = rot_z(q, o.r[0], o.r[1]);
+    q = rot_y(q, o.r[2], o.r[3]);
+    q = rot_x(q, o.r[4], o.r[5]);
+  }
+  switch (kind) {
+    case FT_NODE_SPHERE: return sdf_sphere(q, o.p[0], o.p[1]);
+    case FT_NODE_BOX: return sdf_box(q, o.p[0], o.p[1], o.p[2], o.p[3], o.p[4]);
+    case FT_NODE_CYLINDER: return sdf_cylinder(q, o.p[0], o.p[1], o.p[2], o.p[3]);
+    case FT_NODE_TORUS: return sdf_torus(q, o.p[0], o.p[1]);
+    default: return sdf_ngon(q, (int)o.p[0], o.p[1], o.p[2], o.p[3], o.p[4]);
+  }
+}
+// the record's AABB lower bound (booleans, slice)
+__device__ __forceinline__ float op_aabb_lb(const DOp& o, float3 p) {
+  return aabb_lb_lit(p, o.c[0], o.c[1], o.c[2], o.r[0], o.r[1], o.r[2]);
+}
+// the blend booleans of sdf_distance_info (renderer.rs:5733-5953); keep_l: the ids come
+// from the left operand
+__device__ __forceinline__ float smooth_op(int kind, float a, float b, float rr, float nn, bool& keep_l) {
+  switch (kind) {
+    case FT_NODE_UNION_ROUND: keep_l = a <= b; return op_union_round(a, b, rr);
+    case FT_NODE_UNION_CHAMFER: keep_l = a <= b; return op_union_chamfer(a, b, rr);
+    case FT_NODE_UNION_COLUMNS: keep_l = a <= b; return op_union_columns(a, b, rr, nn);
+    case FT_NODE_UNION_STAIRS: keep_l = a <= b; return op_union_stairs(a, b, rr, nn);
+    case FT_NODE_UNION_SOFT: keep_l = a <= b; return op_union_soft(a, b, rr);
+    case FT_NODE_INTERSECT_ROUND: keep_l = a >= b; return op_intersect_round(a, b, rr);
+    case FT_NODE_INTERSECT_CHAMFER: keep_l = a >= b; return op_intersect_chamfer(a, b, rr);
+    case FT_NODE_INTERSECT_COLUMNS: keep_l = a >= b; return -op_union_columns(-a, -b, rr, nn);
+    case FT_NODE_INTERSECT_STAIRS: keep_l = a >= b; return -op_union_stairs(-a, -b, rr, nn);
+    case FT_NODE_DIFF_ROUND: keep_l = a >= -b; return op_intersect_round(a, -b, rr);
+    case FT_NODE_DIFF_CHAMFER: keep_l = a >= -b; return op_intersect_chamfer(a, -b, rr);
+    case FT_NODE_DIFF_COLUMNS: keep_l = a >= -b; return -op_union_columns(-a, b, rr, nn);
+    case FT_NODE_DIFF_STAIRS: keep_l = a >= -b; return -op_union_stairs(-a, b, rr, nn);
+    case FT_NODE_PIPE: keep_l = true; return fsqrt(a * a + b * b) - fabsf(rr);
+    case FT_NODE_ENGRAVE: keep_l = true; return fmaxf(a, (a + fabsf(rr) - fabsf(b)) * kInvSqrt2);
+    case FT_NODE_GROOVE: keep_l = true; return fmaxf(a, fminf(a + fabsf(rr), fabsf(nn) - fabsf(b)));
+    default: keep_l = true; return fminf(a, fmaxf(a - fabsf(rr), fabsf(b) - fabsf(nn)));  // TONGUE
+  }
+}
+
 // Value stack of the node program: three LDS planes (distance, lower bound, id of the
 // primitive node that produced the value), one column per thread, so a push/pop is one
 // bank-conflict-free ds access even when lanes evaluate different programs.
@@ -234,10 +279,11 @@ extern __shared__ float ft_lds[];
 // `base`: first value-stack entry this evaluation may use (entries below belong to a
 // caller that is itself walking a node program).
 __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int node, float3 p, int base = 0) {
-  // the bounds and every instruction word are wave-uniform; readfirstlane makes that
-  // visible to the compiler so node fetches are scalar loads
+  // the bounds and every record are wave-uniform; readfirstlane makes that visible to the
+  // compiler so record fetches are scalar loads.  Records are streamed in program order and
+  // the next one is requested before the current one is evaluated.
   const auto nodes = uniform_ptr(S.nodes);
-  const auto prog = uniform_ptr(S.prog);
+  const auto ops = uniform_ptr(S.ops);
   const int pf = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_first);
   const int pl = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_last);
   const int nth = blockDim.x * blockDim.y;
@@ -247,37 +293,38 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
   int* __restrict__ sid = reinterpret_cast<int*>(ft_lds + (2 * S.stack_depth + base) * nth) + tid;
   float3 pst[kMaxPointStack];
   int sp = 0, pp = 0;
+  DOp o = load_uniform(ops + pf);
   for (int k = pf; k <= pl; ++k) {
-    const unsigned ins = prog[k];
-    const int ni = (int)(ins & ~kBeginFlag);
-    const DNode n = load_uniform(nodes + ni);
-    if (ins & kBeginFlag) {
+    // read the current record first, then request the next one: the wait for the current
+    // record's load then no longer covers the next, whose latency hides behind this node
+    const int code = o.code;
+    const int kind = code & kOpKindMask;
+    asm volatile("" ::"s"(code));
+    __builtin_amdgcn_sched_barrier(0);
+    const DOp next = load_uniform(ops + k + 1);
+    if (code & (int)kBeginFlag) {
+      const DNode n = load_uniform(nodes + o.node);
       pst[pp++] = p;
       float3 q = node_local(n, p);
       if (n.kind == FT_NODE_DOMAIN_MODIFIER && n.fn_domain >= 0) q = eval_fn(slot, n.fn_domain, &q);
       p = q;
-      continue;
-    }
-    const int kind = n.kind;
-    if (is_prim(kind)) {
-      const float d = prim_distance(n, p);
+    } else if (is_prim(kind)) {
+      const float d = op_prim_distance(o, kind, p);
       sd[sp * nth] = d;
       slb[sp * nth] = d;
-      sid[sp * nth] = ni;
+      sid[sp * nth] = o.node;
       ++sp;
-      continue;
-    }
-    if (kind == FT_NODE_CUSTOM) {
+    } else if (kind == FT_NODE_CUSTOM) {
+      const DNode n = load_uniform(nodes + o.node);
       float3 q = node_local(n, p);
       float mn[3] = {-n.p[0], -n.p[1], -n.p[2]}, mx[3] = {n.p[0], n.p[1], n.p[2]};
       sd[sp * nth] = custom_distance(slot, n, q);
       slb[sp * nth] = aabb_lb(q, mn, mx);
-      sid[sp * nth] = ni;
+      sid[sp * nth] = o.node;
       ++sp;
-      continue;
-    }
-    const int t = (sp - 1) * nth;
-    if (kind == FT_NODE_DOMAIN_MODIFIER || kind == FT_NODE_DISTANCE_POST_MODIFIER || kind == FT_NODE_NOISE) {
+    } else if (kind == FT_NODE_DOMAIN_MODIFIER || kind == FT_NODE_DISTANCE_POST_MODIFIER || kind == FT_NODE_NOISE) {
+      const int t = (sp - 1) * nth;
+      const DNode n = load_uniform(nodes + o.node);
       float3 q = p;
       p = pst[--pp];
       float d = sd[t];
@@ -289,65 +336,40 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
       }
       sd[t] = d;
       if (n.flags & 2) slb[t] = aabb_lb(p, n.bmin, n.bmax);
-      continue;
+    } else if (kind == FT_NODE_SLICE) {
+      const int t = (sp - 1) * nth;
+      float c = comp(p, (int)o.p[0]);
+      sd[t] = fmaxf(sd[t], fmaxf(o.p[1] - c, c - o.p[2]));
+      if (code & kOpNeedsLb) slb[t] = op_aabb_lb(o, p);
+    } else if (kind == FT_NODE_SMOOTH) {
+      const int t = (sp - 1) * nth;
+      sd[t] -= o.p[0] * 0.1f;
+      slb[t] -= o.p[0] * 0.1f;
+    } else {
+      // binary: lhs at l, rhs at r
+      const int r = (sp - 1) * nth, l = r - nth;
+      --sp;
+      const float a = sd[l], b = sd[r];
+      if (kind == FT_NODE_UNION) {
+        // lower-bound short circuit (renderer.rs:5711-5731) evaluated eagerly
+        const float llb = slb[l], rlb = slb[r];
+        const bool take_l = llb <= rlb ? (a <= rlb || a <= b) : (!(b <= llb) && a <= b);
+        if (!take_l) { sd[l] = b; sid[l] = sid[r]; }
+        slb[l] = fminf(llb, rlb);
+      } else if (kind == FT_NODE_INTERSECT) {
+        if (!(a >= b)) { sd[l] = b; sid[l] = sid[r]; }
+        slb[l] = fmaxf(slb[l], slb[r]);
+      } else if (kind == FT_NODE_SUBTRACT) {
+        if (!(a >= -b)) { sd[l] = -b; sid[l] = sid[r]; }
+      } else {
+        bool keep_l;
+        const float d = smooth_op(kind, a, b, o.p[0], o.p[1], keep_l);
+        if (!keep_l) sid[l] = sid[r];
+        sd[l] = d;
+        if (code & kOpNeedsLb) slb[l] = op_aabb_lb(o, p);
+      }
     }
-    if (kind == FT_NODE_SLICE) {
-      float c = comp(p, (int)n.p[0]);
-      sd[t] = fmaxf(sd[t], fmaxf(n.p[1] - c, c - n.p[2]));
-      if (n.flags & 2) slb[t] = aabb_lb(p, n.bmin, n.bmax);
-      continue;
-    }
-    if (kind == FT_NODE_SMOOTH) {
-      sd[t] -= n.p[0] * 0.1f;
-      slb[t] -= n.p[0] * 0.1f;
-      continue;
-    }
-    // binary: lhs at t - nth, rhs at t
-    --sp;
-    const int l = t - nth, r = t;
-    const float a = sd[l], b = sd[r];
-    if (kind == FT_NODE_UNION) {
-      // lower-bound short circuit (renderer.rs:5711-5731) evaluated eagerly
-      const float llb = slb[l], rlb = slb[r];
-      const bool take_l = llb <= rlb ? (a <= rlb || a <= b) : (!(b <= llb) && a <= b);
-      if (!take_l) { sd[l] = b; sid[l] = sid[r]; }
-      slb[l] = fminf(llb, rlb);
-      continue;
-    }
-    if (kind == FT_NODE_INTERSECT) {
-      if (!(a >= b)) { sd[l] = b; sid[l] = sid[r]; }
-      slb[l] = fmaxf(slb[l], slb[r]);
-      continue;
-    }
-    if (kind == FT_NODE_SUBTRACT) {
-      if (!(a >= -b)) { sd[l] = -b; sid[l] = sid[r]; }
-      continue;
-    }
-    const float rr = n.p[0], nn = n.p[1];
-    float d;
-    bool keep_l;
-    switch (kind) {
-      case FT_NODE_UNION_ROUND: d = op_union_round(a, b, rr); keep_l = a <= b; break;
-      case FT_NODE_UNION_CHAMFER: d = op_union_chamfer(a, b, rr); keep_l = a <= b; break;
-      case FT_NODE_UNION_COLUMNS: d = op_union_columns(a, b, rr, nn); keep_l = a <= b; break;
-      case FT_NODE_UNION_STAIRS: d = op_union_stairs(a, b, rr, nn); keep_l = a <= b; break;
-      case FT_NODE_UNION_SOFT: d = op_union_soft(a, b, rr); keep_l = a <= b; break;
-      case FT_NODE_INTERSECT_ROUND: d = op_intersect_round(a, b, rr); keep_l = a >= b; break;
-      case FT_NODE_INTERSECT_CHAMFER: d = op_intersect_chamfer(a, b, rr); keep_l = a >= b; break;
-      case FT_NODE_INTERSECT_COLUMNS: d = -op_union_columns(-a, -b, rr, nn); keep_l = a >= b; break;
-      case FT_NODE_INTERSECT_STAIRS: d = -op_union_stairs(-a, -b, rr, nn); keep_l = a >= b; break;
-      case FT_NODE_DIFF_ROUND: d = op_intersect_round(a, -b, rr); keep_l = a >= -b; break;
-      case FT_NODE_DIFF_CHAMFER: d = op_intersect_chamfer(a, -b, rr); keep_l = a >= -b; break;
-      case FT_NODE_DIFF_COLUMNS: d = -op_union_columns(-a, b, rr, nn); keep_l = a >= -b; break;
-      case FT_NODE_DIFF_STAIRS: d = -op_union_stairs(-a, b, rr, nn); keep_l = a >= -b; break;
-      case FT_NODE_PIPE: d = sqrtf(a * a + b * b) - fabsf(rr); keep_l = true; break;
-      case FT_NODE_ENGRAVE: d = fmaxf(a, (a + fabsf(rr) - fabsf(b)) * kInvSqrt2); keep_l = true; break;
-      case FT_NODE_GROOVE: d = fmaxf(a, fminf(a + fabsf(rr), fabsf(nn) - fabsf(b))); keep_l = true; break;
-      default: d = fminf(a, fmaxf(a - fabsf(rr), fabsf(b) - fabsf(nn))); keep_l = true; break;  // TONGUE
-    }
-    if (!keep_l) sid[l] = sid[r];
-    sd[l] = d;
-    if (n.flags & 2) slb[l] = aabb_lb(p, n.bmin, n.bmax);
+    o = next;
   }
   if (S.flops) atomicAdd(S.flops, (unsigned long long)S.nodes[S.prog[pl] & ~kBeginFlag].cost);
   Info out;

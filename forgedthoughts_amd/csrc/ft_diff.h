@@ -126,7 +126,7 @@ __device__ __forceinline__ float diff_reverse(const DiffArgs& a, float* __restri
     const float r = fmaxf(fabsf(kk), 1.0e-6f);
     const float ux = fmaxf(r - av, 0.0f), uy = fmaxf(r - bv, 0.0f);
     const float m = fminf(av, bv);
-    const float L = sqrtf(ux * ux + uy * uy);
+    const float L = fsqrt(ux * ux + uy * uy);
     float da = (m > r && av <= bv) ? 1.0f : 0.0f, db = (m > r && !(av <= bv)) ? 1.0f : 0.0f;
     float dr = m > r ? 0.0f : 1.0f;
     if (L > 0.0f) {

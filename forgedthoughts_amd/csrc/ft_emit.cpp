@@ -330,7 +330,7 @@ class Emitter {
       case FT_NODE_DIFF_CHAMFER: expr = "op_intersect_chamfer(" + a.d + ", -" + b.d + ", " + r + ")"; keep = a.d + " >= -" + b.d; break;
       case FT_NODE_DIFF_COLUMNS: expr = "-op_union_columns(-" + a.d + ", " + b.d + ", " + r + ", " + nn + ")"; keep = a.d + " >= -" + b.d; break;
       case FT_NODE_DIFF_STAIRS: expr = "-op_union_stairs(-" + a.d + ", " + b.d + ", " + r + ", " + nn + ")"; keep = a.d + " >= -" + b.d; break;
-      case FT_NODE_PIPE: expr = "(sqrtf(" + a.d + " * " + a.d + " + " + b.d + " * " + b.d + ") - fabsf(" + r + "))"; keep = "true"; break;
+      case FT_NODE_PIPE: expr = "(fsqrt(" + a.d + " * " + a.d + " + " + b.d + " * " + b.d + ") - fabsf(" + r + "))"; keep = "true"; break;
       case FT_NODE_ENGRAVE: expr = "fmaxf(" + a.d + ", (" + a.d + " + fabsf(" + r + ") - fabsf(" + b.d + ")) * kInvSqrt2)"; keep = "true"; break;
       case FT_NODE_GROOVE: expr = "fmaxf(" + a.d + ", fminf(" + a.d + " + fabsf(" + r + "), fabsf(" + nn + ") - fabsf(" + b.d + ")))"; keep = "true"; break;
       default: expr = "fminf(" + a.d + ", fmaxf(" + a.d + " - fabsf(" + r + "), fabsf(" + b.d + ") - fabsf(" + nn + ")))"; keep = "true"; break;

@@ -36,6 +36,24 @@ struct DNode {
 };
 static_assert(sizeof(DNode) == 160, "DNode layout");
 
+// One word of the node program as a self-contained 64-byte record (one scalar load), in
+// program order, so the evaluator streams records without a dependent node fetch.
+//   code: node kind | kOpRotated | kOpNeedsLb, or kBeginFlag for a modifier's BEGIN word
+//   node: the DNode index (ids, materials, the slow path of modifiers and custom SDFs)
+//   primitives: c = centre, p = parameters, r = sin/cos of (-rot_z), (-rot_y), (-rot_x)
+//   booleans, slice, smooth: p = parameters; with kOpNeedsLb, c / r[0..2] = AABB min / max
+struct DOp {
+  int code;
+  int node;
+  float c[3];
+  float p[5];
+  float r[6];
+};
+static_assert(sizeof(DOp) == 64, "DOp layout");
+constexpr int kOpKindMask = 0xff;
+constexpr int kOpRotated = 0x100;
+constexpr int kOpNeedsLb = 0x200;
+
 struct DParams {  // MaterialParams (materials/mod.rs:67-87)
   float color[3];
   float roughness;

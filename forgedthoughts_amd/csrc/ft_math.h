@@ -17,7 +17,12 @@ __device__ __forceinline__ float3 sub(float3 a, float3 b) { return mk(a.x - b.x,
 __device__ __forceinline__ float3 mul(float3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
 __device__ __forceinline__ float3 cmul(float3 a, float3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 __device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-__device__ __forceinline__ float len(float3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Square root on the hardware instruction (v_sqrt_f32, within 1 ulp) rather than the
+// correctly rounded sequence sqrtf expands to (about fifteen instructions, most of a
+// primitive's cost).  Distances differ from IEEE sqrt by at most an ulp; arguments here are
+// sums of squares far above the denormal range or exactly zero.
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float len(float3 a) { return fsqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 __device__ __forceinline__ float3 cross(float3 a, float3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
@@ -49,7 +54,7 @@ __device__ __forceinline__ float aabb_lb(float3 p, const float* mn, const float*
   float dx = p.x < mn[0] ? mn[0] - p.x : (p.x > mx[0] ? p.x - mx[0] : 0.0f);
   float dy = p.y < mn[1] ? mn[1] - p.y : (p.y > mx[1] ? p.y - mx[1] : 0.0f);
   float dz = p.z < mn[2] ? mn[2] - p.z : (p.z > mx[2] ? p.z - mx[2] : 0.0f);
-  return sqrtf(dx * dx + dy * dy + dz * dz);
+  return fsqrt(dx * dx + dy * dy + dz * dz);
 }
 
 __device__ __forceinline__ float aabb_lb_lit(float3 p, float x0, float y0, float z0, float x1, float y1, float z1) {
@@ -70,12 +75,12 @@ __device__ __forceinline__ float op_smin(float a, float b, float k) {
 __device__ __forceinline__ float op_union_round(float a, float b, float r) {
   r = fmaxf(fabsf(r), 1.0e-6f);
   float ux = fmaxf(r - a, 0.0f), uy = fmaxf(r - b, 0.0f);
-  return fmaxf(r, fminf(a, b)) - sqrtf(ux * ux + uy * uy);
+  return fmaxf(r, fminf(a, b)) - fsqrt(ux * ux + uy * uy);
 }
 __device__ __forceinline__ float op_intersect_round(float a, float b, float r) {
   r = fmaxf(fabsf(r), 1.0e-6f);
   float ux = fmaxf(r + a, 0.0f), uy = fmaxf(r + b, 0.0f);
-  return fminf(-r, fmaxf(a, b)) + sqrtf(ux * ux + uy * uy);
+  return fminf(-r, fmaxf(a, b)) + fsqrt(ux * ux + uy * uy);
 }
 constexpr float kInvSqrt2 = 0.70710678118654752440f;
 __device__ __forceinline__ float op_union_chamfer(float a, float b, float r) {
@@ -100,7 +105,7 @@ __device__ __forceinline__ float op_union_columns(float a, float b, float r, flo
     if (ni % 2 == 1) y += cr;
     float size = fmaxf(cr * 2.0f, 1.0e-6f);
     y = rem_euclid(y + size * 0.5f, size) - size * 0.5f;
-    return sqrtf(x * x + y * y) - cr;
+    return fsqrt(x * x + y * y) - cr;
   }
   return fminf(a, b);
 }
@@ -226,13 +231,13 @@ __device__ __forceinline__ float box_dist_builtin(float3 a, float3 b) {
   return len(mk(fmaxf(q.x, 0), fmaxf(q.y, 0), fmaxf(q.z, 0))) + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f);
 }
 __device__ __forceinline__ float cylinder_dist_builtin(float3 a, float radius, float half_height) {
-  const float radial = sqrtf(a.x * a.x + a.z * a.z) - radius, axial = fabsf(a.y) - half_height;
+  const float radial = fsqrt(a.x * a.x + a.z * a.z) - radius, axial = fabsf(a.y) - half_height;
   const float ox = fmaxf(radial, 0.0f), oy = fmaxf(axial, 0.0f);
-  return sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, axial), 0.0f);
+  return fsqrt(ox * ox + oy * oy) + fminf(fmaxf(radial, axial), 0.0f);
 }
 __device__ __forceinline__ float torus_dist_builtin(float3 a, float major, float minor) {
-  const float qx = sqrtf(a.x * a.x + a.z * a.z) - major;
-  return sqrtf(qx * qx + a.y * a.y) - minor;
+  const float qx = fsqrt(a.x * a.x + a.z * a.z) - major;
+  return fsqrt(qx * qx + a.y * a.y) - minor;
 }
 
 
@@ -249,20 +254,20 @@ __device__ __forceinline__ float sdf_box(float3 q, float hx, float hy, float hz,
   return shell_op(outside + inside - rd, shell);
 }
 __device__ __forceinline__ float sdf_cylinder(float3 q, float radius, float hh, float rd, float shell) {
-  float dx = sqrtf(q.x * q.x + q.z * q.z) - fmaxf(radius - rd, 0.0f);
+  float dx = fsqrt(q.x * q.x + q.z * q.z) - fmaxf(radius - rd, 0.0f);
   float dy = fabsf(q.y) - fmaxf(hh - rd, 0.0f);
   float ox = fmaxf(dx, 0.0f), oy = fmaxf(dy, 0.0f);
-  return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(dx, dy), 0.0f) - rd, shell);
+  return shell_op(fsqrt(ox * ox + oy * oy) + fminf(fmaxf(dx, dy), 0.0f) - rd, shell);
 }
 __device__ __forceinline__ float sdf_torus(float3 q, float major, float minor) {
-  float qx = sqrtf(q.x * q.x + q.z * q.z) - major;
-  return sqrtf(qx * qx + q.y * q.y) - minor;
+  float qx = fsqrt(q.x * q.x + q.z * q.z) - major;
+  return fsqrt(qx * qx + q.y * q.y) - minor;
 }
 __device__ __noinline__ float sdf_ngon(float3 q, int sides, float radius, float hh, float rd, float shell) {
   float radial = sd_ngon(mk(q.x, 0.0f, q.z), sides, fmaxf(radius - rd, 0.0f));
   float dy = fabsf(q.y) - fmaxf(hh - rd, 0.0f);
   float ox = fmaxf(radial, 0.0f), oy = fmaxf(dy, 0.0f);
-  return shell_op(sqrtf(ox * ox + oy * oy) + fminf(fmaxf(radial, dy), 0.0f) - rd, shell);
+  return shell_op(fsqrt(ox * ox + oy * oy) + fminf(fmaxf(radial, dy), 0.0f) - rd, shell);
 }
 
 }  // namespace ftd

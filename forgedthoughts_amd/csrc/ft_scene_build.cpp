@@ -267,6 +267,30 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
       S.nodes[n.rhs].flags |= 2;
     }
   }
+  // the program as self-contained records
+  S.ops.assign(S.prog.size() + 1, ftd::DOp{});
+  for (size_t k = 0; k < S.prog.size(); ++k) {
+    const unsigned w = S.prog[k];
+    const int i = (int)(w & ~ftd::kBeginFlag);
+    const ftd::DNode& d = S.nodes[i];
+    ftd::DOp& o = S.ops[k];
+    std::memset(&o, 0, sizeof(o));
+    o.node = i;
+    if (w & ftd::kBeginFlag) {
+      o.code = (int)ftd::kBeginFlag;
+      continue;
+    }
+    o.code = d.kind | ((d.flags & 1) ? ftd::kOpRotated : 0) | ((d.flags & 2) ? ftd::kOpNeedsLb : 0);
+    for (int m = 0; m < 5; ++m) o.p[m] = d.p[m];
+    if (d.kind <= FT_NODE_EXTRUDE_POLYGON) {
+      o.c[0] = d.cx; o.c[1] = d.cy; o.c[2] = d.cz;
+      o.r[0] = d.szs; o.r[1] = d.szc; o.r[2] = d.sys; o.r[3] = d.syc; o.r[4] = d.sxs; o.r[5] = d.sxc;
+    } else {
+      for (int m = 0; m < 3; ++m) { o.c[m] = d.bmin[m]; o.r[m] = d.bmax[m]; }
+    }
+  }
+  S.ops.back().code = FT_NODE_UNION;  // padding: read ahead, never executed
+
   // FLOPs of one evaluation of each subtree (work accounting of instrumented runs):
   // FMA = 2, add/mul/min/max/abs/compare = 1, sqrt/sin/cos = 1.
   auto fn_cost = [&](int32_t f) -> float {

@@ -11,6 +11,7 @@ namespace fth {
 struct HostScene {
   std::vector<ftd::DNode> nodes;
   std::vector<unsigned> prog;
+  std::vector<ftd::DOp> ops;  // prog as records, plus one padding record
   std::vector<unsigned> lbprog;
   std::vector<ftd::DParams> mats;
   std::vector<float> objects;  // 9 per object

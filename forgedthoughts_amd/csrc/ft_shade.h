@@ -275,16 +275,17 @@ __device__ __forceinline__ float3 tetra_normal(F&& f, float3 p, float eps) {
 // lerps them field by field (blend_params); at t == 0 that is the left set itself and at
 // t == 1 the right one up to rounding (a + (b - a) for b), so only 0 < t < 1 computes
 // parameters.  A two-model blend takes each side's dominant set by handle.  Every pool
-// entry has at most one live reference, so a lerp overwrites the entry of an operand.
+// entry has at most one live reference: a lerp overwrites the entry of an operand, and an
+// entry whose reference is dropped returns to the pool.
 struct MRef {
   int blend, a_model, b_model;
   float t;
   int ha, hb;
 };
-constexpr int kMatPool = 4;
+constexpr int kMatPool = 8;
 struct MatPool {
   DParams e[kMatPool];
-  int n;
+  unsigned used;  // bit i: entry i is referenced
   int overflow;
 };
 __device__ __forceinline__ void ref_params(int slot, const MatPool& pool, int h, float3 p, DParams& out) {
@@ -305,8 +306,18 @@ __device__ inline void ref_to_mat(int slot, const MatPool& pool, const MRef& r, 
   if (r.blend) ref_params(slot, pool, r.hb, p, m.b);
 }
 __device__ __forceinline__ int pool_take(MatPool& pool) {
-  if (pool.n >= kMatPool) { pool.overflow = 1; return -1; }
-  return pool.n++;
+  const unsigned free_ = ~pool.used & ((1u << kMatPool) - 1u);
+  if (free_ == 0u) { pool.overflow = 1; return -1; }
+  const int i = __builtin_ctz(free_);
+  pool.used |= 1u << i;
+  return i;
+}
+__device__ __forceinline__ void pool_drop(MatPool& pool, int h) {
+  if (h < 0) pool.used &= ~(1u << (-1 - h));
+}
+__device__ __forceinline__ void drop_ref(MatPool& pool, const MRef& r) {
+  pool_drop(pool, r.ha);
+  if (r.blend) pool_drop(pool, r.hb);
 }
 // a walk material (modifier terminal) into the pool
 __device__ inline MRef ref_of_mat(MatPool& pool, const Mat& m) {
@@ -323,11 +334,11 @@ __device__ inline MRef ref_of_mat(MatPool& pool, const Mat& m) {
   }
   return r;
 }
-// blend_materials on references
+// blend_materials on references; r is consumed
 __device__ inline void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
   if (!l.blend && !r.blend && l.a_model == r.a_model) {
-    if (t == 0.0f) return;
-    if (t == 1.0f) { l = r; return; }
+    if (t == 0.0f) { pool_drop(pool, r.ha); return; }
+    if (t == 1.0f) { pool_drop(pool, l.ha); l = r; return; }
     const int dst = l.ha < 0 ? -1 - l.ha : (r.ha < 0 ? -1 - r.ha : pool_take(pool));
     if (dst < 0) return;
     DParams a, b;
@@ -335,10 +346,13 @@ __device__ inline void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& 
     ref_params(slot, pool, r.ha, p, b);
     blend_params(a, b, t);
     pool.e[dst] = a;
+    if (l.ha < 0 && r.ha < 0) pool_drop(pool, r.ha);  // dst is l's entry
     l.ha = -1 - dst;
     return;
   }
   const bool la = !l.blend || l.t < 0.5f, ra = !r.blend || r.t < 0.5f;
+  if (l.blend) pool_drop(pool, la ? l.hb : l.ha);
+  if (r.blend) pool_drop(pool, ra ? r.hb : r.ha);
   MRef out;
   out.blend = 1;
   out.a_model = la ? l.a_model : l.b_model;
@@ -374,7 +388,7 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
   MRef ms[kMaxWalkMats];
   float3 ns[kMaxWalkMats];
   MatPool pool;
-  pool.n = 0;
+  pool.used = 0u;
   pool.overflow = 0;
   int sp = 0;
   w.split = false;
@@ -448,7 +462,10 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
       // the walks' side choice (renderer.rs:4473-4498, 7000-7025)
       if (!hard_left(kind, a, b)) {
         ns[il] = kind == FT_NODE_SUBTRACT ? mul(ns[ir], -1.0f) : ns[ir];
+        drop_ref(pool, ms[il]);
         ms[il] = ms[ir];
+      } else {
+        drop_ref(pool, ms[ir]);
       }
       continue;
     }
@@ -468,7 +485,7 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
       case FT_NODE_DIFF_CHAMFER: d = op_intersect_chamfer(a, -b, rr); break;
       case FT_NODE_DIFF_COLUMNS: d = -op_union_columns(-a, b, rr, nn); break;
       case FT_NODE_DIFF_STAIRS: d = -op_union_stairs(-a, b, rr, nn); break;
-      case FT_NODE_PIPE: d = sqrtf(a * a + b * b) - fabsf(rr); break;
+      case FT_NODE_PIPE: d = fsqrt(a * a + b * b) - fabsf(rr); break;
       case FT_NODE_ENGRAVE: d = fmaxf(a, (a + fabsf(rr) - fabsf(b)) * kInvSqrt2); break;
       case FT_NODE_GROOVE: d = fmaxf(a, fminf(a + fabsf(rr), fabsf(nn) - fabsf(b))); break;
       default: d = fminf(a, fmaxf(a - fabsf(rr), fabsf(b) - fabsf(nn))); break;  // TONGUE
@@ -728,8 +745,9 @@ struct RayEntry {
   float tr, tg, tb;
   float m_ior, m_ax, m_ay, m_az, m_density;
   int depth;
-  int pad;
+  int owner;  // lane of the wave whose sample this ray contributes to
 };
+static_assert(sizeof(RayEntry) == 64, "RayEntry layout");
 
 struct TraceParams {
   int max_steps, max_depth;
@@ -741,6 +759,7 @@ struct RayStack {
   RayEntry e[kRayStack];
   int sp;
   int overflow;
+  int owner;  // owner of the rays pushed while the current ray is processed
   __device__ void push(float3 o, float3 d, float3 thr, const Medium& m, int depth) {
     if (sp >= kRayStack) { overflow = 1; return; }
     RayEntry& r = e[sp++];
@@ -748,6 +767,7 @@ struct RayStack {
     r.tr = thr.x; r.tg = thr.y; r.tb = thr.z;
     r.m_ior = m.ior; r.m_ax = m.ax; r.m_ay = m.ay; r.m_az = m.az; r.m_density = m.density;
     r.depth = depth;
+    r.owner = owner;
   }
 };
 
@@ -871,45 +891,114 @@ __device__ __noinline__ float3 hit_material(int slot, const TraceParams& tp, con
   return cmul(color, thr);
 }
 
-// trace_ray_recursive (ray.rs:3-38) with an explicit stack.  Returns the radiance.
+// Index of the k-th (0-based) set bit of m; m has more than k set bits.
+__device__ __forceinline__ int kth_set_bit(unsigned long long m, int k) {
+  int pos = 0;
+#pragma unroll
+  for (int w = 32; w >= 1; w >>= 1) {
+    const unsigned long long low = m & ((1ull << w) - 1ull);
+    const int c = __popcll(low);
+    if (k >= c) {
+      k -= c;
+      m >>= w;
+      pos += w;
+    } else {
+      m = low;
+    }
+  }
+  return pos;
+}
+__device__ __forceinline__ int lanes_below(unsigned long long m) {
+  return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+__device__ __forceinline__ int lane_read(int v, int lane) { return __builtin_amdgcn_ds_bpermute(lane << 2, v); }
+__device__ __forceinline__ float lane_read(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(lane << 2, __float_as_int(v)));
+}
+
+// Per-wave radiance accumulators: one RGB triple per lane's sample (blocks of 64 x 4).
+__shared__ float ft_whitted_acc[4][64 * 3];
+
+// trace_ray_recursive (ray.rs:3-38) with explicit stacks, for the 64 samples of one wave.
+// Every term of the recursion is linear in the child radiance, so the sample radiance is
+// the sum over its ray tree of path throughput x local term, in any order.  Each lane keeps
+// a stack of pending rays tagged with the lane that owns their sample; before every round
+// lanes whose stack ran empty take the top ray of lanes holding two or more (a cross-lane
+// exchange), so the wave's rays are traced by all of its lanes instead of by the lanes of
+// the deepest trees alone.  Contributions go to the owner's accumulator in LDS.  Returns
+// the radiance of this lane's sample.  All lanes of the wave that are active on entry
+// must call it together.
 template <int kRayStack>
 __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, float3 o, float3 d, int* overflow) {
   const DevScene& S = scene_of(slot);
+  const int me = (int)threadIdx.x;
+  float* const acc = ft_whitted_acc[threadIdx.y];
+  acc[me * 3] = 0.0f;
+  acc[me * 3 + 1] = 0.0f;
+  acc[me * 3 + 2] = 0.0f;
   RayStack<kRayStack> st;
   st.sp = 0;
   st.overflow = 0;
+  st.owner = me;
   st.push(o, d, mk(1, 1, 1), Medium{1.0f, 1.0f, 1.0f, 1.0f, 0.0f}, 0);
-  float3 acc = mk(0, 0, 0);
-  while (st.sp > 0) {
+  const unsigned long long active = __builtin_amdgcn_ballot_w64(true);
+  for (;;) {
+    const unsigned long long idle = __builtin_amdgcn_ballot_w64(st.sp == 0);
+    if (idle == active) break;
+    const unsigned long long rich = __builtin_amdgcn_ballot_w64(st.sp >= 2);
+    if (idle != 0ull && rich != 0ull) {
+      const int n = min(__popcll(idle), __popcll(rich));
+      const int ri = lanes_below(idle), rr = lanes_below(rich);
+      const bool take = st.sp == 0 && ri < n, give = st.sp >= 2 && rr < n;
+      const int src = take ? kth_set_bit(rich, ri) : me;
+      const RayEntry top = st.e[st.sp > 0 ? st.sp - 1 : 0];
+      RayEntry got;
+      got.ox = lane_read(top.ox, src); got.oy = lane_read(top.oy, src); got.oz = lane_read(top.oz, src);
+      got.dx = lane_read(top.dx, src); got.dy = lane_read(top.dy, src); got.dz = lane_read(top.dz, src);
+      got.tr = lane_read(top.tr, src); got.tg = lane_read(top.tg, src); got.tb = lane_read(top.tb, src);
+      got.m_ior = lane_read(top.m_ior, src); got.m_ax = lane_read(top.m_ax, src);
+      got.m_ay = lane_read(top.m_ay, src); got.m_az = lane_read(top.m_az, src);
+      got.m_density = lane_read(top.m_density, src);
+      got.depth = lane_read(top.depth, src); got.owner = lane_read(top.owner, src);
+      if (give) --st.sp;
+      if (take) st.e[st.sp++] = got;
+    }
+    if (st.sp == 0) continue;
     const RayEntry r = st.e[--st.sp];
     if (r.depth >= tp.max_depth) continue;
+    st.owner = r.owner;
     const float3 ro = mk(r.ox, r.oy, r.oz), rd = mk(r.dx, r.dy, r.dz);
     const float3 thr = mk(r.tr, r.tg, r.tb);
     const Medium med{r.m_ior, r.m_ax, r.m_ay, r.m_az, r.m_density};
     const float min_t = r.depth == 0 ? 0.0f : tp.eps * 2.0f;
+    float3 c;
     Hit hit;
     if (!raymarch(slot, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
-      acc = add(acc, cmul(cmul(environment(slot, rd), transmittance(med, tp.max_dist)), thr));
-      continue;
-    }
-    const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
-    // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449), and
-    // the hit's material or split materials, from one shading sweep
-    ShadeWalk w;
-    shade_walks(slot, start, hit.pos, 1.0e-4f, w);
-    float3 sn = normalize(hit.front ? w.normal : mul(w.normal, -1.0f));
-    const float3 att = cmul(thr, transmittance(med, hit.t));
-    if (w.split) {
-      float tt = clampf(w.split_t, 0.0f, 1.0f);
-      if (1.0f - tt != 0.0f)
-        acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, mul(att, 1.0f - tt), st));
-      if (tt != 0.0f) acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.b, mul(att, tt), st));
+      c = cmul(cmul(environment(slot, rd), transmittance(med, tp.max_dist)), thr);
     } else {
-      acc = add(acc, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, att, st));
+      const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
+      // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449), and
+      // the hit's material or split materials, from one shading sweep
+      ShadeWalk w;
+      shade_walks(slot, start, hit.pos, 1.0e-4f, w);
+      const float3 sn = normalize(hit.front ? w.normal : mul(w.normal, -1.0f));
+      const float3 att = cmul(thr, transmittance(med, hit.t));
+      c = mk(0, 0, 0);
+      if (w.split) {
+        const float tt = clampf(w.split_t, 0.0f, 1.0f);
+        if (1.0f - tt != 0.0f)
+          c = add(c, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, mul(att, 1.0f - tt), st));
+        if (tt != 0.0f) c = add(c, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.b, mul(att, tt), st));
+      } else {
+        c = hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, att, st);
+      }
     }
+    atomicAdd(&acc[r.owner * 3], c.x);
+    atomicAdd(&acc[r.owner * 3 + 1], c.y);
+    atomicAdd(&acc[r.owner * 3 + 2], c.z);
   }
   if (st.overflow) *overflow = 1;
-  return acc;
+  return mk(acc[me * 3], acc[me * 3 + 1], acc[me * 3 + 2]);
 }
 
 // trace_ray_debug_aov (ray.rs:224-285)

@@ -531,6 +531,7 @@ __device__ __noinline__ float bvh_lower_bound(int slot, float3 p) {
 #if defined(FT_EMITTED)
 // Provided by the scene emitter (runtime-compiled scene-specialised code).
 __device__ __forceinline__ Info ftx_scene_info(const DevScene& S, float3 p);
+__device__ __forceinline__ float ftx_scene_dist(const DevScene& S, float3 p);
 __device__ __forceinline__ float ftx_lower_bound(const DevScene& S, float3 p);
 #endif
 
@@ -541,6 +542,13 @@ __device__ __forceinline__ Info scene_info(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
   if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_info(slot, p);
   return eval_node(slot, S.root, p);
+}
+// The distance of scene_info alone (emitted scenes skip the winning-primitive selects).
+__device__ __forceinline__ float scene_dist(int slot, float3 p) {
+#if defined(FT_EMITTED) && FT_EMITTED_SCENE
+  return ftx_scene_dist(scene_of(slot), p);
+#endif
+  return scene_info(slot, p).d;
 }
 __device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
 #if defined(FT_EMITTED) && FT_EMITTED_SCENE
@@ -556,6 +564,7 @@ __device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
 struct SceneEval {
   int slot;
   __device__ __forceinline__ Info operator()(float3 p) const { return scene_info(slot, p); }
+  __device__ __forceinline__ float dist(float3 p) const { return scene_dist(slot, p); }
 };
 
 }  // namespace ftd

@@ -78,6 +78,7 @@ struct DiffEval {
     i.leaf = 0;
     return i;
   }
+  __device__ __forceinline__ float dist(float3 p) const { return diff_eval(*a, vals, p); }
 };
 
 __device__ __forceinline__ float wave_sum(float x) {

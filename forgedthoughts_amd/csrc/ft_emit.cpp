@@ -66,6 +66,19 @@ class Emitter {
     return o.str();
   }
 
+  // The same distance without the winning primitive (the march's inner steps).
+  std::string subtree_dist(int node, const std::string& fname) {
+    body_.str("");
+    tmp_ = 0;
+    ids_ = false;
+    Val v = emit(node, "p", false);
+    ids_ = true;
+    std::ostringstream o;
+    o << "__device__ __forceinline__ float " << fname << "(float3 p) {\n" << body_.str() << "  return " << v.d
+      << ";\n}\n";
+    return o.str();
+  }
+
   std::string lower_bound(int node) {
     body_.str("");
     tmp_ = 0;
@@ -81,6 +94,7 @@ class Emitter {
   const ft_scene_ir* ir_;
   std::ostringstream body_;
   int tmp_ = 0;
+  bool ids_ = true;  // false: distances only (the id selects are left out)
 
   std::string fresh(const char* pfx) { return std::string(pfx) + std::to_string(tmp_++); }
 
@@ -282,13 +296,13 @@ class Emitter {
     Val a = emit(n.lhs, p, child_lb || (need_lb && k == FT_NODE_SUBTRACT));
     Val b = emit(n.rhs, p, child_lb);
     out.d = fresh("d");
-    out.id = fresh("id");
+    out.id = ids_ ? fresh("id") : std::string("0");
     if (k == FT_NODE_UNION) {
       std::string tl = fresh("tl");
       body_ << "  const bool " << tl << " = " << a.lb << " <= " << b.lb << " ? (" << a.d << " <= " << b.lb << " || " << a.d
             << " <= " << b.d << ") : (!(" << b.d << " <= " << a.lb << ") && " << a.d << " <= " << b.d << ");\n";
       body_ << "  const float " << out.d << " = " << tl << " ? " << a.d << " : " << b.d << ";\n";
-      body_ << "  const int " << out.id << " = " << tl << " ? " << a.id << " : " << b.id << ";\n";
+      if (ids_) body_ << "  const int " << out.id << " = " << tl << " ? " << a.id << " : " << b.id << ";\n";
       if (need_lb) {
         out.lb = fresh("lb");
         body_ << "  const float " << out.lb << " = fminf(" << a.lb << ", " << b.lb << ");\n";
@@ -299,7 +313,7 @@ class Emitter {
       std::string tl = fresh("tl");
       body_ << "  const bool " << tl << " = " << a.d << " >= " << b.d << ";\n";
       body_ << "  const float " << out.d << " = " << tl << " ? " << a.d << " : " << b.d << ";\n";
-      body_ << "  const int " << out.id << " = " << tl << " ? " << a.id << " : " << b.id << ";\n";
+      if (ids_) body_ << "  const int " << out.id << " = " << tl << " ? " << a.id << " : " << b.id << ";\n";
       if (need_lb) {
         out.lb = fresh("lb");
         body_ << "  const float " << out.lb << " = fmaxf(" << a.lb << ", " << b.lb << ");\n";
@@ -310,7 +324,7 @@ class Emitter {
       std::string tl = fresh("tl");
       body_ << "  const bool " << tl << " = " << a.d << " >= -" << b.d << ";\n";
       body_ << "  const float " << out.d << " = " << tl << " ? " << a.d << " : -" << b.d << ";\n";
-      body_ << "  const int " << out.id << " = " << tl << " ? " << a.id << " : " << b.id << ";\n";
+      if (ids_) body_ << "  const int " << out.id << " = " << tl << " ? " << a.id << " : " << b.id << ";\n";
       if (need_lb) out.lb = a.lb;
       return out;
     }
@@ -336,7 +350,7 @@ class Emitter {
       default: expr = "fminf(" + a.d + ", fmaxf(" + a.d + " - fabsf(" + r + "), fabsf(" + b.d + ") - fabsf(" + nn + ")))"; keep = "true"; break;
     }
     body_ << "  const float " << out.d << " = " << expr << ";\n";
-    body_ << "  const int " << out.id << " = (" << keep << ") ? " << a.id << " : " << b.id << ";\n";
+    if (ids_) body_ << "  const int " << out.id << " = (" << keep << ") ? " << a.id << " : " << b.id << ";\n";
     if (need_lb) {
       out.lb = fresh("lb");
       body_ << "  const float " << out.lb << " = " << aabb(n, p) << ";\n";
@@ -396,6 +410,7 @@ std::string emit_scene_source(const HostScene& S, const ft_scene_ir* ir, const E
   o << E.functions();
   if (plan.scene) {
     o << E.subtree_info(S.root, "ftx_root");
+    o << E.subtree_dist(S.root, "ftx_root_d");
     // Accelerator::distance_info over a one-leaf BVH is that leaf's accel_leaf_distance_info
     const bool bvh_path = S.accel != FT_ACCEL_NAIVE && !S.bvh.empty();
     const bool custom_leaf_shortcut = bvh_path && S.leaves.size() == 1 && S.leaves[0].is_custom;
@@ -416,6 +431,20 @@ std::string emit_scene_source(const HostScene& S, const ft_scene_ir* ir, const E
     o << "  float d, lb;\n  int id;\n  ftx_root(p, d, lb, id);\n  Info o;\n  o.d = d;\n  o.lb = lb;\n"
       << "  const DNode& w = S.nodes[id];\n  o.obj = w.obj;\n  o.mat = w.mat;\n  o.leaf = "
       << (bvh_path ? "0" : "w.leaf") << ";\n  return o;\n}\n";
+    // the distance alone, as ftx_scene_info computes it
+    o << "__device__ __forceinline__ float ftx_scene_dist(const DevScene& S, float3 p) {\n";
+    if (bvh_path) o << "  if (S.flops) atomicAdd(S.flops, 3ull * kAabbFlops);\n";
+    if (custom_leaf_shortcut) {
+      const ftd::DLeaf& L = S.leaves[0];
+      float ex = L.bmax[0] - L.bmin[0], ey = L.bmax[1] - L.bmin[1], ez = L.bmax[2] - L.bmin[2];
+      float far_t = std::sqrt(ex * ex + ey * ey + ez * ez) * 0.1f;
+      far_t = far_t < 0.05f ? 0.05f : (far_t > 0.5f ? 0.5f : far_t);
+      o << "  const float flb = aabb_lb_lit(p, " << lit(L.bmin[0]) << ", " << lit(L.bmin[1]) << ", " << lit(L.bmin[2])
+        << ", " << lit(L.bmax[0]) << ", " << lit(L.bmax[1]) << ", " << lit(L.bmax[2]) << ");\n"
+        << "  if (flb > " << lit(far_t) << ") return flb;\n";
+    }
+    o << "  if (S.flops) atomicAdd(S.flops, (unsigned long long)S.nodes[" << S.root << "].cost);\n";
+    o << "  return ftx_root_d(p);\n}\n";
     if (!bvh_path) {
       o << E.lower_bound(S.root);
     } else {
@@ -427,6 +456,7 @@ std::string emit_scene_source(const HostScene& S, const ft_scene_ir* ir, const E
     }
   } else {
     o << "__device__ __forceinline__ Info ftx_scene_info(const DevScene& S, float3 p) { return Info(); }\n"
+      << "__device__ __forceinline__ float ftx_scene_dist(const DevScene& S, float3 p) { return 0.0f; }\n"
       << "__device__ __forceinline__ float ftx_lower_bound(const DevScene& S, float3 p) { return 0.0f; }\n";
   }
   o << "}  // namespace ftd\n";

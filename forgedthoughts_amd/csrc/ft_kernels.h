@@ -159,7 +159,7 @@ __device__ __forceinline__ void atlas_cells_body(int slot, AtlasGrid g, float* _
   const int cx = ci % g.nx, cy = (ci / g.nx) % g.ny, cz = ci / (g.nx * g.ny);
   const float cell = 8.0f * g.voxel;
   const float3 p = mk(g.ox + ((float)cx + 0.5f) * cell, g.oy + ((float)cy + 0.5f) * cell, g.oz + ((float)cz + 0.5f) * cell);
-  const float d = scene_info(slot, p).d;  // every lane evaluates (wave-uniform programs)
+  const float d = scene_dist(slot, p);  // every lane evaluates (wave-uniform programs)
   if (i >= n) return;
   cell_d[i] = d;
   int b = -1;
@@ -188,7 +188,7 @@ __device__ __forceinline__ void atlas_bricks_body(int slot, AtlasGrid g, const i
   const float cell = 8.0f * g.voxel;
   const float3 p = mk(g.ox + (float)cx * cell + (float)sx * g.voxel, g.oy + (float)cy * cell + (float)sy * g.voxel,
                       g.oz + (float)cz * cell + (float)sz * g.voxel);
-  const float d = scene_info(slot, p).d;
+  const float d = scene_dist(slot, p);
   if (i >= n) return;
   if (fp16) ((_Float16*)bricks)[i] = (_Float16)d;
   else ((float*)bricks)[i] = d;

@@ -36,9 +36,10 @@ __device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, co
 }
 
 // raymarch_hit + refine_hit_distance + estimate_normal (renderer.rs:4091-4215).
-// One evaluation site: the march, the 10 bisection steps, the 6 central differences and
-// the final distance_info are phases of a single loop, so a specialised scene function is
-// inlined once and lanes in different phases still share every evaluation.
+// One evaluation site for the distance: the march, the 10 bisection steps and the 6 central
+// differences are phases of a single loop, so a specialised scene function is inlined once
+// and lanes in different phases still share every evaluation; the final distance_info at
+// the hit (which also names the primitive) is taken once per ray after the loop.
 // No coarse distance: every march step evaluates the scene.
 struct NoCoarse {
   __device__ __forceinline__ bool enabled() const { return false; }
@@ -46,9 +47,10 @@ struct NoCoarse {
   float switch_dist = 0.0f;
 };
 
-// `ev(p)` returns the scene's distance info at p (fields d, obj, mat, leaf); bmin/bmax
-// are the scene bounds the ray is clipped to.  `coarse(p)`, when enabled, returns an
-// approximate distance (or a negative value where it has none): while it exceeds
+// `ev(p)` returns the scene's distance info at p (fields d, obj, mat, leaf), `ev.dist(p)`
+// its distance alone; bmin/bmax are the scene bounds the ray is clipped to.  `coarse(p)`,
+// when enabled, returns an approximate distance (or a negative value where it has none):
+// while it exceeds
 // coarse.switch_dist the march steps on it without evaluating the scene; from the first
 // step below, the lane marches, refines and shades on exact evaluations only.
 template <class Eval, class Coarse = NoCoarse>
@@ -85,53 +87,51 @@ __device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin,
     } else if (phase == 1) {
       mid = 0.5f * (lo + hi);
       p = add(o, mul(d, mid));
-    } else if (phase == 2) {
+    } else {
       const float off = (k & 1) ? -e : e;
       const int axis = k >> 1;
       p = mk(pos.x + (axis == 0 ? off : 0.0f), pos.y + (axis == 1 ? off : 0.0f), pos.z + (axis == 2 ? off : 0.0f));
-    } else {
-      p = pos;
     }
-    const auto inf = ev(p);
+    const float dist = ev.dist(p);
     if (phase == 0) {
-      if (fabsf(inf.d) < eps) {
+      if (fabsf(dist) < eps) {
         phase = 1;
         k = 0;
         lo = prev;
         hi = traveled;
       } else {
         prev = traveled;
-        traveled += fmaxf(fabsf(inf.d) * sscale, min_step);
+        traveled += fmaxf(fabsf(dist) * sscale, min_step);
         ++steps;
       }
     } else if (phase == 1) {
-      if (fabsf(inf.d) < eps) hi = mid; else lo = mid;
+      if (fabsf(dist) < eps) hi = mid; else lo = mid;
       if (++k == 10) {
         pos = add(o, mul(d, hi));
         phase = 2;
         k = 0;
       }
-    } else if (phase == 2) {
+    } else {
       if ((k & 1) == 0) {
-        dplus = inf.d;
+        dplus = dist;
       } else {
-        const float g = dplus - inf.d;
+        const float g = dplus - dist;
         const int axis = k >> 1;
         grad = mk(axis == 0 ? g : grad.x, axis == 1 ? g : grad.y, axis == 2 ? g : grad.z);
       }
-      if (++k == 6) phase = 3;
-    } else {
-      const float3 nrm = len(grad) > 1.0e-8f ? normalize(grad) : mk(0.0f, 1.0f, 0.0f);
-      hit.t = hi;
-      hit.pos = pos;
-      hit.n = nrm;
-      hit.front = dot(nrm, d) < 0.0f;
-      hit.obj = inf.obj;
-      hit.mat = inf.mat;
-      hit.leaf = inf.leaf;
-      return true;
+      if (++k == 6) break;
     }
   }
+  const auto inf = ev(pos);
+  const float3 nrm = len(grad) > 1.0e-8f ? normalize(grad) : mk(0.0f, 1.0f, 0.0f);
+  hit.t = hi;
+  hit.pos = pos;
+  hit.n = nrm;
+  hit.front = dot(nrm, d) < 0.0f;
+  hit.obj = inf.obj;
+  hit.mat = inf.mat;
+  hit.leaf = inf.leaf;
+  return true;
 }
 
 }  // namespace ftd

@@ -629,7 +629,7 @@ __device__ __noinline__ float shadow_visibility(int slot, float3 o, float3 d, fl
     if (t >= max_t) return clampf(vis, 0.0f, 1.0f);
     float3 p = add(o, mul(d, t));
     float lower = fabsf(scene_lower_bound(slot, p));
-    float h = lower > lb_gate ? lower : fabsf(scene_info(slot, p).d);
+    float h = lower > lb_gate ? lower : fabsf(scene_dist(slot, p));
     if (h < eps * 4.0f) return 0.0f;
     vis = fminf(vis, clampf(10.0f * h / t, 0.0f, 1.0f));
     t += fmaxf(h, eps * 2.0f);

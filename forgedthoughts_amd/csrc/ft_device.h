@@ -41,6 +41,7 @@ struct DevScene {
   int n_nodes, root, n_prog, n_lbprog;
   int n_mats, n_objects, n_lights, env_fn;
   int accel, n_leaves, n_bvh, stack_depth;   // stack_depth: LDS value-stack entries per thread
+  int simple;                            // no custom SDF, modifier or noise node
   unsigned long long* flops;             // non-null: count evaluated FLOPs (instrumented runs)
   float bmin[3], bmax[3];                // scene bounds
   float env_rad[3];                      // sum of EnvLight radiance
@@ -278,6 +279,9 @@ extern __shared__ float ft_lds[];
 // subtree root (renderer.rs:5413-5955).
 // `base`: first value-stack entry this evaluation may use (entries below belong to a
 // caller that is itself walking a node program).
+// kSimple: the scene has no custom SDF, modifier or noise node, so the loop calls no
+// function and keeps its state in registers.
+template <bool kSimple = false>
 __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int node, float3 p, int base = 0) {
   // the bounds and every record are wave-uniform; readfirstlane makes that visible to the
   // compiler so record fetches are scalar loads.  Records are streamed in program order and
@@ -302,7 +306,7 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
     asm volatile("" ::"s"(code));
     __builtin_amdgcn_sched_barrier(0);
     const DOp next = load_uniform(ops + k + 1);
-    if (code & (int)kBeginFlag) {
+    if (!kSimple && (code & (int)kBeginFlag)) {
       const DNode n = load_uniform(nodes + o.node);
       pst[pp++] = p;
       float3 q = node_local(n, p);
@@ -314,7 +318,7 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
       slb[sp * nth] = d;
       sid[sp * nth] = o.node;
       ++sp;
-    } else if (kind == FT_NODE_CUSTOM) {
+    } else if (!kSimple && kind == FT_NODE_CUSTOM) {
       const DNode n = load_uniform(nodes + o.node);
       float3 q = node_local(n, p);
       float mn[3] = {-n.p[0], -n.p[1], -n.p[2]}, mx[3] = {n.p[0], n.p[1], n.p[2]};
@@ -322,7 +326,8 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
       slb[sp * nth] = aabb_lb(q, mn, mx);
       sid[sp * nth] = o.node;
       ++sp;
-    } else if (kind == FT_NODE_DOMAIN_MODIFIER || kind == FT_NODE_DISTANCE_POST_MODIFIER || kind == FT_NODE_NOISE) {
+    } else if (!kSimple &&
+               (kind == FT_NODE_DOMAIN_MODIFIER || kind == FT_NODE_DISTANCE_POST_MODIFIER || kind == FT_NODE_NOISE)) {
       const int t = (sp - 1) * nth;
       const DNode n = load_uniform(nodes + o.node);
       float3 q = p;
@@ -382,6 +387,12 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
   return out;
 }
 
+// True, for every active lane, when at least half of the active lanes hold v.
+__device__ __forceinline__ bool wave_majority(bool v) {
+  const unsigned long long yes = __builtin_amdgcn_ballot_w64(v), all = __builtin_amdgcn_ballot_w64(true);
+  return __builtin_amdgcn_readfirstlane(2 * __popcll(yes) >= __popcll(all) ? 1 : 0) != 0;
+}
+
 // Runs f(key) once per distinct value of `key` among the active lanes, each time for the
 // lanes holding that value (a waterfall).  The leader's value is read with a lane index
 // from the ballot, so the compiler cannot mistake `key` for a uniform value.
@@ -404,7 +415,10 @@ __device__ __noinline__ Info eval_node(int slot, int node, float3 p, int base = 
   const DevScene& S = scene_of(slot);
   const int b = __builtin_amdgcn_readfirstlane(base);
   Info out;
-  for_each_uniform(node, [&](int lead) { out = eval_uniform(S, slot, lead, p, b); });
+  if (S.simple)
+    for_each_uniform(node, [&](int lead) { out = eval_uniform<true>(S, slot, lead, p, b); });
+  else
+    for_each_uniform(node, [&](int lead) { out = eval_uniform<false>(S, slot, lead, p, b); });
   return out;
 }
 
@@ -437,27 +451,26 @@ __device__ __noinline__ float lower_bound_root(int slot, float3 p) {
   return st(0);
 }
 
-// accel_leaf_distance_info (renderer.rs:629-657)
-__device__ __forceinline__ Info leaf_info(int slot, int li, float3 p) {
-  const DevScene& S = scene_of(slot);
-  const DLeaf& L = S.leaves[li];
+// accel_leaf_distance_info (renderer.rs:629-657) of leaf li, the same leaf for every lane
+// (li is wave-uniform): record and node fetches are scalar loads.
+__device__ __forceinline__ Info leaf_info_uniform(const DevScene& S, int slot, int li, float3 p) {
+  const DLeaf L = load_uniform(uniform_ptr(S.leaves) + li);
+  Info o;
   if (L.is_custom) {
-    float lb = aabb_lb(p, L.bmin, L.bmax);
-    float ex = L.bmax[0] - L.bmin[0], ey = L.bmax[1] - L.bmin[1], ez = L.bmax[2] - L.bmin[2];
-    float far_t = clampf(sqrtf(ex * ex + ey * ey + ez * ez) * 0.1f, 0.05f, 0.5f);
+    const float lb = aabb_lb(p, L.bmin, L.bmax);
+    const float ex = L.bmax[0] - L.bmin[0], ey = L.bmax[1] - L.bmin[1], ez = L.bmax[2] - L.bmin[2];
+    const float far_t = clampf(sqrtf(ex * ex + ey * ey + ez * ez) * 0.1f, 0.05f, 0.5f);
     if (lb > far_t) {
-      const DNode& n = S.nodes[L.node];
-      Info o;
+      const auto& n = uniform_ptr(S.nodes)[L.node];
       o.d = lb; o.lb = lb; o.obj = n.obj; o.mat = n.mat; o.leaf = li;
       return o;
     }
   }
-  const int node = L.node;
-  if (is_prim(S.nodes[node].kind)) {
-    // a single-primitive leaf: evaluated per lane, no program walk
-    const DNode n = S.nodes[node];
-    Info o;
-    o.d = prim_distance(n, p);
+  if (L.op >= 0) {
+    // a single-primitive leaf: one record, no program walk
+    const DOp r = load_uniform(uniform_ptr(S.ops) + L.op);
+    const auto& n = uniform_ptr(S.nodes)[L.node];
+    o.d = op_prim_distance(r, r.code & kOpKindMask, p);
     o.lb = o.d;
     o.obj = n.obj;
     o.mat = n.mat;
@@ -465,62 +478,75 @@ __device__ __forceinline__ Info leaf_info(int slot, int li, float3 p) {
     if (S.flops) atomicAdd(S.flops, (unsigned long long)n.cost);
     return o;
   }
-  Info o = eval_node(slot, node, p);
+  o = eval_node(slot, L.node, p);
   o.leaf = li;
   return o;
 }
 
-// bvh_distance_info (renderer.rs:743-786), iterative near-first
+// Per-wave traversal stacks (blocks of 64 x 4 threads).
+__shared__ int ft_bvh_stack[4][kMaxBvhStack];
+
+// bvh_distance_info (renderer.rs:743-786) for the wave's 64 points at once.  The wave walks
+// one stack of BVH nodes: a node is entered when its lower bound does not exceed some lane's
+// best distance, each lane testing the reference's `node_lb > best` against its own best and
+// evaluating only the leaves that pass it; of two children the one most lanes find nearer is
+// visited first.  Every lane still gets the minimum over the leaves its own pruning leaves
+// open (the leaves are visited in an order common to the wave instead of each lane's
+// near-first order, which can only change which of two equally distant leaves names the
+// hit).  Nodes and leaves are fetched with scalar loads; no per-lane stack.  All lanes of the
+// wave that are active on entry must call it together.
 __device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
-  int stack[kMaxBvhStack];
+  const auto bvh = uniform_ptr(S.bvh);
+  int* const stack = ft_bvh_stack[threadIdx.y];
   int sp = 0;
   float best = __builtin_huge_valf();
   Info bi;
   bool have = false;
   stack[sp++] = 0;
   while (sp > 0) {
-    const int ni = stack[--sp];
-    const DBvh& nd = S.bvh[ni];
+    const int ni = __builtin_amdgcn_readfirstlane(stack[--sp]);
+    const DBvh nd = load_uniform(bvh + ni);
     if (S.flops) atomicAdd(S.flops, 3ull * kAabbFlops);
-    if (aabb_lb(p, nd.bmin, nd.bmax) > best) continue;
+    const bool need = !(aabb_lb(p, nd.bmin, nd.bmax) > best);
+    if (__builtin_amdgcn_ballot_w64(need) == 0ull) continue;
     if (nd.left < 0) {
-      Info info = leaf_info(slot, -nd.left - 1, p);
-      if (info.d < best) best = info.d;
-      if (!have || info.d < bi.d) { bi = info; have = true; }
+      if (need) {
+        const Info info = leaf_info_uniform(S, slot, -nd.left - 1, p);
+        if (info.d < best) best = info.d;
+        if (!have || info.d < bi.d) { bi = info; have = true; }
+      }
       continue;
     }
-    const DBvh& L = S.bvh[nd.left];
-    const DBvh& R = S.bvh[nd.right];
-    float llb = aabb_lb(p, L.bmin, L.bmax), rlb = aabb_lb(p, R.bmin, R.bmax);
-    int first = nd.left, second = nd.right;
-    if (!(llb <= rlb)) { first = nd.right; second = nd.left; }
-    if (sp + 2 <= kMaxBvhStack) {
-      stack[sp++] = second;
-      stack[sp++] = first;
-    }
+    const DBvh L = load_uniform(bvh + nd.left);
+    const DBvh R = load_uniform(bvh + nd.right);
+    const float llb = aabb_lb(p, L.bmin, L.bmax), rlb = aabb_lb(p, R.bmin, R.bmax);
+    const bool left_first = wave_majority(!need || llb <= rlb);
+    stack[sp++] = left_first ? nd.right : nd.left;
+    stack[sp++] = left_first ? nd.left : nd.right;
   }
-  if (!have) {
-    bi = eval_node(slot, S.root, p);
-  }
+  if (!have) bi = eval_node(slot, S.root, p);
   return bi;
 }
-
+// bvh_lower_bound (renderer.rs:788-822) for the wave's points, one node stack per wave.
 __device__ __noinline__ float bvh_lower_bound(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
-  int stack[kMaxBvhStack];
+  const auto bvh = uniform_ptr(S.bvh);
+  int* const stack = ft_bvh_stack[threadIdx.y];
   int sp = 0;
   float best = __builtin_huge_valf();
   stack[sp++] = 0;
   while (sp > 0) {
-    const DBvh& nd = S.bvh[stack[--sp]];
-    float lb = aabb_lb(p, nd.bmin, nd.bmax);
-    if (lb > best) continue;
-    if (nd.left < 0) { best = fminf(best, lb); continue; }
-    if (sp + 2 <= kMaxBvhStack) {
-      stack[sp++] = nd.right;
-      stack[sp++] = nd.left;
+    const DBvh nd = load_uniform(bvh + __builtin_amdgcn_readfirstlane(stack[--sp]));
+    const float lb = aabb_lb(p, nd.bmin, nd.bmax);
+    const bool need = !(lb > best);
+    if (__builtin_amdgcn_ballot_w64(need) == 0ull) continue;
+    if (nd.left < 0) {
+      if (need) best = fminf(best, lb);
+      continue;
     }
+    stack[sp++] = nd.right;
+    stack[sp++] = nd.left;
   }
   return best;
 }

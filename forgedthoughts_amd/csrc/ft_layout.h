@@ -93,6 +93,9 @@ struct DLeaf {
   int node;
   float bmax[3];
   int is_custom;
+  int op;       // the leaf's program record when the leaf is one primitive, else -1
+  int pad[3];
 };
+static_assert(sizeof(DLeaf) == 48, "DLeaf layout");
 
 }  // namespace ftd

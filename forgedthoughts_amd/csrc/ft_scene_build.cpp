@@ -267,6 +267,8 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
       S.nodes[n.rhs].flags |= 2;
     }
   }
+  for (uint32_t i = 0; i < N; ++i)
+    if (ir->nodes[i].kind == FT_NODE_CUSTOM || is_modifier(ir->nodes[i].kind)) S.simple = false;
   // the program as self-contained records
   S.ops.assign(S.prog.size() + 1, ftd::DOp{});
   for (size_t k = 0; k < S.prog.size(); ++k) {
@@ -386,6 +388,8 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     for (int k = 0; k < 3; ++k) { L.bmin[k] = bounds[ln].mn[k]; L.bmax[k] = bounds[ln].mx[k]; }
     L.node = ln;
     L.is_custom = ir->nodes[ln].kind == FT_NODE_CUSTOM ? 1 : 0;
+    L.op = ir->nodes[ln].kind <= FT_NODE_EXTRUDE_POLYGON ? S.nodes[ln].prog_first : -1;
+    L.pad[0] = L.pad[1] = L.pad[2] = 0;
     S.leaves.push_back(L);
     // every node of the leaf's subtree occupies prog[prog_first .. prog_last]
     for (int k = S.nodes[ln].prog_first; k <= S.nodes[ln].prog_last; ++k) S.nodes[S.prog[k] & ~ftd::kBeginFlag].leaf = (int)li;

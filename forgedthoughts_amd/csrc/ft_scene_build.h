@@ -29,6 +29,7 @@ struct HostScene {
   ft_camera camera;
   int max_stack = 0;        // value-stack depth the program needs
   int max_points = 0;       // modifier nesting
+  bool simple = true;       // no custom SDF, modifier or noise node: the program calls no function
   int max_walk_frames = 0;  // frames of the per-hit walks
   int max_walk_mats = 0;    // material results held by the material walk
   int max_fn_exprs = 0;

@@ -25,22 +25,29 @@ def cpu_baseline(workload, scene, seconds: float):
     from oracle import ftref
     opts = workload.options
     threads = os.cpu_count() or 1
+    if workload.kind == "whitted":
+        spp = workload.settings.aa_samples
+        render = lambda r: ftref.render_whitted(scene, workload.accel, opts, workload.settings, threads=threads,
+                                                rows=(r, r + 1))
+    else:
+        spp = workload.aa
+        render = lambda r: ftref.render_depth(scene, workload.accel, opts, workload.aa, threads=threads, rows=(r, r + 1))
     # evenly spaced single rows, as many as fit the budget (first measure one row)
     rows_all = list(range(0, opts.height))
     probe = rows_all[len(rows_all) // 2]
     t0 = time.perf_counter()
-    ftref.render_whitted(scene, workload.accel, opts, workload.settings, threads=threads, rows=(probe, probe + 1))
+    render(probe)
     per_row = max(time.perf_counter() - t0, 1e-3)
     n_rows = max(1, min(opts.height, int(seconds / per_row)))
     step = max(1, opts.height // n_rows)
     picked = rows_all[step // 2::step][:n_rows]
     t0 = time.perf_counter()
     for r in picked:
-        ftref.render_whitted(scene, workload.accel, opts, workload.settings, threads=threads, rows=(r, r + 1))
+        render(r)
     dt = time.perf_counter() - t0
-    rays = len(picked) * opts.width * workload.settings.aa_samples
+    rays = len(picked) * opts.width * spp
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(picked)} evenly spaced rows x {opts.width} px x {workload.settings.aa_samples} spp "
+            "sample": f"{len(picked)} evenly spaced rows x {opts.width} px x {spp} spp "
                       f"of the same frame ({rays} primary rays, {dt:.1f} s), C++ restatement of the "
                       f"reference renderer, {threads} threads"}
 

@@ -62,29 +62,6 @@ __device__ __forceinline__ float diff_eval(const DiffArgs& a, float* __restrict_
   return f;
 }
 
-// f(p) alone, for the march: the post-order node table on a value stack held in registers
-// (a balanced tree of at most 48 spheres needs 7 entries), no LDS traffic.
-constexpr int kDiffStack = 8;
-__device__ __forceinline__ float diff_dist(const DiffArgs& a, float3 p) {
-  const auto nodes = uniform_ptr(a.nodes);
-  const auto th = uniform_ptr(a.th);
-  const int nn = __builtin_amdgcn_readfirstlane(a.n_nodes);
-  const int np = __builtin_amdgcn_readfirstlane(a.n_prims);
-  float st[kDiffStack];
-  int sp = 0;
-  for (int k = 0; k < nn; ++k) {
-    const int4 d = nodes[k];
-    if (d.x >= 0) {
-      const float3 q = mk(p.x - th[4 * d.x], p.y - th[4 * d.x + 1], p.z - th[4 * d.x + 2]);
-      st[sp++] = len(q) - th[4 * d.x + 3];
-    } else {
-      --sp;
-      st[sp - 1] = op_union_round(st[sp - 1], st[sp], th[4 * np + d.y]);
-    }
-  }
-  return st[0];
-}
-
 struct DiffInfo {
   float d;
   unsigned obj, mat;
@@ -95,13 +72,13 @@ struct DiffEval {
   float* vals;
   __device__ __forceinline__ DiffInfo operator()(float3 p) const {
     DiffInfo i;
-    i.d = diff_dist(*a, p);
+    i.d = diff_eval(*a, vals, p);
     i.obj = 0;
     i.mat = 0;
     i.leaf = 0;
     return i;
   }
-  __device__ __forceinline__ float dist(float3 p) const { return diff_dist(*a, p); }
+  __device__ __forceinline__ float dist(float3 p) const { return diff_eval(*a, vals, p); }
 };
 
 __device__ __forceinline__ float wave_sum(float x) {
@@ -201,9 +178,9 @@ __device__ __forceinline__ void diff_forward_body(DiffArgs a, float* __restrict_
   const float ts = h ? (fabsf(gd) > 1.0e-6f ? hit.t - f / gd : hit.t) : -1.0f;
   const float3 xs = add(o, mul(d, h ? ts : 0.0f));
   const float e = a.normal_step;
-  const float gx = diff_dist(a, mk(xs.x + e, xs.y, xs.z)) - diff_dist(a, mk(xs.x - e, xs.y, xs.z));
-  const float gy = diff_dist(a, mk(xs.x, xs.y + e, xs.z)) - diff_dist(a, mk(xs.x, xs.y - e, xs.z));
-  const float gz = diff_dist(a, mk(xs.x, xs.y, xs.z + e)) - diff_dist(a, mk(xs.x, xs.y, xs.z - e));
+  const float gx = diff_eval(a, vals, mk(xs.x + e, xs.y, xs.z)) - diff_eval(a, vals, mk(xs.x - e, xs.y, xs.z));
+  const float gy = diff_eval(a, vals, mk(xs.x, xs.y + e, xs.z)) - diff_eval(a, vals, mk(xs.x, xs.y - e, xs.z));
+  const float gz = diff_eval(a, vals, mk(xs.x, xs.y, xs.z + e)) - diff_eval(a, vals, mk(xs.x, xs.y, xs.z - e));
   const float3 G = mk(gx, gy, gz);
   const float gl = len(G);
   const float3 n = gl > 1.0e-12f ? mul(G, 1.0f / gl) : mk(0.0f, 1.0f, 0.0f);
@@ -258,7 +235,7 @@ __device__ __forceinline__ void diff_adjoint_body(DiffArgs a, const float* __res
       for (int k = 0; k < 6; ++k) {
         const float off = (k & 1) ? -e : e;
         const int axis = k >> 1;
-        fv[k] = diff_dist(a, mk(xs.x + (axis == 0 ? off : 0.0f), xs.y + (axis == 1 ? off : 0.0f),
+        fv[k] = diff_eval(a, vals, mk(xs.x + (axis == 0 ? off : 0.0f), xs.y + (axis == 1 ? off : 0.0f),
                                       xs.z + (axis == 2 ? off : 0.0f)));
       }
       const float3 G = mk(fv[0] - fv[1], fv[2] - fv[3], fv[4] - fv[5]);

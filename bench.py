@@ -39,12 +39,15 @@ def cpu_baseline(workload, scene, seconds: float):
     render(probe)
     per_row = max(time.perf_counter() - t0, 1e-3)
     n_rows = max(1, min(opts.height, int(seconds / per_row)))
-    step = max(1, opts.height // n_rows)
-    picked = rows_all[step // 2::step][:n_rows]
-    t0 = time.perf_counter()
-    for r in picked:
-        render(r)
-    dt = time.perf_counter() - t0
+    if n_rows == 1:  # one row already fills the budget: the probe is the sample
+        picked, dt = [probe], per_row
+    else:
+        step = max(1, opts.height // n_rows)
+        picked = rows_all[step // 2::step][:n_rows]
+        t0 = time.perf_counter()
+        for r in picked:
+            render(r)
+        dt = time.perf_counter() - t0
     rays = len(picked) * opts.width * spp
     return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{len(picked)} evenly spaced rows x {opts.width} px x {spp} spp "
@@ -240,13 +243,15 @@ def copy_calibration(mib: int):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "whitted_1080p_counters.md")
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of `kernel` as measured with the FETCH_SIZE / WRITE_SIZE counters
-    (scaled by a plain copy of known size) and recorded in the counters profile, or None."""
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` as measured with the FETCH_SIZE /
+    WRITE_SIZE counters (scaled by a plain copy of known size) and recorded in the counters
+    profile, or None when that workload has no counter profile."""
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch") and kernel in line:
+                fields = line.split()
+                if line.startswith("traffic_bytes_per_launch") and fields[1:3] == [workload, kernel]:
                     return int(line.split("=")[-1].strip())
     except OSError:
         pass
@@ -370,7 +375,7 @@ def main():
                                    "compile_ms": round(st.compile_ms, 1)}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic(f"ft_k_{w.kind}_16" if w.kind == "whitted" else "ft_k_depth"),
+                     "traffic": measured_traffic(w.name, f"ft_k_{w.kind}_16" if w.kind == "whitted" else "ft_k_depth"),
                      "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
                      "kernel_ms": round(kernel_ms / args.steps, 3), "stream_ms": round(stream_ms / args.steps, 3),
                      "sdf_flops_per_launch": int(flops)},

@@ -358,6 +358,175 @@ class Emitter {
     return out;
   }
 
+ public:
+  // ---------------------------------------------------------- shading sweep
+  // shade_sweep_uniform of node `root` as straight-line code: every entry of the sweep
+  // (distance, lower bound, walk normal, material reference) is an SSA value.
+  std::string sweep(int root) {
+    body_.str("");
+    tmp_ = 0;
+    prim_cost_ = 0.0;
+    split_ = S_.nodes[root].walk_split;
+    SVal v = sweep_node(root, false);
+    std::ostringstream o;
+    o << "__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w) {\n"
+      << "  MatPool pool;\n  pool.used = 0u;\n  pool.overflow = 0;\n  w.split = false;\n" << body_.str()
+      << "  if (S.flops) atomicAdd(S.flops, " << (unsigned long long)(prim_cost_ + 0.5) << "ull);\n"
+      << "  w.normal = " << v.n << ";\n  w.mat_overflow = pool.overflow != 0;\n"
+      << "  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, " << v.m << ", p, w.a);\n}\n";
+    return o.str();
+  }
+
+ private:
+  struct SVal {
+    std::string d, lb, n, m;
+  };
+  double prim_cost_ = 0.0;
+  int split_ = -1;
+
+  std::string local_expr(const ftd::DNode& n, const std::string& v) {
+    std::string q = "mk(" + v + ".x - " + lit(n.cx) + ", " + v + ".y - " + lit(n.cy) + ", " + v + ".z - " + lit(n.cz) + ")";
+    if (n.flags & 1)
+      q = "rot_x(rot_y(rot_z(" + q + ", " + lit(n.szs) + ", " + lit(n.szc) + "), " + lit(n.sys) + ", " + lit(n.syc) +
+          "), " + lit(n.sxs) + ", " + lit(n.sxc) + ")";
+    return q;
+  }
+
+  SVal sweep_node(int i, bool need_lb) {
+    const ftd::DNode& n = S_.nodes[i];
+    const int k = n.kind;
+    SVal out;
+    if (k <= FT_NODE_EXTRUDE_POLYGON) {
+      out.d = fresh("sd");
+      out.n = fresh("sn");
+      out.m = fresh("sm");
+      body_ << "  const float " << out.d << " = " << prim(n, local_expr(n, "p")) << ";\n";
+      body_ << "  const float3 " << out.n << " = tetra_normal([&](float3 r_) { return " << prim(n, local_expr(n, "r_"))
+            << "; }, p, eps);\n";
+      const unsigned id = n.mat < S_.mats.size() ? n.mat : 0u;
+      const int model = S_.mats.empty() ? 0 : (int)S_.mats[id].model;
+      body_ << "  MRef " << out.m << "{0, " << model << ", 0, 0.0f, " << i << ", 0};\n";
+      out.lb = out.d;
+      prim_cost_ += 5.0 * n.cost;
+      return out;
+    }
+    if (k == FT_NODE_SLICE || k == FT_NODE_SMOOTH) {
+      SVal c = sweep_node(n.lhs, k == FT_NODE_SMOOTH && need_lb);
+      out.n = c.n;
+      out.m = c.m;
+      out.d = fresh("sd");
+      if (k == FT_NODE_SLICE) {
+        const char* ax = (int)n.p[0] == 0 ? ".x" : ((int)n.p[0] == 1 ? ".y" : ".z");
+        body_ << "  const float " << out.d << " = fmaxf(" << c.d << ", fmaxf(" << lit(n.p[1]) << " - p" << ax << ", p" << ax
+              << " - " << lit(n.p[2]) << "));\n";
+        if (need_lb) {
+          out.lb = fresh("sl");
+          body_ << "  const float " << out.lb << " = " << aabb(n, "p") << ";\n";
+        }
+      } else {
+        body_ << "  const float " << out.d << " = " << c.d << " - " << lit(n.p[0] * 0.1f) << ";\n";
+        if (need_lb) {
+          out.lb = fresh("sl");
+          body_ << "  const float " << out.lb << " = " << c.lb << " - " << lit(n.p[0] * 0.1f) << ";\n";
+        }
+      }
+      return out;
+    }
+    const bool hard = k == FT_NODE_UNION || k == FT_NODE_INTERSECT || k == FT_NODE_SUBTRACT;
+    if (hard) {
+      const bool child_lb = k != FT_NODE_SUBTRACT;
+      SVal a = sweep_node(n.lhs, child_lb || need_lb), b = sweep_node(n.rhs, child_lb);
+      out.d = fresh("sd");
+      out.n = fresh("sn");
+      out.m = fresh("sm");
+      const std::string hl = fresh("hl");
+      if (k == FT_NODE_UNION) {
+        const std::string tl = fresh("tl");
+        body_ << "  const bool " << tl << " = " << a.lb << " <= " << b.lb << " ? (" << a.d << " <= " << b.lb << " || " << a.d
+              << " <= " << b.d << ") : (!(" << b.d << " <= " << a.lb << ") && " << a.d << " <= " << b.d << ");\n";
+        body_ << "  const float " << out.d << " = " << tl << " ? " << a.d << " : " << b.d << ";\n";
+        if (need_lb) {
+          out.lb = fresh("sl");
+          body_ << "  const float " << out.lb << " = fminf(" << a.lb << ", " << b.lb << ");\n";
+        }
+        body_ << "  const bool " << hl << " = " << a.d << " <= " << b.d << ";\n";
+      } else if (k == FT_NODE_INTERSECT) {
+        body_ << "  const float " << out.d << " = " << a.d << " >= " << b.d << " ? " << a.d << " : " << b.d << ";\n";
+        if (need_lb) {
+          out.lb = fresh("sl");
+          body_ << "  const float " << out.lb << " = fmaxf(" << a.lb << ", " << b.lb << ");\n";
+        }
+        body_ << "  const bool " << hl << " = " << a.d << " >= " << b.d << ";\n";
+      } else {
+        body_ << "  const float " << out.d << " = " << a.d << " >= -" << b.d << " ? " << a.d << " : -" << b.d << ";\n";
+        if (need_lb) out.lb = a.lb;
+        body_ << "  const bool " << hl << " = " << a.d << " >= -" << b.d << ";\n";
+      }
+      body_ << "  const float3 " << out.n << " = " << hl << " ? " << a.n << " : "
+            << (k == FT_NODE_SUBTRACT ? "mul(" + b.n + ", -1.0f)" : b.n) << ";\n";
+      body_ << "  MRef " << out.m << ";\n  if (" << hl << ") { drop_ref(pool, " << b.m << "); " << out.m << " = " << a.m
+            << "; } else { drop_ref(pool, " << a.m << "); " << out.m << " = " << b.m << "; }\n";
+      return out;
+    }
+    // smooth booleans: value as in emit(), then blend_surface_normals and blend_materials
+    SVal a = sweep_node(n.lhs, false), b = sweep_node(n.rhs, false);
+    const std::string r = lit(n.p[0]), nn = lit(n.p[1]);
+    const std::string& A = a.d;
+    const std::string& B = b.d;
+    std::string expr;
+    switch (k) {
+      case FT_NODE_UNION_ROUND: expr = "op_union_round(" + A + ", " + B + ", " + r + ")"; break;
+      case FT_NODE_UNION_CHAMFER: expr = "op_union_chamfer(" + A + ", " + B + ", " + r + ")"; break;
+      case FT_NODE_UNION_COLUMNS: expr = "op_union_columns(" + A + ", " + B + ", " + r + ", " + nn + ")"; break;
+      case FT_NODE_UNION_STAIRS: expr = "op_union_stairs(" + A + ", " + B + ", " + r + ", " + nn + ")"; break;
+      case FT_NODE_UNION_SOFT: expr = "op_union_soft(" + A + ", " + B + ", " + r + ")"; break;
+      case FT_NODE_INTERSECT_ROUND: expr = "op_intersect_round(" + A + ", " + B + ", " + r + ")"; break;
+      case FT_NODE_INTERSECT_CHAMFER: expr = "op_intersect_chamfer(" + A + ", " + B + ", " + r + ")"; break;
+      case FT_NODE_INTERSECT_COLUMNS: expr = "-op_union_columns(-" + A + ", -" + B + ", " + r + ", " + nn + ")"; break;
+      case FT_NODE_INTERSECT_STAIRS: expr = "-op_union_stairs(-" + A + ", -" + B + ", " + r + ", " + nn + ")"; break;
+      case FT_NODE_DIFF_ROUND: expr = "op_intersect_round(" + A + ", -" + B + ", " + r + ")"; break;
+      case FT_NODE_DIFF_CHAMFER: expr = "op_intersect_chamfer(" + A + ", -" + B + ", " + r + ")"; break;
+      case FT_NODE_DIFF_COLUMNS: expr = "-op_union_columns(-" + A + ", " + B + ", " + r + ", " + nn + ")"; break;
+      case FT_NODE_DIFF_STAIRS: expr = "-op_union_stairs(-" + A + ", " + B + ", " + r + ", " + nn + ")"; break;
+      case FT_NODE_PIPE: expr = "(fsqrt(" + A + " * " + A + " + " + B + " * " + B + ") - fabsf(" + r + "))"; break;
+      case FT_NODE_ENGRAVE: expr = "fmaxf(" + A + ", (" + A + " + fabsf(" + r + ") - fabsf(" + B + ")) * kInvSqrt2)"; break;
+      case FT_NODE_GROOVE: expr = "fmaxf(" + A + ", fminf(" + A + " + fabsf(" + r + "), fabsf(" + nn + ") - fabsf(" + B + ")))"; break;
+      default: expr = "fminf(" + A + ", fmaxf(" + A + " - fabsf(" + r + "), fabsf(" + B + ") - fabsf(" + nn + ")))"; break;
+    }
+    out.d = fresh("sd");
+    body_ << "  const float " << out.d << " = " << expr << ";\n";
+    if (need_lb) {
+      out.lb = fresh("sl");
+      body_ << "  const float " << out.lb << " = " << aabb(n, "p") << ";\n";
+    }
+    // blend_t: smoothstep of the children's distance difference over max(|radius|, 1e-6)
+    const bool grooved = k == FT_NODE_GROOVE || k == FT_NODE_TONGUE;
+    float rad = grooved ? std::fmax(n.p[0], n.p[1]) : n.p[0];
+    rad = std::fabs(rad);
+    if (!(rad >= 1.0e-6f)) rad = 1.0e-6f;
+    const bool diff_class = k == FT_NODE_DIFF_ROUND || k == FT_NODE_DIFF_CHAMFER || k == FT_NODE_PIPE ||
+                            k == FT_NODE_ENGRAVE || k == FT_NODE_DIFF_COLUMNS || k == FT_NODE_DIFF_STAIRS ||
+                            k == FT_NODE_GROOVE || k == FT_NODE_TONGUE;
+    const std::string t = fresh("st"), rn = fresh("rn"), bn = fresh("bn");
+    if (diff_class)
+      body_ << "  const float " << t << " = smoothstepf(0.5f + 0.5f * (((-" << B << ") - " << A << ") / " << lit(rad) << "));\n";
+    else
+      body_ << "  const float " << t << " = smoothstepf(0.5f + 0.5f * ((" << A << " - " << B << ") / " << lit(rad) << "));\n";
+    body_ << "  const float3 " << rn << " = " << (diff_class ? "mul(" + b.n + ", -1.0f)" : b.n) << ";\n";
+    body_ << "  const float3 " << bn << " = add(mul(" << a.n << ", 1.0f - " << t << "), mul(" << rn << ", " << t << "));\n";
+    out.n = fresh("sn");
+    body_ << "  const float3 " << out.n << " = len(" << bn << ") > 1.0e-6f ? normalize(" << bn << ") : (" << t
+          << " < 0.5f ? " << a.n << " : " << rn << ");\n";
+    if (i == split_) {
+      body_ << "  ref_to_mat(slot, pool, " << a.m << ", p, w.a);\n  ref_to_mat(slot, pool, " << b.m << ", p, w.b);\n"
+            << "  w.split_t = " << t << ";\n  w.split = true;\n";
+    }
+    out.m = fresh("sm");
+    body_ << "  MRef " << out.m << " = " << a.m << ";\n  blend_refs(slot, pool, " << out.m << ", " << b.m << ", " << t
+          << ", p);\n";
+    return out;
+  }
+
   // sdf_lower_bound (renderer.rs:5413-5543)
   std::string emit_lb(int i, const std::string& p) {
     const ftd::DNode& n = S_.nodes[i];
@@ -393,6 +562,9 @@ EmitPlan plan_emission(const HostScene& S, const ft_scene_ir* ir) {
   const int root_nodes = S.nodes[S.root].prog_last - S.nodes[S.root].prog_first + 1;
   const bool single = S.accel == FT_ACCEL_NAIVE || S.leaves.size() <= 1;
   plan.scene = single && root_nodes <= kMaxEmittedNodes && root_nodes >= kMinEmittedNodes;
+  const ftd::DNode& r = S.nodes[S.root];
+  const bool hard_root = r.kind == FT_NODE_UNION || r.kind == FT_NODE_INTERSECT || r.kind == FT_NODE_SUBTRACT;
+  plan.sweep = plan.scene && S.simple && !hard_root && r.walk_ok && r.walk_split != -2;
   return plan;
 }
 
@@ -406,7 +578,8 @@ std::string emit_scene_source(const HostScene& S, const ft_scene_ir* ir, const E
   Emitter E(S, ir);
   std::ostringstream o;
   o << "// generated by ft_emit.cpp\n#define FT_EMITTED 1\n#define FT_EMITTED_SCENE " << (plan.scene ? 1 : 0)
-    << "\n#define FT_EMITTED_FUNCTIONS " << (plan.functions ? 1 : 0) << "\n#include \"ft_kernels.h\"\nnamespace ftd {\n";
+    << "\n#define FT_EMITTED_FUNCTIONS " << (plan.functions ? 1 : 0) << "\n#define FT_EMITTED_SWEEP "
+    << (plan.sweep ? 1 : 0) << "\n#include \"ft_kernels.h\"\nnamespace ftd {\n";
   o << E.functions();
   if (plan.scene) {
     o << E.subtree_info(S.root, "ftx_root");
@@ -454,6 +627,7 @@ std::string emit_scene_source(const HostScene& S, const ft_scene_ir* ir, const E
         << lit(L.bmin[0]) << ", " << lit(L.bmin[1]) << ", " << lit(L.bmin[2]) << ", " << lit(L.bmax[0]) << ", "
         << lit(L.bmax[1]) << ", " << lit(L.bmax[2]) << ");\n}\n";
     }
+    if (plan.sweep) o << E.sweep(S.root);
   } else {
     o << "__device__ __forceinline__ Info ftx_scene_info(const DevScene& S, float3 p) { return Info(); }\n"
       << "__device__ __forceinline__ float ftx_scene_dist(const DevScene& S, float3 p) { return 0.0f; }\n"

@@ -16,6 +16,7 @@ constexpr int kMaxEmittedNodes = 4096;
 struct EmitPlan {
   bool functions = false;  // Forge functions become ftx_fn_<k>
   bool scene = false;      // the scene distance / lower bound become ftx_scene_info / ftx_lower_bound
+  bool sweep = false;      // the shading sweep of the root becomes ftx_sweep (scene and no modifier nodes)
   bool any() const { return functions || scene; }
 };
 

@@ -259,12 +259,16 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
   if (S.max_stack > ftd::kMaxStack) { err = "scene tree needs a deeper evaluation stack than supported"; return false; }
   if (S.max_points > ftd::kMaxPointStack) { err = "modifier nesting deeper than supported"; return false; }
 
-  // nodes whose lower bound a parent reads (Union / Intersect children)
-  for (uint32_t i = 0; i < N; ++i) {
+  // nodes whose lower bound a parent reads (Union / Intersect children); a Smooth or
+  // Subtract node's bound is its lhs's, so the need passes down to that lhs.  Parents follow
+  // their children in the IR, so one backward pass reaches every level.
+  for (uint32_t i = N; i-- > 0;) {
     const ft_node& n = ir->nodes[i];
     if (n.kind == FT_NODE_UNION || n.kind == FT_NODE_INTERSECT) {
       S.nodes[n.lhs].flags |= 2;
       S.nodes[n.rhs].flags |= 2;
+    } else if ((n.kind == FT_NODE_SMOOTH || n.kind == FT_NODE_SUBTRACT) && (S.nodes[i].flags & 2)) {
+      S.nodes[n.lhs].flags |= 2;
     }
   }
   for (uint32_t i = 0; i < N; ++i)

@@ -334,20 +334,24 @@ __device__ inline MRef ref_of_mat(MatPool& pool, const Mat& m) {
   }
   return r;
 }
+// the lerp of two single parameter sets of one model, 0 < t < 1, into a pool entry
+__device__ __noinline__ void blend_refs_lerp(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
+  const int dst = l.ha < 0 ? -1 - l.ha : (r.ha < 0 ? -1 - r.ha : pool_take(pool));
+  if (dst < 0) return;
+  DParams a, b;
+  ref_params(slot, pool, l.ha, p, a);
+  ref_params(slot, pool, r.ha, p, b);
+  blend_params(a, b, t);
+  pool.e[dst] = a;
+  if (l.ha < 0 && r.ha < 0) pool_drop(pool, r.ha);  // dst is l's entry
+  l.ha = -1 - dst;
+}
 // blend_materials on references; r is consumed
-__device__ inline void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
+__device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
   if (!l.blend && !r.blend && l.a_model == r.a_model) {
     if (t == 0.0f) { pool_drop(pool, r.ha); return; }
     if (t == 1.0f) { pool_drop(pool, l.ha); l = r; return; }
-    const int dst = l.ha < 0 ? -1 - l.ha : (r.ha < 0 ? -1 - r.ha : pool_take(pool));
-    if (dst < 0) return;
-    DParams a, b;
-    ref_params(slot, pool, l.ha, p, a);
-    ref_params(slot, pool, r.ha, p, b);
-    blend_params(a, b, t);
-    pool.e[dst] = a;
-    if (l.ha < 0 && r.ha < 0) pool_drop(pool, r.ha);  // dst is l's entry
-    l.ha = -1 - dst;
+    blend_refs_lerp(slot, pool, l, r, t, p);
     return;
   }
   const bool la = !l.blend || l.t < 0.5f, ra = !r.blend || r.t < 0.5f;
@@ -511,6 +515,11 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
   if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, ms[0], p, w.a);
 }
 
+#if defined(FT_EMITTED) && FT_EMITTED_SWEEP
+// Provided by the scene emitter: shade_sweep_uniform of the root as straight-line code.
+__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w);
+#endif
+
 // The three walks from `start` for one hit: hard booleans first (per lane, the walks'
 // side choice), then the sweep of the node reached, lanes grouped by that node.
 __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float eps, ShadeWalk& w) {
@@ -534,7 +543,14 @@ __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float ep
     if (!w.split) walk_material(slot, start, p, w.a);
     return;
   }
-  for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w); });
+#if defined(FT_EMITTED) && FT_EMITTED_SWEEP
+  if (node == S.root) {
+    ftx_sweep(S, slot, p, eps, w);
+  } else
+#endif
+  {
+    for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w); });
+  }
   w.normal = mul(w.normal, sign);
   if (w.mat_overflow) {
     w.split = split_material(slot, start, p, w.a, w.b, w.split_t);

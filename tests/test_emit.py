@@ -67,6 +67,18 @@ def test_plan_selection():
     assert "#define FT_EMITTED_SCENE 1" in src
 
 
+def test_sweep_emission():
+    """The shading sweep is emitted for a one-leaf scene whose root is a smooth boolean and
+    that has no custom or modifier node; a hard-union root keeps the interpreted sweep."""
+    big = compile_src(synthetic.whitted_1080p(n_prims=8).source)
+    src = render.emit_scene_source(big, AccelMode.Bvh)
+    assert "#define FT_EMITTED_SWEEP 1" in src and "void ftx_sweep(" in src
+    assert src.count("tetra_normal(") == 8 and src.count("blend_refs(") == 7
+    hard = compile_src(synthetic.depth_512().source)
+    src = render.emit_scene_source(hard, AccelMode.Naive)
+    assert "#define FT_EMITTED_SWEEP 0" in src and "void ftx_sweep(" not in src
+
+
 def test_emit_rejects_bad_ir():
     from forgedthoughts_amd.scene import RenderError
     bad = compile_src("let scene = Sphere { radius: 1.0 };")

@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 ACCEL_NAIVE, ACCEL_BVH, ACCEL_BRICKS = 0, 1, 2
 
@@ -57,6 +57,13 @@ MATERIAL_DTYPE = np.dtype([
 ])
 assert MATERIAL_DTYPE.itemsize == 144
 
+HOOKS_DTYPE = np.dtype([
+    ("material", "<u4"), ("color", "<i4"), ("roughness", "<i4"), ("ior", "<i4"), ("thin_walled", "<i4"),
+    ("emission_color", "<i4"), ("emission_strength", "<i4"), ("pad", "<i4"),
+])
+assert HOOKS_DTYPE.itemsize == 32
+HOOK_PARAMS = 13
+
 LIGHT_DTYPE = np.dtype([
     ("kind", "<u4"), ("samples", "<u4"), ("radius", "<f4"), ("pad", "<f4"),
     ("position", "<f4", 3), ("pad1", "<f4"), ("radiance", "<f4", 3), ("pad2", "<f4"),
@@ -81,6 +88,7 @@ class SceneIR(C.Structure):
         ("nodes", C.c_void_p), ("materials", C.c_void_p), ("object_transforms", C.c_void_p),
         ("lights", C.c_void_p), ("functions", C.c_void_p), ("exprs", C.c_void_p),
         ("camera", Camera),
+        ("n_material_hooks", C.c_uint32), ("pad2", C.c_uint32), ("material_hooks", C.c_void_p),
     ]
 
 
@@ -117,6 +125,7 @@ BATCH_DEPTH, BATCH_WHITTED, BATCH_SEPARATE, BATCH_GRAPH = 1, 2, 4, 8
 STRUCT_SIZES = {
     "ft_node": NODE_DTYPE.itemsize, "ft_expr": EXPR_DTYPE.itemsize,
     "ft_function": FUNCTION_DTYPE.itemsize, "ft_material": MATERIAL_DTYPE.itemsize,
+    "ft_material_hooks": HOOKS_DTYPE.itemsize,
     "ft_light": LIGHT_DTYPE.itemsize, "ft_camera": C.sizeof(Camera),
     "ft_scene_ir": C.sizeof(SceneIR), "ft_render_options": C.sizeof(RenderOptionsC),
     "ft_ray_settings": C.sizeof(RaySettingsC), "ft_render_stats": C.sizeof(RenderStatsC),

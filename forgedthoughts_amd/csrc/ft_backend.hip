@@ -77,7 +77,7 @@ struct ft_scene {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevBuf nodes, prog, ops, lbprog, mats, objects, lights, fns, exprs, bvh, leaves;
+  DevBuf nodes, prog, ops, lbprog, mats, hooks, objects, lights, fns, exprs, bvh, leaves;
   DevBuf rgb, tbuf, samples, hdr, overflow, flops;
   DevBuf atlas_cell_d, atlas_cells, atlas_brick_cell, atlas_bricks, atlas_counter;
   ft_render_stats stats{};
@@ -369,7 +369,7 @@ static int create_scene(const ft_scene_ir* ir, uint32_t accel_mode, bool batch_m
   fth::HostScene& h = s->host;
   if ((rc = upload(s->nodes, h.nodes)) || (rc = upload(s->prog, h.prog)) || (rc = upload(s->ops, h.ops)) ||
       (rc = upload(s->lbprog, h.lbprog)) ||
-      (rc = upload(s->mats, h.mats)) || (rc = upload(s->objects, h.objects)) || (rc = upload(s->lights, h.lights)) ||
+      (rc = upload(s->mats, h.mats)) || (rc = upload(s->hooks, h.hooks)) || (rc = upload(s->objects, h.objects)) || (rc = upload(s->lights, h.lights)) ||
       (rc = upload(s->fns, h.fns)) || (rc = upload(s->exprs, h.exprs)) || (rc = upload(s->bvh, h.bvh)) ||
       (rc = upload(s->leaves, h.leaves))) {
     std::string e = g_err;
@@ -383,6 +383,7 @@ static int create_scene(const ft_scene_ir* ir, uint32_t accel_mode, bool batch_m
   d.ops = (const DOp*)s->ops.ptr;
   d.lbprog = (const unsigned*)s->lbprog.ptr;
   d.mats = (const DParams*)s->mats.ptr;
+  d.hooks = (const DHooks*)s->hooks.ptr;
   d.objects = (const float*)s->objects.ptr;
   d.lights = (const DLight*)s->lights.ptr;
   d.fns = (const ft_function*)s->fns.ptr;
@@ -433,7 +434,7 @@ int ft_scene_destroy(ft_scene* s) {
   if (!s) return FT_OK;
   (void)hipSetDevice(s->device);
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  for (DevBuf* b : {&s->nodes, &s->prog, &s->ops, &s->lbprog, &s->mats, &s->objects, &s->lights, &s->fns, &s->exprs, &s->bvh,
+  for (DevBuf* b : {&s->nodes, &s->prog, &s->ops, &s->lbprog, &s->mats, &s->hooks, &s->objects, &s->lights, &s->fns, &s->exprs, &s->bvh,
                     &s->leaves, &s->rgb, &s->tbuf, &s->samples, &s->hdr, &s->overflow, &s->flops, &s->atlas_cell_d,
                     &s->atlas_cells, &s->atlas_brick_cell, &s->atlas_bricks, &s->atlas_counter})
     if (b->ptr) (void)hipFree(b->ptr);
@@ -645,6 +646,7 @@ int ft_abi_struct_size(const char* name) {
   if (n == "ft_expr") return (int)sizeof(ft_expr);
   if (n == "ft_function") return (int)sizeof(ft_function);
   if (n == "ft_material") return (int)sizeof(ft_material);
+  if (n == "ft_material_hooks") return (int)sizeof(ft_material_hooks);
   if (n == "ft_light") return (int)sizeof(ft_light);
   if (n == "ft_camera") return (int)sizeof(ft_camera);
   if (n == "ft_scene_ir") return (int)sizeof(ft_scene_ir);

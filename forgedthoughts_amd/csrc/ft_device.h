@@ -31,6 +31,7 @@ struct DevScene {
   const unsigned* __restrict__ prog;
   const DOp* __restrict__ ops;           // prog as records (plus one padding record)
   const DParams* __restrict__ mats;
+  const DHooks* __restrict__ hooks;      // per-hit material hooks (DParams::hook)
   const float* __restrict__ objects;     // 9 floats per object: centre, sin/cos z,y,x
   const DLight* __restrict__ lights;
   const ft_function* __restrict__ fns;

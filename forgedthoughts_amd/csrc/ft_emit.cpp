@@ -45,9 +45,9 @@ class Emitter {
     o << "__device__ float3 ftx_fn(int fn, const float3* a) {\n  switch (fn) {\n";
     for (uint32_t f = 0; f < ir_->n_functions; ++f) {
       const ft_function& F = ir_->functions[f];
-      if (F.n_params == 1) o << "    case " << f << ": return ftx_fn_" << f << "(a[0]);\n";
-      else if (F.n_params == 2) o << "    case " << f << ": return ftx_fn_" << f << "(a[0], a[1]);\n";
-      else o << "    case " << f << ": return ftx_fn_" << f << "();\n";
+      o << "    case " << f << ": return ftx_fn_" << f << "(";
+      for (uint32_t a = 0; a < F.n_params; ++a) o << (a ? ", " : "") << "a[" << a << "]";
+      o << ");\n";
     }
     o << "    default: return mk(0.0f, 0.0f, 0.0f);\n  }\n}\n";
     return o.str();
@@ -369,11 +369,12 @@ class Emitter {
     split_ = S_.nodes[root].walk_split;
     SVal v = sweep_node(root, false);
     std::ostringstream o;
-    o << "__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w) {\n"
+    o << "__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w,\n"
+      << "                                          const HookCtx* hc) {\n"
       << "  MatPool pool;\n  pool.used = 0u;\n  pool.overflow = 0;\n  w.split = false;\n" << body_.str()
       << "  if (S.flops) atomicAdd(S.flops, " << (unsigned long long)(prim_cost_ + 0.5) << "ull);\n"
       << "  w.normal = " << v.n << ";\n  w.mat_overflow = pool.overflow != 0;\n"
-      << "  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, " << v.m << ", p, w.a);\n}\n";
+      << "  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, " << v.m << ", p, w.a, hc);\n}\n";
     return o.str();
   }
 
@@ -518,12 +519,12 @@ class Emitter {
     body_ << "  const float3 " << out.n << " = len(" << bn << ") > 1.0e-6f ? normalize(" << bn << ") : (" << t
           << " < 0.5f ? " << a.n << " : " << rn << ");\n";
     if (i == split_) {
-      body_ << "  ref_to_mat(slot, pool, " << a.m << ", p, w.a);\n  ref_to_mat(slot, pool, " << b.m << ", p, w.b);\n"
+      body_ << "  ref_to_mat(slot, pool, " << a.m << ", p, w.a, hc);\n  ref_to_mat(slot, pool, " << b.m << ", p, w.b, hc);\n"
             << "  w.split_t = " << t << ";\n  w.split = true;\n";
     }
     out.m = fresh("sm");
     body_ << "  MRef " << out.m << " = " << a.m << ";\n  blend_refs(slot, pool, " << out.m << ", " << b.m << ", " << t
-          << ", p);\n";
+          << ", p, hc);\n";
     return out;
   }
 

@@ -67,11 +67,18 @@ struct DParams {  // MaterialParams (materials/mod.rs:67-87)
   float medium_absorption[3];
   float model;
   float pattern_a[3];
-  float pad0;
+  float hook;       // index into the hook table (DHooks), -1: no per-hit hooks
   float pattern_b[3];
   float pad1;
 };
 static_assert(sizeof(DParams) == 144, "DParams layout");
+
+// Per-hit hook functions of one material (ft_material_hooks without the material index).
+struct DHooks {
+  int color, roughness, ior, thin_walled, emission_color, emission_strength;
+  int pad[2];
+};
+static_assert(sizeof(DHooks) == 32, "DHooks layout");
 
 struct DLight {
   int kind;

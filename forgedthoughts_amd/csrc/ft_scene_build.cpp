@@ -61,6 +61,9 @@ bool is_blend(uint32_t k) {
 bool validate(const ft_scene_ir* ir, std::string& err) {
   if (!ir) { err = "scene IR is null"; return false; }
   if (ir->abi_version != FT_ABI_VERSION) { err = "scene IR ABI version mismatch"; return false; }
+  if (ir->n_material_hooks > 0 && !ir->material_hooks) { err = "material hook table is null"; return false; }
+  for (uint32_t h = 0; h < ir->n_material_hooks; ++h)
+    if (ir->material_hooks[h].material >= ir->n_materials) { err = "material hook names no material"; return false; }
   if (ir->n_nodes == 0 || !ir->nodes || ir->root >= ir->n_nodes) {
     err = "no suitable scene root found (expected binding like scene/result/b/s)";
     return false;
@@ -103,7 +106,10 @@ bool validate(const ft_scene_ir* ir, std::string& err) {
       const ft_expr& e = ir->exprs[F.first + k];
       if (e.op >= FT_EXPR_OP_COUNT) { err = "unsupported expression op"; return false; }
       if (e.op == FT_EXPR_PARAM) {
-        if (e.a < 0 || (uint32_t)e.a >= F.n_params || F.n_params > 2) { err = "expression parameter out of range"; return false; }
+        if (e.a < 0 || (uint32_t)e.a >= F.n_params || (F.n_params > 2 && F.n_params != FT_HOOK_PARAMS)) {
+          err = "expression parameter out of range";
+          return false;
+        }
         continue;
       }
       for (int32_t o : {e.a, e.b, e.c, e.d})
@@ -503,8 +509,24 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     p.has_medium = a.has_medium ? 1.0f : 0.0f; p.medium_ior = a.medium_ior;
     p.medium_density = a.medium_density; p.pattern = (float)a.pattern; p.pattern_scale = a.pattern_scale;
     p.model = (float)a.model;
+    p.hook = -1.0f;
+    for (uint32_t h = 0; h < ir->n_material_hooks; ++h) {
+      const ft_material_hooks& H = ir->material_hooks[h];
+      if (H.material != m) continue;
+      const int32_t ids[6] = {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength};
+      for (int32_t f : ids) {
+        if (f >= (int32_t)ir->n_functions || (f >= 0 && ir->functions[f].n_params != FT_HOOK_PARAMS)) {
+          err = "material hook is not a function of the shading context";
+          return false;
+        }
+      }
+      ftd::DHooks d{H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength, {0, 0}};
+      p.hook = (float)S.hooks.size();
+      S.hooks.push_back(d);
+      break;
+    }
     S.mats.push_back(p);
-    if (a.dynamic_material_id >= 0 && S.dynamic_material < 0) S.dynamic_material = (int)m;
+    if (a.dynamic_material_id >= 0 && p.hook < 0.0f && S.dynamic_material < 0) S.dynamic_material = (int)m;
     bool transmissive = a.model == FT_MODEL_DIELECTRIC || (a.model == FT_MODEL_STANDARD && a.transmission > 0.0f);
     any_transmissive = any_transmissive || transmissive;
     if (transmissive && a.roughness > 0.06f) any_rough_transmissive = true;

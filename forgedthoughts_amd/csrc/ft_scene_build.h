@@ -14,6 +14,7 @@ struct HostScene {
   std::vector<ftd::DOp> ops;  // prog as records, plus one padding record
   std::vector<unsigned> lbprog;
   std::vector<ftd::DParams> mats;
+  std::vector<ftd::DHooks> hooks;  // per-hit hook functions, indexed by DParams::hook
   std::vector<float> objects;  // 9 per object
   std::vector<ftd::DLight> lights;
   std::vector<ft_function> fns;
@@ -35,7 +36,7 @@ struct HostScene {
   int max_fn_exprs = 0;
   int max_children = 2;     // rays a single hit can spawn (both split materials together)
   int max_bvh_depth = 0;
-  int dynamic_material = -1;  // first material with per-hit hook functions, or -1
+  int dynamic_material = -1;  // first material with per-hit hooks the GPU does not evaluate, or -1
 };
 
 // Validates `ir` and builds the device layout.  Returns false with `err` set on failure.

@@ -88,27 +88,66 @@ __device__ inline void blend_materials(Mat& left, const Mat& right, float t) {
   left = out;
 }
 
-// resolve_leaf_material, static part (renderer.rs:7164-7536): the parameter set of a leaf
-// material at `pos`; returns its model
-__device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float3 pos, DParams& a) {
+// The shading context of one hit as material hooks see it (make_shading_context
+// renderer.rs:7875-7911): hit position, front-facing unit normal, unit view direction.
+struct HookCtx {
+  float3 pos, normal, view;
+  float front;
+};
+
+// resolve_leaf_material (renderer.rs:7164-7536): the parameter set of a leaf material at
+// `pos`; returns its model.  With a shading context, the material's per-hit hooks (the arms
+// for definition models other than Standard / Layered, renderer.rs:7472-7530) replace the
+// static colour, roughness, ior, thin_walled and emission before the clamps.
+__device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float3 pos, DParams& a,
+                                  const HookCtx* hc = nullptr) {
   const DevScene& S = scene_of(slot);
   unsigned id = mat_id < (unsigned)S.n_mats ? mat_id : 0u;
   a = S.mats[id];
   const int model = (int)a.model;
-  if (a.pattern == (float)FT_PATTERN_CHECKER3D) {  // resolve_pattern_color (renderer.rs:7855-7873)
-    float3 lp = object_local(slot, obj, pos);
-    int sx = (int)floorf(lp.x * a.pattern_scale), sy = (int)floorf(lp.y * a.pattern_scale),
-        sz = (int)floorf(lp.z * a.pattern_scale);
-    set3(a.color, ((sx + sy + sz) & 1) == 0 ? v3(a.pattern_a) : v3(a.pattern_b));
+  const int hk = (int)a.hook;
+  const bool hooked = hk >= 0 && hc != nullptr;
+  const bool checker = a.pattern == (float)FT_PATTERN_CHECKER3D;
+  if (checker || hooked) {
+    const float3 lp = object_local(slot, obj, pos);
+    if (hooked) {
+      const DHooks H = S.hooks[hk];
+      float3 args[FT_HOOK_PARAMS];
+      args[0] = hc->pos;
+      args[1] = lp;
+      args[2] = hc->normal;
+      args[3] = hc->view;
+      args[4] = splat3(hc->front);
+      args[5] = splat3((float)obj);
+      args[6] = splat3((float)mat_id);
+      for (int k = 7; k < (int)FT_HOOK_PARAMS; ++k) args[k] = mk(0.0f, 0.0f, 0.0f);
+      if (H.color >= 0) {
+        set3(a.color, eval_fn(slot, H.color, args));
+      } else if (checker) {  // resolve_pattern_color (renderer.rs:7855-7873)
+        const int sx = (int)floorf(lp.x * a.pattern_scale), sy = (int)floorf(lp.y * a.pattern_scale),
+                  sz = (int)floorf(lp.z * a.pattern_scale);
+        set3(a.color, ((sx + sy + sz) & 1) == 0 ? v3(a.pattern_a) : v3(a.pattern_b));
+      }
+      if (H.roughness >= 0) a.roughness = eval_fn(slot, H.roughness, args).x;
+      if (H.ior >= 0) a.ior = eval_fn(slot, H.ior, args).x;
+      if (H.thin_walled >= 0) a.thin_walled = eval_fn(slot, H.thin_walled, args).x >= 0.5f ? 1.0f : 0.0f;
+      if (H.emission_color >= 0) set3(a.emission_color, eval_fn(slot, H.emission_color, args));
+      if (H.emission_strength >= 0) a.emission_strength = eval_fn(slot, H.emission_strength, args).x;
+    } else {  // resolve_pattern_color (renderer.rs:7855-7873)
+      const int sx = (int)floorf(lp.x * a.pattern_scale), sy = (int)floorf(lp.y * a.pattern_scale),
+                sz = (int)floorf(lp.z * a.pattern_scale);
+      set3(a.color, ((sx + sy + sz) & 1) == 0 ? v3(a.pattern_a) : v3(a.pattern_b));
+    }
   }
   if (model != TAG_LAMBERT) a.roughness = clampf(a.roughness, 0.0f, 1.0f);
   if (model == TAG_STANDARD || model == TAG_DIELECTRIC) a.ior = clampf(a.ior, 1.0f, 3.0f);
   a.emission_strength = fmaxf(a.emission_strength, 0.0f);
   return model;
 }
-__device__ inline void leaf_material(int slot, unsigned mat_id, unsigned obj, float3 pos, Mat& m) {
+__device__ inline void leaf_material(int slot, unsigned mat_id, unsigned obj, float3 pos, Mat& m,
+                                     const HookCtx* hc = nullptr) {
   m.blend = 0;
-  m.a_model = leaf_params(slot, mat_id, obj, pos, m.a);
+  m.a_model = leaf_params(slot, mat_id, obj, pos, m.a, hc);
   m.b_model = 0;
   m.t = 0.0f;
   m.pad = 0;
@@ -195,7 +234,8 @@ __device__ __noinline__ float3 walk_normal(int slot, int start, float3 p, float 
 }
 
 // resolve_material_from_node (renderer.rs:6945-7059), from `start`
-__device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& out, int base = 0) {
+__device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& out, int base = 0,
+                                          const HookCtx* hc = nullptr) {
   const DevScene& S = scene_of(slot);
   Frame fr[kMaxWalk];
   Mat res[kMaxWalkMats];
@@ -204,7 +244,7 @@ __device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& o
   while (fsp > 0) {
     Frame f = fr[--fsp];
     const DNode& n = S.nodes[f.node];
-    if (n.kind <= FT_NODE_CUSTOM) { leaf_material(slot, n.mat, n.obj, p, res[rsp++]); continue; }
+    if (n.kind <= FT_NODE_CUSTOM) { leaf_material(slot, n.mat, n.obj, p, res[rsp++], hc); continue; }
     if (n.kind <= FT_NODE_NOISE || n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) {
       fr[fsp++] = {n.lhs, 0};
       continue;
@@ -224,7 +264,8 @@ __device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& o
 }
 
 // resolve_split_material_from_node (renderer.rs:7061-7120)
-__device__ __noinline__ bool split_material(int slot, int start, float3 p, Mat& a, Mat& b, float& t) {
+__device__ __noinline__ bool split_material(int slot, int start, float3 p, Mat& a, Mat& b, float& t,
+                                           const HookCtx* hc = nullptr) {
   const DevScene& S = scene_of(slot);
   int node = start;
   for (;;) {
@@ -237,8 +278,8 @@ __device__ __noinline__ bool split_material(int slot, int start, float3 p, Mat& 
     if (n.kind == FT_NODE_SMOOTH || n.kind == FT_NODE_NOISE) { node = n.lhs; continue; }
     if (blend_class(n.kind) == 0) return false;
     float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
-    walk_material(slot, n.lhs, p, a);
-    walk_material(slot, n.rhs, p, b);
+    walk_material(slot, n.lhs, p, a, 0, hc);
+    walk_material(slot, n.rhs, p, b, 0, hc);
     t = blend_t(n, dl, dr);
     return true;
   }
@@ -288,22 +329,24 @@ struct MatPool {
   unsigned used;  // bit i: entry i is referenced
   int overflow;
 };
-__device__ __forceinline__ void ref_params(int slot, const MatPool& pool, int h, float3 p, DParams& out) {
+__device__ __forceinline__ void ref_params(int slot, const MatPool& pool, int h, float3 p, DParams& out,
+                                           const HookCtx* hc) {
   if (h < 0) {
     out = pool.e[-1 - h];
   } else {
     const DNode& n = scene_of(slot).nodes[h];
-    leaf_params(slot, n.mat, n.obj, p, out);
+    leaf_params(slot, n.mat, n.obj, p, out, hc);
   }
 }
-__device__ inline void ref_to_mat(int slot, const MatPool& pool, const MRef& r, float3 p, Mat& m) {
+__device__ inline void ref_to_mat(int slot, const MatPool& pool, const MRef& r, float3 p, Mat& m,
+                                  const HookCtx* hc) {
   m.blend = r.blend;
   m.a_model = r.a_model;
   m.b_model = r.b_model;
   m.t = r.t;
   m.pad = 0;
-  ref_params(slot, pool, r.ha, p, m.a);
-  if (r.blend) ref_params(slot, pool, r.hb, p, m.b);
+  ref_params(slot, pool, r.ha, p, m.a, hc);
+  if (r.blend) ref_params(slot, pool, r.hb, p, m.b, hc);
 }
 __device__ __forceinline__ int pool_take(MatPool& pool) {
   const unsigned free_ = ~pool.used & ((1u << kMatPool) - 1u);
@@ -335,23 +378,25 @@ __device__ inline MRef ref_of_mat(MatPool& pool, const Mat& m) {
   return r;
 }
 // the lerp of two single parameter sets of one model, 0 < t < 1, into a pool entry
-__device__ __noinline__ void blend_refs_lerp(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
+__device__ __noinline__ void blend_refs_lerp(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p,
+                                            const HookCtx* hc) {
   const int dst = l.ha < 0 ? -1 - l.ha : (r.ha < 0 ? -1 - r.ha : pool_take(pool));
   if (dst < 0) return;
   DParams a, b;
-  ref_params(slot, pool, l.ha, p, a);
-  ref_params(slot, pool, r.ha, p, b);
+  ref_params(slot, pool, l.ha, p, a, hc);
+  ref_params(slot, pool, r.ha, p, b, hc);
   blend_params(a, b, t);
   pool.e[dst] = a;
   if (l.ha < 0 && r.ha < 0) pool_drop(pool, r.ha);  // dst is l's entry
   l.ha = -1 - dst;
 }
 // blend_materials on references; r is consumed
-__device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p) {
+__device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p,
+                                           const HookCtx* hc) {
   if (!l.blend && !r.blend && l.a_model == r.a_model) {
     if (t == 0.0f) { pool_drop(pool, r.ha); return; }
     if (t == 1.0f) { pool_drop(pool, l.ha); l = r; return; }
-    blend_refs_lerp(slot, pool, l, r, t, p);
+    blend_refs_lerp(slot, pool, l, r, t, p, hc);
     return;
   }
   const bool la = !l.blend || l.t < 0.5f, ra = !r.blend || r.t < 0.5f;
@@ -379,7 +424,7 @@ struct ShadeWalk {
 // The sweep of node N (wave-uniform).  Stack entries: distance / lower bound in the LDS
 // value planes, normal and material per lane in registers / private memory.
 __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot, int N, float3 p, float eps,
-                                                    ShadeWalk& w) {
+                                                    ShadeWalk& w, const HookCtx* hc) {
   const auto nodes = uniform_ptr(S.nodes);
   const auto prog = uniform_ptr(S.prog);
   const int pf = __builtin_amdgcn_readfirstlane(S.nodes[N].prog_first);
@@ -407,7 +452,7 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
       slb[sp * nth] = v.lb;
       ns[sp] = tetra_normal([&](float3 q) { return eval_uniform(S, slot, ni, q, sp).d; }, p, eps);
       Mat tm;
-      walk_material(slot, ni, p, tm, sp);
+      walk_material(slot, ni, p, tm, sp, hc);
       ms[sp] = ref_of_mat(pool, tm);
       ++sp;
       k = n.prog_last;
@@ -503,26 +548,28 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
     const float3 bn = add(mul(left, 1.0f - tt), mul(right, tt));
     ns[il] = len(bn) > 1.0e-6f ? normalize(bn) : (tt < 0.5f ? left : right);
     if (ni == split_node) {  // resolve_split_material_from_node (renderer.rs:7061-7120)
-      ref_to_mat(slot, pool, ms[il], p, w.a);
-      ref_to_mat(slot, pool, ms[ir], p, w.b);
+      ref_to_mat(slot, pool, ms[il], p, w.a, hc);
+      ref_to_mat(slot, pool, ms[ir], p, w.b, hc);
       w.split_t = tt;
       w.split = true;
     }
-    blend_refs(slot, pool, ms[il], ms[ir], tt, p);
+    blend_refs(slot, pool, ms[il], ms[ir], tt, p, hc);
   }
   w.normal = ns[0];
   w.mat_overflow = pool.overflow != 0;
-  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, ms[0], p, w.a);
+  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, ms[0], p, w.a, hc);
 }
 
 #if defined(FT_EMITTED) && FT_EMITTED_SWEEP
 // Provided by the scene emitter: shade_sweep_uniform of the root as straight-line code.
-__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w);
+__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w,
+                                          const HookCtx* hc);
 #endif
 
 // The three walks from `start` for one hit: hard booleans first (per lane, the walks'
 // side choice), then the sweep of the node reached, lanes grouped by that node.
-__device__ __noinline__ void shade_walks(int slot, int start, float3 p, float eps, ShadeWalk& w) {
+__device__ __noinline__ void shade_walks(int slot, int start, float3 p, float eps, ShadeWalk& w,
+                                        const HookCtx* hc) {
   const DevScene& S = scene_of(slot);
   int node = start;
   float sign = 1.0f;
@@ -539,22 +586,22 @@ __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float ep
   }
   if (S.nodes[node].walk_split == -2 || !S.nodes[node].walk_ok) {
     w.normal = walk_normal(slot, start, p, eps);
-    w.split = split_material(slot, start, p, w.a, w.b, w.split_t);
-    if (!w.split) walk_material(slot, start, p, w.a);
+    w.split = split_material(slot, start, p, w.a, w.b, w.split_t, hc);
+    if (!w.split) walk_material(slot, start, p, w.a, 0, hc);
     return;
   }
 #if defined(FT_EMITTED) && FT_EMITTED_SWEEP
   if (node == S.root) {
-    ftx_sweep(S, slot, p, eps, w);
+    ftx_sweep(S, slot, p, eps, w, hc);
   } else
 #endif
   {
-    for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w); });
+    for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w, hc); });
   }
   w.normal = mul(w.normal, sign);
   if (w.mat_overflow) {
-    w.split = split_material(slot, start, p, w.a, w.b, w.split_t);
-    if (!w.split) walk_material(slot, start, p, w.a);
+    w.split = split_material(slot, start, p, w.a, w.b, w.split_t, hc);
+    if (!w.split) walk_material(slot, start, p, w.a, 0, hc);
   }
 }
 
@@ -996,7 +1043,9 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
       // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449), and
       // the hit's material or split materials, from one shading sweep
       ShadeWalk w;
-      shade_walks(slot, start, hit.pos, 1.0e-4f, w);
+      const HookCtx hc{hit.pos, hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f)),
+                       normalize(mul(rd, -1.0f)), hit.front ? 1.0f : 0.0f};
+      shade_walks(slot, start, hit.pos, 1.0e-4f, w, &hc);
       const float3 sn = normalize(hit.front ? w.normal : mul(w.normal, -1.0f));
       const float3 att = cmul(thr, transmittance(med, hit.t));
       c = mk(0, 0, 0);
@@ -1024,7 +1073,9 @@ __device__ __noinline__ float3 trace_aov(int slot, const TraceParams& tp, float3
   if (!raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit)) return mk(0, 0, 0);
   const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
   Mat mat;
-  walk_material(slot, start, hit.pos, mat);
+  const HookCtx hc{hit.pos, hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f)),
+                   normalize(mul(d, -1.0f)), hit.front ? 1.0f : 0.0f};
+  walk_material(slot, start, hit.pos, mat, 0, &hc);
   float3 normal = hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f));
   int model = dom_model(mat);
   float ior = (model == TAG_DIELECTRIC || model == TAG_STANDARD) ? clampf(dom_params(mat).ior, 1, 3) : 1.0f;

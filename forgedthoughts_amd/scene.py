@@ -35,6 +35,13 @@ from .forge.evaluator import Obj, Func, is_num, f32
 
 # material-definition functions the reference evaluates per hit (resolve_leaf_material
 # renderer.rs:7164-7536, eval_bsdf hooks renderer.rs:4952-5030)
+# hooks evaluated on the GPU, in ft_material_hooks order: the arms of resolve_leaf_material
+# for definition models other than Standard / Layered (renderer.rs:7472-7530)
+HOOK_NAMES = ("color", "roughness", "ior", "thin_walled", "emission_color", "emission_strength")
+NUMBER_HOOKS = ("roughness", "ior", "thin_walled", "emission_strength")
+# hooks that act outside resolve_leaf_material's simple arms (BSDF overrides, bump / normal,
+# media): scenes with them are not rendered with Whitted on the GPU
+UNSUPPORTED_HOOKS = ("bump", "normal", "eval", "pdf", "sample", "medium", "subsurface")
 MATERIAL_HOOKS = ("color", "roughness", "metallic", "specular", "ior", "thin_walled", "medium", "subsurface",
                   "emission", "emission_color", "emission_strength", "bump", "normal", "eval", "pdf", "sample",
                   "transmission", "clearcoat", "clearcoat_roughness", "specular_color")
@@ -193,6 +200,7 @@ class Material:
     pattern: Optional[Tuple[np.ndarray, np.ndarray, float]] = None
     dynamic_material_id: int = -1
     dynamic_override_id: int = -1
+    hooks: Optional[tuple] = None  # lowered hook functions (HOOK_NAMES order), or None
 
     def key(self):
         def t(a):
@@ -204,7 +212,7 @@ class Material:
                 float(self.emission_strength),
                 None if self.medium is None else (float(self.medium[0]), t(self.medium[1]), float(self.medium[2])),
                 None if self.pattern is None else (t(self.pattern[0]), t(self.pattern[1]), float(self.pattern[2])),
-                self.dynamic_material_id, self.dynamic_override_id)
+                self.dynamic_material_id, self.dynamic_override_id, self.hooks)
 
 
 def _black():
@@ -289,6 +297,7 @@ class SceneCompiler:
         self.functions: List[Tuple[list, int, int]] = []  # (nodes, result_type, n_params)
         self.dynamic_overrides: List[dict] = []
         self.dynamic_hooks: List[str] = []
+        self._hook_cache: dict = {}
         self.sorted_material_names = sorted(state.material_defs.keys())
 
     # -------------------------------------------------------- materials
@@ -396,6 +405,8 @@ class SceneCompiler:
         if hooks:
             m.dynamic_material_id = self.sorted_material_names.index(name)
             self.dynamic_hooks.extend(f"{name}.{h}" for h in hooks if f"{name}.{h}" not in self.dynamic_hooks)
+            if model not in ("Standard", "Layered") and not any(h in UNSUPPORTED_HOOKS for h in hooks):
+                m.hooks = self.lower_hooks(d, name, model, overrides)
         if hooks and with_ctx and overrides.fields:
             key = repr(sorted((k, repr(v)) for k, v in overrides.fields.items()))
             if key not in self.dynamic_overrides:
@@ -408,6 +419,27 @@ class SceneCompiler:
         else:
             m.model = abi.MODEL_STANDARD
         return m
+
+    def lower_hooks(self, d, name: str, model: str, overrides: Obj) -> Optional[tuple]:
+        """The definition's hook functions of the shading context, lowered once per (definition,
+        override); -1 where absent, not read by the model's arm, or (number hooks) not a number.
+        None when a hook cannot be lowered (the scene then keeps the Whitted refusal)."""
+        key = (name, repr(sorted((k, repr(v)) for k, v in overrides.fields.items())))
+        if key in self._hook_cache:
+            return self._hook_cache[key]
+        read = set(HOOK_NAMES) if model != "Metal" else {"color", "roughness", "emission_color", "emission_strength"}
+        ids = []
+        try:
+            for h in HOOK_NAMES:
+                fid = self.lower_block_fn(d, h, ["ctx"], overrides) if h in read else -1
+                if fid >= 0 and h in NUMBER_HOOKS and self.functions[fid][1] != 0:
+                    fid = -1
+                ids.append(fid)
+            out = tuple(ids)
+        except E.ForgeEvalError:
+            out = None
+        self._hook_cache[key] = out
+        return out
 
     def _material_properties(self, d, overrides: Obj) -> dict:
         locals_ = dict(overrides.fields)
@@ -436,7 +468,8 @@ class SceneCompiler:
         for st in block.statements:
             if st[0] == "fn" and st[1] == fn_name and len(st[2]) == len(param_types):
                 nodes, rt = self.ev.lower_function(st[2], st[3], param_types, None, block, overrides)
-                return self.add_function(nodes, rt, len(param_types))
+                n_params = abi.HOOK_PARAMS if list(param_types) == ["ctx"] else len(param_types)
+                return self.add_function(nodes, rt, n_params)
         return -1
 
     def lower_func_value(self, fn: Func, param_types) -> int:
@@ -833,6 +866,7 @@ class CompiledScene:
     root: int
     leaf_count: int = 0
     dynamic_hooks: list = field(default_factory=list)  # "Material.hook" evaluated per hit
+    material_hooks: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=abi.HOOKS_DTYPE))
     _ir: object = None
 
     def ir(self) -> abi.SceneIR:
@@ -853,6 +887,8 @@ class CompiledScene:
             ir.lights = self.lights.ctypes.data if len(self.lights) else None
             ir.functions = self.functions.ctypes.data if len(self.functions) else None
             ir.exprs = self.exprs.ctypes.data if len(self.exprs) else None
+            ir.n_material_hooks = len(self.material_hooks)
+            ir.material_hooks = self.material_hooks.ctypes.data if len(self.material_hooks) else None
             cam = self.camera
             ir.camera.present = 1 if cam["present"] else 0
             ir.camera.has_origin = 1 if cam.get("origin") is not None else 0
@@ -941,6 +977,12 @@ def compile_scene(state: E.EvalState) -> CompiledScene:
     node_arr = np.array(nodes, dtype=abi.NODE_DTYPE)
 
     mats = np.array([_material_record(m) for m in comp.materials], dtype=abi.MATERIAL_DTYPE)
+    hooked = [(i, m.hooks) for i, m in enumerate(comp.materials) if m.hooks is not None]
+    hooks_arr = np.zeros(len(hooked), dtype=abi.HOOKS_DTYPE)
+    for k, (i, h) in enumerate(hooked):
+        hooks_arr[k]["material"] = i
+        for name, fid in zip(HOOK_NAMES, h):
+            hooks_arr[k][name] = fid
     ot = np.zeros((len(comp.object_transforms), 6), dtype=np.float32)
     for i, (c, r) in enumerate(comp.object_transforms):
         ot[i, :3] = c
@@ -979,7 +1021,7 @@ def compile_scene(state: E.EvalState) -> CompiledScene:
 
     leaves = int(np.sum(node_arr["kind"] <= abi.NODE["CUSTOM"]))
     return CompiledScene(node_arr, mats, ot, light_arr, fn_arr, expr_arr, env_fn, cam, root, leaves,
-                         dynamic_hooks=list(comp.dynamic_hooks))
+                         dynamic_hooks=list(comp.dynamic_hooks), material_hooks=hooks_arr)
 
 
 # ---------------------------------------------------------- render settings

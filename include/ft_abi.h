@@ -41,7 +41,7 @@ typedef __hip_internal::uint8_t uint8_t;
 extern "C" {
 #endif
 
-#define FT_ABI_VERSION 1u
+#define FT_ABI_VERSION 2u
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -237,10 +237,31 @@ typedef struct ft_material {
   float pattern_color_a[3];
   int32_t dynamic_material_id;   /* index into the sorted material-definition names when the
                                     definition has per-hit hook functions, else -1; Whitted
-                                    renders of such scenes fail with FT_ERR_UNSUPPORTED_OBJECT */
+                                    renders fail with FT_ERR_UNSUPPORTED_OBJECT unless the
+                                    hooks are listed in ft_scene_ir.material_hooks */
   float pattern_color_b[3];
   int32_t dynamic_override_id;   /* interned override object of a dynamic material, or -1 */
 } ft_material;
+
+/* Per-hit material hooks (resolve_leaf_material renderer.rs:7164-7536) of one material
+ * whose definition model is not Standard / Layered: Forge functions of the shading context,
+ * -1 where the definition has none (or, for a number hook, returns a vector).  The Metal arm
+ * reads color, roughness and the emission hooks; the Dielectric and Standard arms also ior
+ * and thin_walled.  Every hook function takes FT_HOOK_PARAMS parameters, the shading
+ * context (make_shading_context renderer.rs:7875-7911) in the order position,
+ * local_position, normal, view_dir, front_face, object_id, material_id, then wo, wi,
+ * current_ior, u1, u2, u3 (zero for these hooks); scalars are splatted. */
+#define FT_HOOK_PARAMS 13u
+typedef struct ft_material_hooks {
+  uint32_t material;          /* index into ft_scene_ir.materials */
+  int32_t color;
+  int32_t roughness;
+  int32_t ior;
+  int32_t thin_walled;
+  int32_t emission_color;
+  int32_t emission_strength;
+  int32_t pad;
+} ft_material_hooks;
 
 /* -------------------------------------------------------------- lights */
 /* PointLight / EnvLight / SphereLight (render_api.rs:171-289), radiance already
@@ -289,6 +310,9 @@ typedef struct ft_scene_ir {
   const ft_function* functions;
   const ft_expr* exprs;
   ft_camera camera;
+  uint32_t n_material_hooks;
+  uint32_t pad2;
+  const ft_material_hooks* material_hooks;  /* may be null when n_material_hooks is 0 */
 } ft_scene_ir;
 
 /* --------------------------------------------------------- render args */

@@ -432,6 +432,8 @@ template <class T> struct Scene {
   std::vector<Xf<T>> xf;        // per node transform
   std::vector<Xf<T>> objects;   // object_transforms
   std::vector<Mat<T>> materials;
+  std::vector<int> mat_hook;                  // per material: index into hooks, or -1
+  std::vector<ft_material_hooks> hooks;
   std::vector<Light<T>> lights;
   FnEval<T> fn;
   uint32_t root;
@@ -956,6 +958,13 @@ bool validate(const ft_scene_ir* ir) {
   if (!ir) { g_err = "null scene"; return false; }
   if (ir->abi_version != FT_ABI_VERSION) { g_err = "abi version mismatch"; return false; }
   if (ir->n_nodes == 0 || ir->root >= ir->n_nodes) { g_err = "no suitable scene root"; return false; }
+  if (ir->n_material_hooks > 0 && !ir->material_hooks) { g_err = "material hook table is null"; return false; }
+  for (uint32_t h = 0; h < ir->n_material_hooks; ++h) {
+    const ft_material_hooks& H = ir->material_hooks[h];
+    if (H.material >= ir->n_materials) { g_err = "material hook names no material"; return false; }
+    for (int32_t f : {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength})
+      if (f >= (int32_t)ir->n_functions) { g_err = "material hook names no function"; return false; }
+  }
   for (uint32_t i = 0; i < ir->n_nodes; ++i) {
     const ft_node& n = ir->nodes[i];
     if (n.kind >= FT_NODE_KIND_COUNT) { g_err = "bad node kind"; return false; }
@@ -1003,6 +1012,12 @@ template <class T> void load_scene(const ft_scene_ir* ir, Scene<T>& S) {
     mm.model = (int)ir->materials[m].model;
     mm.p = params_from_ir<T>(ir->materials[m]);
     S.materials.push_back(mm);
+    S.mat_hook.push_back(-1);
+  }
+  for (uint32_t h = 0; h < ir->n_material_hooks; ++h) {
+    const ft_material_hooks& H = ir->material_hooks[h];
+    S.mat_hook[H.material] = (int)S.hooks.size();
+    S.hooks.push_back(H);
   }
   for (uint32_t l = 0; l < ir->n_lights; ++l) {
     const ft_light& L = ir->lights[l];
@@ -1061,6 +1076,7 @@ template <class T> struct Hit {
   V3<T> position, normal;
   bool front_face;
   uint32_t object_id, material_id;
+  V3<T> view_dir;  // normalize(-dir) of the ray (the shading context of material hooks)
 };
 
 // ray_aabb_intersection (R:4141-4173)
@@ -1367,12 +1383,45 @@ template <class T> Spec<T> pattern_color(const Params<T>& p, V3<T> lp) {
   return ((sx + sy + sz) & 1) == 0 ? p.pattern_a : p.pattern_b;
 }
 
-// resolve_leaf_material (R:7164-7536), static part (no Forge material hooks)
+// resolve_leaf_material (R:7164-7536): the static material, or for a definition with per-hit
+// hooks the arms for models other than Standard / Layered (R:7472-7530): each hook is a
+// function of the shading context (make_shading_context R:7875-7911) replacing the static
+// value before the clamps.
 template <class T> Mat<T> resolve_leaf_material(const Ctx<T>& c, Mat<T> m, const Hit<T>& hit) {
   if (m.blend) return m;
-  Xf<T> xf = hit.object_id < c.S->objects.size() ? c.S->objects[hit.object_id] : Xf<T>{};
+  const Scene<T>& S = *c.S;
+  Xf<T> xf = hit.object_id < S.objects.size() ? S.objects[hit.object_id] : Xf<T>{};
   V3<T> lp = to_local(hit.position, xf);
-  m.p.color = pattern_color(m.p, lp);
+  const uint32_t mi = hit.material_id < S.materials.size() ? hit.material_id : 0u;
+  const int hk = S.mat_hook.empty() ? -1 : S.mat_hook[mi];
+  if (hk >= 0) {
+    const ft_material_hooks& H = S.hooks[hk];
+    V3<T> args[FT_HOOK_PARAMS];
+    args[0] = hit.position;
+    args[1] = lp;
+    args[2] = hit.front_face ? hit.normal.normalize() : (hit.normal * T(-1)).normalize();
+    args[3] = hit.view_dir;
+    args[4] = V3<T>::splat(hit.front_face ? T(1) : T(0));
+    args[5] = V3<T>::splat((T)hit.object_id);
+    args[6] = V3<T>::splat((T)hit.material_id);
+    for (uint32_t k = 7; k < FT_HOOK_PARAMS; ++k) args[k] = V3<T>();
+    if (H.color >= 0) {
+      V3<T> v = S.fn(H.color, args);
+      m.p.color = Spec<T>(v.x, v.y, v.z);
+    } else {
+      m.p.color = pattern_color(m.p, lp);
+    }
+    if (H.roughness >= 0) m.p.roughness = S.fn(H.roughness, args).x;
+    if (H.ior >= 0) m.p.ior = S.fn(H.ior, args).x;
+    if (H.thin_walled >= 0) m.p.thin_walled = S.fn(H.thin_walled, args).x >= T(0.5);
+    if (H.emission_color >= 0) {
+      V3<T> v = S.fn(H.emission_color, args);
+      m.p.emission_color = Spec<T>(v.x, v.y, v.z);
+    }
+    if (H.emission_strength >= 0) m.p.emission_strength = S.fn(H.emission_strength, args).x;
+  } else {
+    m.p.color = pattern_color(m.p, lp);
+  }
   if (m.model != TAG_LAMBERT) m.p.roughness = rclamp(m.p.roughness, T(0), T(1));
   if (m.model == TAG_STANDARD || m.model == TAG_DIELECTRIC) m.p.ior = rclamp(m.p.ior, T(1), T(3));
   m.p.emission_strength = rmax(m.p.emission_strength, T(0));
@@ -1775,6 +1824,7 @@ template <class T> Spec<T> trace_ray(const Ctx<T>& c, V3<T> o, V3<T> d, const Me
   Hit<T> hit;
   if (!raymarch_hit(*c.A, o, d, c.op, min_t, c.op.max_dist, hit))
     return attenuate(environment(c, d), med, c.op.max_dist);
+  hit.view_dir = (d * T(-1)).normalize();
   Mat<T> a, b;
   T t;
   if (resolve_split(c, c.S->root, hit, a, b, t)) {
@@ -1790,6 +1840,7 @@ template <class T> Spec<T> trace_ray(const Ctx<T>& c, V3<T> o, V3<T> d, const Me
 template <class T> Spec<T> trace_aov(const Ctx<T>& c, V3<T> o, V3<T> d, int aov) {
   Hit<T> hit;
   if (!raymarch_hit(*c.A, o, d, c.op, T(0), c.op.max_dist, hit)) return Spec<T>();
+  hit.view_dir = (d * T(-1)).normalize();
   Mat<T> mat = resolve_material_from_node(c, c.S->root, hit);
   V3<T> normal = hit.front_face ? hit.normal.normalize() : (hit.normal * T(-1)).normalize();
   int model = dominant_model(mat);

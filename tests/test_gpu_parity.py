@@ -217,16 +217,68 @@ def test_error_paths():
         GpuScene(bad)
 
 
-def test_per_hit_material_hooks_fail_loudly():
-    """Whitted renders of scenes whose materials evaluate per-hit hooks are refused (the
-    backend does not run them); the depth preview of the same scene is unaffected."""
-    from forgedthoughts_amd.scene import RenderError
-    name, _, opts, _ = REGRESSION_CASES[3]
+HOOKS_SCENE = """
+let camera = Camera { origin: vec3(0.0, 1.2, 6.0), target: vec3(0.0, 0.3, 0.0), fov_y: 38.0 };
+let key = PointLight { position: vec3(3.0, 5.0, 4.0), color: #fff2d8, intensity: 90.0 };
+material Stripes {
+  model: Lambert;
+  let base = #eceef4;
+  let accent = #c25f56;
+  fn color(ctx) { return mix(base, accent, step(0.0, sin(ctx.local_position.x * 9.0))); }
+  fn emission_strength(ctx) { return clamp(ctx.position.y - 0.9, 0.0, 1.0) * 0.6; }
+  fn emission_color(ctx) { return ctx.normal * 0.5 + vec3(0.5, 0.5, 0.5); }
+};
+material Brushed {
+  model: Metal;
+  color = #c8ccd4;
+  fn roughness(ctx) { return 0.05 + 0.3 * step(0.0, sin(ctx.local_position.y * 12.0)); }
+};
+material Glass {
+  model: Dielectric;
+  fn ior(ctx) { return 1.2 + 0.4 * clamp(ctx.view_dir.y + 0.5, 0.0, 1.0); }
+  fn thin_walled(ctx) { return step(0.3, ctx.local_position.x); }
+};
+let floor_mat = Lambert { color: #d7dce5 };
+var floor = Box { size: vec3(10.0, 0.5, 10.0), material: floor_mat };
+floor.pos.y = -1.15;
+var a = Sphere { radius: 0.8, material: Stripes {} };
+a.pos.x = -1.7;
+var b = Sphere { radius: 0.8, material: Brushed {} };
+var c = Sphere { radius: 0.8, material: Glass {} };
+c.pos.x = 1.7;
+let scene = floor + a + b.union_round(c, 0.3);
+"""
+
+
+@pytest.mark.parametrize("emit", ["0", "1"], ids=["interp", "emitted"])
+def test_material_hooks(emit, monkeypatch):
+    """Per-hit hooks of Lambert-, Metal- and Dielectric-model definitions (colour, roughness,
+    ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530), evaluated on the
+    GPU with the shading context of each hit, against the oracle; and the reference's own
+    hooked regression scene in Whitted."""
+    monkeypatch.setenv("FT_EMIT", emit)
+    scene = compile_src(HOOKS_SCENE)
+    assert len(scene.material_hooks) == 3
+    opts = RenderOptions(width=160, height=96, max_steps=256, max_dist=40.0, epsilon=1.0e-3)
+    whitted_parity(scene, AccelMode.Bvh, opts, RaySettings(4, 64, 2), "material hooks", None)
+    name, _, ropts, _ = REGRESSION_CASES[3]
     scene = compile_ft(os.path.join(REGRESSION, name + ".ft"))
+    whitted_parity(scene, AccelMode.Bvh, ropts, RaySettings(3, 96, 1), name + " (Whitted)", None)
+
+
+def test_unsupported_material_hooks_fail_loudly():
+    """Hooks outside the simple arms (bump here; also normal, eval / pdf / sample, medium,
+    subsurface, and any hook of a Standard / Layered definition) are refused for Whitted;
+    the depth preview of the same scene is unaffected."""
+    from forgedthoughts_amd.scene import RenderError
+    scene = compile_src("material Bumpy { model: Lambert; fn bump(ctx) { return 0.0; } };\n"
+                        "let scene = Sphere { radius: 1.0, material: Bumpy {} };")
+    assert len(scene.material_hooks) == 0
+    opts = RenderOptions(width=32, height=32)
     with GpuScene(scene, AccelMode.Bvh) as gs:
         gs.render_depth(opts, 1)
         with pytest.raises(RenderError) as e:
-            gs.render_whitted(opts, RaySettings(2, 96, 1))
+            gs.render_whitted(opts, RaySettings(2, 32, 1))
         assert e.value.kind == "UnsupportedObjectType"
 
 

@@ -292,3 +292,69 @@ def test_diff_forward_f32_against_f64_and_smoothness():
         g4 = (loss(p + 1e-4 * e) - loss(p - 1e-4 * e)) / 2e-4
         g5 = (loss(p + 1e-5 * e) - loss(p - 1e-5 * e)) / 2e-5
         assert abs(g4 - g5) <= 1e-4 * max(abs(g4), 1.0), (j, g4, g5)
+
+
+# -------------------------------------------------------------- material hooks
+# Per-hit hooks of a material definition (resolve_leaf_material renderer.rs:7164-7536, the
+# arms for models other than Standard / Layered) evaluated with the shading context of
+# make_shading_context (renderer.rs:7875-7911).  A black, unlit surface whose emission
+# colour is a hook of the context radiates exactly that colour, so the HDR output of a
+# one-bounce Whitted render reads the context fields back.
+_DARK = "let key = PointLight { position: vec3(0.0, 0.0, 3.0), color: #ffffff, intensity: 0.0 };\n"
+_OPTS9 = RenderOptions(9, 9, 256, 40.0, 1.0e-4, 0.7, 6.0, 20.0)
+
+
+def _emitter(expr, shape="Sphere { radius: 1.0, material: E {} }", place=""):
+    return compile_src(_axis_camera(
+        _DARK + "material E { model: Lambert; color = vec3(0.0, 0.0, 0.0); emission_strength = 1.0;\n"
+        f"  fn emission_color(ctx) {{ return {expr}; }} }};\n" f"var s = {shape}; {place}\nlet scene = s;"))
+
+
+def test_hook_context_normal_matches_normal_aov():
+    scene = _emitter("ctx.normal * 0.5 + vec3(0.5, 0.5, 0.5)")
+    _, hdr = ftref.render_whitted(scene, AccelMode.Bvh, _OPTS9, RaySettings(1, 9, 1))
+    _, aov = ftref.render_whitted(scene, AccelMode.Bvh, _OPTS9, RaySettings(1, 9, 1, "normal"))
+    assert np.abs(hdr - aov).max() < 1e-6 and aov.std() > 0.01
+
+
+def test_hook_context_points_and_view_direction():
+    """Centre pixel of the axis camera hits a sphere of radius r at z_c on (0, 0, z_c + r)."""
+    r, zc = 0.75, 0.5
+    place = f"s.pos.z = {zc};"
+    shape = f"Sphere {{ radius: {r}, material: E {{}} }}"
+    for expr, want in (("ctx.position * 0.1 + vec3(0.5, 0.5, 0.5)", (0.5, 0.5, 0.5 + 0.1 * (zc + r))),
+                       ("ctx.local_position * 0.1 + vec3(0.5, 0.5, 0.5)", (0.5, 0.5, 0.5 + 0.1 * r)),
+                       ("ctx.view_dir * 0.25 + vec3(0.5, 0.5, 0.5)", (0.5, 0.5, 0.75))):
+        scene = _emitter(expr, shape, place)
+        _, hdr = ftref.render_whitted(scene, AccelMode.Bvh, _OPTS9, RaySettings(1, 9, 1))
+        assert np.abs(hdr[4, 4] - np.array(want)).max() < 2e-4, expr
+
+
+def test_hook_context_ids():
+    scene = _emitter("vec3(ctx.front_face, ctx.object_id + 0.5, ctx.material_id + 0.25) * 0.5")
+    leaf = int(np.flatnonzero(scene.nodes["kind"] == 0)[0])
+    obj, mat = int(scene.nodes["object_id"][leaf]), int(scene.nodes["material_id"][leaf])
+    _, hdr = ftref.render_whitted(scene, AccelMode.Bvh, _OPTS9, RaySettings(1, 9, 1))
+    assert np.abs(hdr[4, 4] - 0.5 * np.array([1.0, obj + 0.5, mat + 0.25])).max() < 1e-6
+
+
+def test_constant_hooks_render_as_static_materials():
+    """A hook returning a constant gives the image of the static material with that value
+    (colour on a lit Lambert-model definition; ior read back by the IOR and Fresnel AOVs)."""
+    lit = "let key = PointLight { position: vec3(0.0, 0.0, 3.0), color: #ffffff, intensity: 9.0 };\n"
+    wall = "Box { size: vec3(20.0, 20.0, 1.0), material: M {} }"
+    hooked = compile_src(_axis_camera(lit + "material M { model: Lambert; color = #ffffff;\n"
+                                      "  fn color(ctx) { return vec3(0.25, 0.5, 0.75); } };\n"
+                                      f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+    static = compile_src(_axis_camera(lit + "material M { model: Lambert; color = vec3(0.25, 0.5, 0.75); };\n"
+                                      f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+    a, ha = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    b, hb = ftref.render_whitted(static, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0 and ha.max() > 0.0
+    glass = compile_src(_axis_camera("material G { model: Dielectric; fn ior(ctx) { return 1.5; } };\n"
+                                     f"var w = {wall.replace('M {}', 'G {}')}; w.pos.z = -0.5; let scene = w;"))
+    assert len(glass.material_hooks) == 1
+    _, hdr = ftref.render_whitted(glass, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1, "ior"))
+    assert abs(hdr[4, 4, 0] - 0.25) < 1e-6
+    _, hdr = ftref.render_whitted(glass, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1, "fresnel"))
+    assert abs(hdr[4, 4, 0] - 0.04) < 1e-5

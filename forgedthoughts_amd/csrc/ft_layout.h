@@ -76,9 +76,10 @@ static_assert(sizeof(DParams) == 144, "DParams layout");
 // Per-hit hook functions of one material (ft_material_hooks without the material index).
 struct DHooks {
   int color, roughness, ior, thin_walled, emission_color, emission_strength;
+  int metallic, specular, specular_weight, transmission, specular_color, clearcoat_roughness, clearcoat, coat_mask;
   int pad[2];
 };
-static_assert(sizeof(DHooks) == 32, "DHooks layout");
+static_assert(sizeof(DHooks) == 64, "DHooks layout");
 
 struct DLight {
   int kind;

@@ -96,9 +96,12 @@ struct HookCtx {
 };
 
 // resolve_leaf_material (renderer.rs:7164-7536): the parameter set of a leaf material at
-// `pos`; returns its model.  With a shading context, the material's per-hit hooks (the arms
-// for definition models other than Standard / Layered, renderer.rs:7472-7530) replace the
-// static colour, roughness, ior, thin_walled and emission before the clamps.
+// `pos`; returns its model.  With a shading context, the material's per-hit hooks replace
+// the static colour, roughness, ior, thin_walled and emission before the clamps (the arms
+// of renderer.rs:7472-7530); a Standard / Layered definition also has its metallic,
+// specular, specular weight, transmission and coat hooks (renderer.rs:7286-7470), the coat
+// hooks listed under the parameter each replaces, the clearcoat being
+// clamp(weight * coat_mask).
 __device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float3 pos, DParams& a,
                                   const HookCtx* hc = nullptr) {
   const DevScene& S = scene_of(slot);
@@ -133,6 +136,16 @@ __device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float
       if (H.thin_walled >= 0) a.thin_walled = eval_fn(slot, H.thin_walled, args).x >= 0.5f ? 1.0f : 0.0f;
       if (H.emission_color >= 0) set3(a.emission_color, eval_fn(slot, H.emission_color, args));
       if (H.emission_strength >= 0) a.emission_strength = eval_fn(slot, H.emission_strength, args).x;
+      if (H.metallic >= 0) a.metallic = clampf(eval_fn(slot, H.metallic, args).x, 0.0f, 1.0f);
+      if (H.specular >= 0) a.specular = clampf(eval_fn(slot, H.specular, args).x, 0.0f, 1.0f);
+      if (H.specular_weight >= 0) a.specular_weight = clampf(eval_fn(slot, H.specular_weight, args).x, 0.0f, 1.0f);
+      if (H.transmission >= 0) a.transmission = clampf(eval_fn(slot, H.transmission, args).x, 0.0f, 1.0f);
+      if (H.specular_color >= 0) set3(a.specular_color, eval_fn(slot, H.specular_color, args));
+      if (H.clearcoat_roughness >= 0)
+        a.clearcoat_roughness = clampf(eval_fn(slot, H.clearcoat_roughness, args).x, 0.0f, 1.0f);
+      if (H.clearcoat >= 0) a.clearcoat = clampf(eval_fn(slot, H.clearcoat, args).x, 0.0f, 1.0f);
+      if (H.coat_mask >= 0)
+        a.clearcoat = clampf(a.clearcoat * clampf(eval_fn(slot, H.coat_mask, args).x, 0.0f, 1.0f), 0.0f, 1.0f);
     } else {  // resolve_pattern_color (renderer.rs:7855-7873)
       const int sx = (int)floorf(lp.x * a.pattern_scale), sy = (int)floorf(lp.y * a.pattern_scale),
                 sz = (int)floorf(lp.z * a.pattern_scale);

@@ -35,13 +35,37 @@ from .forge.evaluator import Obj, Func, is_num, f32
 
 # material-definition functions the reference evaluates per hit (resolve_leaf_material
 # renderer.rs:7164-7536, eval_bsdf hooks renderer.rs:4952-5030)
-# hooks evaluated on the GPU, in ft_material_hooks order: the arms of resolve_leaf_material
-# for definition models other than Standard / Layered (renderer.rs:7472-7530)
-HOOK_NAMES = ("color", "roughness", "ior", "thin_walled", "emission_color", "emission_strength")
-NUMBER_HOOKS = ("roughness", "ior", "thin_walled", "emission_strength")
-# hooks that act outside resolve_leaf_material's simple arms (BSDF overrides, bump / normal,
-# media): scenes with them are not rendered with Whitted on the GPU
-UNSUPPORTED_HOOKS = ("bump", "normal", "eval", "pdf", "sample", "medium", "subsurface")
+# ft_material_hooks entries, in order: the parameter each hook replaces in
+# resolve_leaf_material (renderer.rs:7164-7536)
+HOOK_NAMES = ("color", "roughness", "ior", "thin_walled", "emission_color", "emission_strength",
+              "metallic", "specular", "specular_weight", "transmission", "specular_color",
+              "clearcoat_roughness", "clearcoat", "coat_mask")
+SPECTRUM_HOOKS = ("color", "emission_color", "specular_color")
+# the hooks each definition model reads: the Metal arm and the Dielectric / other arms
+# (renderer.rs:7472-7530), and the Standard / Layered block (7259-7470), whose coat hooks
+# replace the ior, specular colour, clearcoat roughness and clearcoat weight
+_ARM_HOOKS = {
+    "Metal": {"color": "color", "roughness": "roughness", "emission_color": "emission_color",
+              "emission_strength": "emission_strength"},
+    "other": {h: h for h in HOOK_NAMES[:6]},
+    "Standard": dict({h: h for h in HOOK_NAMES}, ior="coat_ior", specular_color="coat_color",
+                     clearcoat_roughness="coat_roughness", clearcoat="coat_weight"),
+}
+
+
+def arm_hooks(model: str) -> dict:
+    """ft_material_hooks entry -> the definition's hook function read for it, per model."""
+    if model in ("Standard", "Layered"):
+        return _ARM_HOOKS["Standard"]
+    return _ARM_HOOKS["Metal" if model == "Metal" else "other"]
+
+
+# hooks that no part of the Whitted render reads: bump / normal feed build_bsdf_context
+# (renderer.rs:4397-4449), eval / pdf / sample the BSDF overrides (4905-5004), medium /
+# subsurface the medium transitions (5125-5161); none of these has a caller outside the
+# reference's unit tests, so they leave the image unchanged, as do hooks a model's arm
+# does not read (a Metal's ior) and helper functions of the shading context: a material
+# with only such hooks is compiled as a static one
 MATERIAL_HOOKS = ("color", "roughness", "metallic", "specular", "ior", "thin_walled", "medium", "subsurface",
                   "emission", "emission_color", "emission_strength", "bump", "normal", "eval", "pdf", "sample",
                   "transmission", "clearcoat", "clearcoat_roughness", "specular_color")
@@ -398,16 +422,30 @@ class SceneCompiler:
                 m.emission_strength = max(value, f32(0.0))
             elif pname == "medium":
                 m.medium = _medium_from_value(value)
+        if model in ("Standard", "Layered"):
+            # resolve_leaf_material's block for Standard / Layered definitions
+            # (renderer.rs:7259-7470) runs at every hit of such a material: the coat properties
+            # replace the specular colour, the ior and the clearcoat terms
+            def num(key, default):
+                v = props.get(key)
+                return v if is_num(v) else f32(default)
+            cc = spectrum_from_value(props["coat_color"]) if "coat_color" in props else None
+            m.specular_color = cc if cc is not None else _v(1, 1, 1)
+            m.clearcoat_roughness = _clamp(num("coat_roughness", 0.08), 0, 1)
+            m.ior = _clamp(num("coat_ior", 1.33), 1, 3)
+            m.clearcoat = _clamp(f32(_clamp(num("coat_weight", 0.0), 0, 1) * f32(1.0)), 0, 1)
         # per-hit hooks (functions of the shading context, resolve_leaf_material
-        # renderer.rs:7164-7536); a material without them is fully static
+        # renderer.rs:7164-7536); a material without hooks its model's arm reads renders
+        # exactly as its static parameters
         hooks = [st[1] for st in d.statements if st[0] == "fn" and
                  (st[1] in MATERIAL_HOOKS or (len(st[2]) >= 1 and st[2][0] == "ctx"))]
+        read = [h for h in hooks if h in arm_hooks(model).values()]
         if hooks:
-            m.dynamic_material_id = self.sorted_material_names.index(name)
             self.dynamic_hooks.extend(f"{name}.{h}" for h in hooks if f"{name}.{h}" not in self.dynamic_hooks)
-            if model not in ("Standard", "Layered") and not any(h in UNSUPPORTED_HOOKS for h in hooks):
-                m.hooks = self.lower_hooks(d, name, model, overrides)
-        if hooks and with_ctx and overrides.fields:
+        if read:
+            m.dynamic_material_id = self.sorted_material_names.index(name)
+            m.hooks = self.lower_hooks(d, name, model, overrides)
+        if read and with_ctx and overrides.fields:
             key = repr(sorted((k, repr(v)) for k, v in overrides.fields.items()))
             if key not in self.dynamic_overrides:
                 self.dynamic_overrides.append(key)
@@ -427,12 +465,12 @@ class SceneCompiler:
         key = (name, repr(sorted((k, repr(v)) for k, v in overrides.fields.items())))
         if key in self._hook_cache:
             return self._hook_cache[key]
-        read = set(HOOK_NAMES) if model != "Metal" else {"color", "roughness", "emission_color", "emission_strength"}
+        arm = arm_hooks(model)
         ids = []
         try:
             for h in HOOK_NAMES:
-                fid = self.lower_block_fn(d, h, ["ctx"], overrides) if h in read else -1
-                if fid >= 0 and h in NUMBER_HOOKS and self.functions[fid][1] != 0:
+                fid = self.lower_block_fn(d, arm[h], ["ctx"], overrides) if h in arm else -1
+                if fid >= 0 and h not in SPECTRUM_HOOKS and self.functions[fid][1] != 0:
                     fid = -1
                 ids.append(fid)
             out = tuple(ids)

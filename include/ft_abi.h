@@ -41,7 +41,7 @@ typedef __hip_internal::uint8_t uint8_t;
 extern "C" {
 #endif
 
-#define FT_ABI_VERSION 2u
+#define FT_ABI_VERSION 3u
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -243,11 +243,15 @@ typedef struct ft_material {
   int32_t dynamic_override_id;   /* interned override object of a dynamic material, or -1 */
 } ft_material;
 
-/* Per-hit material hooks (resolve_leaf_material renderer.rs:7164-7536) of one material
- * whose definition model is not Standard / Layered: Forge functions of the shading context,
- * -1 where the definition has none (or, for a number hook, returns a vector).  The Metal arm
- * reads color, roughness and the emission hooks; the Dielectric and Standard arms also ior
- * and thin_walled.  Every hook function takes FT_HOOK_PARAMS parameters, the shading
+/* Per-hit material hooks (resolve_leaf_material renderer.rs:7164-7536) of one material from
+ * a definition: Forge functions of the shading context, -1 where the definition has none
+ * (or, for a number hook, returns a vector).  The Metal arm reads color, roughness and the
+ * emission hooks; the Dielectric and Lambert arms also ior and thin_walled.  Standard /
+ * Layered definitions (renderer.rs:7259-7470) read every entry: their coat block replaces
+ * the ior, specular colour, clearcoat and clearcoat roughness, so for them `ior` holds the
+ * coat_ior hook, `specular_color` coat_color, `clearcoat_roughness` coat_roughness and
+ * `clearcoat` coat_weight, the clearcoat being clamp(weight * coat_mask) with the static
+ * weight where `clearcoat` is -1.  Every hook function takes FT_HOOK_PARAMS parameters, the shading
  * context (make_shading_context renderer.rs:7875-7911) in the order position,
  * local_position, normal, view_dir, front_face, object_id, material_id, then wo, wi,
  * current_ior, u1, u2, u3 (zero for these hooks); scalars are splatted. */
@@ -260,6 +264,14 @@ typedef struct ft_material_hooks {
   int32_t thin_walled;
   int32_t emission_color;
   int32_t emission_strength;
+  int32_t metallic;           /* Standard / Layered definitions only, from here on */
+  int32_t specular;
+  int32_t specular_weight;
+  int32_t transmission;
+  int32_t specular_color;
+  int32_t clearcoat_roughness;
+  int32_t clearcoat;
+  int32_t coat_mask;
   int32_t pad;
 } ft_material_hooks;
 

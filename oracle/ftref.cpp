@@ -962,7 +962,9 @@ bool validate(const ft_scene_ir* ir) {
   for (uint32_t h = 0; h < ir->n_material_hooks; ++h) {
     const ft_material_hooks& H = ir->material_hooks[h];
     if (H.material >= ir->n_materials) { g_err = "material hook names no material"; return false; }
-    for (int32_t f : {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength})
+    for (int32_t f : {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
+                      H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
+                      H.clearcoat_roughness, H.clearcoat, H.coat_mask})
       if (f >= (int32_t)ir->n_functions) { g_err = "material hook names no function"; return false; }
   }
   for (uint32_t i = 0; i < ir->n_nodes; ++i) {
@@ -1384,9 +1386,11 @@ template <class T> Spec<T> pattern_color(const Params<T>& p, V3<T> lp) {
 }
 
 // resolve_leaf_material (R:7164-7536): the static material, or for a definition with per-hit
-// hooks the arms for models other than Standard / Layered (R:7472-7530): each hook is a
-// function of the shading context (make_shading_context R:7875-7911) replacing the static
-// value before the clamps.
+// hooks the arm of its model: each hook is a function of the shading context
+// (make_shading_context R:7875-7911) replacing the static value before the clamps.  The
+// Standard / Layered block (R:7259-7470) also reads metallic, specular, specular_weight and
+// transmission, then its coat hooks (listed under the parameter each replaces); its
+// clearcoat is clamp(coat_weight * coat_mask), the weight static where it has no hook.
 template <class T> Mat<T> resolve_leaf_material(const Ctx<T>& c, Mat<T> m, const Hit<T>& hit) {
   if (m.blend) return m;
   const Scene<T>& S = *c.S;
@@ -1419,6 +1423,17 @@ template <class T> Mat<T> resolve_leaf_material(const Ctx<T>& c, Mat<T> m, const
       m.p.emission_color = Spec<T>(v.x, v.y, v.z);
     }
     if (H.emission_strength >= 0) m.p.emission_strength = S.fn(H.emission_strength, args).x;
+    if (H.metallic >= 0) m.p.metallic = rclamp(S.fn(H.metallic, args).x, T(0), T(1));
+    if (H.specular >= 0) m.p.specular = rclamp(S.fn(H.specular, args).x, T(0), T(1));
+    if (H.specular_weight >= 0) m.p.specular_weight = rclamp(S.fn(H.specular_weight, args).x, T(0), T(1));
+    if (H.transmission >= 0) m.p.transmission = rclamp(S.fn(H.transmission, args).x, T(0), T(1));
+    if (H.specular_color >= 0) {
+      V3<T> v = S.fn(H.specular_color, args);
+      m.p.specular_color = Spec<T>(v.x, v.y, v.z);
+    }
+    if (H.clearcoat_roughness >= 0) m.p.clearcoat_roughness = rclamp(S.fn(H.clearcoat_roughness, args).x, T(0), T(1));
+    if (H.clearcoat >= 0) m.p.clearcoat = rclamp(S.fn(H.clearcoat, args).x, T(0), T(1));
+    if (H.coat_mask >= 0) m.p.clearcoat = rclamp(m.p.clearcoat * rclamp(S.fn(H.coat_mask, args).x, T(0), T(1)), T(0), T(1));
   } else {
     m.p.color = pattern_color(m.p, lp);
   }

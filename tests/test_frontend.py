@@ -105,3 +105,20 @@ def test_per_hit_material_hooks_are_recorded():
     s = compile_src("material Plain { model: Lambert; let c = #ff0000; color = c; };\n"
                     "let scene = Sphere { radius: 1.0, material: Plain {} };")
     assert s.dynamic_hooks == [] and (s.materials["dynamic_material_id"] == -1).all()
+
+
+def test_standard_definition_coat_properties():
+    """Standard / Layered definitions take their specular colour, ior and clearcoat from the
+    coat properties at every hit (renderer.rs:7386-7470): defaults white, 1.33, 0 x 1, 0.08."""
+    s = compile_src("material P { model: Standard; color = #ff0000; ior = 1.8; specular_color = #00ff00; };\n"
+                    "let scene = Sphere { radius: 1.0, material: P {} };")
+    m = s.materials[int(s.nodes["material_id"][0])]
+    assert m["ior"] == np.float32(1.33) and tuple(m["specular_color"]) == (1.0, 1.0, 1.0)
+    assert m["clearcoat"] == 0.0 and m["clearcoat_roughness"] == np.float32(0.08)
+    assert tuple(m["color"]) == (1.0, 0.0, 0.0)
+    s = compile_src("material C { model: Layered; coat_weight = 0.5; coat_ior = 1.6; coat_roughness = 0.2;\n"
+                    "  coat_color = vec3(0.5, 0.6, 0.7); };\nlet scene = Sphere { radius: 1.0, material: C {} };")
+    m = s.materials[int(s.nodes["material_id"][0])]
+    assert m["ior"] == np.float32(1.6) and m["clearcoat"] == np.float32(0.5)
+    assert m["clearcoat_roughness"] == np.float32(0.2)
+    assert np.allclose(m["specular_color"], (0.5, 0.6, 0.7))

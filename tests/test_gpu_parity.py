@@ -246,19 +246,34 @@ a.pos.x = -1.7;
 var b = Sphere { radius: 0.8, material: Brushed {} };
 var c = Sphere { radius: 0.8, material: Glass {} };
 c.pos.x = 1.7;
-let scene = floor + a + b.union_round(c, 0.3);
+material Lacquer {
+  model: Standard;
+  color = #3a6ea5;
+  coat_weight = 0.8;
+  fn metallic(ctx) { return step(0.0, sin(ctx.local_position.y * 10.0)); }
+  fn specular_weight(ctx) { return 0.5 + 0.5 * ctx.normal.y; }
+  fn coat_color(ctx) { return mix(#ffffff, #ffd08a, clamp(ctx.local_position.x + 0.5, 0.0, 1.0)); }
+  fn coat_roughness(ctx) { return 0.04 + 0.2 * step(0.0, ctx.local_position.z); }
+  fn coat_mask(ctx) { return clamp(ctx.position.y + 0.4, 0.0, 1.0); }
+  fn bump(ctx) { return sin(ctx.local_position.x * 30.0) * 0.01; }
+};
+var d = Sphere { radius: 0.5, material: Lacquer {} };
+d.pos.z = 1.6;
+d.pos.y = -0.3;
+let scene = floor + a + b.union_round(c, 0.3) + d;
 """
 
 
 @pytest.mark.parametrize("emit", ["0", "1"], ids=["interp", "emitted"])
 def test_material_hooks(emit, monkeypatch):
     """Per-hit hooks of Lambert-, Metal- and Dielectric-model definitions (colour, roughness,
-    ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530), evaluated on the
-    GPU with the shading context of each hit, against the oracle; and the reference's own
-    hooked regression scene in Whitted."""
+    ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530) and of a
+    Standard-model one (metallic, specular weight and the coat hooks, renderer.rs:7259-7470),
+    evaluated on the GPU with the shading context of each hit, against the oracle; and the
+    reference's own hooked regression scene in Whitted."""
     monkeypatch.setenv("FT_EMIT", emit)
     scene = compile_src(HOOKS_SCENE)
-    assert len(scene.material_hooks) == 3
+    assert len(scene.material_hooks) == 4
     opts = RenderOptions(width=160, height=96, max_steps=256, max_dist=40.0, epsilon=1.0e-3)
     whitted_parity(scene, AccelMode.Bvh, opts, RaySettings(4, 64, 2), "material hooks", None)
     name, _, ropts, _ = REGRESSION_CASES[3]
@@ -267,12 +282,11 @@ def test_material_hooks(emit, monkeypatch):
 
 
 def test_unsupported_material_hooks_fail_loudly():
-    """Hooks outside the simple arms (bump here; also normal, eval / pdf / sample, medium,
-    subsurface, and any hook of a Standard / Layered definition) are refused for Whitted;
-    the depth preview of the same scene is unaffected."""
+    """A hook that the lowering to device functions cannot express (a string-valued colour
+    here) is refused for Whitted; the depth preview of the same scene is unaffected."""
     from forgedthoughts_amd.scene import RenderError
-    scene = compile_src("material Bumpy { model: Lambert; fn bump(ctx) { return 0.0; } };\n"
-                        "let scene = Sphere { radius: 1.0, material: Bumpy {} };")
+    scene = compile_src('material Odd { model: Lambert; fn color(ctx) { return "red"; } };\n'
+                        "let scene = Sphere { radius: 1.0, material: Odd {} };")
     assert len(scene.material_hooks) == 0
     opts = RenderOptions(width=32, height=32)
     with GpuScene(scene, AccelMode.Bvh) as gs:

@@ -338,6 +338,78 @@ def test_hook_context_ids():
     assert np.abs(hdr[4, 4] - 0.5 * np.array([1.0, obj + 0.5, mat + 0.25])).max() < 1e-6
 
 
+def test_constant_standard_hooks_render_as_static_materials():
+    """Constant hooks of a Standard-model definition give the image of the static definition
+    with those values: metallic, specular, specular weight and transmission replace their
+    parameters, the coat hooks replace the specular colour, ior and clearcoat roughness, and
+    the clearcoat is the static coat weight times the coat mask (0.4 * 0.5 = 0.2); an ior hook
+    is overridden by the coat ior (renderer.rs:7259-7470)."""
+    lit = "let key = PointLight { position: vec3(0.5, 0.7, 3.0), color: #ffffff, intensity: 9.0 };\n"
+    wall = "Box { size: vec3(20.0, 20.0, 1.0), material: M {} }"
+    hooked = compile_src(_axis_camera(
+        lit + "material M { model: Standard; color = #ffffff; coat_weight = 0.4;\n"
+        "  fn color(ctx) { return vec3(0.25, 0.5, 0.75); }\n"
+        "  fn metallic(ctx) { return 0.3; }  fn specular(ctx) { return 0.6; }\n"
+        "  fn specular_weight(ctx) { return 0.8; }  fn transmission(ctx) { return 0.2; }\n"
+        "  fn coat_color(ctx) { return vec3(0.9, 0.8, 0.7); }  fn coat_roughness(ctx) { return 0.2; }\n"
+        "  fn coat_ior(ctx) { return 1.6; }  fn ior(ctx) { return 2.5; }  fn coat_mask(ctx) { return 0.5; } };\n"
+        f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+    static = compile_src(_axis_camera(
+        lit + "material M { model: Standard; color = vec3(0.25, 0.5, 0.75); metallic = 0.3;\n"
+        "  specular = 0.6; specular_weight = 0.8; transmission = 0.2; coat_color = vec3(0.9, 0.8, 0.7);\n"
+        "  coat_roughness = 0.2; coat_ior = 1.6; coat_weight = 0.2; };\n"
+        f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+    assert len(hooked.material_hooks) == 1 and len(static.material_hooks) == 0
+    h = hooked.material_hooks[0]
+    assert h["ior"] >= 0 and h["clearcoat"] < 0 and h["coat_mask"] >= 0
+    for aov in (None, "ior", "transmission", "fresnel"):
+        a, ha = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1, aov))
+        b, hb = ftref.render_whitted(static, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1, aov))
+        assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0 and ha.max() > 0.0, aov
+    _, hdr = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1, "ior"))
+    assert abs(hdr[4, 4, 0] - 0.3) < 1e-6
+    # the image does depend on the clearcoat (an unmasked weight) and on the metallic term
+    _, ha = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    for change in (("coat_weight = 0.2", "coat_weight = 0.4"), ("metallic = 0.3", "metallic = 0.0")):
+        other = compile_src(_axis_camera(
+            lit + "material M { model: Standard; color = vec3(0.25, 0.5, 0.75); metallic = 0.3;\n"
+            "  specular = 0.6; specular_weight = 0.8; transmission = 0.2; coat_color = vec3(0.9, 0.8, 0.7);\n"
+            "  coat_roughness = 0.2; coat_ior = 1.6; coat_weight = 0.2; };\n".replace(*change) +
+            f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+        _, hc = ftref.render_whitted(other, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+        assert np.abs(ha - hc).max() > 1e-4, change
+    # a coat weight hook without a mask is the clearcoat itself
+    weight = compile_src(_axis_camera(
+        lit + "material M { model: Standard; fn coat_weight(ctx) { return 0.2; } };\n"
+        f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+    plain = compile_src(_axis_camera(
+        lit + "material M { model: Standard; coat_weight = 0.2; };\n"
+        f"var w = {wall}; w.pos.z = -0.5; let scene = w;"))
+    a, ha = ftref.render_whitted(weight, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    b, hb = ftref.render_whitted(plain, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    assert np.abs(ha - hb).max() == 0.0
+
+
+def test_hooks_outside_the_whitted_arms_leave_materials_static():
+    """bump / normal, eval / pdf / sample, medium / subsurface hooks, hooks a model's arm does
+    not read (a Metal's ior) and helper functions are never called by the Whitted render:
+    such a material compiles as its static parameters."""
+    lit = "let key = PointLight { position: vec3(0.0, 0.0, 3.0), color: #ffffff, intensity: 9.0 };\n"
+    wall = "var w = Box { size: vec3(20.0, 20.0, 1.0), material: M {} }; w.pos.z = -0.5; let scene = w;"
+    hooked = compile_src(_axis_camera(
+        lit + "material M { model: Metal; color = #c0a080; roughness = 0.3;\n"
+        "  fn helper(ctx) { return ctx.position.x; }  fn ior(ctx) { return 2.0; }\n"
+        "  fn bump(ctx) { return helper(ctx) * 0.1; }  fn normal(ctx) { return vec3(0.0, 1.0, 0.0); }\n"
+        "  fn eval(ctx) { return #ffffff; }  fn pdf(ctx) { return 1.0; }\n"
+        "  fn medium(ctx) { return 0.0; }  fn subsurface(ctx) { return 0.0; } };\n" + wall))
+    static = compile_src(_axis_camera(lit + "material M { model: Metal; color = #c0a080; roughness = 0.3; };\n" + wall))
+    assert len(hooked.material_hooks) == 0
+    assert all(int(m["dynamic_material_id"]) < 0 for m in hooked.materials)
+    a, ha = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    b, hb = ftref.render_whitted(static, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+    assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0 and ha.max() > 0.0
+
+
 def test_constant_hooks_render_as_static_materials():
     """A hook returning a constant gives the image of the static material with that value
     (colour on a lit Lambert-model definition; ior read back by the IOR and Fresnel AOVs)."""

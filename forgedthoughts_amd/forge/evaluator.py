@@ -286,6 +286,8 @@ class Evaluator:
             fields = {k: self.eval(v, sc) for k, v in e.fields}
             if e.type_name == "vec3" and sc.lowering is not None and any(isinstance(v, Sym) for v in fields.values()):
                 return Obj("vec3", fields)
+            if e.type_name in self.state.sdf_defs:
+                self.augment_sdf_instance(e.type_name, fields)
             return Obj(e.type_name, fields)
         if isinstance(e, S.Binary):
             return self.binary(self.eval(e.lhs, sc), e.op, self.eval(e.rhs, sc), sc)
@@ -302,6 +304,26 @@ class Evaluator:
             return self.call(e, sc)
         raise ForgeEvalError(f"unsupported expression {type(e).__name__}")
 
+    def augment_sdf_instance(self, type_name: str, fields: Dict[str, object]) -> None:
+        """augment_object_literal_fields (eval.rs:547-577): an instance of an sdf definition
+        carries the definition's `bounds()` (as `__bounds`) and `anchors` for layout."""
+        d = self.state.sdf_defs[type_name]
+        overrides = Obj(type_name, dict(fields))
+        if "__bounds" not in fields:
+            try:
+                v = self.eval_block_zero_arg(d, "bounds", overrides)
+            except ForgeEvalError:
+                v = None
+            if v is not None:
+                fields["__bounds"] = v
+        if "anchors" not in fields:
+            try:
+                v = self.block_binding(d, "anchors", Scope({}, d, overrides))
+            except ForgeEvalError:
+                v = None
+            if v is not None:
+                fields["anchors"] = v
+
     def lookup(self, name: str, sc: Scope):
         if name in sc.locals:
             return sc.locals[name]
@@ -309,6 +331,9 @@ class Evaluator:
             v = self.block_binding(sc.block, name, sc)
             if v is not None:
                 return v
+        sym = _layout.symbol_value(name)
+        if sym is not None:
+            return sym
         if name in self.state.bindings:
             return self.state.bindings[name].value
         raise ForgeEvalError(f"undefined identifier '{name}'")
@@ -379,6 +404,10 @@ class Evaluator:
                 fn = {"+": lambda a, b: a + b, "-": lambda a, b: a - b,
                       "*": lambda a, b: a * b, "/": lambda a, b: a / b}[op]
                 return vec3(*(fn(f32(a), f32(b)) for a, b in zip(lv, rv)))
+        if op in "+-":
+            moved = _layout.anchor_offset(l, op, r)
+            if moved is not None:
+                return moved
         if op in "+-&":
             return Obj({"+": "add", "-": "sub", "&": "intersect"}[op], {"lhs": l, "rhs": r})
         raise ForgeEvalError("binary type mismatch")
@@ -413,10 +442,19 @@ class Evaluator:
             if isinstance(target, S.Ident) and target.name in ("Box", "Sphere", "Cylinder", "Torus") \
                     and callee.field == "distance" and target.name not in sc.locals:
                 return self.primitive_distance(target.name, args, sc)
+            flat = _flatten_member(callee)
+            fdef = self.state.function_defs.get(flat) if flat is not None else None
+            if fdef is not None and callee.field not in _layout.LAYOUT_METHODS:
+                # a function of a namespaced import (`palette.make_gold()`, eval.rs:1857-1866)
+                return self.apply(fdef.params, fdef.body, args, Scope({}, None, None, sc.lowering, sc.depth + 1))
             base = self.eval(target, sc)
             out = build_sdf_member_value(callee.field, base, args)
             if out is not None:
                 return out
+            if callee.field in _layout.LAYOUT_METHODS:
+                out = _layout.build_layout_member_value(callee.field, base, args)
+                if out is not None:
+                    return out
             if isinstance(base, Obj) and isinstance(base.fields.get(callee.field), Func):
                 fn = base.fields[callee.field]
                 return self.apply(fn.params, fn.body, args, Scope(dict(fn.captures), None, None, sc.lowering, sc.depth + 1))
@@ -683,3 +721,6 @@ def eval_program(program: list) -> EvalState:
 
 def load_scene_source(source: str) -> EvalState:
     return eval_program(S.parse_program(source))
+
+
+from . import layout as _layout  # noqa: E402  (layout uses the value types above)

@@ -525,6 +525,54 @@ class SceneCompiler:
         self.object_transforms[oid] = (transform[0].copy(), transform[1].copy())
         return oid
 
+    def compile_room(self, obj: Obj, default_mat: Material) -> dict:
+        """compile_room (renderer.rs:3359-3520): the shown floor, ceiling and walls as boxes
+        offset from the room centre along world axes, joined by hard unions in that order."""
+        center, rot = read_transform(obj)
+
+        def dim(name, default, lo):
+            return max(_or(read_number_field(obj, [name]), f32(default)), f32(lo))
+
+        def flag(name, default):  # room_flag (renderer.rs:2403-2411)
+            v = obj.fields.get(name)
+            return bool(v >= f32(0.5)) if is_num(v) else default
+
+        def material(specific):  # room_material_id (renderer.rs:2339-2355)
+            for name in (specific, "wall_material"):
+                mv = obj.fields.get(name)
+                if isinstance(mv, Obj):
+                    return self.intern_material(self.material_from_object(mv, True))
+            return self.intern_material(default_mat)
+
+        w, h, d = dim("width", 8.0, 0.1), dim("height", 4.0, 0.1), dim("depth", 8.0, 0.1)
+        t = dim("wall_thickness", 0.18, 0.01)
+        hw, hh, hd, ht = (f32(v * f32(0.5)) for v in (w, h, d, t))
+        nhw, nhh, nhd = (f32(-v * f32(0.5)) for v in (w, h, d))
+        mids = {k: material(k) for k in ("floor_material", "ceiling_material", "back_wall_material",
+                                         "front_wall_material", "left_wall_material", "right_wall_material")}
+        specs = [("show_floor", True, "floor_material", (0.0, f32(nhh + ht), 0.0), (hw, ht, hd)),
+                 ("show_ceiling", False, "ceiling_material", (0.0, f32(hh - ht), 0.0), (hw, ht, hd)),
+                 ("show_back_wall", True, "back_wall_material", (0.0, 0.0, f32(nhd + ht)), (hw, hh, ht)),
+                 ("show_front_wall", False, "front_wall_material", (0.0, 0.0, f32(hd - ht)), (hw, hh, ht)),
+                 ("show_left_wall", False, "left_wall_material", (f32(nhw + ht), 0.0, 0.0), (ht, hh, hd)),
+                 ("show_right_wall", True, "right_wall_material", (f32(hw - ht), 0.0, 0.0), (ht, hh, hd))]
+        parts = []
+        for fl, default, mat, off, half in specs:
+            if not flag(fl, default):
+                continue
+            tr = ((center + _v(*off)).astype(np.float32), rot)
+            parts.append(dict(kind=abi.NODE["BOX"], transform=tr, param=[*half, f32(0.0), f32(0.0)],
+                              object_id=self.alloc_object(tr), material_id=mids[mat]))
+        if not parts:
+            oid = self.alloc_object((center, rot))
+            return dict(kind=abi.NODE["BOX"], transform=(center, rot),
+                        param=[f32(0.001), f32(0.001), f32(0.001), f32(0.0), f32(0.0)],
+                        object_id=oid, material_id=self.intern_material(default_mat))
+        root = parts[0]
+        for part in parts[1:]:
+            root = dict(kind=abi.NODE["UNION"], children=[root, part])
+        return root
+
     def compile_sdf(self, value, default_mat: Material) -> dict:
         if not isinstance(value, Obj):
             raise RenderError("ExpectedObject", "expected object value for rendering")
@@ -577,6 +625,8 @@ class SceneCompiler:
             mid = self.primitive_material_id(obj, default_mat)
             node = dict(kind=abi.NODE["EXTRUDE_POLYGON" if tn == "ExtrudePolygon" else tn.upper()],
                         transform=transform, param=params, object_id=oid, material_id=mid)
+        elif tn == "Room":
+            node = self.compile_room(obj, default_mat)
         elif tn in self.state.sdf_defs:
             block = self.state.sdf_defs[tn]
             transform = read_transform(obj)
@@ -906,6 +956,47 @@ class CompiledScene:
     dynamic_hooks: list = field(default_factory=list)  # "Material.hook" evaluated per hit
     material_hooks: np.ndarray = field(default_factory=lambda: np.zeros(0, dtype=abi.HOOKS_DTYPE))
     _ir: object = None
+
+    # the scene IR as a file: every array of ft_scene_ir in its C layout plus the camera,
+    # written with numpy's .npz container (no pickled objects), tagged with the ABI version
+    _ARRAYS = ("nodes", "materials", "object_transforms", "lights", "functions", "exprs", "material_hooks")
+
+    def save(self, path: str, **extra) -> None:
+        """Write the IR to `path` (.npz); `extra` adds named arrays (e.g. render settings)."""
+        cam = self.camera
+        camera = np.array([cam.get("present", False), cam.get("origin") is not None,
+                           cam.get("target") is not None, cam.get("fov") is not None], dtype=np.uint8)
+        cam_vals = np.zeros(7, dtype=np.float32)
+        if cam.get("origin") is not None:
+            cam_vals[0:3] = cam["origin"]
+        if cam.get("target") is not None:
+            cam_vals[3:6] = cam["target"]
+        if cam.get("fov") is not None:
+            cam_vals[6] = cam["fov"]
+        np.savez(path, abi_version=np.uint32(abi.ABI_VERSION),
+                 scalars=np.array([self.root, self.env_fn, self.leaf_count], dtype=np.int64),
+                 camera=camera, camera_values=cam_vals, dynamic_hooks=np.array(self.dynamic_hooks, dtype=str),
+                 **{k: getattr(self, k) for k in self._ARRAYS}, **extra)
+
+    @classmethod
+    def load(cls, path: str) -> Tuple["CompiledScene", dict]:
+        """Read an IR written by `save`; returns the scene and the extra arrays."""
+        with np.load(path, allow_pickle=False) as z:
+            if int(z["abi_version"]) != abi.ABI_VERSION:
+                raise RenderError("Image", f"scene IR file {path} has ABI version {int(z['abi_version'])}, "
+                                           f"this build reads {abi.ABI_VERSION}")
+            arrays = {k: np.ascontiguousarray(z[k]) for k in cls._ARRAYS}
+            root, env_fn, leaf_count = (int(v) for v in z["scalars"])
+            flags, vals = z["camera"], z["camera_values"]
+            camera = {"present": bool(flags[0]),
+                      "origin": vals[0:3].copy() if flags[1] else None,
+                      "target": vals[3:6].copy() if flags[2] else None,
+                      "fov": np.float32(vals[6]) if flags[3] else None}
+            extra = {k: z[k] for k in z.files if k not in cls._ARRAYS and
+                     k not in ("abi_version", "scalars", "camera", "camera_values", "dynamic_hooks")}
+            hooks = [str(h) for h in z["dynamic_hooks"]]
+        return cls(env_fn=env_fn, camera=camera, root=root, leaf_count=leaf_count, dynamic_hooks=hooks,
+                   **arrays), extra
 
     def ir(self) -> abi.SceneIR:
         if self._ir is None:

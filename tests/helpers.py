@@ -4,6 +4,7 @@ import os
 import numpy as np
 
 from forgedthoughts_amd.forge.evaluator import load_scene_source
+from forgedthoughts_amd.forge.imports import load_scene_file
 from forgedthoughts_amd.options import RaySettings, RenderOptions
 from forgedthoughts_amd.scene import compile_scene
 
@@ -19,9 +20,10 @@ REGRESSION_CASES = [
 ]
 
 
-def load_ft(path):
-    with open(path) as f:
-        return load_scene_source(f.read())
+def load_ft(path, library_dirs=None):
+    """A scene file with its imports (file imports relative to it, library imports from
+    `library_dirs` or FT_LIBRARY_PATH)."""
+    return load_scene_file(path, library_dirs)
 
 
 def compile_ft(path):

@@ -400,6 +400,7 @@ static int create_scene(const ft_scene_ir* ir, uint32_t accel_mode, bool batch_m
   d.env_fn = h.env_fn;
   d.accel = h.accel;
   d.simple = h.simple ? 1 : 0;
+  d.root_walks = h.root_walks ? 1 : 0;
   d.n_leaves = (int)h.leaves.size();
   d.n_bvh = (int)h.bvh.size();
   d.stack_depth = std::max(1, h.max_stack);

@@ -57,6 +57,9 @@ struct DevScene {
   int atlas_dims[3], atlas_fp16;
   int atlas_on;
   float atlas_switch;                    // exact evaluation below this distance (2 voxels)
+  // the per-hit walks start at the root instead of the hit's accelerator leaf: the scene has
+  // a node whose distance can fall below the bounds the accelerators prune with
+  int root_walks;
 };
 
 // Scene descriptors live in constant memory, one slot per scene handle.  Device functions

@@ -77,7 +77,7 @@ static_assert(sizeof(DParams) == 144, "DParams layout");
 struct DHooks {
   int color, roughness, ior, thin_walled, emission_color, emission_strength;
   int metallic, specular, specular_weight, transmission, specular_color, clearcoat_roughness, clearcoat, coat_mask;
-  int pad[2];
+  int bump, normal;
 };
 static_assert(sizeof(DHooks) == 64, "DHooks layout");
 

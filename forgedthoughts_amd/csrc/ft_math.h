@@ -123,7 +123,11 @@ __device__ __forceinline__ float op_union_soft(float a, float b, float r) {
 // noise modifier (renderer.rs:5957-6009)
 __device__ __forceinline__ float fractf_(float x) { return x - floorf(x); }
 constexpr float kInvPi = 0.31830988618379067154f;
+// The hash multiplies lattice coordinates up to ~2.5e5 before taking the fraction, so one
+// rounding step decides its value: no multiply-add fusion here (the reference's Rust never
+// fuses), or the noise field would differ from the reference's almost everywhere.
 __device__ __forceinline__ float hash_noise(float3 p) {
+#pragma clang fp contract(off)
   float qx = fractf_(p.x * kInvPi + 0.11f) * 17.0f;
   float qy = fractf_(p.y * kInvPi + 0.17f) * 17.0f;
   float qz = fractf_(p.z * kInvPi + 0.13f) * 17.0f;

@@ -277,8 +277,16 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
       S.nodes[n.lhs].flags |= 2;
     }
   }
-  for (uint32_t i = 0; i < N; ++i)
-    if (ir->nodes[i].kind == FT_NODE_CUSTOM || is_modifier(ir->nodes[i].kind)) S.simple = false;
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint32_t k = ir->nodes[i].kind;
+    if (k == FT_NODE_CUSTOM || is_modifier(k)) S.simple = false;
+    // hg_sdf's columns formulas (op_union_columns renderer.rs:6072-6099) run on negated
+    // inputs for the intersection and difference, so far from the shape they return values
+    // far below the distance to its bounds: the accelerators prune such a leaf where the
+    // reference's root walk (resolve_material_from_node from setup.root) still enters it
+    if (k == FT_NODE_UNION_COLUMNS || k == FT_NODE_INTERSECT_COLUMNS || k == FT_NODE_DIFF_COLUMNS)
+      S.root_walks = true;
+  }
   // the program as self-contained records
   S.ops.assign(S.prog.size() + 1, ftd::DOp{});
   for (size_t k = 0; k < S.prog.size(); ++k) {
@@ -513,9 +521,9 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     for (uint32_t h = 0; h < ir->n_material_hooks; ++h) {
       const ft_material_hooks& H = ir->material_hooks[h];
       if (H.material != m) continue;
-      const int32_t ids[14] = {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
+      const int32_t ids[16] = {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                                H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
-                               H.clearcoat_roughness, H.clearcoat, H.coat_mask};
+                               H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal};
       for (int32_t f : ids) {
         if (f >= (int32_t)ir->n_functions || (f >= 0 && ir->functions[f].n_params != FT_HOOK_PARAMS)) {
           err = "material hook is not a function of the shading context";
@@ -524,7 +532,7 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
       }
       ftd::DHooks d{H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                     H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
-                    H.clearcoat_roughness, H.clearcoat, H.coat_mask, {0, 0}};
+                    H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal};
       p.hook = (float)S.hooks.size();
       S.hooks.push_back(d);
       break;

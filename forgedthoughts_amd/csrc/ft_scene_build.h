@@ -31,6 +31,7 @@ struct HostScene {
   int max_stack = 0;        // value-stack depth the program needs
   int max_points = 0;       // modifier nesting
   bool simple = true;       // no custom SDF, modifier or noise node: the program calls no function
+  bool root_walks = false;  // a columns boolean: its distance can fall below its bounds
   int max_walk_frames = 0;  // frames of the per-hit walks
   int max_walk_mats = 0;    // material results held by the material walk
   int max_fn_exprs = 0;

@@ -157,6 +157,53 @@ __device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float
   a.emission_strength = fmaxf(a.emission_strength, 0.0f);
   return model;
 }
+// resolve_dynamic_bump / resolve_dynamic_normal / orthonormal_tangent (renderer.rs:7749-7827):
+// the material of the hit's id reshapes the shading normal `gn` of build_bsdf_context
+// (renderer.rs:4397-4449) through its bump hook (finite differences at 0.01 along a tangent
+// frame) and then its normal hook, each result flipped to the side of the normal it
+// started from.  Context: make_shading_context_with_normal (renderer.rs:7884-7911), the
+// normal being the one reshaped and view_dir = -dir as build_bsdf_context receives it.
+__device__ __noinline__ float3 apply_normal_hooks(int slot, float3 pos, bool front, unsigned obj, unsigned mat_id,
+                                                  float3 view, float3 gn) {
+  const DevScene& S = scene_of(slot);
+  const unsigned id = mat_id < (unsigned)S.n_mats ? mat_id : 0u;
+  const int hk = (int)S.mats[id].hook;
+  if (hk < 0) return gn;
+  const DHooks H = S.hooks[hk];
+  if (H.bump < 0 && H.normal < 0) return gn;
+  const float3 lp = object_local(slot, obj, pos);
+  float3 args[FT_HOOK_PARAMS];
+  args[0] = pos;
+  args[1] = lp;
+  args[2] = gn;
+  args[3] = view;
+  args[4] = splat3(front ? 1.0f : 0.0f);
+  args[5] = splat3((float)obj);
+  args[6] = splat3((float)mat_id);
+  for (int k = 7; k < (int)FT_HOOK_PARAMS; ++k) args[k] = mk(0.0f, 0.0f, 0.0f);
+  float3 n = gn;
+  if (H.bump >= 0) {
+    const float center = eval_fn(slot, H.bump, args).x;
+    const float3 up = fabsf(gn.y) < 0.999f ? mk(0.0f, 1.0f, 0.0f) : mk(1.0f, 0.0f, 0.0f);
+    const float3 tangent = normalize(cross(gn, up));
+    const float3 bitangent = normalize(cross(gn, tangent));
+    const float eps = 0.01f;
+    args[1] = add(lp, mul(tangent, eps));
+    const float dx = eval_fn(slot, H.bump, args).x - center;
+    args[1] = add(lp, mul(bitangent, eps));
+    const float dy = eval_fn(slot, H.bump, args).x - center;
+    args[1] = lp;
+    const float3 nb = normalize(sub(sub(gn, mul(tangent, dx / eps)), mul(bitangent, dy / eps)));
+    if (len(nb) > 1.0e-6f) n = dot(nb, gn) < 0.0f ? mul(nb, -1.0f) : nb;
+  }
+  if (H.normal >= 0) {
+    args[2] = n;
+    const float3 nn = normalize(eval_fn(slot, H.normal, args));
+    if (len(nn) > 1.0e-6f) n = dot(nn, n) < 0.0f ? mul(nn, -1.0f) : nn;
+  }
+  return n;
+}
+
 __device__ inline void leaf_material(int slot, unsigned mat_id, unsigned obj, float3 pos, Mat& m,
                                      const HookCtx* hc = nullptr) {
   m.blend = 0;
@@ -1052,14 +1099,15 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
     if (!raymarch(slot, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
       c = cmul(cmul(environment(slot, rd), transmittance(med, tp.max_dist)), thr);
     } else {
-      const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
+      const int start = hit.leaf >= 0 && !S.root_walks ? S.leaves[hit.leaf].node : S.root;
       // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449), and
       // the hit's material or split materials, from one shading sweep
       ShadeWalk w;
       const HookCtx hc{hit.pos, hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f)),
                        normalize(mul(rd, -1.0f)), hit.front ? 1.0f : 0.0f};
       shade_walks(slot, start, hit.pos, 1.0e-4f, w, &hc);
-      const float3 sn = normalize(hit.front ? w.normal : mul(w.normal, -1.0f));
+      const float3 sn = apply_normal_hooks(slot, hit.pos, hit.front, hit.obj, hit.mat, mul(rd, -1.0f),
+                                           normalize(hit.front ? w.normal : mul(w.normal, -1.0f)));
       const float3 att = cmul(thr, transmittance(med, hit.t));
       c = mk(0, 0, 0);
       if (w.split) {
@@ -1084,7 +1132,7 @@ __device__ __noinline__ float3 trace_aov(int slot, const TraceParams& tp, float3
   const DevScene& S = scene_of(slot);
   Hit hit;
   if (!raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit)) return mk(0, 0, 0);
-  const int start = hit.leaf >= 0 ? S.leaves[hit.leaf].node : S.root;
+  const int start = hit.leaf >= 0 && !S.root_walks ? S.leaves[hit.leaf].node : S.root;
   Mat mat;
   const HookCtx hc{hit.pos, hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f)),
                    normalize(mul(d, -1.0f)), hit.front ? 1.0f : 0.0f};

@@ -39,15 +39,17 @@ from .forge.evaluator import Obj, Func, is_num, f32
 # resolve_leaf_material (renderer.rs:7164-7536)
 HOOK_NAMES = ("color", "roughness", "ior", "thin_walled", "emission_color", "emission_strength",
               "metallic", "specular", "specular_weight", "transmission", "specular_color",
-              "clearcoat_roughness", "clearcoat", "coat_mask")
-SPECTRUM_HOOKS = ("color", "emission_color", "specular_color")
+              "clearcoat_roughness", "clearcoat", "coat_mask", "bump", "normal")
+SPECTRUM_HOOKS = ("color", "emission_color", "specular_color", "normal")
 # the hooks each definition model reads: the Metal arm and the Dielectric / other arms
-# (renderer.rs:7472-7530), and the Standard / Layered block (7259-7470), whose coat hooks
-# replace the ior, specular colour, clearcoat roughness and clearcoat weight
+# (renderer.rs:7472-7530), the Standard / Layered block (7259-7470), whose coat hooks replace
+# the ior, specular colour, clearcoat roughness and clearcoat weight, and for every model
+# the bump and normal hooks of build_bsdf_context (4397-4449)
+_NORMAL = {"bump": "bump", "normal": "normal"}
 _ARM_HOOKS = {
-    "Metal": {"color": "color", "roughness": "roughness", "emission_color": "emission_color",
-              "emission_strength": "emission_strength"},
-    "other": {h: h for h in HOOK_NAMES[:6]},
+    "Metal": dict({"color": "color", "roughness": "roughness", "emission_color": "emission_color",
+                   "emission_strength": "emission_strength"}, **_NORMAL),
+    "other": dict({h: h for h in HOOK_NAMES[:6]}, **_NORMAL),
     "Standard": dict({h: h for h in HOOK_NAMES}, ior="coat_ior", specular_color="coat_color",
                      clearcoat_roughness="coat_roughness", clearcoat="coat_weight"),
 }
@@ -60,12 +62,14 @@ def arm_hooks(model: str) -> dict:
     return _ARM_HOOKS["Metal" if model == "Metal" else "other"]
 
 
-# hooks that no part of the Whitted render reads: bump / normal feed build_bsdf_context
-# (renderer.rs:4397-4449), eval / pdf / sample the BSDF overrides (4905-5004), medium /
-# subsurface the medium transitions (5125-5161); none of these has a caller outside the
-# reference's unit tests, so they leave the image unchanged, as do hooks a model's arm
-# does not read (a Metal's ior) and helper functions of the shading context: a material
-# with only such hooks is compiled as a static one
+# hooks the Whitted render (renderer/ray.rs) reads that the GPU does not evaluate: eval
+# replaces the BSDF value in shade_color (renderer.rs:4217-4276, 4905-4920), medium enters
+# transition_medium (5125-5161, ray.rs:179).  A material with one of them keeps its
+# dynamic id without a hook table entry, and Whitted renders of its scene fail loudly.  pdf / sample serve only the path tracer and
+# subsurface nothing the renderers read; like hooks a model's arm does not read (a
+# Metal's ior) and helper functions of the shading context they leave the image unchanged,
+# and a material with only such hooks compiles as a static one.
+REFUSED_HOOKS = ("eval", "medium")
 MATERIAL_HOOKS = ("color", "roughness", "metallic", "specular", "ior", "thin_walled", "medium", "subsurface",
                   "emission", "emission_color", "emission_strength", "bump", "normal", "eval", "pdf", "sample",
                   "transmission", "clearcoat", "clearcoat_roughness", "specular_color")
@@ -439,12 +443,13 @@ class SceneCompiler:
         # exactly as its static parameters
         hooks = [st[1] for st in d.statements if st[0] == "fn" and
                  (st[1] in MATERIAL_HOOKS or (len(st[2]) >= 1 and st[2][0] == "ctx"))]
-        read = [h for h in hooks if h in arm_hooks(model).values()]
+        read = [h for h in hooks if h in arm_hooks(model).values() or h in REFUSED_HOOKS]
         if hooks:
             self.dynamic_hooks.extend(f"{name}.{h}" for h in hooks if f"{name}.{h}" not in self.dynamic_hooks)
         if read:
             m.dynamic_material_id = self.sorted_material_names.index(name)
-            m.hooks = self.lower_hooks(d, name, model, overrides)
+            if not any(h in REFUSED_HOOKS for h in read):
+                m.hooks = self.lower_hooks(d, name, model, overrides)
         if read and with_ctx and overrides.fields:
             key = repr(sorted((k, repr(v)) for k, v in overrides.fields.items()))
             if key not in self.dynamic_overrides:

@@ -41,7 +41,7 @@ typedef __hip_internal::uint8_t uint8_t;
 extern "C" {
 #endif
 
-#define FT_ABI_VERSION 3u
+#define FT_ABI_VERSION 4u
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -251,7 +251,8 @@ typedef struct ft_material {
  * the ior, specular colour, clearcoat and clearcoat roughness, so for them `ior` holds the
  * coat_ior hook, `specular_color` coat_color, `clearcoat_roughness` coat_roughness and
  * `clearcoat` coat_weight, the clearcoat being clamp(weight * coat_mask) with the static
- * weight where `clearcoat` is -1.  Every hook function takes FT_HOOK_PARAMS parameters, the shading
+ * weight where `clearcoat` is -1.  bump and normal hooks of any model reshape the normal the
+ * direct lighting uses.  Every hook function takes FT_HOOK_PARAMS parameters, the shading
  * context (make_shading_context renderer.rs:7875-7911) in the order position,
  * local_position, normal, view_dir, front_face, object_id, material_id, then wo, wi,
  * current_ior, u1, u2, u3 (zero for these hooks); scalars are splatted. */
@@ -272,6 +273,8 @@ typedef struct ft_material_hooks {
   int32_t clearcoat_roughness;
   int32_t clearcoat;
   int32_t coat_mask;
+  int32_t bump;               /* any model: the shading normal of build_bsdf_context */
+  int32_t normal;             /* (renderer.rs:4397-4449, 7749-7827), a number and a vector */
   int32_t pad;
 } ft_material_hooks;
 

@@ -964,7 +964,7 @@ bool validate(const ft_scene_ir* ir) {
     if (H.material >= ir->n_materials) { g_err = "material hook names no material"; return false; }
     for (int32_t f : {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                       H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
-                      H.clearcoat_roughness, H.clearcoat, H.coat_mask})
+                      H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal})
       if (f >= (int32_t)ir->n_functions) { g_err = "material hook names no function"; return false; }
   }
   for (uint32_t i = 0; i < ir->n_nodes; ++i) {
@@ -1443,6 +1443,50 @@ template <class T> Mat<T> resolve_leaf_material(const Ctx<T>& c, Mat<T> m, const
   return m;
 }
 
+// resolve_dynamic_bump / resolve_dynamic_normal / orthonormal_tangent (R:7749-7827): the
+// material of the hit's id reshapes the shading normal `gn` of build_bsdf_context
+// (R:4397-4449) through its bump hook (finite differences at 0.01 along a tangent frame)
+// and then its normal hook, each result flipped to the side of the normal it started from.
+// The context is make_shading_context_with_normal (R:7884-7911): normal = the normal being
+// reshaped, view_dir = -dir as build_bsdf_context receives it.
+template <class T> V3<T> apply_normal_hooks(const Scene<T>& S, const Hit<T>& hit, V3<T> lp, V3<T> view, V3<T> gn) {
+  const uint32_t mi = hit.material_id < S.materials.size() ? hit.material_id : 0u;
+  const int hk = S.mat_hook.empty() ? -1 : S.mat_hook[mi];
+  if (hk < 0) return gn;
+  const ft_material_hooks& H = S.hooks[hk];
+  if (H.bump < 0 && H.normal < 0) return gn;
+  V3<T> args[FT_HOOK_PARAMS];
+  args[0] = hit.position;
+  args[1] = lp;
+  args[2] = gn;
+  args[3] = view;
+  args[4] = V3<T>::splat(hit.front_face ? T(1) : T(0));
+  args[5] = V3<T>::splat((T)hit.object_id);
+  args[6] = V3<T>::splat((T)hit.material_id);
+  for (uint32_t k = 7; k < FT_HOOK_PARAMS; ++k) args[k] = V3<T>();
+  V3<T> n = gn;
+  if (H.bump >= 0) {
+    const T center = S.fn(H.bump, args).x;
+    const V3<T> up = std::fabs(gn.y) < T(0.999) ? V3<T>(0, 1, 0) : V3<T>(1, 0, 0);
+    const V3<T> tangent = gn.cross(up).normalize();
+    const V3<T> bitangent = gn.cross(tangent).normalize();
+    const T eps = T(0.01);
+    args[1] = lp + tangent * eps;
+    const T dx = S.fn(H.bump, args).x - center;
+    args[1] = lp + bitangent * eps;
+    const T dy = S.fn(H.bump, args).x - center;
+    args[1] = lp;
+    V3<T> nb = (gn - tangent * (dx / eps) - bitangent * (dy / eps)).normalize();
+    if (nb.length() > T(1.0e-6)) n = nb.dot(gn) < T(0) ? nb * T(-1) : nb;
+  }
+  if (H.normal >= 0) {
+    args[2] = n;
+    const V3<T> nn = S.fn(H.normal, args).normalize();
+    if (nn.length() > T(1.0e-6)) n = nn.dot(n) < T(0) ? nn * T(-1) : nn;
+  }
+  return n;
+}
+
 // resolve_material_from_node (R:6945-7059)
 template <class T> Mat<T> resolve_material_from_node(const Ctx<T>& c, uint32_t i, const Hit<T>& hit);
 template <class T> Mat<T> blend_node_materials(const Ctx<T>& c, const ft_node& n, const Hit<T>& hit, T radius, bool diff) {
@@ -1747,7 +1791,7 @@ template <class T> Spec<T> trace_hit(const Ctx<T>& c, const Hit<T>& hit, V3<T> d
   b.local_position = to_local(hit.position, xf);
   V3<T> sn = surface_normal(S, S.root, hit.position, T(1.0e-4));
   sn = hit.front_face ? sn : sn * T(-1);
-  b.normal = sn.normalize();
+  b.normal = apply_normal_hooks(S, hit, b.local_position, dir * T(-1), sn.normalize());
   b.wo = (dir * T(-1)).normalize();
   b.current_ior = med.ior;
 

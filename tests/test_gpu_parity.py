@@ -69,7 +69,7 @@ def depth_parity(scene, accel, opts, aa=1, label="", emit=None):
     return rgb_g, t_g, (mism, float(dt.max()) if both.any() else 0.0)
 
 
-def whitted_parity(scene, accel, opts, settings, label="", emit=None):
+def whitted_parity(scene, accel, opts, settings, label="", emit=None, min_frac=0.999):
     print(f"[whitted parity] {label} accel={int(accel)} {opts.width}x{opts.height} spp={settings.aa_samples} "
           f"emit={emit}", flush=True)
     rgb_c, hdr_c = ftref.render_whitted(scene, accel, opts, settings)
@@ -78,7 +78,7 @@ def whitted_parity(scene, accel, opts, settings, label="", emit=None):
         check_path(gs, emit)
     d = np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2)
     frac = float((d <= 1).mean())
-    assert frac >= 0.999, f"{label}: only {frac:.5f} of pixels within 1/255 (max {d.max()})"
+    assert frac >= min_frac, f"{label}: only {frac:.5f} of pixels within 1/255 (max {d.max()})"
     return rgb_g, frac
 
 
@@ -232,6 +232,7 @@ material Brushed {
   model: Metal;
   color = #c8ccd4;
   fn roughness(ctx) { return 0.05 + 0.3 * step(0.0, sin(ctx.local_position.y * 12.0)); }
+  fn bump(ctx) { return sin(ctx.local_position.x * 14.0) * 0.03; }
 };
 material Glass {
   model: Dielectric;
@@ -255,7 +256,7 @@ material Lacquer {
   fn coat_color(ctx) { return mix(#ffffff, #ffd08a, clamp(ctx.local_position.x + 0.5, 0.0, 1.0)); }
   fn coat_roughness(ctx) { return 0.04 + 0.2 * step(0.0, ctx.local_position.z); }
   fn coat_mask(ctx) { return clamp(ctx.position.y + 0.4, 0.0, 1.0); }
-  fn bump(ctx) { return sin(ctx.local_position.x * 30.0) * 0.01; }
+  fn normal(ctx) { return normalize(ctx.normal + vec3(0.3 * sin(ctx.local_position.y * 20.0), 0.0, 0.0)); }
 };
 var d = Sphere { radius: 0.5, material: Lacquer {} };
 d.pos.z = 1.6;
@@ -267,8 +268,9 @@ let scene = floor + a + b.union_round(c, 0.3) + d;
 @pytest.mark.parametrize("emit", ["0", "1"], ids=["interp", "emitted"])
 def test_material_hooks(emit, monkeypatch):
     """Per-hit hooks of Lambert-, Metal- and Dielectric-model definitions (colour, roughness,
-    ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530) and of a
-    Standard-model one (metallic, specular weight and the coat hooks, renderer.rs:7259-7470),
+    ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530), of a
+    Standard-model one (metallic, specular weight and the coat hooks, renderer.rs:7259-7470)
+    and bump / normal hooks (renderer.rs:7749-7827),
     evaluated on the GPU with the shading context of each hit, against the oracle; and the
     reference's own hooked regression scene in Whitted."""
     monkeypatch.setenv("FT_EMIT", emit)

@@ -390,24 +390,53 @@ def test_constant_standard_hooks_render_as_static_materials():
     assert np.abs(ha - hb).max() == 0.0
 
 
+def test_bump_and_normal_hooks_reshape_the_shading_normal():
+    """resolve_dynamic_bump / resolve_dynamic_normal (renderer.rs:7749-7827) on a wall facing
+    +z: a constant bump and a normal hook returning the context normal or its negation leave
+    the image of the static material; a bump linear in local x (0.5 x) tilts the normal by
+    finite differences along the tangent frame ((-1,0,0), (0,-1,0)) to normalize(-0.5, 0, 1),
+    the image of a normal hook returning that vector."""
+    lit = "let key = PointLight { position: vec3(0.6, 0.4, 3.0), color: #ffffff, intensity: 9.0 };\n"
+    wall = "var w = Box { size: vec3(20.0, 20.0, 1.0), material: M {} }; w.pos.z = -0.5; let scene = w;"
+
+    def render(hooks):
+        sc = compile_src(_axis_camera(lit + f"material M {{ model: Lambert; color = #c0c0c0; {hooks} }};\n" + wall))
+        return ftref.render_whitted(sc, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
+
+    a, ha = render("")
+    for hooks in ("fn bump(ctx) { return 0.3; }", "fn normal(ctx) { return ctx.normal; }",
+                  "fn normal(ctx) { return ctx.normal * -2.0; }"):
+        b, hb = render(hooks)
+        assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0, hooks
+    b, hb = render("fn bump(ctx) { return ctx.local_position.x * 0.5; }")
+    c, hc = render("fn normal(ctx) { return vec3(-0.5, 0.0, 1.0); }")
+    assert np.abs(hb - hc).max() < 1e-4 * max(1.0, float(hc.max()))
+    assert np.abs(ha - hb).max() > 1e-3
+
+
 def test_hooks_outside_the_whitted_arms_leave_materials_static():
-    """bump / normal, eval / pdf / sample, medium / subsurface hooks, hooks a model's arm does
-    not read (a Metal's ior) and helper functions are never called by the Whitted render:
-    such a material compiles as its static parameters."""
+    """pdf / sample (path tracer only) and subsurface hooks, hooks a model's arm does not
+    read (a Metal's ior) and helper functions are never called by the Whitted render: such a
+    material compiles as its static parameters.  eval / medium hooks are read
+    (renderer/ray.rs:135, 179) but not evaluated on the GPU: they keep the material dynamic
+    without a hook entry."""
     lit = "let key = PointLight { position: vec3(0.0, 0.0, 3.0), color: #ffffff, intensity: 9.0 };\n"
     wall = "var w = Box { size: vec3(20.0, 20.0, 1.0), material: M {} }; w.pos.z = -0.5; let scene = w;"
     hooked = compile_src(_axis_camera(
         lit + "material M { model: Metal; color = #c0a080; roughness = 0.3;\n"
         "  fn helper(ctx) { return ctx.position.x; }  fn ior(ctx) { return 2.0; }\n"
-        "  fn bump(ctx) { return helper(ctx) * 0.1; }  fn normal(ctx) { return vec3(0.0, 1.0, 0.0); }\n"
-        "  fn eval(ctx) { return #ffffff; }  fn pdf(ctx) { return 1.0; }\n"
-        "  fn medium(ctx) { return 0.0; }  fn subsurface(ctx) { return 0.0; } };\n" + wall))
+        "  fn pdf(ctx) { return helper(ctx); }  fn sample(ctx) { return 1.0; }\n"
+        "  fn subsurface(ctx) { return 0.0; } };\n" + wall))
     static = compile_src(_axis_camera(lit + "material M { model: Metal; color = #c0a080; roughness = 0.3; };\n" + wall))
     assert len(hooked.material_hooks) == 0
     assert all(int(m["dynamic_material_id"]) < 0 for m in hooked.materials)
     a, ha = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
     b, hb = ftref.render_whitted(static, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
     assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0 and ha.max() > 0.0
+    for hook in ("fn eval(ctx) { return #ffffff; }", "fn medium(ctx) { return 0.0; }"):
+        read = compile_src(_axis_camera(lit + f"material M {{ model: Metal; {hook} }};\n" + wall))
+        assert len(read.material_hooks) == 0, hook
+        assert any(int(m["dynamic_material_id"]) >= 0 for m in read.materials), hook
 
 
 def test_constant_hooks_render_as_static_materials():

@@ -288,6 +288,8 @@ class Evaluator:
                 return Obj("vec3", fields)
             if e.type_name in self.state.sdf_defs:
                 self.augment_sdf_instance(e.type_name, fields)
+            if e.type_name in self.state.skeleton_defs:
+                fields.update(self.instantiate_skeleton(e.type_name, fields))
             return Obj(e.type_name, fields)
         if isinstance(e, S.Binary):
             return self.binary(self.eval(e.lhs, sc), e.op, self.eval(e.rhs, sc), sc)
@@ -323,6 +325,28 @@ class Evaluator:
                 v = None
             if v is not None:
                 fields["anchors"] = v
+
+    def instantiate_skeleton(self, type_name: str, fields: Dict[str, object]) -> Dict[str, object]:
+        """instantiate_skeleton_fields (eval.rs:586-750): the definition's bindings and joints
+        evaluated in order with the instance fields in scope, then IK and the rig fields."""
+        d = self.state.skeleton_defs[type_name]
+        locals_ = dict(fields)
+        joints, bones, chains = {}, {}, {}
+        for st in d.statements:
+            if st[0] == "let":
+                locals_[st[1]] = self.eval(st[2], Scope(dict(locals_)))
+            elif st[0] == "joint":
+                v = self.eval(st[2], Scope(dict(locals_)))
+                comps = as_vec3(v)
+                if comps is None or not all(is_num(c) for c in comps):
+                    raise _layout.LayoutError("unsupported layout object")
+                joints[st[1]] = comps
+                locals_[st[1]] = vec3(*comps)
+            elif st[0] == "bone":
+                bones[st[1]] = (st[2], st[3])
+            elif st[0] == "chain":
+                chains[st[1]] = (st[2], st[3], st[4])
+        return _layout.skeleton_fields(joints, bones, chains, fields)
 
     def lookup(self, name: str, sc: Scope):
         if name in sc.locals:
@@ -362,6 +386,9 @@ class Evaluator:
             op = {"x": "GET_X", "y": "GET_Y", "z": "GET_Z", "r": "GET_X", "g": "GET_Y", "b": "GET_Z"}[fld]
             return sc.lowering.add(op, 0, base.idx)
         if isinstance(base, Obj):
+            part = _layout.semantic_part_value(base, fld)
+            if part is not None:
+                return part
             if fld in base.fields:
                 return base.fields[fld]
             if base.type_name == "vec3" and fld in ("r", "g", "b"):
@@ -565,6 +592,8 @@ class Evaluator:
                 raise ForgeEvalError(f"'{name}' outside a lowered function is not supported")
             a, b = lo.lift(args[0]), lo.lift(args[1])
             return lo.add(name.upper(), 1, a.idx, b.idx)
+        if name in _NATIVE_SDF:
+            return self.native_sdf(name, args, sc)
         if name in ("value_noise_3d", "fbm_3d"):
             lo = sc.lowering
             if lo is None:
@@ -572,6 +601,59 @@ class Evaluator:
             ids = [lo.lift(a) for a in args]
             return lo.add("VALUE_NOISE" if name == "value_noise_3d" else "FBM", 0, *[i.idx for i in ids])
         return None
+
+    # native SDF helpers of asset definitions (eval.rs:1943-2009, 2435-2600), written with the
+    # generic operations so they fold on numbers and lower on symbolic arguments alike
+    def _box_dist(self, p, b, sc):  # box_distance_native
+        d = self.binary(self.builtin("abs", [p], sc), "-", b, sc)
+        outside = self.builtin("length", [self.builtin("max", [d, f32(0.0)], sc)], sc)
+        m = self.builtin("max", [self.member(d, "y", sc), self.member(d, "z", sc)], sc)
+        inside = self.builtin("min", [self.builtin("max", [self.member(d, "x", sc), m], sc), f32(0.0)], sc)
+        return self.binary(outside, "+", inside, sc)
+
+    def _len2(self, a, b, sc):  # vec2_len
+        return self.builtin("sqrt", [self.binary(self.binary(a, "*", a, sc), "+", self.binary(b, "*", b, sc), sc)], sc)
+
+    def _axis_cylinder(self, p, axis, radius, half_len, sc):  # axis_cylinder_distance_native
+        c = [self.member(p, k, sc) for k in "xyz"]
+        axial = self.binary(self.builtin("abs", [c[axis]], sc), "-", half_len, sc)
+        ra, rb = [c[k] for k in range(3) if k != axis]
+        radial = self.binary(self._len2(ra, rb, sc), "-", radius, sc)
+        outside = self._len2(self.builtin("max", [radial, f32(0.0)], sc), self.builtin("max", [axial, f32(0.0)], sc), sc)
+        inside = self.builtin("min", [self.builtin("max", [radial, axial], sc), f32(0.0)], sc)
+        return self.binary(outside, "+", inside, sc)
+
+    def native_sdf(self, name: str, args: list, sc: Scope):
+        need = 4 if name == "box_shell_sdf" else (3 if name.startswith("cylinder") else 5)
+        if len(args) != need:
+            raise ForgeEvalError("unsupported call")
+        if name == "box_shell_sdf":  # box_shell_distance_native
+            p, half, wall, rnd = args
+            outer_half = self.builtin("max", [self.binary(half, "-", rnd, sc), f32(0.001)], sc)
+            outer = self.binary(self._box_dist(p, outer_half, sc), "-", rnd, sc)
+            inner_half = self.builtin("max", [self.binary(self.binary(half, "-", wall, sc), "-", rnd, sc),
+                                              f32(0.001)], sc)
+            inner_round = self.builtin("max", [self.binary(rnd, "-", self.binary(wall, "*", f32(0.4), sc), sc),
+                                               f32(0.0)], sc)
+            inner = self.binary(self._box_dist(p, inner_half, sc), "-", inner_round, sc)
+            return self.builtin("max", [outer, self.neg(inner, sc)], sc)
+        axis = "xyz".index(name.split("_")[-2])
+        if name.startswith("cylinder"):
+            return self._axis_cylinder(args[0], axis, args[1], args[2], sc)
+        p, radius, half_len, spacing, count = args  # hole_line_distance_native
+        if not is_num(count):
+            raise ForgeEvalError(f"'{name}' needs a constant hole count")
+        c = float(count)
+        n = int(min(max(math.copysign(math.floor(abs(c) + 0.5), c), 1.0), 32.0))
+        start = self.binary(f32(f32(-0.5) * f32(n - 1)), "*", spacing, sc)
+        best = None
+        for i in range(n):
+            off = self.binary(start, "+", self.binary(f32(i), "*", spacing, sc), sc)
+            q = self.builtin("vec3", [self.member(p, "x", sc), self.member(p, "y", sc),
+                                      self.binary(self.member(p, "z", sc), "-", off, sc)], sc)
+            d = self._axis_cylinder(q, axis, radius, half_len, sc)
+            best = d if best is None else self.builtin("min", [best, d], sc)
+        return best
 
     def primitive_distance(self, type_name: str, args: list, sc: Scope):
         lo = sc.lowering
@@ -608,6 +690,10 @@ class Evaluator:
             if st[0] == "fn" and st[1] == fn_name and not st[2]:
                 return self.apply([], st[3], [], Scope(self.block_locals(Scope({}, block, overrides)), block, overrides, None, 0))
         return None
+
+
+_NATIVE_SDF = ("box_shell_sdf", "cylinder_x_sdf", "cylinder_y_sdf", "cylinder_z_sdf",
+               "hole_line_x_sdf", "hole_line_y_sdf", "hole_line_z_sdf")
 
 
 def _flatten_member(e) -> Optional[str]:

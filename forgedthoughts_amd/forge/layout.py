@@ -11,6 +11,8 @@ Mirrors the layout layer of the reference evaluator:
   attach_object / align_object_axis /
   place_object_relative                               eval.rs:4934-5039
   build_layout_member_value                           eval.rs:1538-1634
+  skeleton rigs: instantiation, two-bone IK, part
+  proxies, bind                                       eval.rs:580-945, 1132-1213, 1662-1712
 
 Placement methods return a new object and leave their receiver unchanged (values are
 copied on use in the reference).  All arithmetic is f32.
@@ -478,13 +480,13 @@ def _unary(args) -> np.float32:
     return _num_arg(args[0])
 
 
-def build_layout_member_value(fld: str, base, args: list, bind=None):
+def build_layout_member_value(fld: str, base, args: list):
     """The placement method `fld` applied to `base` (eval.rs:1538-1634); None for other
-    names.  `bind` handles `.bind(bone)` (skeleton proxies)."""
+    names."""
     if fld == "bind":
-        if len(args) != 1 or bind is None:
+        if len(args) != 1:
             raise LayoutError("unsupported call")
-        return bind(base, args[0])
+        return bind_object_to_bone(base, args[0])
     if fld == "attach":
         if len(args) not in (2, 3):
             raise LayoutError("unsupported call")
@@ -515,3 +517,198 @@ def build_layout_member_value(fld: str, base, args: list, bind=None):
             raise LayoutError("unsupported call")
         return face_object_to(base, args[0], args[1] if len(args) == 2 else None)
     return None
+
+
+# ---------------------------------------------------------------- skeleton rigs
+# skeleton instantiation / IK / part proxies / bind (eval.rs:580-1230, 1660-1712)
+
+def _v3(v) -> List[np.float32]:
+    return [f32(c) for c in v]
+
+
+def _add3(a, b):
+    return [f32(a[k] + b[k]) for k in range(3)]
+
+
+def _sub3(a, b):
+    return [f32(a[k] - b[k]) for k in range(3)]
+
+
+def _scale3(v, s):
+    return [f32(c * s) for c in v]
+
+
+def _dot3(a, b):
+    return f32(f32(f32(a[0] * b[0]) + f32(a[1] * b[1])) + f32(a[2] * b[2]))
+
+
+def _length3(v):
+    return f32(np.sqrt(_dot3(v, v)))
+
+
+def _normalize3(v):
+    ln = _fmax(_length3(v), f32(1.0e-9))
+    return [f32(c / ln) for c in v]
+
+
+def _cross3(a, b):
+    return [f32(f32(a[1] * b[2]) - f32(a[2] * b[1])), f32(f32(a[2] * b[0]) - f32(a[0] * b[2])),
+            f32(f32(a[0] * b[1]) - f32(a[1] * b[0]))]
+
+
+def solve_two_bone_chain(start, mid, end, target):
+    """Analytic two-bone IK (eval.rs:806-842): the new mid and end joints."""
+    upper_len = _fmax(_length3(_sub3(mid, start)), f32(1.0e-6))
+    lower_len = _fmax(_length3(_sub3(end, mid)), f32(1.0e-6))
+    target_vec = _sub3(target, start)
+    target_len = _fmax(_length3(target_vec), f32(1.0e-6))
+    max_reach = _fmax(f32(f32(upper_len + lower_len) - f32(1.0e-6)), f32(1.0e-6))
+    min_reach = f32(f32(abs(f32(upper_len - lower_len))) + f32(1.0e-6))
+    clamped = min(max(target_len, min_reach), max_reach)
+    d = _scale3(target_vec, f32(f32(1.0) / target_len))
+    original_plane = _cross3(_sub3(mid, start), _sub3(end, start))
+    sm = _sub3(mid, start)
+    bend = _sub3(sm, _scale3(d, _dot3(sm, d)))
+    if _length3(bend) < f32(1.0e-6):
+        if _length3(original_plane) > f32(1.0e-6):
+            plane_normal = _normalize3(original_plane)
+        else:  # fallback_perpendicular (eval.rs:877-884)
+            seed = [f32(0), f32(1), f32(0)] if abs(d[1]) < f32(0.9) else [f32(1), f32(0), f32(0)]
+            plane_normal = _normalize3(_cross3(d, seed))
+        bend = _cross3(plane_normal, d)
+    bend = _normalize3(bend)
+    along = f32(f32(f32(f32(clamped * clamped) + f32(upper_len * upper_len)) - f32(lower_len * lower_len))
+                / f32(f32(2.0) * _fmax(clamped, f32(1.0e-6))))
+    height = f32(np.sqrt(_fmax(f32(f32(upper_len * upper_len) - f32(along * along)), f32(0.0))))
+    new_mid = _add3(_add3(start, _scale3(d, along)), _scale3(bend, height))
+    new_end = _add3(start, _scale3(d, clamped))
+    return new_mid, new_end
+
+
+def _ik_target(value, bounds_center):  # skeleton_ik_target_point (eval.rs:793-804)
+    local = _num_vec3(value)
+    if local is not None:
+        return _add3(local, bounds_center)
+    p = object_anchor_point(value, "Center")
+    return p if p is not None else object_anchor_point(value, "Top")
+
+
+def _resolve_chain(chains: dict, target_name: str):  # resolve_skeleton_chain (eval.rs:776-791)
+    if target_name in chains:
+        return chains[target_name]
+    for start, mid, end in chains.values():
+        if end == target_name:
+            return start, mid, end
+    return None
+
+
+def skeleton_fields(joints: dict, bones: dict, chains: dict, instance: dict) -> dict:
+    """The fields a skeleton instance gets (eval.rs:600-750) from its evaluated joints (world
+    points, definition order), bones and chains: joints relative to the centre of their
+    bounds, anchors at the joints, `__bounds` half extents, `pos` the bounds centre."""
+    pts = {k: _v3(v) for k, v in joints.items()}
+    ik = instance.get("ik")
+    if isinstance(ik, Obj):
+        if not pts:
+            raise LayoutError("unsupported layout object")
+        lo = [min(p[k] for p in pts.values()) for k in range(3)]
+        hi = [max(p[k] for p in pts.values()) for k in range(3)]
+        center0 = [f32(f32(lo[k] + hi[k]) * f32(0.5)) for k in range(3)]
+        for name, target in ik.fields.items():
+            chain = _resolve_chain(chains, name)
+            if chain is None or any(j not in pts for j in chain):
+                continue
+            t = _ik_target(target, center0)
+            if t is None:
+                raise LayoutError("unsupported layout object")
+            pts[chain[1]], pts[chain[2]] = solve_two_bone_chain(pts[chain[0]], pts[chain[1]], pts[chain[2]], t)
+    if pts:
+        lo = [min(p[k] for p in pts.values()) for k in range(3)]
+        hi = [max(p[k] for p in pts.values()) for k in range(3)]
+    else:
+        lo = hi = [f32(0.0)] * 3
+    center = [f32(f32(lo[k] + hi[k]) * f32(0.5)) for k in range(3)]
+    half = [f32(f32(hi[k] - lo[k]) * f32(0.5)) for k in range(3)]
+    rel = {k: vec3(*_sub3(p, center)) for k, p in pts.items()}
+    out = {}
+    if "anchors" not in instance:
+        out["anchors"] = Obj(None, dict(rel))
+    if "__bounds" not in instance:
+        out["__bounds"] = vec3(*half)
+    if "pos" not in instance:
+        out["pos"] = vec3(*center)
+    out["__skeleton_joints"] = Obj(None, dict(rel))
+    out["__skeleton_bones"] = Obj(None, {k: Obj("SkeletonBone", {"start": s, "end": e}) for k, (s, e) in bones.items()})
+    if chains:
+        out["__skeleton_chains"] = Obj(None, {k: Obj("SkeletonChain", {"start": s, "mid": m, "end": e})
+                                              for k, (s, m, e) in chains.items()})
+    return out
+
+
+def _proxy_joint(pos, rot, local, radius) -> Obj:  # proxy_skeleton_joint (eval.rs:1132-1158)
+    return Obj("SkeletonJoint", {"pos": vec3(*_add3(pos, rotate_xyz(local, rot))),
+                                 "__bounds": vec3(radius, radius, radius),
+                                 "anchors": Obj(None, {"Center": vec3(0.0, 0.0, 0.0)})})
+
+
+def _proxy_bone(pos, rot, local_start, local_end, radius) -> Obj:  # proxy_skeleton_bone (eval.rs:1160-1213)
+    start = _add3(pos, rotate_xyz(local_start, rot))
+    end = _add3(pos, rotate_xyz(local_end, rot))
+    center = [f32(f32(start[k] + end[k]) * f32(0.5)) for k in range(3)]
+    half = [f32(f32(f32(abs(f32(start[k] - end[k]))) * f32(0.5)) + f32(radius)) for k in range(3)]
+    return Obj("SkeletonBone", {"pos": vec3(*center), "__bounds": vec3(*half),
+                                "anchors": Obj(None, {"Start": vec3(*_sub3(start, center)),
+                                                      "End": vec3(*_sub3(end, center)),
+                                                      "Center": vec3(0.0, 0.0, 0.0)})})
+
+
+def semantic_part_value(base, fld: str) -> Optional[Obj]:
+    """`rig.torso`, `rig.knee_l`, `rig.arm_l` of a skeleton instance (eval.rs:886-945):
+    proxies with bounds and Start / End / Center anchors in world space."""
+    if not isinstance(base, Obj) or base.type_name is None:
+        return None
+    if "__skeleton_joints" not in base.fields and "__skeleton_bones" not in base.fields:
+        return None
+    pos, rot = object_position(base), object_rotation(base)
+    joints = base.fields.get("__skeleton_joints")
+    joints = joints.fields if isinstance(joints, Obj) else {}
+    p = _num_vec3(joints.get(fld))
+    if p is not None:
+        return _proxy_joint(pos, rot, p, f32(0.04))
+    for key, radius in (("__skeleton_bones", f32(0.04)), ("__skeleton_chains", f32(0.05))):
+        table = base.fields.get(key)
+        part = table.fields.get(fld) if isinstance(table, Obj) else None
+        if isinstance(part, Obj):
+            s, e = part.fields.get("start"), part.fields.get("end")
+            if not isinstance(s, str) or not isinstance(e, str):
+                return None
+            ps, pe = _num_vec3(joints.get(s)), _num_vec3(joints.get(e))
+            if ps is None or pe is None:
+                return None
+            return _proxy_bone(pos, rot, ps, pe, radius)
+    return None
+
+
+def bind_object_to_bone(value, target) -> Obj:
+    """`.bind(bone)` (eval.rs:1662-1690): centre the object on the bone, point its +z along
+    it and stretch its length (fit_object_length_to_bone eval.rs:1692-1712)."""
+    start = object_anchor_point(target, "Start") or object_anchor_point(target, "Bottom")
+    end = object_anchor_point(target, "End") or object_anchor_point(target, "Top")
+    if start is None or end is None:
+        raise LayoutError("unsupported layout object")
+    center = [f32(f32(start[k] + end[k]) * f32(0.5)) for k in range(3)]
+    d = _sub3(end, start)
+    length = _fmax(f32(np.sqrt(_dot3(d, d))), f32(1.0e-6))
+    yaw, pitch = _yaw_pitch(d)
+    out = _with_field(value, "pos", vec3(*center))
+    out.fields["rot"] = vec3(pitch, yaw, 0.0)
+    fit = f32(length * f32(1.03))
+    if out.type_name == "Box":
+        size = _broadcast_vec3(out.fields.get("size"))
+        if size is not None:
+            out.fields["size"] = vec3(size[0], size[1], fit)
+    elif out.type_name == "Cylinder":
+        out.fields["height"] = fit
+    elif "length" in out.fields:
+        out.fields["length"] = fit
+    return out

@@ -218,3 +218,71 @@ def test_room_compiles_to_its_shown_parts():
     st = load_scene_source("let room = Room { width: 6.0, height: 3.0, depth: 5.0 };\n"
                            "let s = Sphere { radius: 0.5 }.attach(room, Top);")
     assert _vec(st, "s")[1] == f32(2.0)
+
+
+# ------------------------------------------------------------------------ skeletons
+_RIG = ("skeleton Rig {\n  joint a = vec3(0.0, 1.6, 0.0);\n  joint b = vec3(-0.3, 1.0, 0.0);\n"
+        "  bone arm = a, b;\n};\n")
+
+
+def test_skeleton_instances_and_part_proxies():  # lib.rs:2589-2635
+    st = load_scene_source(_RIG + "let rig = Rig {};\nlet j = rig.b;\nlet arm = rig.arm;")
+    rig = st.bindings["rig"].value
+    assert {"anchors", "__skeleton_joints", "__skeleton_bones", "__bounds"} <= set(rig.fields)
+    assert _vec(st, "rig") == [f32(-0.15), f32(1.3), f32(0.0)]      # bounds centre
+    j = st.bindings["j"].value
+    assert j.type_name == "SkeletonJoint" and "__bounds" in j.fields
+    assert _vec(st, "j") == [f32(-0.3), f32(1.0), f32(0.0)]
+    arm = st.bindings["arm"].value
+    assert arm.type_name == "SkeletonBone" and {"Start", "End"} <= set(arm.fields["anchors"].fields)
+
+
+def test_bind_aligns_box_long_axis_to_bone():  # lib.rs:2638-2755
+    from forgedthoughts_amd.forge.layout import rotate_xyz
+    st = load_scene_source(_RIG + "let rig = Rig {};\nlet part = Box { size: vec3(0.1, 0.1, 0.5) }.bind(rig.arm);")
+    pos, rot, size = (np.array(_vec(st, "part", k), dtype=np.float64) for k in ("pos", "rot", "size"))
+    half = size[2] * 0.5
+    ends = [pos + np.array(rotate_xyz([f32(0), f32(0), f32(s * half)], [f32(r) for r in rot]), dtype=np.float64)
+            for s in (-1, 1)]
+    a, b = np.array([0.0, 1.6, 0.0]), np.array([-0.3, 1.0, 0.0])
+    best = min(np.linalg.norm(ends[0] - a) + np.linalg.norm(ends[1] - b),
+               np.linalg.norm(ends[0] - b) + np.linalg.norm(ends[1] - a))
+    assert best < 0.08
+    assert abs(size[2] - 1.03 * np.hypot(0.3, 0.6)) < 1e-5      # stretched to 1.03 x the bone
+
+
+def test_ik_keeps_two_bone_chain_lengths():  # lib.rs:2758-2814
+    st = load_scene_source("skeleton Rig {\n  joint shoulder = vec3(0.0, 1.6, 0.0);\n"
+                           "  joint elbow = vec3(-0.2, 1.0, 0.0);\n  joint hand = vec3(-0.4, 0.5, 0.0);\n"
+                           "  chain arm = shoulder, elbow, hand;\n};\n"
+                           "let rig = Rig { ik: { arm: vec3(-0.6, 0.1, 0.2) } };")
+    joints = st.bindings["rig"].value.fields["__skeleton_joints"].fields
+    j = {k: np.array([float(v.fields[c]) for c in "xyz"]) for k, v in joints.items()}
+    assert abs(np.linalg.norm(j["shoulder"] - j["elbow"]) - 0.6324555) < 0.02
+    assert abs(np.linalg.norm(j["elbow"] - j["hand"]) - 0.53851646) < 0.02
+    assert np.linalg.norm(j["hand"] - j["elbow"] - (np.array([-0.2, -0.5, 0.0]))) > 0.05   # the chain moved
+
+
+def test_native_sdf_helpers_fold_and_lower_alike():
+    """box_shell_sdf / cylinder_*_sdf / hole_line_*_sdf (eval.rs:1943-2009): on numbers they
+    fold to the closed forms; inside a custom distance function they lower to the expression
+    DAG, which the oracle then renders."""
+    from oracle import ftref
+    st = load_scene_source("let a = box_shell_sdf(vec3(0.0, 0.0, 2.0), vec3(1.0, 1.0, 1.0), 0.1, 0.05);\n"
+                           "let b = cylinder_y_sdf(vec3(3.0, 0.0, 0.0), 1.0, 2.0);\n"
+                           "let c = cylinder_x_sdf(vec3(0.0, 0.0, 0.0), 0.5, 1.0);\n"
+                           "let d = hole_line_z_sdf(vec3(0.0, 0.0, 0.3), 0.1, 1.0, 1.0, 3.0);")
+    v = {k: float(st.bindings[k].value) for k in "abcd"}
+    assert abs(v["a"] - 1.0) < 1e-6          # outside the outer box face at z = 1
+    assert abs(v["b"] - 2.0) < 1e-6          # radial distance 3 - 1
+    assert abs(v["c"] + 0.5) < 1e-6          # inside: max(-0.5, -1) = -0.5
+    assert abs(v["d"] + 0.1) < 1e-6          # on the axis of z-holes shifted along z: -radius
+    cs = compile_scene(load_scene_source(
+        "sdf Part { fn bounds() { return vec3(1.5, 1.5, 1.5); }\n"
+        "  fn distance(p) { return min(box_shell_sdf(p, vec3(1.0, 0.8, 0.6), 0.1, 0.05),\n"
+        "                            hole_line_x_sdf(p - vec3(0.0, 0.9, 0.0), 0.08, 0.6, 0.3, 4.0)); } };\n"
+        "let camera = Camera { origin: vec3(2.0, 1.5, 4.0), target: vec3(0.0, 0.0, 0.0), fov_y: 40.0 };\n"
+        "let scene = Part {};"))
+    from forgedthoughts_amd.options import RenderOptions
+    _, t = ftref.render_depth(cs, 1, RenderOptions(32, 24, 256), 1)
+    assert (t >= 0).mean() > 0.1

@@ -662,13 +662,83 @@ def _proxy_bone(pos, rot, local_start, local_end, radius) -> Obj:  # proxy_skele
                                                       "Center": vec3(0.0, 0.0, 0.0)})})
 
 
+def _proxy(type_name, pos, rot, offset, **fields) -> Obj:  # proxy_box / proxy_cylinder (eval.rs:1065-1112)
+    out = Obj(type_name, {"pos": vec3(*_add3(pos, rotate_xyz(offset, rot))), "rot": vec3(*rot)})
+    out.fields.update(fields)
+    return out
+
+
+def _asset_part(base, fld: str) -> Optional[Obj]:
+    """The parts of the Table / Cupboard / Lamp assets (eval.rs:946-1063)."""
+    pos, rot, t = object_position(base), object_rotation(base), base.type_name
+
+    def num(name, default):
+        v = _numeric_field(base, (name,))
+        return v if v is not None else f32(default)
+
+    def clamp(x, lo, hi):
+        return min(max(x, lo), hi)
+
+    h = lambda v: f32(v * f32(0.5))  # noqa: E731
+    z = f32(0.0)
+    if t == "Table":
+        width, depth = max(num("width", 1.8), f32(0.2)), max(num("depth", 0.9), f32(0.2))
+        height = max(num("height", 0.76), f32(0.1))
+        top_t = clamp(num("top_thickness", 0.08), f32(0.01), max(height, f32(0.01)))
+        leg_r = max(num("leg_radius", 0.05), f32(0.005))
+        inset = max(num("leg_inset", 0.12), f32(0.0))
+        if fld == "top":
+            return _proxy("Box", pos, rot, [z, f32(h(height) - h(top_t)), z], size=vec3(width, top_t, depth))
+        if fld == "legs":
+            leg_hh = max(f32(f32(height - top_t) * f32(0.5)), f32(0.001))
+            leg_cy = f32(f32(-height * f32(0.5)) + leg_hh)
+            leg_x = max(f32(h(width) - inset), leg_r)
+            leg_z = max(f32(h(depth) - inset), leg_r)
+            return _proxy("Box", pos, rot, [z, leg_cy, z], size=vec3(f32(2.0) * f32(leg_x + leg_r),
+                                                                      f32(2.0) * leg_hh, f32(2.0) * f32(leg_z + leg_r)))
+    elif t == "Cupboard":
+        width, height = max(num("width", 1.6), f32(0.1)), max(num("height", 2.0), f32(0.1))
+        depth = max(num("depth", 0.6), f32(0.1))
+        wall = max(num("wall_thickness", 0.05), f32(0.005))
+        open_ = clamp(num("open_amount", 0.0), f32(0.0), f32(1.0))
+        if fld == "body":
+            return _proxy("Box", pos, rot, [z, z, z], size=vec3(width, height, depth))
+        if fld == "door":
+            door_t = max(f32(wall * f32(f32(1.0) - open_)), f32(0.001))
+            return _proxy("Box", pos, rot, [z, z, f32(h(depth) - h(door_t))], size=vec3(width, height, door_t))
+    elif t == "Lamp":
+        height = max(num("height", 0.72), f32(0.1))
+        base_r = max(num("base_radius", 0.16), f32(0.02))
+        base_h = clamp(num("base_height", 0.05), f32(0.01), max(height, f32(0.01)))
+        stem_r = max(num("stem_radius", 0.025), f32(0.005))
+        shade_r = max(num("shade_radius", 0.2), f32(0.02))
+        shade_h = clamp(num("shade_height", 0.22), f32(0.02), max(height, f32(0.02)))
+        bulb_r = max(num("bulb_radius", 0.065), f32(0.01))
+        base_cy = f32(f32(-height * f32(0.5)) + h(base_h))
+        shade_cy = f32(h(height) - h(shade_h))
+        stem_hh = f32(max(f32(f32(shade_cy - h(shade_h)) - f32(base_cy + h(base_h))), f32(0.001)) * f32(0.5))
+        stem_cy = f32(f32(f32(f32(shade_cy - h(shade_h)) + base_cy) + h(base_h)) * f32(0.5))
+        bulb_cy = f32(shade_cy - f32(shade_h * f32(0.15)))
+        if fld == "body":
+            return _proxy("Box", pos, rot, [z, z, z], size=vec3(f32(2.0) * shade_r, height, f32(2.0) * shade_r))
+        cyl = {"base": (base_cy, base_r, base_h), "stem": (stem_cy, stem_r, f32(f32(2.0) * stem_hh)),
+               "shade": (shade_cy, shade_r, shade_h)}
+        if fld in cyl:
+            cy, r, hgt = cyl[fld]
+            return _proxy("Cylinder", pos, rot, [z, cy, z], radius=r, height=hgt)
+        if fld == "bulb":  # proxy_sphere: no rotation of the offset
+            return Obj("Sphere", {"pos": vec3(pos[0], f32(pos[1] + bulb_cy), pos[2]), "radius": bulb_r})
+    return None
+
+
 def semantic_part_value(base, fld: str) -> Optional[Obj]:
-    """`rig.torso`, `rig.knee_l`, `rig.arm_l` of a skeleton instance (eval.rs:886-945):
-    proxies with bounds and Start / End / Center anchors in world space."""
+    """`rig.torso`, `rig.knee_l`, `rig.arm_l` of a skeleton instance and `table.top`,
+    `cupboard.door`, `lamp.bulb` of the library assets (eval.rs:886-1063): proxies with
+    bounds (and, for rigs, Start / End / Center anchors) in world space."""
     if not isinstance(base, Obj) or base.type_name is None:
         return None
     if "__skeleton_joints" not in base.fields and "__skeleton_bones" not in base.fields:
-        return None
+        return _asset_part(base, fld) if base.type_name in ("Table", "Cupboard", "Lamp") else None
     pos, rot = object_position(base), object_rotation(base)
     joints = base.fields.get("__skeleton_joints")
     joints = joints.fields if isinstance(joints, Obj) else {}

@@ -325,6 +325,7 @@ class SceneCompiler:
         self.functions: List[Tuple[list, int, int]] = []  # (nodes, result_type, n_params)
         self.dynamic_overrides: List[dict] = []
         self.dynamic_hooks: List[str] = []
+        self.semantic_lights: list = []  # lights of compiled Lamp assets (renderer.rs:2729-2738)
         self._hook_cache: dict = {}
         self.sorted_material_names = sorted(state.material_defs.keys())
 
@@ -530,6 +531,107 @@ class SceneCompiler:
         self.object_transforms[oid] = (transform[0].copy(), transform[1].copy())
         return oid
 
+    def compile_semantic_asset(self, tn: str, obj: Obj, default_mat: Material) -> dict:
+        """compile_lowered_library_object (renderer.rs:2413-2770): the Table, Cupboard and
+        Lamp library definitions render as fixed primitive assemblies (lower_*_asset), each
+        part offset in the object's frame; a Lamp adds a sphere light at its bulb."""
+        base = read_transform(obj)
+
+        def num(name, default):
+            return _or(read_number_field(obj, [name]), f32(default))
+
+        def material(specific, fallback=None):  # object_material_id(_with_fallback)
+            for name in (specific, fallback):
+                mv = obj.fields.get(name) if name else None
+                if isinstance(mv, Obj):
+                    return self.intern_material(self.material_from_object(mv, True))
+            return self.primitive_material_id(obj, default_mat)
+
+        h = lambda v: f32(v * f32(0.5))  # noqa: E731
+        specs = []
+        if tn == "Table":
+            width, depth = max(num("width", 1.8), f32(0.2)), max(num("depth", 0.9), f32(0.2))
+            height = max(num("height", 0.76), f32(0.1))
+            top_t = _clamp(num("top_thickness", 0.08), f32(0.01), max(height, f32(0.01)))
+            leg_r = max(num("leg_radius", 0.05), f32(0.005))
+            inset = max(num("leg_inset", 0.12), f32(0.0))
+            top_mat, leg_mat = material("top_material"), material("leg_material")
+            specs.append(("box", (0.0, f32(h(height) - h(top_t)), 0.0), (h(width), h(top_t), h(depth)), top_mat))
+            leg_hh = max(f32(f32(height - top_t) * f32(0.5)), f32(0.001))
+            leg_cy = f32(f32(-height * f32(0.5)) + leg_hh)
+            leg_x = max(f32(h(width) - inset), leg_r)
+            leg_z = max(f32(h(depth) - inset), leg_r)
+            for sx in (-1.0, 1.0):
+                for sz in (-1.0, 1.0):
+                    specs.append(("cyl", (f32(sx * leg_x), leg_cy, f32(sz * leg_z)), (leg_r, leg_hh), leg_mat))
+            root = ("union", specs)
+        elif tn == "Cupboard":
+            width, height = max(num("width", 1.6), f32(0.1)), max(num("height", 2.0), f32(0.1))
+            depth = max(num("depth", 0.6), f32(0.1))
+            wall = max(num("wall_thickness", 0.05), f32(0.005))
+            open_ = _clamp(num("open_amount", 0.0), f32(0.0), f32(1.0))
+            body, door_mat = material("body_material"), material("door_material")
+            inner = tuple(max(f32(h(v) - wall), f32(0.001)) for v in (width, height, depth))
+            shell = ("sub", ("box", (0.0, 0.0, 0.0), (h(width), h(height), h(depth)), body),
+                     ("box", (0.0, 0.0, f32(-wall)), inner, body))
+            door_t = max(f32(wall * f32(f32(1.0) - open_)), f32(0.001))
+            door = ("box", (0.0, 0.0, f32(h(depth) - h(door_t))), (h(width), h(height), h(door_t)), door_mat)
+            root = ("union", [shell, door])
+        else:
+            height = max(num("height", 0.72), f32(0.1))
+            base_r = max(num("base_radius", 0.16), f32(0.02))
+            base_h = _clamp(num("base_height", 0.05), f32(0.01), max(height, f32(0.01)))
+            stem_r = max(num("stem_radius", 0.025), f32(0.005))
+            shade_r = max(num("shade_radius", 0.2), f32(0.02))
+            shade_h = _clamp(num("shade_height", 0.22), f32(0.02), max(height, f32(0.02)))
+            bulb_r = max(num("bulb_radius", 0.065), f32(0.01))
+            base_mat = material("base_material", "body_material")
+            stem_mat = material("stem_material", "body_material")
+            shade_mat = material("shade_material", "body_material")
+            bulb_mat = material("bulb_material")
+            inten = _or(_read_light_spectrum(obj, ["light_intensity"], "light_color", "light_intensity"),
+                        _v(18.0, 17.2, 15.6))
+            light_r = max(_or(read_number_field(obj, ["light_radius"]), max(f32(bulb_r * f32(0.85)), f32(0.02))),
+                          f32(0.0))
+            ls = read_number_field(obj, ["light_samples"])
+            samples = int(max(ls, f32(1.0))) if ls is not None else 6
+            base_cy = f32(f32(-height * f32(0.5)) + h(base_h))
+            shade_cy = f32(h(height) - h(shade_h))
+            stem_hh = f32(max(f32(f32(shade_cy - h(shade_h)) - f32(base_cy + h(base_h))), f32(0.001)) * f32(0.5))
+            stem_cy = f32(f32(f32(f32(shade_cy - h(shade_h)) + base_cy) + h(base_h)) * f32(0.5))
+            bulb_cy = f32(shade_cy - f32(shade_h * f32(0.15)))
+            if inten[0] > 0 or inten[1] > 0 or inten[2] > 0:
+                pos = (base[0] + _transform_offset(base, _v(0.0, bulb_cy, 0.0))).astype(np.float32)
+                self.semantic_lights.append((abi.LIGHT_SPHERE, pos, inten, light_r, samples))
+            root = ("union", [("cyl", (0.0, base_cy, 0.0), (base_r, h(base_h)), base_mat),
+                              ("cyl", (0.0, stem_cy, 0.0), (stem_r, stem_hh), stem_mat),
+                              ("cyl", (0.0, shade_cy, 0.0), (shade_r, h(shade_h)), shade_mat),
+                              ("sph", (0.0, bulb_cy, 0.0), (bulb_r,), bulb_mat)])
+        return self._instantiate_lowered(root, base)
+
+    def _instantiate_lowered(self, spec, base) -> dict:
+        """instantiate_lowered_node (renderer.rs:2480-2559)."""
+        kind = spec[0]
+        if kind == "union":
+            parts = [self._instantiate_lowered(p, base) for p in spec[1]]
+            root = parts[0]
+            for part in parts[1:]:
+                root = dict(kind=abi.NODE["UNION"], children=[root, part])
+            return root
+        if kind == "sub":
+            return dict(kind=abi.NODE["SUBTRACT"],
+                        children=[self._instantiate_lowered(spec[1], base), self._instantiate_lowered(spec[2], base)])
+        _, off, dims, mid = spec
+        tr = ((base[0] + _transform_offset(base, _v(*off))).astype(np.float32), base[1].copy())
+        oid = self.alloc_object(tr)
+        if kind == "box":
+            return dict(kind=abi.NODE["BOX"], transform=tr, param=[*dims, f32(0.0), f32(0.0)], object_id=oid,
+                        material_id=mid)
+        if kind == "cyl":
+            return dict(kind=abi.NODE["CYLINDER"], transform=tr, param=[*dims, f32(0.0), f32(0.0)], object_id=oid,
+                        material_id=mid)
+        return dict(kind=abi.NODE["SPHERE"], transform=tr, param=[dims[0], f32(0.0)], object_id=oid, material_id=mid)
+
     def compile_room(self, obj: Obj, default_mat: Material) -> dict:
         """compile_room (renderer.rs:3359-3520): the shown floor, ceiling and walls as boxes
         offset from the room centre along world axes, joined by hard unions in that order."""
@@ -632,6 +734,8 @@ class SceneCompiler:
                         transform=transform, param=params, object_id=oid, material_id=mid)
         elif tn == "Room":
             node = self.compile_room(obj, default_mat)
+        elif tn in ("Table", "Cupboard", "Lamp") and tn in self.state.sdf_defs:
+            node = self.compile_semantic_asset(tn, obj, default_mat)
         elif tn in self.state.sdf_defs:
             block = self.state.sdf_defs[tn]
             transform = read_transform(obj)
@@ -897,10 +1001,10 @@ def parse_material(comp: SceneCompiler, root) -> Material:
     return m if m is not None else default_material()
 
 
-def _read_light_spectrum(obj: Obj, legacy):
-    col = read_spectrum_field(obj, "color")
+def _read_light_spectrum(obj: Obj, legacy, color_field="color", scalar_field="intensity"):
+    col = read_spectrum_field(obj, color_field)
     if col is not None:
-        inten = _or(read_number_field(obj, ["intensity"]), f32(1.0))
+        inten = _or(read_number_field(obj, [scalar_field]), f32(1.0))
         return (col * max(inten, f32(0.0))).astype(np.float32)
     for f in legacy:
         s = read_spectrum_field(obj, f)
@@ -909,7 +1013,9 @@ def _read_light_spectrum(obj: Obj, legacy):
     return None
 
 
-def parse_lights(state: E.EvalState) -> list:
+def parse_lights(state: E.EvalState, semantic_lights=()) -> list:
+    """parse_lights (renderer.rs:6288-6380); `semantic_lights` (the bulbs of compiled Lamp
+    assets) follow the scene's own lights as sphere lights."""
     lights = []
     for b in state.bindings.values():
         v = b.value
@@ -929,6 +1035,7 @@ def parse_lights(state: E.EvalState) -> list:
             sv = read_number_field(v, ["samples"])
             samples = int(max(sv, f32(1.0))) if sv is not None else 8
             lights.append((abi.LIGHT_SPHERE, pos, rad, radius, samples))
+    lights.extend(semantic_lights)
     if not lights:
         lights.append((abi.LIGHT_POINT, _v(3, 3, 6), _v(6, 6, 6), 0.0, 1))
         lights.append((abi.LIGHT_ENV, _v(0, 0, 0), _v(0.1, 0.1, 0.1), 0.0, 1))
@@ -1127,7 +1234,7 @@ def compile_scene(state: E.EvalState) -> CompiledScene:
     if env_name is not None:
         env_fn = comp.lower_block_fn(state.environment_defs[env_name], "color", [1], None)
 
-    lights = parse_lights(state)
+    lights = parse_lights(state, comp.semantic_lights)
     light_arr = np.zeros(len(lights), dtype=abi.LIGHT_DTYPE)
     for i, (kind, pos, rad, radius, samples) in enumerate(lights):
         light_arr[i]["kind"] = kind

@@ -30,8 +30,8 @@ def _load(name):
 
 
 def test_example_fixtures_present():
-    assert len(EXAMPLES) >= 40
-    for n in ("boolean_union_round", "boolean_tongue", "stone", "room_statue_test", "imports", "ft_material_wax"):
+    assert len(EXAMPLES) >= 45
+    for n in ("boolean_union_round", "boolean_tongue", "stone", "room", "robot_on_stone", "imports", "ft_material_wax"):
         assert n in NAMES
 
 

@@ -286,3 +286,34 @@ def test_native_sdf_helpers_fold_and_lower_alike():
     from forgedthoughts_amd.options import RenderOptions
     _, t = ftref.render_depth(cs, 1, RenderOptions(32, 24, 256), 1)
     assert (t >= 0).mean() > 0.1
+
+
+def test_layout_against_semantic_part_proxies():  # lib.rs:1534-1558
+    st = load_scene_source("var table = Table { width: 1.6, depth: 0.8, height: 0.75, top_thickness: 0.1 };\n"
+                           "var vase = Sphere { radius: 0.2 }.attach(table.top, Top);")
+    assert _vec(st, "vase")[1] == f32(0.575)
+
+
+def test_semantic_assets_compile_to_their_parts():
+    """compile_lowered_library_object (renderer.rs:2413-2770): a Table is a top box and four
+    leg cylinders, a Cupboard a shell (box minus cavity) and a door, a Lamp three cylinders
+    and a bulb sphere plus a sphere light at the bulb."""
+    defs = ("sdf Table { fn bounds() { return vec3(1.0); } fn distance(p) { return length(p) - 1.0; } };\n"
+            "sdf Cupboard { fn bounds() { return vec3(1.0); } fn distance(p) { return length(p) - 1.0; } };\n"
+            "sdf Lamp { fn bounds() { return vec3(1.0); } fn distance(p) { return length(p) - 1.0; } };\n")
+    kinds = lambda cs: sorted(abi.NODE_KINDS[int(n["kind"])] for n in cs.nodes)  # noqa: E731
+    t = compile_scene(load_scene_source(defs + "let scene = Table { width: 1.6, height: 0.75 };"))
+    assert kinds(t) == sorted(["BOX"] + ["CYLINDER"] * 4 + ["UNION"] * 4)
+    top = [n for n in t.nodes if n["kind"] == abi.NODE["BOX"]][0]
+    assert abs(float(t.object_transforms[top["object_id"], 1]) - (0.375 - 0.04)) < 1e-6
+    c = compile_scene(load_scene_source(defs + "let scene = Cupboard { };"))
+    assert kinds(c) == sorted(["BOX"] * 3 + ["SUBTRACT", "UNION"])
+    n_lights = len(compile_scene(load_scene_source(defs + "let k = PointLight { position: vec3(0.0, 3.0, 0.0) };\n"
+                                                          "let scene = Sphere { radius: 1.0 };")).lights)
+    lamp = compile_scene(load_scene_source(defs + "let k = PointLight { position: vec3(0.0, 3.0, 0.0) };\n"
+                                                  "var lamp = Lamp { height: 0.72 }; lamp.pos.x = 1.0;\nlet scene = lamp;"))
+    assert kinds(lamp) == sorted(["CYLINDER"] * 3 + ["SPHERE"] + ["UNION"] * 3)
+    assert len(lamp.lights) == n_lights + 1
+    bulb = lamp.lights[-1]
+    assert bulb["kind"] == abi.LIGHT_SPHERE and abs(float(bulb["position"][0]) - 1.0) < 1e-6
+    assert np.allclose(bulb["radiance"], [18.0, 17.2, 15.6])

@@ -192,9 +192,9 @@ __device__ __noinline__ float3 eval_fn(int slot, int fn, const float3* params) {
       }
       default: r = mk(0, 0, 0); break;
     }
-    v[k] = r;
+    v[(int)e.pad] = r;  // the node's value slot (assigned at scene build)
   }
-  return v[count - 1];
+  return v[(int)S.exprs[F.first + count - 1].pad];
 }
 
 // eval_custom_sdf_distance (renderer.rs:6139-6179)

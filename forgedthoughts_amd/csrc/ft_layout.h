@@ -12,7 +12,7 @@ namespace ftd {
 
 constexpr int kMaxStack = 24;      // value-stack depth of the node program
 constexpr int kMaxPointStack = 4;  // nested modifier depth
-constexpr int kMaxExpr = 96;       // nodes per Forge function
+constexpr int kMaxExpr = 96;       // values live at once in a Forge function
 constexpr int kMaxWalk = 48;       // frames of the per-hit tree walks
 constexpr int kMaxWalkMats = 12;   // pending materials of the material walk
 constexpr int kMaxBvhStack = 48;

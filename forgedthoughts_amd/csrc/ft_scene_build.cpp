@@ -11,6 +11,7 @@
 #include "ft_scene_build.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -98,7 +99,7 @@ bool validate(const ft_scene_ir* ir, std::string& err) {
   }
   for (uint32_t f = 0; f < ir->n_functions; ++f) {
     const ft_function& F = ir->functions[f];
-    if (F.count == 0 || F.first + F.count > ir->n_exprs || F.count > (uint32_t)ftd::kMaxExpr) {
+    if (F.count == 0 || F.first + F.count > ir->n_exprs) {
       err = "function " + std::to_string(f) + " has an invalid or too long expression range";
       return false;
     }
@@ -573,7 +574,53 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
   // functions
   if (ir->n_functions) S.fns.assign(ir->functions, ir->functions + ir->n_functions);
   if (ir->n_exprs) S.exprs.assign(ir->exprs, ir->exprs + ir->n_exprs);
-  for (const ft_function& F : S.fns) S.max_fn_exprs = std::max(S.max_fn_exprs, (int)F.count);
+  // The interpreter holds live values only.  Each node's result goes to a value slot that a
+  // value past its last use has freed, so a function may have any number of nodes while at
+  // most kMaxExpr values are live at once.  The device copy of the expressions names slots:
+  // operands are rewritten to the slots of the nodes they read, and the slot a node writes
+  // is kept in its `pad` field (a small integer, exact as a float).
+  for (uint32_t f = 0; f < (uint32_t)S.fns.size(); ++f) {
+    const ft_function& F = S.fns[f];
+    const int n = (int)F.count;
+    std::vector<int> last(n, -1), slot(n, -1), free_slots;
+    auto refs = [](const ft_expr& e) {
+      std::array<int32_t, 4> r{e.a, e.b, e.c, e.d};
+      if (e.op == FT_EXPR_PARAM) r[0] = -1;  // a parameter index, not a node
+      return r;
+    };
+    for (int k = 0; k < n; ++k)
+      for (int32_t o : refs(S.exprs[F.first + k]))
+        if (o >= 0) last[o] = k;
+    last[n - 1] = n;  // the result
+    int top = 0;
+    for (int k = 0; k < n; ++k) {
+      ft_expr& e = S.exprs[F.first + k];
+      const std::array<int32_t, 4> r = refs(e);
+      int32_t* fields[4] = {&e.a, &e.b, &e.c, &e.d};
+      for (int q = 0; q < 4; ++q)
+        if (r[q] >= 0) *fields[q] = slot[r[q]];
+      for (int q = 0; q < 4; ++q) {  // operands read for the last time free their slots
+        if (r[q] < 0 || last[r[q]] != k) continue;
+        bool seen = false;
+        for (int p = 0; p < q; ++p) seen = seen || r[p] == r[q];
+        if (!seen) free_slots.push_back(slot[r[q]]);
+      }
+      if (free_slots.empty()) {
+        slot[k] = top++;
+      } else {
+        slot[k] = free_slots.back();
+        free_slots.pop_back();
+      }
+      if (top > ftd::kMaxExpr) {
+        err = "function " + std::to_string(f) + " keeps more than " + std::to_string(ftd::kMaxExpr) +
+              " values live";
+        return false;
+      }
+      e.pad = (float)slot[k];
+      if (last[k] < 0) free_slots.push_back(slot[k]);  // never read
+    }
+    S.max_fn_exprs = std::max(S.max_fn_exprs, top);
+  }
   return true;
 }
 

@@ -81,7 +81,10 @@ def test_example_gpu_parity(name):
     from forgedthoughts_amd.scene import RenderError
     from test_gpu_parity import depth_parity, whitted_parity
     scene, opts, accel = _load(name)
-    depth_parity(scene, accel, opts, 1, name)
+    # the refine step bisects the last march step six times, so when GPU and CPU rounding
+    # lead to march steps of different lengths the hit t can differ by a fraction of the
+    # step; the examples' own epsilon bounds that for their long custom distance functions
+    depth_parity(scene, accel, opts, 1, name, t_tol=max(1.0e-4, opts.epsilon))
     if _whitted_refused(scene):
         with GpuScene(scene, accel) as gs:
             with pytest.raises(RenderError) as e:

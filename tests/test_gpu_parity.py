@@ -49,7 +49,7 @@ def check_path(gs, emit):
         assert flags == want, f"specialised code requested: flags {flags}, expected {want}"
 
 
-def depth_parity(scene, accel, opts, aa=1, label="", emit=None):
+def depth_parity(scene, accel, opts, aa=1, label="", emit=None, t_tol=T_TOL):
     print(f"[depth parity] {label} accel={int(accel)} {opts.width}x{opts.height} aa={aa} emit={emit}", flush=True)
     rgb_c, t_c = ftref.render_depth(scene, accel, opts, aa)
     with GpuScene(scene, accel) as gs:
@@ -62,7 +62,7 @@ def depth_parity(scene, accel, opts, aa=1, label="", emit=None):
     dt = np.abs(t_c[both] - t_g[both])
     assert mism <= max(1, int(GRAZING_FRAC * n)), f"{label}: mask mismatch {mism}/{n}"
     if both.any():
-        assert float(dt.max()) <= T_TOL, f"{label}: max |dt| {dt.max()}"
+        assert float(dt.max()) <= t_tol, f"{label}: max |dt| {dt.max()}"
     same = (hit_c == hit_g)
     db = np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2)
     assert int(db[same].max()) <= 1, f"{label}: byte diff {db[same].max()}"

@@ -78,8 +78,10 @@ struct DHooks {
   int color, roughness, ior, thin_walled, emission_color, emission_strength;
   int metallic, specular, specular_weight, transmission, specular_color, clearcoat_roughness, clearcoat, coat_mask;
   int bump, normal;
+  int medium_ior, medium_absorption, medium_density;
+  int pad;
 };
-static_assert(sizeof(DHooks) == 64, "DHooks layout");
+static_assert(sizeof(DHooks) == 80, "DHooks layout");
 
 struct DLight {
   int kind;

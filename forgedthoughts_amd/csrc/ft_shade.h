@@ -146,6 +146,12 @@ __device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float
       if (H.clearcoat >= 0) a.clearcoat = clampf(eval_fn(slot, H.clearcoat, args).x, 0.0f, 1.0f);
       if (H.coat_mask >= 0)
         a.clearcoat = clampf(a.clearcoat * clampf(eval_fn(slot, H.coat_mask, args).x, 0.0f, 1.0f), 0.0f, 1.0f);
+      if (H.medium_ior >= 0) {  // medium_from_value (renderer.rs:6834-6845) of the hook's Medium
+        a.has_medium = 1.0f;
+        a.medium_ior = clampf(eval_fn(slot, H.medium_ior, args).x, 1.0f, 3.0f);
+        set3(a.medium_absorption, eval_fn(slot, H.medium_absorption, args));
+        a.medium_density = fmaxf(eval_fn(slot, H.medium_density, args).x, 0.0f);
+      }
     } else {  // resolve_pattern_color (renderer.rs:7855-7873)
       const int sx = (int)floorf(lp.x * a.pattern_scale), sy = (int)floorf(lp.y * a.pattern_scale),
                 sz = (int)floorf(lp.z * a.pattern_scale);

@@ -685,6 +685,31 @@ class Evaluator:
             res = lo.add("ADD" if res.ty == 0 else "ADD", res.ty, res.idx, lo.const(f32(0.0)).idx)
         return lo.nodes, res.ty
 
+    def lower_function_fields(self, params, body, param_types, fields, block=None, overrides=None):
+        """Lower a function returning an object to one DAG per requested field: `fields` is
+        [(name, default)], a field the object lacks taking its default.  Returns
+        [(nodes, result_type)] per field, or None when the function does not return an
+        object."""
+        lo = Lowering()
+        args = []
+        for i, (pname, ty) in enumerate(zip(params, param_types)):
+            if ty == "ctx":
+                ctx = Obj(None, {})
+                for k, (fname, fty) in enumerate(CTX_FIELDS):
+                    ctx.fields[fname] = lo.param(k, fty)
+                args.append(ctx)
+            else:
+                args.append(lo.param(i, ty))
+        out = self.apply(params, body, args, Scope({}, block, overrides, lo, 0))
+        if not isinstance(out, Obj) or out.type_name == "vec3":
+            return None
+        result = []
+        for name, default in fields:
+            res = lo.lift(out.fields.get(name, default))
+            res = lo.add("ADD", res.ty, res.idx, lo.const(f32(0.0)).idx)
+            result.append((list(lo.nodes), res.ty))
+        return result
+
     def eval_block_zero_arg(self, block: S.BlockDef, fn_name: str, overrides: Optional[Obj]):
         for st in block.statements:
             if st[0] == "fn" and st[1] == fn_name and not st[2]:

@@ -39,19 +39,24 @@ from .forge.evaluator import Obj, Func, is_num, f32
 # resolve_leaf_material (renderer.rs:7164-7536)
 HOOK_NAMES = ("color", "roughness", "ior", "thin_walled", "emission_color", "emission_strength",
               "metallic", "specular", "specular_weight", "transmission", "specular_color",
-              "clearcoat_roughness", "clearcoat", "coat_mask", "bump", "normal")
+              "clearcoat_roughness", "clearcoat", "coat_mask", "bump", "normal",
+              "medium_ior", "medium_absorption", "medium_density")
+# the medium hook returns a Medium object: its fields, with medium_from_value's defaults
+# (renderer.rs:6834-6845), become three functions
+MEDIUM_FIELDS = (("ior", 1.0), ("absorption_color", None), ("density", 0.0))
 SPECTRUM_HOOKS = ("color", "emission_color", "specular_color", "normal")
 # the hooks each definition model reads: the Metal arm and the Dielectric / other arms
 # (renderer.rs:7472-7530), the Standard / Layered block (7259-7470), whose coat hooks replace
 # the ior, specular colour, clearcoat roughness and clearcoat weight, and for every model
-# the bump and normal hooks of build_bsdf_context (4397-4449)
-_NORMAL = {"bump": "bump", "normal": "normal"}
+# the bump and normal hooks of build_bsdf_context (4397-4449) and the medium hook
+_NORMAL = {"bump": "bump", "normal": "normal", "medium_ior": "medium", "medium_absorption": "medium",
+           "medium_density": "medium"}
 _ARM_HOOKS = {
     "Metal": dict({"color": "color", "roughness": "roughness", "emission_color": "emission_color",
                    "emission_strength": "emission_strength"}, **_NORMAL),
     "other": dict({h: h for h in HOOK_NAMES[:6]}, **_NORMAL),
-    "Standard": dict({h: h for h in HOOK_NAMES}, ior="coat_ior", specular_color="coat_color",
-                     clearcoat_roughness="coat_roughness", clearcoat="coat_weight"),
+    "Standard": dict({h: h for h in HOOK_NAMES[:14]}, ior="coat_ior", specular_color="coat_color",
+                     clearcoat_roughness="coat_roughness", clearcoat="coat_weight", **_NORMAL),
 }
 
 
@@ -62,14 +67,14 @@ def arm_hooks(model: str) -> dict:
     return _ARM_HOOKS["Metal" if model == "Metal" else "other"]
 
 
-# hooks the Whitted render (renderer/ray.rs) reads that the GPU does not evaluate: eval
-# replaces the BSDF value in shade_color (renderer.rs:4217-4276, 4905-4920), medium enters
-# transition_medium (5125-5161, ray.rs:179).  A material with one of them keeps its
-# dynamic id without a hook table entry, and Whitted renders of its scene fail loudly.  pdf / sample serve only the path tracer and
+# a hook the Whitted render (renderer/ray.rs) reads that the GPU does not evaluate: eval
+# replaces the BSDF value in shade_color (renderer.rs:4217-4276, 4905-4920).  A material
+# with it keeps its dynamic id without a hook table entry, and Whitted renders of its
+# scene fail loudly.  pdf / sample serve only the path tracer and
 # subsurface nothing the renderers read; like hooks a model's arm does not read (a
 # Metal's ior) and helper functions of the shading context they leave the image unchanged,
 # and a material with only such hooks compiles as a static one.
-REFUSED_HOOKS = ("eval", "medium")
+REFUSED_HOOKS = ("eval",)
 MATERIAL_HOOKS = ("color", "roughness", "metallic", "specular", "ior", "thin_walled", "medium", "subsurface",
                   "emission", "emission_color", "emission_strength", "bump", "normal", "eval", "pdf", "sample",
                   "transmission", "clearcoat", "clearcoat_roughness", "specular_color")
@@ -474,7 +479,11 @@ class SceneCompiler:
         arm = arm_hooks(model)
         ids = []
         try:
+            medium = self.lower_medium_hook(d, overrides)
             for h in HOOK_NAMES:
+                if h.startswith("medium_"):
+                    ids.append(medium[("medium_ior", "medium_absorption", "medium_density").index(h)])
+                    continue
                 fid = self.lower_block_fn(d, arm[h], ["ctx"], overrides) if h in arm else -1
                 if fid >= 0 and h not in SPECTRUM_HOOKS and self.functions[fid][1] != 0:
                     fid = -1
@@ -507,6 +516,19 @@ class SceneCompiler:
     def add_function(self, nodes, result_type, n_params) -> int:
         self.functions.append((nodes, result_type, n_params))
         return len(self.functions) - 1
+
+    def lower_medium_hook(self, block, overrides) -> tuple:
+        """The medium hook's Medium object as three functions (ior, absorption colour,
+        density); (-1, -1, -1) without the hook or when it returns no object."""
+        for st in block.statements:
+            if st[0] == "fn" and st[1] == "medium" and len(st[2]) == 1:
+                white = E.vec3(1.0, 1.0, 1.0)
+                fields = [(k, white if v is None else f32(v)) for k, v in MEDIUM_FIELDS]
+                lowered = self.ev.lower_function_fields(st[2], st[3], ["ctx"], fields, block, overrides)
+                if lowered is None:
+                    return (-1, -1, -1)
+                return tuple(self.add_function(nodes, rt, abi.HOOK_PARAMS) for nodes, rt in lowered)
+        return (-1, -1, -1)
 
     def lower_block_fn(self, block, fn_name, param_types, overrides) -> int:
         for st in block.statements:

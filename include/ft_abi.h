@@ -41,7 +41,7 @@ typedef __hip_internal::uint8_t uint8_t;
 extern "C" {
 #endif
 
-#define FT_ABI_VERSION 4u
+#define FT_ABI_VERSION 5u
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -275,7 +275,9 @@ typedef struct ft_material_hooks {
   int32_t coat_mask;
   int32_t bump;               /* any model: the shading normal of build_bsdf_context */
   int32_t normal;             /* (renderer.rs:4397-4449, 7749-7827), a number and a vector */
-  int32_t pad;
+  int32_t medium_ior;         /* the fields of the Medium object a medium hook returns */
+  int32_t medium_absorption;  /* (medium_from_value renderer.rs:6834-6845): all three or none */
+  int32_t medium_density;
 } ft_material_hooks;
 
 /* -------------------------------------------------------------- lights */

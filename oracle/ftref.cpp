@@ -964,7 +964,8 @@ bool validate(const ft_scene_ir* ir) {
     if (H.material >= ir->n_materials) { g_err = "material hook names no material"; return false; }
     for (int32_t f : {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                       H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
-                      H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal})
+                      H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal,
+                      H.medium_ior, H.medium_absorption, H.medium_density})
       if (f >= (int32_t)ir->n_functions) { g_err = "material hook names no function"; return false; }
   }
   for (uint32_t i = 0; i < ir->n_nodes; ++i) {
@@ -1434,6 +1435,13 @@ template <class T> Mat<T> resolve_leaf_material(const Ctx<T>& c, Mat<T> m, const
     if (H.clearcoat_roughness >= 0) m.p.clearcoat_roughness = rclamp(S.fn(H.clearcoat_roughness, args).x, T(0), T(1));
     if (H.clearcoat >= 0) m.p.clearcoat = rclamp(S.fn(H.clearcoat, args).x, T(0), T(1));
     if (H.coat_mask >= 0) m.p.clearcoat = rclamp(m.p.clearcoat * rclamp(S.fn(H.coat_mask, args).x, T(0), T(1)), T(0), T(1));
+    if (H.medium_ior >= 0) {  // medium_from_value (R:6834-6845) of the hook's Medium object
+      const V3<T> ab = S.fn(H.medium_absorption, args);
+      m.p.has_medium = true;
+      m.p.medium_ior = rclamp(S.fn(H.medium_ior, args).x, T(1), T(3));
+      m.p.medium_absorption = Spec<T>(ab.x, ab.y, ab.z);
+      m.p.medium_density = rmax(S.fn(H.medium_density, args).x, T(0));
+    }
   } else {
     m.p.color = pattern_color(m.p, lp);
   }

@@ -238,6 +238,8 @@ material Glass {
   model: Dielectric;
   fn ior(ctx) { return 1.2 + 0.4 * clamp(ctx.view_dir.y + 0.5, 0.0, 1.0); }
   fn thin_walled(ctx) { return step(0.3, ctx.local_position.x); }
+  fn medium(ctx) { return Medium { ior: 1.3, absorption_color: vec3(0.7, 0.85, 1.0),
+                                   density: 0.2 + 0.6 * step(0.0, ctx.local_position.y) }; }
 };
 let floor_mat = Lambert { color: #d7dce5 };
 var floor = Box { size: vec3(10.0, 0.5, 10.0), material: floor_mat };
@@ -270,7 +272,7 @@ def test_material_hooks(emit, monkeypatch):
     """Per-hit hooks of Lambert-, Metal- and Dielectric-model definitions (colour, roughness,
     ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530), of a
     Standard-model one (metallic, specular weight and the coat hooks, renderer.rs:7259-7470)
-    and bump / normal hooks (renderer.rs:7749-7827),
+    bump / normal hooks (renderer.rs:7749-7827) and a medium hook (renderer.rs:6834-6845),
     evaluated on the GPU with the shading context of each hit, against the oracle; and the
     reference's own hooked regression scene in Whitted."""
     monkeypatch.setenv("FT_EMIT", emit)

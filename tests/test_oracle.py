@@ -414,12 +414,36 @@ def test_bump_and_normal_hooks_reshape_the_shading_normal():
     assert np.abs(ha - hb).max() > 1e-3
 
 
+def test_medium_hook_matches_a_static_medium():
+    """A medium hook returning a constant Medium object gives the image of the static
+    definition with that medium (medium_from_value renderer.rs:6834-6845: ior clamped to
+    [1, 3], absorption colour, density >= 0; missing fields take 1, white, 0), and the image
+    depends on it (the refracted rays cross the absorbing medium, transition_medium)."""
+    lit = "let key = PointLight { position: vec3(0.0, 2.0, 4.0), color: #ffffff, intensity: 20.0 };\n"
+    scene = ("var w = Box { size: vec3(20.0, 20.0, 1.0), material: Lambert { color: #d0c0a0 } }; w.pos.z = -3.0;\n"
+             "let scene = w + Sphere { radius: 1.0, material: M {} };")
+
+    def render(body):
+        sc = compile_src(_axis_camera(lit + f"material M {{ model: Dielectric; ior = 1.5; {body} }};\n" + scene))
+        return sc, ftref.render_whitted(sc, AccelMode.Bvh, _OPTS9, RaySettings(4, 9, 1))
+
+    hooked, (a, ha) = render("fn medium(ctx) { return Medium { ior: 1.4, absorption_color: vec3(0.5, 0.6, 0.7), "
+                             "density: 0.3 }; }")
+    assert len(hooked.material_hooks) == 1 and hooked.material_hooks[0]["medium_density"] >= 0
+    _, (b, hb) = render("medium = Medium { ior: 1.4, absorption_color: vec3(0.5, 0.6, 0.7), density: 0.3 };")
+    assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0
+    _, (c, hc) = render("")
+    assert np.abs(ha - hc).max() > 1e-3
+    _, (d, hd) = render("fn medium(ctx) { return Medium { density: 0.0 }; }")
+    _, (e, he) = render("medium = Medium { ior: 1.0, absorption_color: vec3(1.0, 1.0, 1.0), density: 0.0 };")
+    assert np.abs(hd - he).max() == 0.0
+
+
 def test_hooks_outside_the_whitted_arms_leave_materials_static():
     """pdf / sample (path tracer only) and subsurface hooks, hooks a model's arm does not
     read (a Metal's ior) and helper functions are never called by the Whitted render: such a
-    material compiles as its static parameters.  eval / medium hooks are read
-    (renderer/ray.rs:135, 179) but not evaluated on the GPU: they keep the material dynamic
-    without a hook entry."""
+    material compiles as its static parameters.  An eval hook is read (renderer/ray.rs:135)
+    but not evaluated on the GPU: it keeps the material dynamic without a hook entry."""
     lit = "let key = PointLight { position: vec3(0.0, 0.0, 3.0), color: #ffffff, intensity: 9.0 };\n"
     wall = "var w = Box { size: vec3(20.0, 20.0, 1.0), material: M {} }; w.pos.z = -0.5; let scene = w;"
     hooked = compile_src(_axis_camera(
@@ -433,7 +457,7 @@ def test_hooks_outside_the_whitted_arms_leave_materials_static():
     a, ha = ftref.render_whitted(hooked, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
     b, hb = ftref.render_whitted(static, AccelMode.Bvh, _OPTS9, RaySettings(2, 9, 1))
     assert np.array_equal(a, b) and np.abs(ha - hb).max() == 0.0 and ha.max() > 0.0
-    for hook in ("fn eval(ctx) { return #ffffff; }", "fn medium(ctx) { return 0.0; }"):
+    for hook in ("fn eval(ctx) { return #ffffff; }",):
         read = compile_src(_axis_camera(lit + f"material M {{ model: Metal; {hook} }};\n" + wall))
         assert len(read.material_hooks) == 0, hook
         assert any(int(m["dynamic_material_id"]) >= 0 for m in read.materials), hook

@@ -72,11 +72,14 @@ def _whitted_refused(scene) -> bool:
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("emit", ["auto", "0"], ids=["default", "interp"])
 @pytest.mark.parametrize("name", NAMES)
-def test_example_gpu_parity(name):
+def test_example_gpu_parity(name, emit, monkeypatch):
     """Depth preview and Whitted (3 bounces, 1 spp) of one example, GPU against the oracle,
-    at 64x48 with the example's own march settings; Whitted of an example with hooks the GPU
-    does not evaluate must raise UnsupportedObjectType."""
+    at 64x48 with the example's own march settings, on the default path (scene-specialised
+    code where the scene allows it) and on the interpreter; Whitted of an example with hooks
+    the GPU does not evaluate must raise UnsupportedObjectType."""
+    monkeypatch.setenv("FT_EMIT", emit)
     from forgedthoughts_amd.render import GpuScene
     from forgedthoughts_amd.scene import RenderError
     from test_gpu_parity import depth_parity, whitted_parity

@@ -10,7 +10,7 @@ import ctypes as C
 
 import numpy as np
 
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 ACCEL_NAIVE, ACCEL_BVH, ACCEL_BRICKS = 0, 1, 2
 
@@ -63,8 +63,9 @@ HOOKS_DTYPE = np.dtype([
     ("specular_weight", "<i4"), ("transmission", "<i4"), ("specular_color", "<i4"),
     ("clearcoat_roughness", "<i4"), ("clearcoat", "<i4"), ("coat_mask", "<i4"), ("bump", "<i4"),
     ("normal", "<i4"), ("medium_ior", "<i4"), ("medium_absorption", "<i4"), ("medium_density", "<i4"),
+    ("eval", "<i4"), ("pad", "<i4"),
 ])
-assert HOOKS_DTYPE.itemsize == 80
+assert HOOKS_DTYPE.itemsize == 88
 HOOK_PARAMS = 13
 
 LIGHT_DTYPE = np.dtype([

@@ -67,7 +67,8 @@ struct DParams {  // MaterialParams (materials/mod.rs:67-87)
   float medium_absorption[3];
   float model;
   float pattern_a[3];
-  float hook;       // index into the hook table (DHooks), -1: no per-hit hooks
+  float hook;       // index into the hook table (DHooks), -1: no per-hit hooks (also after a
+                    // lerp of two sets: blend_params drops the dynamic id, renderer.rs:7624)
   float pattern_b[3];
   float pad1;
 };
@@ -79,7 +80,7 @@ struct DHooks {
   int metallic, specular, specular_weight, transmission, specular_color, clearcoat_roughness, clearcoat, coat_mask;
   int bump, normal;
   int medium_ior, medium_absorption, medium_density;
-  int pad;
+  int eval;
 };
 static_assert(sizeof(DHooks) == 80, "DHooks layout");
 

@@ -522,10 +522,10 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     for (uint32_t h = 0; h < ir->n_material_hooks; ++h) {
       const ft_material_hooks& H = ir->material_hooks[h];
       if (H.material != m) continue;
-      const int32_t ids[19] = {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
+      const int32_t ids[20] = {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                                H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
                                H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal,
-                               H.medium_ior, H.medium_absorption, H.medium_density};
+                               H.medium_ior, H.medium_absorption, H.medium_density, H.eval};
       for (int32_t f : ids) {
         if (f >= (int32_t)ir->n_functions || (f >= 0 && ir->functions[f].n_params != FT_HOOK_PARAMS)) {
           err = "material hook is not a function of the shading context";
@@ -535,7 +535,7 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
       ftd::DHooks d{H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                     H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
                     H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal,
-                    H.medium_ior, H.medium_absorption, H.medium_density, 0};
+                    H.medium_ior, H.medium_absorption, H.medium_density, H.eval};
       p.hook = (float)S.hooks.size();
       S.hooks.push_back(d);
       break;

@@ -51,6 +51,7 @@ __device__ inline void blend_params(DParams& a, const DParams& b, float t) {
   a.clearcoat_roughness = a.clearcoat_roughness + (b.clearcoat_roughness - a.clearcoat_roughness) * t;
   a.transmission = a.transmission + (b.transmission - a.transmission) * t;
   a.thin_walled = t < 0.5f ? a.thin_walled : b.thin_walled;
+  a.hook = -1.0f;  // a.dynamic_material_id = None (renderer.rs:7624): no eval hook after a lerp
   set3(a.emission_color, lerp_spec(v3(a.emission_color), v3(b.emission_color), t));
   a.emission_strength = a.emission_strength + (b.emission_strength - a.emission_strength) * t;
   if (a.has_medium != 0.0f && b.has_medium != 0.0f) {  // blend_medium (renderer.rs:7543-7566)
@@ -395,13 +396,21 @@ struct MatPool {
   unsigned used;  // bit i: entry i is referenced
   int overflow;
 };
+// A leaf handle with this bit names a leaf set that a smooth blend at t == 0 or 1 passed
+// through: its parameters are the leaf's, its dynamic id (hook entry) is dropped.
+constexpr int kLerped = 1 << 30;
+__device__ __forceinline__ void mark_lerped(MatPool& pool, int& h) {
+  if (h < 0) pool.e[-1 - h].hook = -1.0f;
+  else h |= kLerped;
+}
 __device__ __forceinline__ void ref_params(int slot, const MatPool& pool, int h, float3 p, DParams& out,
                                            const HookCtx* hc) {
   if (h < 0) {
     out = pool.e[-1 - h];
   } else {
-    const DNode& n = scene_of(slot).nodes[h];
+    const DNode& n = scene_of(slot).nodes[h & ~kLerped];
     leaf_params(slot, n.mat, n.obj, p, out, hc);
+    if (h & kLerped) out.hook = -1.0f;
   }
 }
 __device__ inline void ref_to_mat(int slot, const MatPool& pool, const MRef& r, float3 p, Mat& m,
@@ -460,8 +469,10 @@ __device__ __noinline__ void blend_refs_lerp(int slot, MatPool& pool, MRef& l, c
 __device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p,
                                            const HookCtx* hc) {
   if (!l.blend && !r.blend && l.a_model == r.a_model) {
-    if (t == 0.0f) { pool_drop(pool, r.ha); return; }
-    if (t == 1.0f) { pool_drop(pool, l.ha); l = r; return; }
+    // t == 0 / 1: the left / right set itself (blend_params' lerp up to rounding), marked as
+    // a lerp result, which has no dynamic id
+    if (t == 0.0f) { pool_drop(pool, r.ha); mark_lerped(pool, l.ha); return; }
+    if (t == 1.0f) { pool_drop(pool, l.ha); l = r; mark_lerped(pool, l.ha); return; }
     blend_refs_lerp(slot, pool, l, r, t, p, hc);
     return;
   }
@@ -807,9 +818,27 @@ __device__ inline void sample_light(const DLight& L, float3 point, unsigned idx,
 
 // shade_color (renderer.rs:4217-4276)
 __device__ __noinline__ float3 shade_color(int slot, const Mat& mat, float3 hit_pos, float3 hit_n, bool front,
-                                           float3 nrm, float3 wo, float eps, float max_dist) {
+                                           float3 nrm, float3 wo, float eps, float max_dist, unsigned obj,
+                                           unsigned mat_id, float cur_ior) {
   const DevScene& S = scene_of(slot);
   if (S.n_lights == 0) return mk(1, 1, 1);
+  // eval_bsdf (renderer.rs:4905-4920): while the material keeps its dynamic id, its eval hook
+  // replaces the BSDF, with the context of bsdf_context_value (renderer.rs:5003-5030)
+  const int hk = (int)dom_params(mat).hook;
+  const int ev = hk >= 0 ? S.hooks[hk].eval : -1;
+  float3 args[FT_HOOK_PARAMS];
+  if (ev >= 0) {
+    args[0] = hit_pos;
+    args[1] = object_local(slot, obj, hit_pos);
+    args[2] = nrm;
+    args[3] = wo;
+    args[4] = splat3(front ? 1.0f : 0.0f);
+    args[5] = splat3((float)obj);
+    args[6] = splat3((float)mat_id);
+    args[7] = wo;
+    args[9] = splat3(cur_ior);
+    for (int k = 10; k < (int)FT_HOOK_PARAMS; ++k) args[k] = mk(0.0f, 0.0f, 0.0f);
+  }
   float3 n = normalize(nrm);
   float3 gn = front ? normalize(hit_n) : normalize(mul(hit_n, -1.0f));
   float3 color = mk(0, 0, 0);
@@ -829,7 +858,13 @@ __device__ __noinline__ float3 shade_color(int slot, const Mat& mat, float3 hit_
       if (max_t <= 0.0f) continue;
       float sh = shadow_visibility(slot, so, wi, max_t, eps);
       if (sh <= 0.0f) continue;
-      float3 f = mat_eval(mat, nrm, wi, wo);
+      float3 f;
+      if (ev >= 0) {
+        args[8] = wi;
+        f = eval_fn(slot, ev, args);
+      } else {
+        f = mat_eval(mat, nrm, wi, wo);
+      }
       color = add(color, mul(cmul(f, rad), ndotl * sh * inv));
     }
   }
@@ -978,7 +1013,8 @@ __device__ __noinline__ float3 hit_material(int slot, const TraceParams& tp, con
   const float local_weight = clampf((1.0f - transmission) * (1.0f - reflect_weight), 0.0f, 1.0f);
   float3 color = mk(0, 0, 0);
   if (local_weight != 0.0f)
-    color = mul(shade_color(slot, mat, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist), local_weight);
+    color = mul(shade_color(slot, mat, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist, hit.obj, hit.mat, med.ior),
+                local_weight);
   color = add(color, mat_emission(mat));
 
   if (reflect_weight > 1.0e-4f) {

@@ -40,17 +40,18 @@ from .forge.evaluator import Obj, Func, is_num, f32
 HOOK_NAMES = ("color", "roughness", "ior", "thin_walled", "emission_color", "emission_strength",
               "metallic", "specular", "specular_weight", "transmission", "specular_color",
               "clearcoat_roughness", "clearcoat", "coat_mask", "bump", "normal",
-              "medium_ior", "medium_absorption", "medium_density")
+              "medium_ior", "medium_absorption", "medium_density", "eval")
 # the medium hook returns a Medium object: its fields, with medium_from_value's defaults
 # (renderer.rs:6834-6845), become three functions
 MEDIUM_FIELDS = (("ior", 1.0), ("absorption_color", None), ("density", 0.0))
-SPECTRUM_HOOKS = ("color", "emission_color", "specular_color", "normal")
+SPECTRUM_HOOKS = ("color", "emission_color", "specular_color", "normal", "eval")
 # the hooks each definition model reads: the Metal arm and the Dielectric / other arms
 # (renderer.rs:7472-7530), the Standard / Layered block (7259-7470), whose coat hooks replace
 # the ior, specular colour, clearcoat roughness and clearcoat weight, and for every model
-# the bump and normal hooks of build_bsdf_context (4397-4449) and the medium hook
+# the bump and normal hooks of build_bsdf_context (4397-4449), the medium hook and the
+# eval hook of eval_bsdf (4905-4920)
 _NORMAL = {"bump": "bump", "normal": "normal", "medium_ior": "medium", "medium_absorption": "medium",
-           "medium_density": "medium"}
+           "medium_density": "medium", "eval": "eval"}
 _ARM_HOOKS = {
     "Metal": dict({"color": "color", "roughness": "roughness", "emission_color": "emission_color",
                    "emission_strength": "emission_strength"}, **_NORMAL),
@@ -67,14 +68,14 @@ def arm_hooks(model: str) -> dict:
     return _ARM_HOOKS["Metal" if model == "Metal" else "other"]
 
 
-# a hook the Whitted render (renderer/ray.rs) reads that the GPU does not evaluate: eval
-# replaces the BSDF value in shade_color (renderer.rs:4217-4276, 4905-4920).  A material
-# with it keeps its dynamic id without a hook table entry, and Whitted renders of its
-# scene fail loudly.  pdf / sample serve only the path tracer and
+# hooks the Whitted render reads that the GPU does not evaluate: none since the eval hook
+# (the BSDF value of shade_color, renderer.rs:4905-4920) is lowered like the others; a
+# material with a refused hook would keep its dynamic id without a hook table entry, and
+# Whitted renders of its scene would fail loudly.  pdf / sample serve only the path tracer and
 # subsurface nothing the renderers read; like hooks a model's arm does not read (a
 # Metal's ior) and helper functions of the shading context they leave the image unchanged,
 # and a material with only such hooks compiles as a static one.
-REFUSED_HOOKS = ("eval",)
+REFUSED_HOOKS: tuple = ()
 MATERIAL_HOOKS = ("color", "roughness", "metallic", "specular", "ior", "thin_walled", "medium", "subsurface",
                   "emission", "emission_color", "emission_strength", "bump", "normal", "eval", "pdf", "sample",
                   "transmission", "clearcoat", "clearcoat_roughness", "specular_color")

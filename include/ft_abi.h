@@ -41,7 +41,7 @@ typedef __hip_internal::uint8_t uint8_t;
 extern "C" {
 #endif
 
-#define FT_ABI_VERSION 5u
+#define FT_ABI_VERSION 6u
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -278,6 +278,10 @@ typedef struct ft_material_hooks {
   int32_t medium_ior;         /* the fields of the Medium object a medium hook returns */
   int32_t medium_absorption;  /* (medium_from_value renderer.rs:6834-6845): all three or none */
   int32_t medium_density;
+  int32_t eval;               /* the BSDF value of shade_color (eval_bsdf renderer.rs:4905-4920):
+                                 context normal = shading normal, view_dir = wo, plus wo, wi,
+                                 current_ior; read while the material keeps its dynamic id */
+  int32_t pad;
 } ft_material_hooks;
 
 /* -------------------------------------------------------------- lights */

@@ -408,6 +408,7 @@ template <class T> struct Params {  // MaterialParams (materials/mod.rs:67-87)
   int pattern;
   Spec<T> pattern_a, pattern_b;
   T pattern_scale;
+  int hook = -1;  // hook table entry while the set keeps its dynamic id (blend_params drops it)
 };
 template <class T> struct Mat {  // Material (materials/mod.rs:214-221)
   bool blend;
@@ -965,7 +966,7 @@ bool validate(const ft_scene_ir* ir) {
     for (int32_t f : {H.color, H.roughness, H.ior, H.thin_walled, H.emission_color, H.emission_strength,
                       H.metallic, H.specular, H.specular_weight, H.transmission, H.specular_color,
                       H.clearcoat_roughness, H.clearcoat, H.coat_mask, H.bump, H.normal,
-                      H.medium_ior, H.medium_absorption, H.medium_density})
+                      H.medium_ior, H.medium_absorption, H.medium_density, H.eval})
       if (f >= (int32_t)ir->n_functions) { g_err = "material hook names no function"; return false; }
   }
   for (uint32_t i = 0; i < ir->n_nodes; ++i) {
@@ -1327,6 +1328,7 @@ template <class T> Params<T> blend_params(Params<T> a, const Params<T>& b, T t) 
   a.clearcoat_roughness = a.clearcoat_roughness + (b.clearcoat_roughness - a.clearcoat_roughness) * t;
   a.transmission = a.transmission + (b.transmission - a.transmission) * t;
   a.thin_walled = t < T(0.5) ? a.thin_walled : b.thin_walled;
+  a.hook = -1;  // a.dynamic_material_id = None (R:7624)
   a.emission_color = lerp_spectrum(a.emission_color, b.emission_color, t);
   a.emission_strength = a.emission_strength + (b.emission_strength - a.emission_strength) * t;
   // blend_medium (R:7543-7566)
@@ -1399,6 +1401,7 @@ template <class T> Mat<T> resolve_leaf_material(const Ctx<T>& c, Mat<T> m, const
   V3<T> lp = to_local(hit.position, xf);
   const uint32_t mi = hit.material_id < S.materials.size() ? hit.material_id : 0u;
   const int hk = S.mat_hook.empty() ? -1 : S.mat_hook[mi];
+  m.p.hook = hk;
   if (hk >= 0) {
     const ft_material_hooks& H = S.hooks[hk];
     V3<T> args[FT_HOOK_PARAMS];
@@ -1692,7 +1695,29 @@ template <class T> Spec<T> shade_color(const Ctx<T>& c, const Mat<T>& mat, const
       if (max_t <= T(0)) continue;
       T sh = shadow_visibility(*c.A, so, wi, max_t, c.op.epsilon);
       if (sh <= T(0)) continue;
-      Spec<T> f = mat_eval(mat, b.normal, wi, b.wo);
+      // eval_bsdf (R:4905-4920): the material's eval hook, with the context of
+      // bsdf_context_value (R:5003-5030), replaces the BSDF while it keeps its dynamic id
+      const Params<T>& dp = dominant_params(mat);
+      const int ev = dp.hook >= 0 ? S.hooks[dp.hook].eval : -1;
+      Spec<T> f;
+      if (ev >= 0) {
+        V3<T> args[FT_HOOK_PARAMS];
+        args[0] = b.hit.position;
+        args[1] = b.local_position;
+        args[2] = b.normal;
+        args[3] = b.wo;
+        args[4] = V3<T>::splat(b.hit.front_face ? T(1) : T(0));
+        args[5] = V3<T>::splat((T)b.hit.object_id);
+        args[6] = V3<T>::splat((T)b.hit.material_id);
+        args[7] = b.wo;
+        args[8] = wi;
+        args[9] = V3<T>::splat(b.current_ior);
+        for (uint32_t k = 10; k < FT_HOOK_PARAMS; ++k) args[k] = V3<T>();
+        const V3<T> v = S.fn(ev, args);
+        f = Spec<T>(v.x, v.y, v.z);
+      } else {
+        f = mat_eval(mat, b.normal, wi, b.wo);
+      }
       color = color + f.cmul(smp.radiance) * (ndotl * sh * inv);
     }
   }

@@ -227,6 +227,7 @@ material Stripes {
   fn color(ctx) { return mix(base, accent, step(0.0, sin(ctx.local_position.x * 9.0))); }
   fn emission_strength(ctx) { return clamp(ctx.position.y - 0.9, 0.0, 1.0) * 0.6; }
   fn emission_color(ctx) { return ctx.normal * 0.5 + vec3(0.5, 0.5, 0.5); }
+  fn eval(ctx) { return mix(base, accent, step(0.0, sin(ctx.local_position.x * 9.0))) * (0.25 + 0.1 * ctx.wi.y); }
 };
 material Brushed {
   model: Metal;
@@ -272,7 +273,8 @@ def test_material_hooks(emit, monkeypatch):
     """Per-hit hooks of Lambert-, Metal- and Dielectric-model definitions (colour, roughness,
     ior, thin_walled, emission; resolve_leaf_material renderer.rs:7472-7530), of a
     Standard-model one (metallic, specular weight and the coat hooks, renderer.rs:7259-7470)
-    bump / normal hooks (renderer.rs:7749-7827) and a medium hook (renderer.rs:6834-6845),
+    bump / normal hooks (renderer.rs:7749-7827), a medium hook (renderer.rs:6834-6845) and an
+    eval hook (renderer.rs:4905-4920),
     evaluated on the GPU with the shading context of each hit, against the oracle; and the
     reference's own hooked regression scene in Whitted."""
     monkeypatch.setenv("FT_EMIT", emit)

@@ -28,6 +28,15 @@ namespace {
 
 thread_local std::string g_err;
 
+// Planted mistakes (test infrastructure).  Compiling with -DFTREF_PLANT=k builds the oracle
+// with mistake k at one site, marked planted(k) below; tests/test_oracle_pins.py builds each
+// variant and checks that the pin of the function holding the mistake fails on it.  The
+// default build (FTREF_PLANT = 0) contains none of them.
+#ifndef FTREF_PLANT
+#define FTREF_PLANT 0
+#endif
+constexpr bool planted(int k) { return FTREF_PLANT == k; }
+
 // ------------------------------------------------------------------- math
 template <class T> struct C;
 template <> struct C<float> {
@@ -99,6 +108,7 @@ template <class T> V3<T> rotate_z(V3<T> v, T deg) {
 template <class T> struct Xf { V3<T> center, rot; };
 template <class T> V3<T> to_local(V3<T> p, const Xf<T>& t) {
   V3<T> q = p - t.center;
+  if (planted(38)) return rotate_z(rotate_y(rotate_x(q, -t.rot.x), -t.rot.y), -t.rot.z);
   q = rotate_z(q, -t.rot.z);
   q = rotate_y(q, -t.rot.y);
   q = rotate_x(q, -t.rot.x);
@@ -128,21 +138,23 @@ template <class T> T op_smin(T a, T b, T k) {
 }
 template <class T> T op_union_round(T a, T b, T r) {
   r = rmax(std::fabs(r), T(1.0e-6));
-  T ux = rmax(r - a, T(0)), uy = rmax(r - b, T(0));
+  T ux = rmax(r - a, T(0)), uy = rmax(r - (planted(19) ? a : b), T(0));
   return rmax(r, rmin(a, b)) - std::sqrt(ux * ux + uy * uy);
 }
 template <class T> T op_intersect_round(T a, T b, T r) {
   r = rmax(std::fabs(r), T(1.0e-6));
   T ux = rmax(r + a, T(0)), uy = rmax(r + b, T(0));
-  return rmin(-r, rmax(a, b)) + std::sqrt(ux * ux + uy * uy);
+  return rmin(planted(20) ? r : -r, rmax(a, b)) + std::sqrt(ux * ux + uy * uy);
 }
-template <class T> T op_diff_round(T a, T b, T r) { return op_intersect_round(a, -b, r); }
+template <class T> T op_diff_round(T a, T b, T r) {
+  return planted(21) ? op_intersect_round(-a, b, r) : op_intersect_round(a, -b, r);
+}
 template <class T> T frac_1_sqrt_2() { return T(0.70710678118654752440); }
 template <class T> T op_union_chamfer(T a, T b, T r) {
-  return rmin(rmin(a, b), (a - r + b) * frac_1_sqrt_2<T>());
+  return rmin(rmin(a, b), (planted(22) ? a + r + b : a - r + b) * frac_1_sqrt_2<T>());
 }
 template <class T> T op_intersect_chamfer(T a, T b, T r) {
-  return rmax(rmax(a, b), (a + r + b) * frac_1_sqrt_2<T>());
+  return rmax(rmax(a, b), (a + r + b) * (planted(23) ? T(0.5) : frac_1_sqrt_2<T>()));
 }
 template <class T> T op_diff_chamfer(T a, T b, T r) { return op_intersect_chamfer(a, -b, r); }
 template <class T> T repeat_centered(T p, T size) {
@@ -153,7 +165,7 @@ template <class T> T op_union_columns(T a, T b, T r, T n) {
   if (a < r && b < r) {
     n = rmax(std::round(n), T(2));
     T s2 = std::sqrt(T(2));
-    T column_radius = r * s2 / ((n - T(1)) * T(2) + s2);
+    T column_radius = r * s2 / ((planted(31) ? n : n - T(1)) * T(2) + s2);
     T x = a + column_radius, y = b + column_radius;
     int ni = (int)n;
     if (ni % 2 == 1) x += column_radius;
@@ -172,25 +184,29 @@ template <class T> T op_intersect_columns(T a, T b, T r, T n) { return -op_union
 template <class T> T op_diff_columns(T a, T b, T r, T n) { return -op_union_columns(-a, b, r, n); }
 template <class T> T op_union_stairs(T a, T b, T r, T n) {
   n = rmax(std::round(n), T(1));
-  T s = r / n, u = b - r;
+  T s = r / (planted(24) ? n + T(1) : n), u = b - r;
   return rmin(rmin(a, b), T(0.5) * (u + a + std::fabs(rem_euclid(u - a + s, T(2) * s) - s)));
 }
-template <class T> T op_intersect_stairs(T a, T b, T r, T n) { return -op_union_stairs(-a, -b, r, n); }
+template <class T> T op_intersect_stairs(T a, T b, T r, T n) {
+  return planted(25) ? -op_union_stairs(-a, b, r, n) : -op_union_stairs(-a, -b, r, n);
+}
 template <class T> T op_diff_stairs(T a, T b, T r, T n) { return -op_union_stairs(-a, b, r, n); }
 template <class T> T op_union_soft(T a, T b, T r) {
   r = rmax(std::fabs(r), T(1.0e-6));
   T e = rmax(r - std::fabs(a - b), T(0));
-  return rmin(a, b) - e * e * T(0.25) / r;
+  return rmin(a, b) - e * e * T(0.25) / (planted(26) ? T(2) * r : r);
 }
-template <class T> T op_pipe(T a, T b, T r) { return std::sqrt(a * a + b * b) - std::fabs(r); }
+template <class T> T op_pipe(T a, T b, T r) {
+  return (planted(27) ? a * a + b * b : std::sqrt(a * a + b * b)) - std::fabs(r);
+}
 template <class T> T op_engrave(T a, T b, T r) {
-  return rmax(a, (a + std::fabs(r) - std::fabs(b)) * frac_1_sqrt_2<T>());
+  return rmax(a, (a + std::fabs(r) - (planted(28) ? b : std::fabs(b))) * frac_1_sqrt_2<T>());
 }
 template <class T> T op_groove(T a, T b, T ra, T rb) {
-  return rmax(a, rmin(a + std::fabs(ra), std::fabs(rb) - std::fabs(b)));
+  return rmax(a, rmin(planted(29) ? a - std::fabs(ra) : a + std::fabs(ra), std::fabs(rb) - std::fabs(b)));
 }
 template <class T> T op_tongue(T a, T b, T ra, T rb) {
-  return rmin(a, rmax(a - std::fabs(ra), std::fabs(b) - std::fabs(rb)));
+  return rmin(a, rmax(a - std::fabs(ra), (planted(30) ? b : std::fabs(b)) - std::fabs(rb)));
 }
 
 // noise modifier (R:5957-6009)
@@ -239,7 +255,7 @@ template <class T> T apply_noise_modifier(V3<T> p, T d, uint32_t octaves, T scal
 template <class T> T sd_regular_ngon(V3<T> p, uint32_t sides, T radius) {
   T n = (T)(sides < 3 ? 3 : sides);
   T an = T(3.14159265358979323846) / n;
-  T apothem = radius * std::cos(an);
+  T apothem = radius * (planted(3) ? std::sin(an) : std::cos(an));
   T angle = std::atan2(p.z, p.x);
   T sector = T(2) * an;
   T wrapped = rem_euclid(angle + an, sector) - an;
@@ -262,12 +278,13 @@ template <class T> T lerp64(T a, T b, T t) { return a * (T(1) - t) + b * t; }
 template <class T> T value_noise_3d(V3<T> p) {  // eval.rs:4292-4317
   V3<T> i(std::floor(p.x), std::floor(p.y), std::floor(p.z));
   V3<T> f(fractf(p.x), fractf(p.y), fractf(p.z));
-  V3<T> u(smoothstep01(f.x), smoothstep01(f.y), smoothstep01(f.z));
+  V3<T> u = planted(34) ? f : V3<T>(smoothstep01(f.x), smoothstep01(f.y), smoothstep01(f.z));
   T n000 = hash_noise3(V3<T>(i.x, i.y, i.z)), n001 = hash_noise3(V3<T>(i.x, i.y, i.z + T(1)));
   T n010 = hash_noise3(V3<T>(i.x, i.y + T(1), i.z)), n011 = hash_noise3(V3<T>(i.x, i.y + T(1), i.z + T(1)));
   T n100 = hash_noise3(V3<T>(i.x + T(1), i.y, i.z)), n101 = hash_noise3(V3<T>(i.x + T(1), i.y, i.z + T(1)));
   T n110 = hash_noise3(V3<T>(i.x + T(1), i.y + T(1), i.z));
   T n111 = hash_noise3(V3<T>(i.x + T(1), i.y + T(1), i.z + T(1)));
+  if (planted(35)) std::swap(n010, n100);
   T nx00 = lerp64(n000, n100, u.x), nx01 = lerp64(n001, n101, u.x);
   T nx10 = lerp64(n010, n110, u.x), nx11 = lerp64(n011, n111, u.x);
   T nxy0 = lerp64(nx00, nx10, u.y), nxy1 = lerp64(nx01, nx11, u.y);
@@ -281,7 +298,7 @@ template <class T> T fbm_3d(V3<T> p, uint32_t octaves, T scale, T lac) {  // eva
   for (uint32_t k = 0; k < n; ++k) {
     sum += amp * value_noise_3d(q);
     q = q * l;
-    amp *= T(0.55);
+    amp *= planted(36) ? T(0.5) : T(0.55);
   }
   return sum;
 }
@@ -545,6 +562,7 @@ template <class T> T prim_distance(const Scene<T>& S, uint32_t i, V3<T> p) {
     case FT_NODE_CYLINDER: {
       V3<T> q = to_local(p, S.xf[i]);
       T rd = S.prm(i, 2);
+      if (planted(2)) std::swap(q.y, q.z);
       T radial = std::sqrt(q.x * q.x + q.z * q.z);
       T dx = radial - rmax(S.prm(i, 0) - rd, T(0));
       T dy = std::fabs(q.y) - rmax(S.prm(i, 1) - rd, T(0));
@@ -555,8 +573,9 @@ template <class T> T prim_distance(const Scene<T>& S, uint32_t i, V3<T> p) {
     }
     case FT_NODE_TORUS: {
       V3<T> q = to_local(p, S.xf[i]);
-      T qx = std::sqrt(q.x * q.x + q.z * q.z) - S.prm(i, 0);
-      return std::sqrt(qx * qx + q.y * q.y) - S.prm(i, 1);
+      const T major = planted(1) ? S.prm(i, 1) : S.prm(i, 0), minor = planted(1) ? S.prm(i, 0) : S.prm(i, 1);
+      T qx = std::sqrt(q.x * q.x + q.z * q.z) - major;
+      return std::sqrt(qx * qx + q.y * q.y) - minor;
     }
     case FT_NODE_EXTRUDE_POLYGON: {
       V3<T> q = to_local(p, S.xf[i]);
@@ -667,13 +686,14 @@ template <class T> DistInfo<T> sdf_distance_info(const Scene<T>& S, uint32_t i, 
       DistInfo<T> info = sdf_distance_info(S, n.lhs, p);
       int axis = (int)n.param[0];
       T coord = p[axis];
-      T slab = rmax(S.prm(i, 1) - coord, coord - S.prm(i, 2));
+      T slab = planted(32) ? rmax(coord - S.prm(i, 1), S.prm(i, 2) - coord)
+                           : rmax(S.prm(i, 1) - coord, coord - S.prm(i, 2));
       info.distance = rmax(info.distance, slab);
       return info;
     }
     case FT_NODE_SMOOTH: {
       DistInfo<T> info = sdf_distance_info(S, n.lhs, p);
-      info.distance -= S.prm(i, 0) * T(0.1);
+      info.distance -= S.prm(i, 0) * (planted(33) ? T(1) : T(0.1));
       return info;
     }
     default: break;
@@ -1231,12 +1251,12 @@ template <class T> T smoothstepf(T e0, T e1, T x) {  // R:7538-7541
 
 // materials/*.rs eval
 template <class T> T ggx_d(T ndoth, T alpha) {
-  T a2 = alpha * alpha, n2 = ndoth * ndoth;
+  T a2 = planted(13) ? alpha : alpha * alpha, n2 = ndoth * ndoth;
   T denom = rmax(n2 * (a2 - T(1)) + T(1), T(1.0e-5));
   return a2 / (T(3.14159265358979323846) * denom * denom);
 }
 template <class T> T smith_g1(T ndotx, T alpha) {
-  T k = (alpha * alpha) * T(0.5);
+  T k = (planted(14) ? alpha : alpha * alpha) * T(0.5);
   return ndotx / rmax(ndotx * (T(1) - k) + k, T(1.0e-5));
 }
 template <class T> Spec<T> metal_eval(const Params<T>& m, V3<T> n, V3<T> wi, V3<T> wo) {  // metal.rs:7-21
@@ -1248,13 +1268,13 @@ template <class T> Spec<T> metal_eval(const Params<T>& m, V3<T> n, V3<T> wi, V3<
   T alpha = rr * rr;
   T d = ggx_d(ndoth, alpha);
   T g = smith_g1(ndotl, alpha) * smith_g1(ndotv, alpha);
-  T mm = powi5(T(1) - rclamp(vdoth, T(0), T(1)));
+  T mm = planted(15) ? std::pow(T(1) - rclamp(vdoth, T(0), T(1)), T(4)) : powi5(T(1) - rclamp(vdoth, T(0), T(1)));
   Spec<T> fres = m.color + Spec<T>(1, 1, 1) * mm + m.color * (-mm);
   return fres * ((d * g) / rmax(T(4) * ndotl * ndotv, T(1.0e-5)));
 }
 template <class T> Spec<T> lambert_eval(const Params<T>& m, V3<T> n, V3<T> wi) {  // lambert.rs:7-13
   if (n.dot(wi) <= T(0)) return Spec<T>();
-  return m.color * (T(1) / T(3.14159265358979323846));
+  return m.color * (T(1) / (T(3.14159265358979323846) * (planted(12) ? T(2) : T(1))));
 }
 template <class T> T fresnel_f0_from_ior(T ior) {
   T i = rmax(ior, T(1.0e-3));
@@ -1275,13 +1295,14 @@ template <class T> Spec<T> standard_eval(const Params<T>& m, V3<T> n, V3<T> wi, 
   T mw = rclamp(T(1) - rclamp(ndotv, T(0), T(1)), T(0), T(1));
   T m2 = mw * mw;
   T schlick = m2 * m2 * mw;
-  T coat_cover = rclamp(clearcoat * (T(0.2) + T(0.8) * schlick) * T(0.65), T(0), T(0.95));
-  T base_kd = rclamp((T(1) - metallic) * (T(1) - transmission), T(0), T(1)) * (T(1) - coat_cover);
+  T coat_cover = rclamp(clearcoat * (T(0.2) + T(0.8) * schlick) * (planted(18) ? T(0.5) : T(0.65)), T(0), T(0.95));
+  T base_kd = rclamp((T(1) - metallic) * (planted(16) ? T(1) : T(1) - transmission), T(0), T(1)) * (T(1) - coat_cover);
   Spec<T> diffuse = m.color * (base_kd / T(3.14159265358979323846));
   T f0d = rclamp(fresnel_f0_from_ior(m.ior), T(0), T(1));
   Spec<T> base_f0 = Spec<T>(f0d, f0d, f0d) * (rclamp(m.specular, T(0), T(1)) * rclamp(m.specular_weight, T(0), T(1)));
   base_f0 = base_f0.cmul(m.specular_color);
-  Spec<T> spec_f0 = base_f0 * (T(1) - metallic) + m.color * metallic;  // standard.rs lerp_spectrum
+  Spec<T> spec_f0 = planted(17) ? base_f0 * metallic + m.color * (T(1) - metallic)
+                                 : base_f0 * (T(1) - metallic) + m.color * metallic;  // standard.rs lerp_spectrum
   Spec<T> specular = metal_eval(metal_params(spec_f0, m.roughness), n, wi, wo) * (T(1) - coat_cover * T(0.85));
   Spec<T> cc;
   if (clearcoat > T(1.0e-5))
@@ -1634,7 +1655,7 @@ template <class T> void hammersley(uint32_t i, uint32_t n, T& u, T& v) {
   uint32_t bits = (i >> 16) | (i << 16);
   bits = ((bits & 0x55555555u) << 1) | ((bits & 0xAAAAAAAAu) >> 1);
   bits = ((bits & 0x33333333u) << 2) | ((bits & 0xCCCCCCCCu) >> 2);
-  bits = ((bits & 0x0F0F0F0Fu) << 4) | ((bits & 0xF0F0F0F0u) >> 4);
+  if (!planted(10)) bits = ((bits & 0x0F0F0F0Fu) << 4) | ((bits & 0xF0F0F0F0u) >> 4);
   bits = ((bits & 0x00FF00FFu) << 8) | ((bits & 0xFF00FF00u) >> 8);
   v = (T)(float)bits * T(2.3283064e-10f);
 }
@@ -1643,14 +1664,15 @@ template <class T> V3<T> concentric_disk(T u1, T u2) {
   if (std::fabs(sx) <= C<T>::EPS && std::fabs(sy) <= C<T>::EPS) return V3<T>();
   T r, th;
   const T pi4 = T(0.78539816339744830962), pi2 = T(1.57079632679489661923);
-  if (std::fabs(sx) > std::fabs(sy)) { r = sx; th = pi4 * (sy / sx); }
+  if (std::fabs(sx) > std::fabs(sy)) { r = sx; th = (planted(11) ? pi2 : pi4) * (sy / sx); }
   else { r = sy; th = pi2 - pi4 * (sx / sy); }
   return V3<T>(r * std::cos(th), r * std::sin(th), T(0));
 }
 
 template <class T> struct LightSample { V3<T> wi; Spec<T> radiance; T distance; };
 template <class T> LightSample<T> sample_light(const Light<T>& L, V3<T> point, uint32_t idx, uint32_t count) {
-  if (L.kind == FT_LIGHT_ENV) return {V3<T>(0, 1, 0), L.radiance, std::numeric_limits<T>::infinity()};
+  if (L.kind == FT_LIGHT_ENV)
+    return {planted(7) ? V3<T>(0, 0, 1) : V3<T>(0, 1, 0), L.radiance, std::numeric_limits<T>::infinity()};
   V3<T> target = L.position;
   if (L.kind == FT_LIGHT_SPHERE && L.radius > T(1.0e-5)) {
     V3<T> normal = (point - L.position).normalize();
@@ -1660,11 +1682,12 @@ template <class T> LightSample<T> sample_light(const Light<T>& L, V3<T> point, u
     T u1, u2;
     hammersley(idx, count < 1 ? 1 : count, u1, u2);
     V3<T> disk = concentric_disk(u1, u2);
-    target = L.position + tangent * (disk.x * L.radius) + bitangent * (disk.y * L.radius);
+    const T rad = planted(9) ? L.radius * T(0.5) : L.radius;
+    target = L.position + tangent * (disk.x * rad) + bitangent * (disk.y * rad);
   }
   V3<T> to = target - point;
   T dist = rmax(to.length(), T(1.0e-4));
-  T att = T(1) / (dist * dist);
+  T att = planted(8) ? T(1) / dist : T(1) / (dist * dist);
   return {to * (T(1) / dist), L.radiance * att, dist};
 }
 
@@ -1734,7 +1757,7 @@ template <class T> struct Medium {
 // medium_transmittance / apply_medium_attenuation (R:5109-5125)
 template <class T> Spec<T> attenuate(Spec<T> c, const Medium<T>& m, T dist) {
   if (m.density <= T(0) || dist <= T(0)) return c;
-  T ex = m.density * rmax(dist, T(0));
+  T ex = m.density * rmax(dist, T(0)) * (planted(6) ? T(0.5) : T(1));
   return c.cmul(Spec<T>(std::pow(rclamp(m.absorption.x, T(0), T(1)), ex), std::pow(rclamp(m.absorption.y, T(0), T(1)), ex),
                         std::pow(rclamp(m.absorption.z, T(0), T(1)), ex)));
 }
@@ -1761,7 +1784,7 @@ template <class T> T fresnel_dielectric(T cos_i, T eta_i, T eta_t) {  // R:5186-
   if (sin2t >= T(1)) return T(1);
   T ct = std::sqrt(T(1) - sin2t);
   T rs = ((ei * ci) - (et * ct)) / rmax((ei * ci) + (et * ct), T(1.0e-6));
-  T rp = ((et * ci) - (ei * ct)) / rmax((et * ci) + (ei * ct), T(1.0e-6));
+  T rp = planted(5) ? rs : ((et * ci) - (ei * ct)) / rmax((et * ci) + (ei * ct), T(1.0e-6));
   return T(0.5) * (rs * rs + rp * rp);
 }
 template <class T> T fresnel_schlick(T f0, T cos_t) { return f0 + (T(1) - f0) * powi5(T(1) - rclamp(cos_t, T(0), T(1))); }
@@ -1856,6 +1879,7 @@ template <class T> Spec<T> trace_hit(const Ctx<T>& c, const Hit<T>& hit, V3<T> d
   if (thin) { eta_i = 1; eta_t = 1; next_ior = med.ior; }
   else if (hit.front_face) { eta_i = med.ior; eta_t = ior; next_ior = ior; }
   else { eta_i = med.ior; eta_t = 1; next_ior = 1; }
+  const T eta_ratio = planted(4) ? eta_t / eta_i : eta_i / eta_t;
   V3<T> wo = b.wo;
   V3<T> gn = normal;
   T fresnel = transmission > T(1.0e-4) ? fresnel_dielectric(std::fabs(normal.dot(wo)), eta_i, eta_t)
@@ -1865,7 +1889,8 @@ template <class T> Spec<T> trace_hit(const Ctx<T>& c, const Hit<T>& hit, V3<T> d
   if (transmission > T(1.0e-4)) reflect_weight = rclamp(fresnel, T(0), T(1));
   else {
     T base_reflect = metallic > T(0.5) ? T(1) : rclamp(spec_strength, T(0), T(1));
-    reflect_weight = rclamp(base_reflect * fresnel + metallic * T(0.8) + clearcoat * T(0.2), T(0), T(1));
+    reflect_weight = rclamp(base_reflect * fresnel + metallic * (planted(39) ? T(0.6) : T(0.8)) + clearcoat * T(0.2),
+                            T(0), T(1));
   }
   T local_weight = rclamp((T(1) - transmission) * (T(1) - reflect_weight), T(0), T(1));
   Spec<T> color = shade_color(c, mat, b) * local_weight;
@@ -1891,11 +1916,11 @@ template <class T> Spec<T> trace_hit(const Ctx<T>& c, const Hit<T>& hit, V3<T> d
     Spec<T> refracted;
     V3<T> rdir;
     if (rough > T(0.06)) {
-      if (refract(dir, normal, eta_i / eta_t, rdir)) {
+      if (refract(dir, normal, eta_ratio, rdir)) {
         uint32_t n = rough < T(0.1) ? 2 : (rough < T(0.24) ? 3 : 4);
         refracted = trace_lobes(c, hp, gn, rdir.normalize(), next, depth, rough, n);
       }
-    } else if (refract(dir, normal, eta_i / eta_t, rdir)) {
+    } else if (refract(dir, normal, eta_ratio, rdir)) {
       V3<T> ro = offset_ray_origin(hp, gn, rdir, c.op.epsilon * (T(8) + rough * T(12)));
       refracted = trace_ray(c, ro, rdir.normalize(), next, depth + 1);
     }
@@ -1903,7 +1928,7 @@ template <class T> Spec<T> trace_hit(const Ctx<T>& c, const Hit<T>& hit, V3<T> d
       Spec<T> tt(T(1) - (T(1) - rclamp(base_color.x, T(0), T(1))) * transmission,
                  T(1) - (T(1) - rclamp(base_color.y, T(0), T(1))) * transmission,
                  T(1) - (T(1) - rclamp(base_color.z, T(0), T(1))) * transmission);
-      color = color + refracted.cmul(tt) * (transmission * (T(1) - fresnel));
+      color = color + refracted.cmul(tt) * (transmission * (planted(37) ? T(1) : T(1) - fresnel));
     }
   }
   return attenuate(color, med, hit.t);
@@ -2144,6 +2169,57 @@ int ftref_eval_function(const ft_scene_ir* ir, int32_t fn, const float* params, 
     out[3 * k] = r.x;
     out[3 * k + 1] = r.y;
     out[3 * k + 2] = r.z;
+  }
+  return FT_OK;
+}
+
+// Unit-level views of single oracle functions, so each can be checked against closed forms.
+
+// sample_light (render_api.rs:171-289) of the scene's light `light` at `point`.
+int ftref_light_sample(const ft_scene_ir* ir, uint32_t light, const float* point, uint32_t sample_index,
+                       uint32_t sample_count, float* wi3, float* radiance3, float* distance) {
+  if (!validate(ir)) return FT_ERR_INVALID_ARGUMENT;
+  Scene<float> S;
+  load_scene(ir, S);
+  if (light >= S.lights.size()) { g_err = "light index out of range"; return FT_ERR_INVALID_ARGUMENT; }
+  LightSample<float> ls = sample_light(S.lights[light], V3<float>(point[0], point[1], point[2]), sample_index,
+                                       sample_count);
+  wi3[0] = ls.wi.x; wi3[1] = ls.wi.y; wi3[2] = ls.wi.z;
+  radiance3[0] = ls.radiance.x; radiance3[1] = ls.radiance.y; radiance3[2] = ls.radiance.z;
+  *distance = ls.distance;
+  return FT_OK;
+}
+
+// The BSDF value (eval_bsdf R:4905-4920 without hooks) of static material `material` for n
+// direction triples (normal, wi, wo), 9 floats each; out: 3 floats each.
+int ftref_material_eval(const ft_scene_ir* ir, uint32_t material, const float* nwiwo, int32_t n, float* out) {
+  if (!validate(ir)) return FT_ERR_INVALID_ARGUMENT;
+  Scene<float> S;
+  load_scene(ir, S);
+  if (material >= S.materials.size()) { g_err = "material index out of range"; return FT_ERR_INVALID_ARGUMENT; }
+  for (int32_t k = 0; k < n; ++k) {
+    const float* v = nwiwo + 9 * k;
+    Spec<float> f = mat_eval(S.materials[material], V3<float>(v[0], v[1], v[2]), V3<float>(v[3], v[4], v[5]),
+                             V3<float>(v[6], v[7], v[8]));
+    out[3 * k] = f.x; out[3 * k + 1] = f.y; out[3 * k + 2] = f.z;
+  }
+  return FT_OK;
+}
+
+// fresnel_dielectric_scalar (R:5186-5204) for n (cos_i, eta_i, eta_t) triples.
+int ftref_fresnel_dielectric(const float* args, int32_t n, float* out) {
+  for (int32_t k = 0; k < n; ++k) out[k] = fresnel_dielectric(args[3 * k], args[3 * k + 1], args[3 * k + 2]);
+  return FT_OK;
+}
+
+// refract (R:5080-5095) for n (incident, normal, eta) records of 7 floats; out: 3 floats
+// each, all zero when refract returns None.
+int ftref_refract(const float* args, int32_t n, float* out) {
+  for (int32_t k = 0; k < n; ++k) {
+    const float* a = args + 7 * k;
+    V3<float> r;
+    if (!refract(V3<float>(a[0], a[1], a[2]), V3<float>(a[3], a[4], a[5]), a[6], r)) r = V3<float>();
+    out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
   }
   return FT_OK;
 }

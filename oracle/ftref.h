@@ -63,6 +63,16 @@ int ftref_diff_forward(uint32_t n_prims, const double* params, const ft_view* vi
                        const ft_render_options* opts, const float light_dir[3], float normal_step, int32_t threads,
                        int32_t precision, double* t_out, double* s_out);
 
+/* Unit-level views of single functions (for closed-form checks):
+ * sample_light of light `light` (wi, radiance, distance), eval_bsdf of a static material
+ * for n (normal, wi, wo) triples, fresnel_dielectric for n (cos_i, eta_i, eta_t), refract
+ * for n (incident[3], normal[3], eta) records (zero vector when it returns None). */
+int ftref_light_sample(const ft_scene_ir* ir, uint32_t light, const float* point, uint32_t sample_index,
+                       uint32_t sample_count, float* wi3, float* radiance3, float* distance);
+int ftref_material_eval(const ft_scene_ir* ir, uint32_t material, const float* nwiwo, int32_t n, float* out);
+int ftref_fresnel_dielectric(const float* args, int32_t n, float* out);
+int ftref_refract(const float* args, int32_t n, float* out);
+
 const char* ftref_last_error(void);
 
 #ifdef __cplusplus

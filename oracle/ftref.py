@@ -18,25 +18,52 @@ def build() -> str:
     return LIB_PATH
 
 
+def _bind(path):
+    L = C.CDLL(path)
+    vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
+    L.ftref_render_depth.argtypes = [vp, u32, vp, u32, i32, i32, i32, i32, vp, vp]
+    L.ftref_render_whitted.argtypes = [vp, u32, vp, vp, i32, i32, i32, i32, vp, vp]
+    L.ftref_distance_info.argtypes = [vp, u32, i32, vp, i32, vp, vp, vp]
+    L.ftref_lower_bound.argtypes = [vp, u32, i32, vp, i32, vp]
+    L.ftref_eval_function.argtypes = [vp, i32, vp, i32, vp]
+    L.ftref_scene_bounds.argtypes = [vp, vp, vp, vp]
+    L.ftref_depth_eval_count.argtypes = [vp, u32, vp, u32, i32, i32, i32, vp]
+    L.ftref_diff_forward.argtypes = [C.c_uint32, vp, vp, C.c_uint32, vp, C.POINTER(C.c_float), C.c_float, i32,
+                                     i32, vp, vp]
+    L.ftref_light_sample.argtypes = [vp, u32, vp, u32, u32, vp, vp, vp]
+    L.ftref_material_eval.argtypes = [vp, u32, vp, i32, vp]
+    L.ftref_fresnel_dielectric.argtypes = [vp, i32, vp]
+    L.ftref_refract.argtypes = [vp, i32, vp]
+    L.ftref_last_error.restype = C.c_char_p
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             build()
-        L = C.CDLL(LIB_PATH)
-        vp, u32, i32 = C.c_void_p, C.c_uint32, C.c_int32
-        L.ftref_render_depth.argtypes = [vp, u32, vp, u32, i32, i32, i32, i32, vp, vp]
-        L.ftref_render_whitted.argtypes = [vp, u32, vp, vp, i32, i32, i32, i32, vp, vp]
-        L.ftref_distance_info.argtypes = [vp, u32, i32, vp, i32, vp, vp, vp]
-        L.ftref_lower_bound.argtypes = [vp, u32, i32, vp, i32, vp]
-        L.ftref_eval_function.argtypes = [vp, i32, vp, i32, vp]
-        L.ftref_scene_bounds.argtypes = [vp, vp, vp, vp]
-        L.ftref_depth_eval_count.argtypes = [vp, u32, vp, u32, i32, i32, i32, vp]
-        L.ftref_diff_forward.argtypes = [C.c_uint32, vp, vp, C.c_uint32, vp, C.POINTER(C.c_float), C.c_float, i32,
-                                         i32, vp, vp]
-        L.ftref_last_error.restype = C.c_char_p
-        _lib = L
+        _lib = _bind(LIB_PATH)
     return _lib
+
+
+class using:
+    """Runs the enclosed calls against another build of the oracle (a library path), then
+    restores the default one; the planted-mistake checks use it."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        global _lib
+        self.saved = _lib
+        _lib = _bind(self.path)
+        return self
+
+    def __exit__(self, *exc):
+        global _lib
+        _lib = self.saved
+        return False
 
 
 def _check(rc):
@@ -136,3 +163,43 @@ def diff_forward(n_prims, params, views, opts, light_dir=(0.4, 1.0, 0.3), normal
     _check(lib().ftref_diff_forward(n_prims, p.ctypes.data, C.addressof(arr), len(views), C.addressof(o), light,
                                     float(normal_step), threads, precision, t.ctypes.data, s.ctypes.data))
     return t, s
+
+
+def light_sample(scene, light, point, sample_index=0, sample_count=1):
+    """sample_light of the scene's light `light` at `point`: (wi, radiance, distance)."""
+    pt = np.ascontiguousarray(point, np.float32)
+    wi = np.zeros(3, np.float32)
+    rad = np.zeros(3, np.float32)
+    dist = np.zeros(1, np.float32)
+    ir = scene.ir()
+    _check(lib().ftref_light_sample(C.addressof(ir), int(light), pt.ctypes.data, int(sample_index),
+                                    int(sample_count), wi.ctypes.data, rad.ctypes.data, dist.ctypes.data))
+    return wi, rad, float(dist[0])
+
+
+def material_eval(scene, material, normal, wi, wo):
+    """BSDF value of static material `material` for arrays of (normal, wi, wo) [n, 3]."""
+    n = np.broadcast_arrays(np.asarray(normal, np.float32), np.asarray(wi, np.float32), np.asarray(wo, np.float32))
+    rec = np.ascontiguousarray(np.concatenate([a.reshape(-1, 3) for a in n], axis=1), np.float32)
+    out = np.zeros((len(rec), 3), np.float32)
+    ir = scene.ir()
+    _check(lib().ftref_material_eval(C.addressof(ir), int(material), rec.ctypes.data, len(rec), out.ctypes.data))
+    return out
+
+
+def fresnel_dielectric(cos_i, eta_i, eta_t):
+    a = np.ascontiguousarray(np.stack(np.broadcast_arrays(np.asarray(cos_i, np.float32), np.asarray(eta_i, np.float32),
+                                                          np.asarray(eta_t, np.float32)), axis=-1).reshape(-1, 3))
+    out = np.zeros(len(a), np.float32)
+    _check(lib().ftref_fresnel_dielectric(a.ctypes.data, len(a), out.ctypes.data))
+    return out
+
+
+def refract(incident, normal, eta):
+    i = np.asarray(incident, np.float32).reshape(-1, 3)
+    nrm = np.broadcast_to(np.asarray(normal, np.float32).reshape(-1, 3), i.shape)
+    e = np.broadcast_to(np.asarray(eta, np.float32).reshape(-1, 1), (len(i), 1))
+    rec = np.ascontiguousarray(np.concatenate([i, nrm, e], axis=1), np.float32)
+    out = np.zeros((len(rec), 3), np.float32)
+    _check(lib().ftref_refract(rec.ctypes.data, len(rec), out.ctypes.data))
+    return out

@@ -19,24 +19,24 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak
 
 
 def cpu_baseline(workload, scene, seconds: float):
-    """Times the oracle on a bounded sample of the same workload on all host cores."""
-    import numpy as np
-
+    """Times the oracle on a bounded sample of the same workload on all host cores.
+    Returns the baseline record and the oracle's image rows it rendered {row: rgb row}."""
     from oracle import ftref
     opts = workload.options
     threads = os.cpu_count() or 1
     if workload.kind == "whitted":
         spp = workload.settings.aa_samples
         render = lambda r: ftref.render_whitted(scene, workload.accel, opts, workload.settings, threads=threads,
-                                                rows=(r, r + 1))
+                                                rows=(r, r + 1))[0][r]
     else:
         spp = workload.aa
-        render = lambda r: ftref.render_depth(scene, workload.accel, opts, workload.aa, threads=threads, rows=(r, r + 1))
+        render = lambda r: ftref.render_depth(scene, workload.accel, opts, workload.aa, threads=threads,
+                                              rows=(r, r + 1))[0][r]
     # evenly spaced single rows, as many as fit the budget (first measure one row)
     rows_all = list(range(0, opts.height))
     probe = rows_all[len(rows_all) // 2]
     t0 = time.perf_counter()
-    render(probe)
+    rows = {probe: render(probe)}
     per_row = max(time.perf_counter() - t0, 1e-3)
     n_rows = max(1, min(opts.height, int(seconds / per_row)))
     if n_rows == 1:  # one row already fills the budget: the probe is the sample
@@ -46,13 +46,49 @@ def cpu_baseline(workload, scene, seconds: float):
         picked = rows_all[step // 2::step][:n_rows]
         t0 = time.perf_counter()
         for r in picked:
-            render(r)
+            rows[r] = render(r)
         dt = time.perf_counter() - t0
     rays = len(picked) * opts.width * spp
-    return {"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{len(picked)} evenly spaced rows x {opts.width} px x {spp} spp "
-                      f"of the same frame ({rays} primary rays, {dt:.1f} s), C++ restatement of the "
-                      f"reference renderer, {threads} threads"}
+    return ({"value": rays / dt / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+             "sample": f"{len(picked)} evenly spaced rows x {opts.width} px x {spp} spp "
+                       f"of the same frame ({rays} primary rays, {dt:.1f} s), C++ restatement of the "
+                       f"reference renderer, {threads} threads"}, rows)
+
+
+def frame_parity(workload, scene, frame, oracle_rows, f64_rows: int):
+    """The timed frame's rows against the oracle's rows of the same frame (bytes within 1 of
+    255), and `f64_rows` of them against the oracle in FP64 (the error of the f32 frame
+    against an f64 march and shading of the same algorithm)."""
+    import numpy as np
+
+    from oracle import ftref
+    opts = workload.options
+    threads = os.cpu_count() or 1
+    rows = sorted(oracle_rows)
+    g = frame[rows].astype(int)
+    c = np.stack([oracle_rows[r] for r in rows]).astype(int)
+    d = np.abs(g - c).max(axis=-1)
+    out = {"rows": rows, "pixels": int(d.size), "frac_within_1": round(float((d <= 1).mean()), 6),
+           "max_abs_diff": int(d.max()), "mean_abs_diff": round(float(np.abs(g - c).mean()), 5)}
+    if f64_rows > 0:
+        pick = rows[len(rows) // 2:len(rows) // 2 + f64_rows]
+        f64 = []
+        for r in pick:
+            if workload.kind == "whitted":
+                img = ftref.render_whitted(scene, workload.accel, opts, workload.settings, threads=threads,
+                                           precision=1, rows=(r, r + 1))[0]
+            else:
+                img = ftref.render_depth(scene, workload.accel, opts, workload.aa, threads=threads, precision=1,
+                                         rows=(r, r + 1))[0]
+            f64.append(img[r])
+        c64 = np.stack(f64).astype(int)
+        d64 = np.abs(frame[pick].astype(int) - c64).max(axis=-1)
+        d3264 = np.abs(np.stack([oracle_rows[r] for r in pick]).astype(int) - c64).max(axis=-1)
+        out["vs_fp64"] = {"rows": pick, "gpu_frac_within_1": round(float((d64 <= 1).mean()), 6),
+                          "gpu_max_abs_diff": int(d64.max()),
+                          "oracle_f32_frac_within_1": round(float((d3264 <= 1).mean()), 6),
+                          "oracle_f32_max_abs_diff": int(d3264.max())}
+    return out
 
 
 def cpu_baseline_batch(w, scenes, seconds: float):
@@ -266,6 +302,8 @@ def main():
     ap.add_argument("--workload", default="whitted_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp64-rows", type=int, default=1,
+                    help="rows of the timed frame also checked against the oracle in FP64")
     ap.add_argument("--scenes", type=int, default=0, help="batch workloads: number of scenes (default: the workload's)")
     ap.add_argument("--atlas", type=int, default=0, choices=(0, 16, 32),
                     help="single-scene workloads: coarse march steps on an FP16 / FP32 distance atlas")
@@ -326,6 +364,7 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     stream_ms = ev0.elapsed_time(ev1)
+    frame = rgb.cpu().numpy().reshape(opts.height, opts.width, 3)  # the last timed frame
 
     spp = w.settings.aa_samples if w.kind == "whitted" else w.aa
     rays_per_step = opts.width * opts.height * spp
@@ -393,7 +432,8 @@ def main():
                                   "frac": round(gbs / 8000.0, 5), "coarse_samples": int(st.atlas_samples),
                                   "bytes_per_sample": per_sample}
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(w, scene, args.cpu_seconds)
+        line["cpu_baseline"], oracle_rows = cpu_baseline(w, scene, args.cpu_seconds)
+        line["parity"] = frame_parity(w, scene, frame, oracle_rows, args.fp64_rows)
     gs.close()
     print(json.dumps(line), flush=True)
 

@@ -498,7 +498,7 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
   }
 
   // materials
-  bool any_transmissive = false, any_rough_transmissive = false;
+  bool dielectric_any = false, dielectric_rough = false, standard_transmissive = false, standard_rough = false;
   for (uint32_t m = 0; m < ir->n_materials; ++m) {
     const ft_material& a = ir->materials[m];
     ftd::DParams p;
@@ -542,10 +542,21 @@ bool build_scene(const ft_scene_ir* ir, uint32_t accel, HostScene& S, std::strin
     }
     S.mats.push_back(p);
     if (a.dynamic_material_id >= 0 && p.hook < 0.0f && S.dynamic_material < 0) S.dynamic_material = (int)m;
-    bool transmissive = a.model == FT_MODEL_DIELECTRIC || (a.model == FT_MODEL_STANDARD && a.transmission > 0.0f);
-    any_transmissive = any_transmissive || transmissive;
-    if (transmissive && a.roughness > 0.06f) any_rough_transmissive = true;
+    // per-hit roughness / transmission hooks and lerps between two parameter sets of one
+    // model can reach any value, so a model counts as (rough) transmissive when any of its
+    // materials can be
+    const ftd::DHooks* hk = p.hook >= 0.0f ? &S.hooks[(size_t)p.hook] : nullptr;
+    const bool rough = a.roughness > 0.06f || (hk && hk->roughness >= 0);
+    if (a.model == FT_MODEL_DIELECTRIC) {
+      dielectric_any = true;
+      dielectric_rough = dielectric_rough || rough;
+    } else if (a.model == FT_MODEL_STANDARD) {
+      standard_transmissive = standard_transmissive || a.transmission > 0.0f || (hk && hk->transmission >= 0);
+      standard_rough = standard_rough || rough;
+    }
   }
+  const bool any_transmissive = dielectric_any || standard_transmissive;
+  const bool any_rough_transmissive = (dielectric_any && dielectric_rough) || (standard_transmissive && standard_rough);
   // children of one hit: reflection (<= 3 lobes) + refraction (<= 4 lobes), per split material
   int per_mat = any_rough_transmissive ? 7 : (any_transmissive ? 2 : 1);
   bool any_blend = false;

@@ -923,8 +923,10 @@ struct RayStack {
   RayEntry e[kRayStack];
   int sp;
   int overflow;
-  int owner;  // owner of the rays pushed while the current ray is processed
+  int owner;      // owner of the rays pushed while the current ray is processed
+  int max_depth;  // rays at this depth return black (ray.rs:12-14) and are never stored
   __device__ void push(float3 o, float3 d, float3 thr, const Medium& m, int depth) {
+    if (depth >= max_depth) return;
     if (sp >= kRayStack) { overflow = 1; return; }
     RayEntry& r = e[sp++];
     r.ox = o.x; r.oy = o.y; r.oz = o.z; r.dx = d.x; r.dy = d.y; r.dz = d.z;
@@ -1030,7 +1032,11 @@ __device__ __noinline__ float3 hit_material(int slot, const TraceParams& tp, con
   }
   if (transmission > 1.0e-4f) {
     Medium next = med;
-    if (!thin) {  // transition_medium (renderer.rs:5153-5175)
+    // transition_medium (renderer.rs:5153-5175): a Standard material changes the medium only
+    // above transmission 0.01 (its own test, not the 1e-4 of the refraction branch)
+    const int model = dom_model(mat);
+    const bool changes_medium = model == TAG_DIELECTRIC || (model == TAG_STANDARD && pr.transmission > 0.01f);
+    if (!thin && changes_medium) {
       if (!hit.front) next = Medium{1.0f, 1.0f, 1.0f, 1.0f, 0.0f};
       else if (pr.has_medium != 0.0f)
         next = Medium{clampf(pr.medium_ior, 1, 3), pr.medium_absorption[0], pr.medium_absorption[1],
@@ -1105,6 +1111,7 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
   st.sp = 0;
   st.overflow = 0;
   st.owner = me;
+  st.max_depth = tp.max_depth;
   st.push(o, d, mk(1, 1, 1), Medium{1.0f, 1.0f, 1.0f, 1.0f, 0.0f}, 0);
   const unsigned long long active = __builtin_amdgcn_ballot_w64(true);
   for (;;) {

@@ -34,6 +34,12 @@ class ForgeEvalError(Exception):
     pass
 
 
+class NonNumericResult(ForgeEvalError):
+    """A function whose result is neither a number nor a vec3 (a string, say): a per-hit
+    material hook of that kind yields no value and the material keeps its static one
+    (spectrum_from_value -> None, renderer.rs:7924-7935)."""
+
+
 # ------------------------------------------------------------------ values
 @dataclass
 class Obj:
@@ -679,6 +685,8 @@ class Evaluator:
                 args.append(lo.param(i, ty))
         sc = Scope(locals_, block, overrides, lo, 0)
         out = self.apply(params, body, args, sc)
+        if not isinstance(out, Sym) and not is_num(out) and as_vec3(out) is None:
+            raise NonNumericResult("function result is neither a number nor a vec3")
         res = lo.lift(out)
         if res.idx != len(lo.nodes) - 1:
             # the result must be the last node; copy it with an identity op

@@ -472,7 +472,8 @@ class SceneCompiler:
 
     def lower_hooks(self, d, name: str, model: str, overrides: Obj) -> Optional[tuple]:
         """The definition's hook functions of the shading context, lowered once per (definition,
-        override); -1 where absent, not read by the model's arm, or (number hooks) not a number.
+        override); -1 where absent, not read by the model's arm, or returning something that is
+        not a number (number hooks) or neither a number nor a vec3 (spectrum hooks).
         None when a hook cannot be lowered (the scene then keeps the Whitted refusal)."""
         key = (name, repr(sorted((k, repr(v)) for k, v in overrides.fields.items())))
         if key in self._hook_cache:
@@ -485,7 +486,10 @@ class SceneCompiler:
                 if h.startswith("medium_"):
                     ids.append(medium[("medium_ior", "medium_absorption", "medium_density").index(h)])
                     continue
-                fid = self.lower_block_fn(d, arm[h], ["ctx"], overrides) if h in arm else -1
+                try:
+                    fid = self.lower_block_fn(d, arm[h], ["ctx"], overrides) if h in arm else -1
+                except E.NonNumericResult:
+                    fid = -1  # no value per hit: the static parameter stays (renderer.rs:7924-7935)
                 if fid >= 0 and h not in SPECTRUM_HOOKS and self.functions[fid][1] != 0:
                     fid = -1
                 ids.append(fid)

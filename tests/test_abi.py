@@ -34,3 +34,25 @@ def test_struct_layout_matches_header():
 
 def test_last_error_is_empty_string_initially():
     assert isinstance(render.library().ft_last_error(), bytes)
+
+
+def _peak_stack(max_depth, children):
+    """Peak size of one lane's ray stack under the Whitted kernel's protocol: pop a ray,
+    push its children unless they would sit at max_depth (those return black, ray.rs:12-14,
+    so the kernel never stores them)."""
+    stack, peak = [0], 1
+    while stack:
+        d = stack.pop()
+        if d + 1 < max_depth:
+            stack.extend([d + 1] * children)
+        peak = max(peak, len(stack))
+    return peak
+
+
+def test_ray_stack_bound_is_exact():
+    """The capacity the host asks of the ray stack, 1 + (max_depth - 1) (children - 1)
+    (whitted_impl in csrc/ft_backend.hip), is the protocol's peak: rough glass (7 children
+    per hit) at the reference's default max_depth 8 needs 43 entries."""
+    for max_depth in range(1, 9):
+        for children in (1, 2, 4, 7):
+            assert _peak_stack(max_depth, children) == 1 + (max_depth - 1) * (children - 1)

@@ -287,19 +287,38 @@ def test_material_hooks(emit, monkeypatch):
     whitted_parity(scene, AccelMode.Bvh, ropts, RaySettings(3, 96, 1), name + " (Whitted)", None)
 
 
-def test_unsupported_material_hooks_fail_loudly():
-    """A hook that the lowering to device functions cannot express (a string-valued colour
-    here) is refused for Whitted; the depth preview of the same scene is unaffected."""
-    from forgedthoughts_amd.scene import RenderError
-    scene = compile_src('material Odd { model: Lambert; fn color(ctx) { return "red"; } };\n'
+def test_whitted_1080p_benchmark_frame_rows():
+    """The benchmark frame itself (1920x1080, 16 spp, depth 4, the 64-primitive scene):
+    two full-width rows of the GPU frame against the oracle's rows of the same frame."""
+    w = synthetic.whitted_1080p()
+    scene = compile_src(w.source)
+    with GpuScene(scene, w.accel) as gs:
+        rgb, _ = gs.render_whitted(w.options, w.settings)
+    for r in (300, 700):
+        print(f"[1080p frame] oracle row {r}", flush=True)
+        rgb_c, _ = ftref.render_whitted(scene, w.accel, w.options, w.settings, rows=(r, r + 1))
+        d = np.abs(rgb[r].astype(int) - rgb_c[r].astype(int)).max(axis=-1)
+        frac = float((d <= 1).mean())
+        assert frac >= 0.999, f"row {r}: {frac:.5f} of pixels within 1/255 (max {d.max()})"
+
+
+def test_non_numeric_hook_keeps_the_static_value():
+    """A hook whose result is no colour (a string here) yields nothing per hit, so the
+    material keeps its static colour (spectrum_from_value -> None, renderer.rs:7924-7935):
+    the Whitted image matches the oracle and the same material without the hook."""
+    hooked = compile_src('material Odd { model: Lambert; color: vec3(0.8, 0.3, 0.2); '
+                         'fn color(ctx) { return "red"; } };\n'
+                         "let scene = Sphere { radius: 1.0, material: Odd {} };")
+    plain = compile_src("material Odd { model: Lambert; color: vec3(0.8, 0.3, 0.2); };\n"
                         "let scene = Sphere { radius: 1.0, material: Odd {} };")
-    assert len(scene.material_hooks) == 0
+    assert len(hooked.material_hooks) == 1
     opts = RenderOptions(width=32, height=32)
-    with GpuScene(scene, AccelMode.Bvh) as gs:
-        gs.render_depth(opts, 1)
-        with pytest.raises(RenderError) as e:
-            gs.render_whitted(opts, RaySettings(2, 32, 1))
-        assert e.value.kind == "UnsupportedObjectType"
+    whitted_parity(hooked, AccelMode.Bvh, opts, RaySettings(2, 32, 1), "non-numeric hook", None)
+    with GpuScene(hooked, AccelMode.Bvh) as gs:
+        a, _ = gs.render_whitted(opts, RaySettings(2, 32, 1))
+    with GpuScene(plain, AccelMode.Bvh) as gs:
+        b, _ = gs.render_whitted(opts, RaySettings(2, 32, 1))
+    np.testing.assert_array_equal(a, b)
 
 
 @pytest.mark.parametrize("emit", ["0", "1"], ids=["interp", "emitted"])

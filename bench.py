@@ -213,6 +213,13 @@ def bench_adjoint(args, w):
     err0 = float(np.abs(np.float32(w.params_init) - np.float32(w.params_true)).mean())
     err1 = float(np.abs(p - np.float32(w.params_true)).mean())
     npix = len(w.views) * w.options.width * w.options.height
+    # work accounting: one untimed step (forward + loss + adjoint) on the FLOP-counting kernels
+    dr.set_flop_counting(True)
+    dr.loss_grad()
+    step_flops = dr.stats().sdf_flops
+    dr.set_flop_counting(False)
+    step_s = kernel_ms / steps / 1e3
+    achieved = step_flops / step_s / 1e12 if step_s > 0 else 0.0
     line = {
         "metric": "adjoint grad-steps/s",
         "value": round(steps / wall, 4),
@@ -228,7 +235,12 @@ def bench_adjoint(args, w):
                 "mean_abs_param_error": [round(err0, 5), round(err1, 5)]},
         "kernel_ms_per_step": round(kernel_ms / steps, 3),
         "pixels_per_step": npix,
-        "roofline": None,
+        # the step's tree-pass FLOPs (forward march + adjoint reverse passes, counted per
+        # lane on the device) over the step's kernel time (loss, reduction and Adam included)
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "traffic": None,
+                     "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2, "flops_per_step": int(step_flops),
+                     "kernel_ms_per_step": round(kernel_ms / steps, 3)},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline_adjoint(w, args.cpu_seconds)
@@ -302,6 +314,7 @@ def main():
     ap.add_argument("--workload", default="whitted_1080p")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--fp64-rows", type=int, default=1,
                     help="rows of the timed frame also checked against the oracle in FP64")
     ap.add_argument("--scenes", type=int, default=0, help="batch workloads: number of scenes (default: the workload's)")
@@ -323,6 +336,7 @@ def main():
 
     from forgedthoughts_amd import synthetic
     from forgedthoughts_amd.forge.evaluator import load_scene_source
+    from forgedthoughts_amd.options import RaySettings
     from forgedthoughts_amd.render import GpuScene
     from forgedthoughts_amd.scene import compile_scene
 
@@ -338,6 +352,19 @@ def main():
     gs = GpuScene(scene, w.accel)
     if args.atlas:
         gs.build_atlas(args.atlas_voxel, args.atlas)
+    # launch shape: the autotuner's choice for this scene and request (outside the timed region)
+    from forgedthoughts_amd import abi
+    kind = abi.KIND_WHITTED if w.kind == "whitted" else abi.KIND_DEPTH
+    tune = None
+    if not args.no_tune:
+        t_tune = time.perf_counter()
+        rep = gs.autotune(kind, opts, w.settings if w.kind == "whitted" else RaySettings(1, 64, w.aa))
+        tune = {"chosen": rep.best.as_dict(), "tuned": True, "cached": bool(rep.cached),
+                "candidates": rep.candidates, "tune_render_spp": rep.tune_aa_samples,
+                "chosen_ms": round(rep.best_ms, 3), "default_ms": round(rep.default_ms, 3),
+                "speedup_vs_default": round(rep.default_ms / rep.best_ms, 4) if rep.best_ms > 0 else None,
+                "search_s": round(time.perf_counter() - t_tune, 1)}
+        print(f"[bench] autotune {tune}", file=sys.stderr, flush=True)
 
     def step():
         if w.kind == "whitted":
@@ -411,7 +438,9 @@ def main():
                    "max_depth": w.settings.max_depth if w.settings else None, "accel": w.accel.name.lower(),
                    "max_steps": opts.max_steps, "epsilon": opts.epsilon, "nodes": int(len(scene.nodes)),
                    "specialised": {"scene": bool(st.flags & 1), "functions": bool(st.flags & 2),
-                                   "compile_ms": round(st.compile_ms, 1)}},
+                                   "compile_ms": round(st.compile_ms, 1)},
+                   "launch": tune if tune is not None else {"chosen": gs.launch_config(kind).as_dict(),
+                                                            "tuned": False}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": measured_traffic(w.name, f"ft_k_{w.kind}_16" if w.kind == "whitted" else "ft_k_depth"),

@@ -126,6 +126,34 @@ class View(C.Structure):
 
 BATCH_DEPTH, BATCH_WHITTED, BATCH_SEPARATE, BATCH_GRAPH = 1, 2, 4, 8
 
+
+class LaunchConfigC(C.Structure):
+    _fields_ = [("block_waves", C.c_uint32), ("tile_width", C.c_uint32), ("waves_per_eu", C.c_uint32),
+                ("march_unroll", C.c_uint32), ("shade_inline", C.c_uint32)]
+
+    def as_dict(self):
+        return {"block_waves": self.block_waves, "tile": f"{self.tile_width}x{64 // self.tile_width}",
+                "waves_per_eu": self.waves_per_eu, "march_unroll": self.march_unroll,
+                "shade_inline": bool(self.shade_inline)}
+
+
+class TuneReportC(C.Structure):
+    _fields_ = [("best", LaunchConfigC), ("best_ms", C.c_double), ("default_ms", C.c_double),
+                ("tune_ms", C.c_double), ("candidates", C.c_uint32), ("cached", C.c_uint32),
+                ("tune_aa_samples", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ProgressC(C.Structure):
+    _fields_ = [("tiles_done", C.c_uint32), ("tiles_total", C.c_uint32), ("elapsed_ms", C.c_uint64),
+                ("tile_x", C.c_uint32), ("tile_y", C.c_uint32), ("tile_w", C.c_uint32), ("tile_h", C.c_uint32),
+                ("width", C.c_uint32), ("height", C.c_uint32), ("image", C.POINTER(C.c_uint8))]
+
+
+PROGRESS_FN = C.CFUNCTYPE(C.c_int, C.POINTER(ProgressC), C.c_void_p)
+
+KIND_DEPTH, KIND_WHITTED = 0, 1
+TUNE_FULL, TUNE_SHAPE_ONLY, TUNE_NO_CACHE = 1, 2, 4
+
 STRUCT_SIZES = {
     "ft_node": NODE_DTYPE.itemsize, "ft_expr": EXPR_DTYPE.itemsize,
     "ft_function": FUNCTION_DTYPE.itemsize, "ft_material": MATERIAL_DTYPE.itemsize,
@@ -133,5 +161,6 @@ STRUCT_SIZES = {
     "ft_light": LIGHT_DTYPE.itemsize, "ft_camera": C.sizeof(Camera),
     "ft_scene_ir": C.sizeof(SceneIR), "ft_render_options": C.sizeof(RenderOptionsC),
     "ft_ray_settings": C.sizeof(RaySettingsC), "ft_render_stats": C.sizeof(RenderStatsC),
-    "ft_view": C.sizeof(View),
+    "ft_view": C.sizeof(View), "ft_launch_config": C.sizeof(LaunchConfigC),
+    "ft_tune_report": C.sizeof(TuneReportC), "ft_progress": C.sizeof(ProgressC),
 }

@@ -11,10 +11,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -70,6 +73,18 @@ int ensure(DevBuf& b, size_t bytes) {
 
 }  // namespace
 
+// Launch shape of a march kernel (ft_launch_config): block_waves x 64 threads per block,
+// tile_w x (64 / tile_w) pixels per wave, the occupancy the code is compiled for and the
+// march steps per loop iteration.  The default is the precompiled kernels' shape.
+struct LaunchCfg {
+  uint32_t block_waves = 4, tile_w = 8, wpe = 4, unroll = 1, shade_inline = 0;
+  bool operator==(const LaunchCfg& o) const {
+    return block_waves == o.block_waves && tile_w == o.tile_w && wpe == o.wpe && unroll == o.unroll &&
+           shade_inline == o.shade_inline;
+  }
+  bool default_code() const { return wpe == 4 && unroll == 1 && shade_inline == 0; }
+};
+
 struct ft_scene {
   fth::HostScene host;
   DevScene dev;
@@ -82,13 +97,16 @@ struct ft_scene {
   DevBuf atlas_cell_d, atlas_cells, atlas_brick_cell, atlas_bricks, atlas_counter;
   ft_render_stats stats{};
   bool timed = false;
+  int* last_overflow = nullptr;  // overflow flag of the last Whitted render, read by finish()
   // scene specialisation
   fth::EmitPlan plan;
   std::string source;                              // emitted translation unit (empty: interpreter)
-  hipModule_t modules[fth::kRtcKernelCount] = {};  // one code object per kernel, built on first use
-  hipFunction_t functions[fth::kRtcKernelCount] = {};
-  hipModule_t counted_modules[fth::kRtcKernelCount] = {};  // *_counted variants (FLOP counting on)
-  hipFunction_t counted_functions[fth::kRtcKernelCount] = {};
+  // runtime-compiled code objects, one per (kernel, counted, occupancy, unroll), built on
+  // first use: the scene-specialised kernels, the *_counted variants (FLOP counting on) and
+  // launch-shape variants of the interpreter kernels
+  std::map<uint32_t, std::pair<hipModule_t, hipFunction_t>> mods;
+  LaunchCfg cfg[2];  // launch shape of the depth [0] and Whitted [1] kernels
+  uint64_t ir_hash = 0;
   double compile_ms = 0.0;
 };
 
@@ -122,14 +140,8 @@ int publish_module(ft_scene* s, hipModule_t m) {
 int publish(ft_scene* s) {
   FT_HIP(hipMemcpyToSymbol(HIP_SYMBOL(ft_c_scenes), &s->dev, sizeof(DevScene), (size_t)s->slot * sizeof(DevScene),
                            hipMemcpyHostToDevice));
-  for (hipModule_t m : s->modules) {
-    if (!m) continue;
-    int rc = publish_module(s, m);
-    if (rc) return rc;
-  }
-  for (hipModule_t m : s->counted_modules) {
-    if (!m) continue;
-    int rc = publish_module(s, m);
+  for (auto& kv : s->mods) {
+    int rc = publish_module(s, kv.second.first);
     if (rc) return rc;
   }
   return FT_OK;
@@ -140,41 +152,62 @@ const char* const kKernelNames[fth::kRtcKernelCount] = {"ft_k_depth", "ft_k_whit
 const void* const kKernelStubs[fth::kRtcKernelCount] = {(const void*)ft_k_depth, (const void*)ft_k_whitted_16,
                                                         (const void*)ft_k_whitted_48, (const void*)ft_k_whitted_128};
 
-// Builds and loads the scene's specialised code object for march kernel `k` (no-op when the
-// scene runs on the interpreter or the object is loaded).  With FLOP counting on, the
-// *_counted variant is used instead; it is built for every scene (from the interpreter
-// alone when the scene is not specialised).
-int prepare(ft_scene* s, int k) {
-  const bool counted = s->dev.flops != nullptr;
-  if (counted ? s->counted_functions[k] != nullptr : (s->source.empty() || s->functions[k] != nullptr)) return FT_OK;
-  const std::string src = s->source.empty() ? std::string("#include \"ft_kernels.h\"\n") : s->source;
-  std::vector<char> code;
-  std::string err;
-  double ms = 0.0;
-  if (!fth::rtc_compile(src, k, counted, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
-  s->compile_ms += ms;
+const LaunchCfg& cfg_of(const ft_scene* s, int k) { return s->cfg[k == fth::kRtcDepth ? 0 : 1]; }
+uint32_t mod_key(int k, bool counted, const LaunchCfg& c) {
+  return (uint32_t)k | (counted ? 1u << 4 : 0u) | (c.wpe << 8) | (c.unroll << 12) | (c.shade_inline << 16);
+}
+// Kernel k of shape c runs from a runtime-compiled code object (else the precompiled one).
+bool needs_module(const ft_scene* s, bool counted, const LaunchCfg& c) {
+  return counted || !s->source.empty() || !c.default_code() || fth::strict_math();
+}
+hipFunction_t module_function(const ft_scene* s, int k, bool counted, const LaunchCfg& c) {
+  auto it = s->mods.find(mod_key(k, counted, c));
+  return it == s->mods.end() ? nullptr : it->second.second;
+}
+
+// Loads a compiled code object for kernel k and publishes the scene table into it.
+int load_module(ft_scene* s, int k, bool counted, const LaunchCfg& c, const std::vector<char>& code) {
   FT_HIP(hipStreamSynchronize(s->stream));
   hipModule_t m = nullptr;
   FT_HIP(hipModuleLoadData(&m, code.data()));
   const std::string name = std::string(kKernelNames[k]) + (counted ? "_counted" : "");
-  if (counted) {
-    s->counted_modules[k] = m;
-    FT_HIP(hipModuleGetFunction(&s->counted_functions[k], m, name.c_str()));
-  } else {
-    s->modules[k] = m;
-    FT_HIP(hipModuleGetFunction(&s->functions[k], m, name.c_str()));
-  }
+  hipFunction_t f = nullptr;
+  FT_HIP(hipModuleGetFunction(&f, m, name.c_str()));
+  s->mods[mod_key(k, counted, c)] = {m, f};
   return publish_module(s, m);
 }
 
-// Launches march kernel `k`: the FLOP-counting variant when counting is on, else the
-// scene's specialised code object when it has one, else the precompiled kernel.  `args`
-// points at each argument.
-int launch(ft_scene* s, int k, dim3 grid, dim3 block, size_t lds, void** args) {
-  int rc = prepare(s, k);
+std::string module_source(const ft_scene* s) {
+  return s->source.empty() ? std::string("#include \"ft_kernels.h\"\n") : s->source;
+}
+
+// Builds and loads the code object kernel `k` needs under shape `c` (no-op when it runs
+// precompiled or is loaded).  With FLOP counting on, the *_counted variant is used; it is
+// built for every scene (from the interpreter alone when the scene is not specialised).
+int prepare(ft_scene* s, int k, const LaunchCfg& c) {
+  const bool counted = s->dev.flops != nullptr;
+  if (!needs_module(s, counted, c) || module_function(s, k, counted, c)) return FT_OK;
+  std::vector<char> code;
+  std::string err;
+  double ms = 0.0;
+  if (!fth::rtc_compile(module_source(s), k, counted, (int)c.wpe, (int)c.unroll, (int)c.shade_inline, code, ms, err))
+    return fail(FT_ERR_COMPILE_FAILED, err);
+  s->compile_ms += ms;
+  return load_module(s, k, counted, c, code);
+}
+int prepare(ft_scene* s, int k) { return prepare(s, k, cfg_of(s, k)); }
+
+// Launches march kernel `k` in the scene's launch shape for it: the FLOP-counting variant
+// when counting is on, else the runtime-compiled object when one is needed, else the
+// precompiled kernel.  `args` points at each argument.
+int launch(ft_scene* s, int k, dim3 grid, size_t lds, void** args) {
+  const LaunchCfg& c = cfg_of(s, k);
+  int rc = prepare(s, k, c);
   if (rc) return rc;
-  hipFunction_t f = s->dev.flops ? s->counted_functions[k] : s->functions[k];
-  if (f) {
+  const bool counted = s->dev.flops != nullptr;
+  const dim3 block(64, c.block_waves);
+  if (needs_module(s, counted, c)) {
+    hipFunction_t f = module_function(s, k, counted, c);
     FT_HIP(hipModuleLaunchKernel(f, grid.x, grid.y, grid.z, block.x, block.y, block.z, (unsigned)lds, s->stream, args,
                                  nullptr));
     return FT_OK;
@@ -191,7 +224,7 @@ int check_options(const ft_render_options* o) {
   return FT_OK;
 }
 
-CamArgs make_cam(const ft_scene* s, const ft_render_options& o, uint32_t aa) {
+CamArgs make_cam(const ft_scene* s, const ft_render_options& o, uint32_t aa, uint32_t tile_w = 8) {
   fth::CamBasis cb = fth::camera_basis(s->host, o);
   CamArgs c;
   c.ox = cb.origin[0]; c.oy = cb.origin[1]; c.oz = cb.origin[2];
@@ -205,6 +238,9 @@ CamArgs make_cam(const ft_scene* s, const ft_render_options& o, uint32_t aa) {
   c.aa = (int)(aa < 1 ? 1 : aa);
   c.grid = 1;
   while (c.grid * c.grid < c.aa) ++c.grid;  // ceil(sqrt(aa))
+  c.tile_w = (int)tile_w;
+  c.y0 = 0;
+  c.rows = c.height;
   return c;
 }
 
@@ -218,12 +254,16 @@ TraceParams make_tp(const ft_render_options& o, uint32_t max_depth) {
   return tp;
 }
 
-// dynamic LDS of a 256-thread block: three value-stack planes
-size_t stack_lds(const ft_scene* s) { return (size_t)3 * s->dev.stack_depth * 256 * sizeof(float); }
+// dynamic LDS of a block of `waves` wavefronts: three value-stack planes
+size_t stack_lds(const ft_scene* s, uint32_t waves = 4) {
+  return (size_t)3 * s->dev.stack_depth * 64 * waves * sizeof(float);
+}
 
-dim3 tile_grid(int width, int height) {
-  int tiles = ((width + 7) / 8) * ((height + 7) / 8);
-  return dim3((tiles + 3) / 4, 1, 1);
+// one wave per pixel tile of the camera's tile shape over its rows, `waves` waves per block
+dim3 tile_grid(const CamArgs& c, uint32_t waves = 4) {
+  const int th = 64 / c.tile_w;
+  const int tiles = ((c.width + c.tile_w - 1) / c.tile_w) * ((c.rows + th - 1) / th);
+  return dim3((unsigned)((tiles + (int)waves - 1) / (int)waves), 1, 1);
 }
 
 
@@ -235,27 +275,37 @@ int clear_flops(ft_scene* s) {
   return FT_OK;
 }
 
-int depth_impl(ft_scene* s, const ft_render_options* o, uint32_t aa, unsigned char* rgb_dev, float* t_dev) {
-  int rc = check_options(o);
-  if (rc) return rc;
-  if ((rc = clear_flops(s))) return rc;
-  CamArgs cam = make_cam(s, *o, aa);
-  TraceParams tp = make_tp(*o, 1);
-  FT_HIP(hipSetDevice(s->device));
-  int slot = s->slot;
+// Enqueues the depth preview of the camera's rows (the kernel must be prepared).
+int depth_enqueue(ft_scene* s, const CamArgs& cam_in, const TraceParams& tp_in, unsigned char* rgb_dev, float* t_dev) {
+  CamArgs cam = cam_in;
+  TraceParams tp = tp_in;
+  const LaunchCfg& lc = s->cfg[0];
+  int slot = s->slot, rc;
   void* args[] = {&slot, &cam, &tp, &rgb_dev, &t_dev};
-  if ((rc = prepare(s, fth::kRtcDepth))) return rc;  // compile outside the timed region
   FT_HIP(hipEventRecord(s->ev0, s->stream));
-  if ((rc = launch(s, fth::kRtcDepth, tile_grid(cam.width, cam.height), dim3(64, 4), stack_lds(s), args))) return rc;
+  if ((rc = launch(s, fth::kRtcDepth, tile_grid(cam, lc.block_waves), stack_lds(s, lc.block_waves), args))) return rc;
   FT_HIP(hipEventRecord(s->ev1, s->stream));
   s->timed = true;
-  s->stats.primary_rays = (uint64_t)cam.width * cam.height * cam.aa;
+  s->stats.primary_rays = (uint64_t)cam.width * cam.rows * cam.aa;
   s->stats.kernel_launches = 1;
   return FT_OK;
 }
 
-int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs, unsigned char* rgb_dev,
-                 float* hdr_dev) {
+int depth_impl(ft_scene* s, const ft_render_options* o, uint32_t aa, unsigned char* rgb_dev, float* t_dev) {
+  int rc = check_options(o);
+  if (rc) return rc;
+  if ((rc = clear_flops(s))) return rc;
+  CamArgs cam = make_cam(s, *o, aa, s->cfg[0].tile_w);
+  TraceParams tp = make_tp(*o, 1);
+  FT_HIP(hipSetDevice(s->device));
+  if ((rc = prepare(s, fth::kRtcDepth))) return rc;  // compile outside the timed region
+  return depth_enqueue(s, cam, tp, rgb_dev, t_dev);
+}
+
+// Checks a Whitted request and picks its kernel: the ray-stack capacity is exact (every hit
+// pops one entry and pushes at most max_children, none at max_depth).
+int whitted_check(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs, CamArgs& cam, TraceParams& tp,
+                  int& kernel) {
   int rc = check_options(o);
   if (rc) return rc;
   if (!rs) return fail(FT_ERR_INVALID_ARGUMENT, "ray settings are null");
@@ -263,42 +313,62 @@ int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings*
   if (s->host.dynamic_material >= 0)
     return fail(FT_ERR_UNSUPPORTED_OBJECT, "material " + std::to_string(s->host.dynamic_material) +
                                                " has per-hit hook functions, which this backend does not evaluate");
-  if ((rc = clear_flops(s))) return rc;
-  CamArgs cam = make_cam(s, *o, rs->aa_samples);
+  cam = make_cam(s, *o, rs->aa_samples, s->cfg[1].tile_w);
   if (cam.aa > 65535) return fail(FT_ERR_INVALID_ARGUMENT, "too many AA samples");
-  TraceParams tp = make_tp(*o, rs->max_depth);
-  // ray-stack capacity: every hit pops one entry and pushes at most max_children
+  tp = make_tp(*o, rs->max_depth);
   const long need = 1 + (long)(tp.max_depth - 1) * (long)(s->host.max_children - 1);
+  if (need <= 16) kernel = fth::kRtcWhitted16;
+  else if (need <= 48) kernel = fth::kRtcWhitted48;
+  else if (need <= 128) kernel = fth::kRtcWhitted128;
+  else return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with this scene's materials");
+  return FT_OK;
+}
+
+// Enqueues a checked Whitted render with the per-sample radiance in `samples`
+// (aa * W * H * 3 floats) and the overflow flag at `ovf`; the kernel must be prepared.
+int whitted_enqueue(ft_scene* s, const CamArgs& cam_in, const TraceParams& tp_in, int kernel, int aov,
+                    unsigned char* rgb_dev, float* hdr_dev, float* samples, int* ovf) {
+  CamArgs cam = cam_in;
+  TraceParams tp = tp_in;
+  int rc;
+  FT_HIP(hipSetDevice(s->device));
+  hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, s->stream, ovf);
+  FT_HIP(hipGetLastError());
+  const LaunchCfg& lc = s->cfg[1];
+  dim3 grid = tile_grid(cam, lc.block_waves);
+  grid.y = (unsigned)cam.aa;
+  int slot = s->slot;
+  void* args[] = {&slot, &cam, &tp, &aov, &samples, &ovf};
+  FT_HIP(hipEventRecord(s->ev0, s->stream));
+  if ((rc = launch(s, kernel, grid, stack_lds(s, lc.block_waves), args))) return rc;
+  const int npix = cam.width * cam.rows;  // pixels of the launch's rows
+  hipLaunchKernelGGL(ft_k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s->stream, cam, aov, samples, rgb_dev,
+                     hdr_dev);
+  FT_HIP(hipGetLastError());
+  FT_HIP(hipEventRecord(s->ev1, s->stream));
+  s->last_overflow = ovf;
+  s->timed = true;
+  s->stats.primary_rays = (uint64_t)npix * cam.aa;
+  s->stats.kernel_launches = 2;
+  return FT_OK;
+}
+
+int whitted_impl(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs, unsigned char* rgb_dev,
+                 float* hdr_dev) {
+  CamArgs cam;
+  TraceParams tp;
+  int kernel;
+  int rc = whitted_check(s, o, rs, cam, tp, kernel);
+  if (rc) return rc;
+  if ((rc = clear_flops(s))) return rc;
   rc = ensure(s->samples, (size_t)cam.aa * cam.width * cam.height * 3 * sizeof(float));
   if (rc) return rc;
   rc = ensure(s->overflow, sizeof(int));
   if (rc) return rc;
   FT_HIP(hipSetDevice(s->device));
-  int kernel;
-  if (need <= 16) kernel = fth::kRtcWhitted16;
-  else if (need <= 48) kernel = fth::kRtcWhitted48;
-  else if (need <= 128) kernel = fth::kRtcWhitted128;
-  else return fail(FT_ERR_INVALID_ARGUMENT, "max_depth too large for the ray stack with this scene's materials");
-  hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, s->stream, (int*)s->overflow.ptr);
-  FT_HIP(hipGetLastError());
-  dim3 grid = tile_grid(cam.width, cam.height);
-  grid.y = (unsigned)cam.aa;
-  float* samples = (float*)s->samples.ptr;
-  int* ovf = (int*)s->overflow.ptr;
-  int slot = s->slot, aov = rs->debug_aov;
-  void* args[] = {&slot, &cam, &tp, &aov, &samples, &ovf};
   if ((rc = prepare(s, kernel))) return rc;  // compile outside the timed region
-  FT_HIP(hipEventRecord(s->ev0, s->stream));
-  if ((rc = launch(s, kernel, grid, dim3(64, 4), stack_lds(s), args))) return rc;
-  int npix = cam.width * cam.height;
-  hipLaunchKernelGGL(ft_k_resolve, dim3((npix + 255) / 256), dim3(256), 0, s->stream, cam, rs->debug_aov, samples, rgb_dev,
-                     hdr_dev);
-  FT_HIP(hipGetLastError());
-  FT_HIP(hipEventRecord(s->ev1, s->stream));
-  s->timed = true;
-  s->stats.primary_rays = (uint64_t)npix * cam.aa;
-  s->stats.kernel_launches = 2;
-  return FT_OK;
+  return whitted_enqueue(s, cam, tp, kernel, rs->debug_aov, rgb_dev, hdr_dev, (float*)s->samples.ptr,
+                         (int*)s->overflow.ptr);
 }
 
 int finish(ft_scene* s) {
@@ -315,11 +385,283 @@ int finish(ft_scene* s) {
     s->stats.last_kernel_ms = ms;
     s->timed = false;
   }
-  if (s->overflow.ptr) {
+  if (s->last_overflow) {
     int ovf = 0;
-    FT_HIP(hipMemcpy(&ovf, s->overflow.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    FT_HIP(hipMemcpy(&ovf, s->last_overflow, sizeof(int), hipMemcpyDeviceToHost));
+    s->last_overflow = nullptr;
     if (ovf) return fail(FT_ERR_LAUNCH_FAILED, "ray stack overflow");
   }
+  return FT_OK;
+}
+
+}  // namespace
+
+
+namespace {
+
+uint64_t fnv_bytes(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = (const unsigned char*)p;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= c[i];
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+// A hash of everything in the IR that decides the image (the tuning cache key's scene part).
+uint64_t ir_hash(const ft_scene_ir* ir, uint32_t accel) {
+  uint64_t h = fnv_bytes(1469598103934665603ull, &accel, sizeof(accel));
+  h = fnv_bytes(h, &ir->root, sizeof(ir->root));
+  h = fnv_bytes(h, ir->nodes, sizeof(ft_node) * ir->n_nodes);
+  h = fnv_bytes(h, ir->materials, sizeof(ft_material) * ir->n_materials);
+  if (ir->object_transforms) h = fnv_bytes(h, ir->object_transforms, sizeof(float) * 6 * ir->n_objects);
+  if (ir->lights) h = fnv_bytes(h, ir->lights, sizeof(ft_light) * ir->n_lights);
+  if (ir->functions) h = fnv_bytes(h, ir->functions, sizeof(ft_function) * ir->n_functions);
+  if (ir->exprs) h = fnv_bytes(h, ir->exprs, sizeof(ft_expr) * ir->n_exprs);
+  if (ir->material_hooks) h = fnv_bytes(h, ir->material_hooks, sizeof(ft_material_hooks) * ir->n_material_hooks);
+  h = fnv_bytes(h, &ir->env_fn, sizeof(ir->env_fn));
+  h = fnv_bytes(h, &ir->camera, sizeof(ir->camera));
+  return h;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ progressive renders
+namespace {
+
+// render_preview_tiled / render_ray_with_accel_progressive (renderer.rs:3635-3724,
+// 3751-3834): the image in rows of tiles of tile_size pixels; after each row of tiles the
+// rows are copied back and the callback runs on the calling thread once per finished tile,
+// in the reference's tile order (tile_jobs renderer.rs:3849-3862), with the image so far
+// (unfinished tiles black).  A non-zero return stops the render.
+int progressive_impl(ft_scene* s, bool whitted, const ft_render_options* o, const ft_ray_settings* rs, uint32_t aa,
+                     uint32_t tile_size, ft_progress_fn cb, void* user, uint8_t* rgb_out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  int rc = check_options(o);
+  if (rc) return rc;
+  if ((rc = clear_flops(s))) return rc;
+  CamArgs cam;
+  TraceParams tp;
+  int kernel = fth::kRtcDepth;
+  if (whitted) {
+    if ((rc = whitted_check(s, o, rs, cam, tp, kernel))) return rc;
+    if ((rc = ensure(s->samples, (size_t)cam.aa * cam.width * cam.height * 3 * sizeof(float)))) return rc;
+    if ((rc = ensure(s->overflow, sizeof(int)))) return rc;
+  } else {
+    cam = make_cam(s, *o, aa, s->cfg[0].tile_w);
+    tp = make_tp(*o, 1);
+  }
+  const size_t W = o->width, H = o->height;
+  if ((rc = ensure(s->rgb, W * H * 3))) return rc;
+  FT_HIP(hipSetDevice(s->device));
+  if ((rc = prepare(s, kernel))) return rc;
+  const uint32_t ts = std::max<uint32_t>(tile_size, 1u);
+  const uint32_t tiles_x = (uint32_t)((W + ts - 1) / ts), tiles_y = (uint32_t)((H + ts - 1) / ts);
+  std::vector<uint8_t> image(W * H * 3, 0);
+  ft_progress pg{};
+  pg.tiles_total = tiles_x * tiles_y;
+  unsigned char* rgb_dev = (unsigned char*)s->rgb.ptr;
+  for (uint32_t by = 0; by < tiles_y; ++by) {
+    cam.y0 = (int)(by * ts);
+    cam.rows = (int)std::min<size_t>(ts, H - cam.y0);
+    rc = whitted ? whitted_enqueue(s, cam, tp, kernel, rs->debug_aov, rgb_dev, nullptr, (float*)s->samples.ptr,
+                                   (int*)s->overflow.ptr)
+                 : depth_enqueue(s, cam, tp, rgb_dev, nullptr);
+    if (rc) return rc;
+    if ((rc = finish(s))) return rc;
+    const size_t off = (size_t)cam.y0 * W * 3, n = (size_t)cam.rows * W * 3;
+    FT_HIP(hipMemcpy(image.data() + off, rgb_dev + off, n, hipMemcpyDeviceToHost));
+    if (!cb) continue;
+    for (uint32_t bx = 0; bx < tiles_x; ++bx) {
+      ++pg.tiles_done;
+      pg.elapsed_ms = (uint64_t)std::chrono::duration_cast<std::chrono::milliseconds>(
+                          std::chrono::steady_clock::now() - t0).count();
+      pg.tile_x = bx * ts;
+      pg.tile_y = (uint32_t)cam.y0;
+      pg.tile_w = (uint32_t)std::min<size_t>(ts, W - pg.tile_x);
+      pg.tile_h = (uint32_t)cam.rows;
+      pg.width = (uint32_t)W;
+      pg.height = (uint32_t)H;
+      pg.image = image.data();
+      if (cb(&pg, user) != 0) return fail(FT_ERR_CALLBACK_ABORT, "render stopped by the progress callback");
+    }
+  }
+  std::memcpy(rgb_out, image.data(), image.size());
+  s->stats.primary_rays = (uint64_t)W * H * cam.aa;
+  s->stats.kernel_launches = (uint64_t)tiles_y * (whitted ? 2 : 1);
+  return FT_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launch-shape autotuner
+namespace {
+
+std::mutex g_tune_mu;
+std::map<uint64_t, LaunchCfg> g_tune_cache;  // (scene, kind, request) -> best shape, per process
+
+uint64_t tune_key(const ft_scene* s, uint32_t kind, const ft_render_options& o, uint32_t aa, uint32_t max_depth) {
+  uint64_t h = fnv_bytes(s->ir_hash, &kind, sizeof(kind));
+  h = fnv_bytes(h, &o, sizeof(o));
+  h = fnv_bytes(h, &aa, sizeof(aa));
+  h = fnv_bytes(h, &max_depth, sizeof(max_depth));
+  const int emitted = s->source.empty() ? 0 : 1;
+  return fnv_bytes(h, &emitted, sizeof(emitted));
+}
+
+bool tune_cache_get(uint64_t key, LaunchCfg& c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune_cache.find(key);
+  if (it != g_tune_cache.end()) {
+    c = it->second;
+    return true;
+  }
+  char name[48];
+  std::snprintf(name, sizeof(name), "tune2_%016llx.txt", (unsigned long long)key);
+  const std::string path = fth::cache_file(name);
+  if (path.empty()) return false;
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return false;
+  unsigned v[5];
+  const bool ok = std::fscanf(f, "%u %u %u %u %u", &v[0], &v[1], &v[2], &v[3], &v[4]) == 5;
+  std::fclose(f);
+  if (!ok) return false;
+  c = LaunchCfg{v[0], v[1], v[2], v[3], v[4]};
+  g_tune_cache[key] = c;
+  return true;
+}
+
+void tune_cache_put(uint64_t key, const LaunchCfg& c) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = c;
+  char name[48];
+  std::snprintf(name, sizeof(name), "tune2_%016llx.txt", (unsigned long long)key);
+  const std::string path = fth::cache_file(name);
+  if (path.empty()) return;
+  const std::string tmp = path + ".tmp";
+  if (FILE* f = std::fopen(tmp.c_str(), "w")) {  // best effort: the directory may not exist
+    std::fprintf(f, "%u %u %u %u %u\n", c.block_waves, c.tile_w, c.wpe, c.unroll, c.shade_inline);
+    std::fclose(f);
+    std::rename(tmp.c_str(), path.c_str());
+  }
+}
+
+// Kernel time (best of `reps` after one warm-up render) of the request under shape c.
+int time_config(ft_scene* s, uint32_t kind, const LaunchCfg& c, const ft_render_options* o,
+                const ft_ray_settings* rs, int reps, double& ms_out) {
+  s->cfg[kind] = c;
+  double best = 1e30;
+  for (int r = 0; r <= reps; ++r) {
+    int rc = kind == FT_KIND_DEPTH ? depth_impl(s, o, rs ? rs->aa_samples : 1, (unsigned char*)s->rgb.ptr, nullptr)
+                                   : whitted_impl(s, o, rs, (unsigned char*)s->rgb.ptr, nullptr);
+    if (rc) return rc;
+    if ((rc = finish(s))) return rc;
+    if (r > 0) best = std::min(best, (double)s->stats.last_kernel_ms);
+  }
+  ms_out = best;
+  return FT_OK;
+}
+
+// Builds the code objects of `variants` for kernel k in parallel host threads and loads them.
+int build_variants(ft_scene* s, int k, const std::vector<LaunchCfg>& variants) {
+  std::vector<std::vector<char>> codes(variants.size());
+  std::vector<std::string> errs(variants.size());
+  std::vector<double> ms(variants.size(), 0.0);
+  std::vector<char> ok(variants.size(), 1), need(variants.size(), 0);
+  const std::string src = module_source(s);
+  std::vector<std::thread> th;
+  for (size_t i = 0; i < variants.size(); ++i) {
+    if (!needs_module(s, false, variants[i]) || module_function(s, k, false, variants[i])) continue;
+    need[i] = 1;
+    th.emplace_back([&, i]() {
+      ok[i] = fth::rtc_compile(src, k, false, (int)variants[i].wpe, (int)variants[i].unroll,
+                               (int)variants[i].shade_inline, codes[i], ms[i], errs[i]);
+    });
+  }
+  for (auto& t : th) t.join();
+  for (size_t i = 0; i < variants.size(); ++i) {
+    if (!need[i]) continue;
+    if (!ok[i]) return fail(FT_ERR_COMPILE_FAILED, errs[i]);
+    s->compile_ms += ms[i];
+    int rc = load_module(s, k, false, variants[i], codes[i]);
+    if (rc) return rc;
+  }
+  return FT_OK;
+}
+
+int autotune_impl(ft_scene* s, uint32_t kind, const ft_render_options* o, const ft_ray_settings* rs_in,
+                  uint32_t flags, ft_tune_report* rep) {
+  const auto t0 = std::chrono::steady_clock::now();
+  std::memset(rep, 0, sizeof(*rep));
+  if (s->dev.flops) return fail(FT_ERR_INVALID_ARGUMENT, "FLOP counting is on");
+  int rc = check_options(o);
+  if (rc) return rc;
+  if (kind == FT_KIND_WHITTED && !rs_in) return fail(FT_ERR_INVALID_ARGUMENT, "ray settings are null");
+  ft_ray_settings rs{};
+  if (rs_in) rs = *rs_in;
+  else rs.aa_samples = 1;
+  // the tuning render: the request itself, or (Whitted) the same frame at <= 4 samples
+  if (kind == FT_KIND_WHITTED && !(flags & FT_TUNE_FULL) && rs.aa_samples > 4) rs.aa_samples = 4;
+  if ((rc = ensure(s->rgb, (size_t)o->width * o->height * 3))) return rc;
+  int k = fth::kRtcDepth;
+  if (kind == FT_KIND_WHITTED) {
+    CamArgs cam;
+    TraceParams tp;
+    if ((rc = whitted_check(s, o, &rs, cam, tp, k))) return rc;
+  }
+  const uint64_t key = tune_key(s, kind, *o, rs.aa_samples, rs.max_depth);
+  const LaunchCfg dflt;
+  LaunchCfg best = dflt;
+  double best_ms = 0.0, dflt_ms = 0.0;
+  uint32_t timed = 0;
+  if (!(flags & FT_TUNE_NO_CACHE) && tune_cache_get(key, best)) {
+    rep->cached = 1;
+    if ((rc = prepare(s, k, best))) return rc;
+    if ((rc = time_config(s, kind, best, o, &rs, 1, best_ms))) return rc;
+    if ((rc = time_config(s, kind, dflt, o, &rs, 1, dflt_ms))) return rc;
+    timed = 2;
+  } else {
+    // stage 1: code variants (occupancy target, march unroll) in the default launch shape
+    std::vector<LaunchCfg> code = {dflt};
+    if (!(flags & FT_TUNE_SHAPE_ONLY))
+      for (LaunchCfg c : {LaunchCfg{4, 8, 3, 1, 0}, LaunchCfg{4, 8, 2, 1, 0}, LaunchCfg{4, 8, 4, 2, 0},
+                          LaunchCfg{4, 8, 2, 2, 0}, LaunchCfg{4, 8, 4, 1, 1}, LaunchCfg{4, 8, 2, 1, 1}})
+        if (kind == FT_KIND_WHITTED || c.shade_inline == 0) code.push_back(c);
+    if ((rc = build_variants(s, k, code))) return rc;
+    best_ms = 1e30;
+    for (const LaunchCfg& c : code) {
+      double ms = 0.0;
+      if ((rc = time_config(s, kind, c, o, &rs, 2, ms))) return rc;
+      ++timed;
+      if (c == dflt) dflt_ms = ms;
+      if (ms < best_ms) { best_ms = ms; best = c; }
+    }
+    // stage 2: launch shapes (waves per block, pixel tile) of the best code
+    const LaunchCfg code_best = best;
+    for (uint32_t bw : {1u, 2u, 4u})
+      for (uint32_t tw : {8u, 16u, 32u}) {
+        LaunchCfg c = code_best;
+        c.block_waves = bw;
+        c.tile_w = tw;
+        if (c == code_best) continue;
+        double ms = 0.0;
+        if ((rc = time_config(s, kind, c, o, &rs, 2, ms))) return rc;
+        ++timed;
+        if (ms < best_ms) { best_ms = ms; best = c; }
+      }
+    tune_cache_put(key, best);
+  }
+  s->cfg[kind] = best;
+  rep->best.block_waves = best.block_waves;
+  rep->best.tile_width = best.tile_w;
+  rep->best.waves_per_eu = best.wpe;
+  rep->best.march_unroll = best.unroll;
+  rep->best.shade_inline = best.shade_inline;
+  rep->best_ms = best_ms;
+  rep->default_ms = dflt_ms;
+  rep->candidates = timed;
+  rep->tune_aa_samples = rs.aa_samples;
+  rep->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return FT_OK;
 }
 
@@ -415,6 +757,7 @@ static int create_scene(const ft_scene_ir* ir, uint32_t accel_mode, bool batch_m
     else if (m != "0" && m != "never") emit = s->plan.any() && fth::emission_pays(h, ir, s->plan);
     if (emit && !batch_member) s->source = fth::emit_scene_source(h, ir, s->plan);
   }
+  s->ir_hash = ir_hash(ir, accel_mode);
   s->slot = acquire_slot();
   if (s->slot < 0) {
     cleanup();
@@ -439,10 +782,7 @@ int ft_scene_destroy(ft_scene* s) {
                     &s->leaves, &s->rgb, &s->tbuf, &s->samples, &s->hdr, &s->overflow, &s->flops, &s->atlas_cell_d,
                     &s->atlas_cells, &s->atlas_brick_cell, &s->atlas_bricks, &s->atlas_counter})
     if (b->ptr) (void)hipFree(b->ptr);
-  for (hipModule_t m : s->modules)
-    if (m) (void)hipModuleUnload(m);
-  for (hipModule_t m : s->counted_modules)
-    if (m) (void)hipModuleUnload(m);
+  for (auto& kv : s->mods) (void)hipModuleUnload(kv.second.first);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -494,6 +834,19 @@ int ft_render_whitted(ft_scene* s, const ft_render_options* o, const ft_ray_sett
   FT_HIP(hipMemcpy(rgb_out, s->rgb.ptr, npix * 3, hipMemcpyDeviceToHost));
   if (hdr_out) FT_HIP(hipMemcpy(hdr_out, s->hdr.ptr, npix * 3 * sizeof(float), hipMemcpyDeviceToHost));
   return FT_OK;
+}
+
+int ft_render_depth_progressive(ft_scene* s, const ft_render_options* o, uint32_t aa, uint32_t tile_size,
+                                ft_progress_fn cb, void* user, uint8_t* rgb_out) {
+  if (!s || !rgb_out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  return progressive_impl(s, false, o, nullptr, aa, tile_size, cb, user, rgb_out);
+}
+
+int ft_render_whitted_progressive(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs,
+                                  ft_progress_fn cb, void* user, uint8_t* rgb_out) {
+  if (!s || !rgb_out) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  if (!rs) return fail(FT_ERR_INVALID_ARGUMENT, "ray settings are null");
+  return progressive_impl(s, true, o, rs, rs->aa_samples, rs->tile_size, cb, user, rgb_out);
 }
 
 int ft_scene_stream(ft_scene* s, void** stream_out) {
@@ -595,6 +948,38 @@ int ft_scene_build_atlas(ft_scene* s, float voxel, uint32_t bits, uint32_t max_b
   return publish(s);
 }
 
+int ft_scene_set_launch_config(ft_scene* s, uint32_t kind, const ft_launch_config* c) {
+  if (!s || !c) return fail(FT_ERR_INVALID_ARGUMENT, "scene or config is null");
+  if (kind > FT_KIND_WHITTED) return fail(FT_ERR_INVALID_ARGUMENT, "unknown kernel kind");
+  if (!(c->block_waves == 1 || c->block_waves == 2 || c->block_waves == 4) ||
+      !(c->tile_width == 8 || c->tile_width == 16 || c->tile_width == 32) || c->waves_per_eu < 1 ||
+      c->waves_per_eu > 4 || !(c->march_unroll == 1 || c->march_unroll == 2 || c->march_unroll == 4) ||
+      c->shade_inline > 1)
+    return fail(FT_ERR_INVALID_ARGUMENT, "launch config out of range");
+  s->cfg[kind] = LaunchCfg{c->block_waves, c->tile_width, c->waves_per_eu, c->march_unroll, c->shade_inline};
+  return FT_OK;
+}
+
+int ft_scene_get_launch_config(ft_scene* s, uint32_t kind, ft_launch_config* c) {
+  if (!s || !c) return fail(FT_ERR_INVALID_ARGUMENT, "scene or config is null");
+  if (kind > FT_KIND_WHITTED) return fail(FT_ERR_INVALID_ARGUMENT, "unknown kernel kind");
+  const LaunchCfg& l = s->cfg[kind];
+  c->block_waves = l.block_waves;
+  c->tile_width = l.tile_w;
+  c->waves_per_eu = l.wpe;
+  c->march_unroll = l.unroll;
+  c->shade_inline = l.shade_inline;
+  return FT_OK;
+}
+
+int ft_scene_autotune(ft_scene* s, uint32_t kind, const ft_render_options* o, const ft_ray_settings* rs,
+                      uint32_t flags, ft_tune_report* report) {
+  if (!s || !report) return fail(FT_ERR_INVALID_ARGUMENT, "scene or report is null");
+  if (kind > FT_KIND_WHITTED) return fail(FT_ERR_INVALID_ARGUMENT, "unknown kernel kind");
+  FT_HIP(hipSetDevice(s->device));
+  return autotune_impl(s, kind, o, rs, flags, report);
+}
+
 int ft_scene_set_flop_counting(ft_scene* s, int enable) {
   if (!s) return fail(FT_ERR_INVALID_ARGUMENT, "scene is null");
   FT_HIP(hipStreamSynchronize(s->stream));
@@ -634,7 +1019,7 @@ int ft_emit_compile(const ft_scene_ir* ir, uint32_t accel_mode, int kernel, doub
   const std::string src = fth::emit_scene_source(h, ir, fth::plan_emission(h, ir));
   std::vector<char> code;
   double ms = 0.0;
-  if (!fth::rtc_compile(src, kernel, false, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
+  if (!fth::rtc_compile(src, kernel, false, 4, 1, 0, code, ms, err)) return fail(FT_ERR_COMPILE_FAILED, err);
   if (compile_ms_out) *compile_ms_out = ms;
   return FT_OK;
 }
@@ -655,6 +1040,9 @@ int ft_abi_struct_size(const char* name) {
   if (n == "ft_ray_settings") return (int)sizeof(ft_ray_settings);
   if (n == "ft_render_stats") return (int)sizeof(ft_render_stats);
   if (n == "ft_view") return (int)sizeof(ft_view);
+  if (n == "ft_launch_config") return (int)sizeof(ft_launch_config);
+  if (n == "ft_tune_report") return (int)sizeof(ft_tune_report);
+  if (n == "ft_progress") return (int)sizeof(ft_progress);
   return -1;
 }
 
@@ -726,13 +1114,13 @@ int batch_enqueue(ft_batch* b, uint32_t n_views, const TraceParams& tp, int aov,
       const CamArgs cam = b->h_cams[v];
       const size_t slds = stack_lds(s);
       if (flags & FT_BATCH_DEPTH) {
-        hipLaunchKernelGGL(ft_k_depth, tile_grid(cam.width, cam.height), dim3(64, 4), slds, b->stream, s->slot, cam, tp,
+        hipLaunchKernelGGL(ft_k_depth, tile_grid(cam), dim3(64, 4), slds, b->stream, s->slot, cam, tp,
                            depth_rgb + (size_t)v * npix * 3, (float*)nullptr);
         ++launches;
       }
       if (flags & FT_BATCH_WHITTED) {
         const long need = 1 + (long)(tp.max_depth - 1) * (long)(s->host.max_children - 1);
-        dim3 grid = tile_grid(cam.width, cam.height);
+        dim3 grid = tile_grid(cam);
         grid.y = (unsigned)cam.aa;
         float* vs = samples + (size_t)v * cam.aa * npix * 3;
         switch (ray_stack_kernel(need)) {
@@ -815,6 +1203,9 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
     c.aa = (int)(aa < 1 ? 1 : aa);
     c.grid = 1;
     while (c.grid * c.grid < c.aa) ++c.grid;
+    c.tile_w = 8;  // the batch kernels' 8x8 tiles (tiles_per_view)
+    c.y0 = 0;
+    c.rows = c.height;
     slots[v] = (int)views[v].scene;  // host side: scene index; device side: constant slot
     order[v] = (int)v;
     cost[v] = s->host.nodes[s->host.root].cost;
@@ -1030,6 +1421,8 @@ struct ft_diff {
   bool have_target = false, have_forward = false;
   int adam_step = 0;
   ft_render_stats stats{};
+  DevBuf flops;  // FLOP counter of the counted kernels (counting on), else unallocated
+  bool counting = false;
 };
 
 namespace {
@@ -1095,8 +1488,12 @@ unsigned diff_blocks(const ft_diff* d) {
 size_t diff_pixels(const ft_diff* d) { return (size_t)d->n_views * d->opts.width * d->opts.height; }
 
 int diff_forward_enqueue(ft_diff* d) {
-  hipLaunchKernelGGL(ft_k_diff_forward, dim3(diff_blocks(d)), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
-                     (float*)d->t.ptr, (float*)d->s.ptr);
+  if (d->counting)
+    hipLaunchKernelGGL(ft_k_diff_forward_counted, dim3(diff_blocks(d)), dim3(64, 2), diff_lds(d), d->stream,
+                       diff_args(d), (float*)d->t.ptr, (float*)d->s.ptr, (unsigned long long*)d->flops.ptr);
+  else
+    hipLaunchKernelGGL(ft_k_diff_forward, dim3(diff_blocks(d)), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
+                       (float*)d->t.ptr, (float*)d->s.ptr);
   FT_HIP(hipGetLastError());
   d->have_forward = true;
   return FT_OK;
@@ -1124,8 +1521,12 @@ int diff_reduce(ft_diff* d, const double* in, int rows, int cols, double* out) {
 
 int diff_backward_enqueue(ft_diff* d, const float* gt, const float* gs) {
   const unsigned blocks = diff_blocks(d);
-  hipLaunchKernelGGL(ft_k_diff_adjoint, dim3(blocks), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
-                     (const float*)d->t.ptr, gt, gs, (double*)d->partials.ptr);
+  if (d->counting)
+    hipLaunchKernelGGL(ft_k_diff_adjoint_counted, dim3(blocks), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
+                       (const float*)d->t.ptr, gt, gs, (double*)d->partials.ptr, (unsigned long long*)d->flops.ptr);
+  else
+    hipLaunchKernelGGL(ft_k_diff_adjoint, dim3(blocks), dim3(64, 2), diff_lds(d), d->stream, diff_args(d),
+                       (const float*)d->t.ptr, gt, gs, (double*)d->partials.ptr);
   FT_HIP(hipGetLastError());
   return diff_reduce(d, (const double*)d->partials.ptr, (int)blocks, (int)d->n_params, (double*)d->grad.ptr);
 }
@@ -1191,6 +1592,9 @@ int ft_diff_create(uint32_t n_prims, const ft_view* views, uint32_t n_views, con
     c.height = (int)o->height;
     c.aa = 1;
     c.grid = 1;
+    c.tile_w = 8;
+    c.y0 = 0;
+    c.rows = c.height;
   }
   const size_t npix = diff_pixels(d);
   const unsigned blocks = diff_blocks(d);
@@ -1222,7 +1626,7 @@ int ft_diff_destroy(ft_diff* d) {
   (void)hipSetDevice(d->device);
   if (d->stream) (void)hipStreamSynchronize(d->stream);
   for (DevBuf* b : {&d->nodes, &d->th, &d->bounds, &d->cams, &d->t, &d->s, &d->tt, &d->st, &d->gt, &d->gs,
-                    &d->partials, &d->red_a, &d->red_b, &d->grad, &d->m, &d->v, &d->lpart, &d->losses})
+                    &d->partials, &d->red_a, &d->red_b, &d->grad, &d->m, &d->v, &d->lpart, &d->losses, &d->flops})
     if (b->ptr) (void)hipFree(b->ptr);
   if (d->ev0) (void)hipEventDestroy(d->ev0);
   if (d->ev1) (void)hipEventDestroy(d->ev1);
@@ -1368,6 +1772,28 @@ int ft_diff_fit(ft_diff* d, uint32_t steps, float lr, float wt, float ws, double
 int ft_diff_stats(ft_diff* d, ft_render_stats* out) {
   if (!d || !out) return fail(FT_ERR_INVALID_ARGUMENT, "diff or output is null");
   *out = d->stats;
+  out->sdf_flops = 0;
+  if (d->counting) {
+    FT_HIP(hipSetDevice(d->device));
+    FT_HIP(hipStreamSynchronize(d->stream));
+    unsigned long long f[2] = {0, 0};
+    FT_HIP(hipMemcpy(f, d->flops.ptr, sizeof(f), hipMemcpyDeviceToHost));
+    out->sdf_flops = f[0];
+  }
+  return FT_OK;
+}
+
+int ft_diff_set_flop_counting(ft_diff* d, int enable) {
+  if (!d) return fail(FT_ERR_INVALID_ARGUMENT, "diff is null");
+  FT_HIP(hipSetDevice(d->device));
+  FT_HIP(hipStreamSynchronize(d->stream));
+  if (enable) {
+    int rc = ensure(d->flops, 2 * sizeof(unsigned long long));
+    if (rc) return rc;
+    hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, d->stream, (unsigned long long*)d->flops.ptr, 2);
+    FT_HIP(hipGetLastError());
+  }
+  d->counting = enable != 0;
   return FT_OK;
 }
 

@@ -456,11 +456,13 @@ __device__ __noinline__ float lower_bound_root(int slot, float3 p) {
 }
 
 // accel_leaf_distance_info (renderer.rs:629-657) of leaf li, the same leaf for every lane
-// (li is wave-uniform): record and node fetches are scalar loads.
-__device__ __forceinline__ Info leaf_info_uniform(const DevScene& S, int slot, int li, float3 p) {
+// (li is wave-uniform): record and node fetches are scalar loads.  far_shortcut = false
+// evaluates a custom leaf exactly wherever p is (sdf_distance_info of the leaf's subtree).
+__device__ __forceinline__ Info leaf_info_uniform(const DevScene& S, int slot, int li, float3 p,
+                                                  bool far_shortcut = true) {
   const DLeaf L = load_uniform(uniform_ptr(S.leaves) + li);
   Info o;
-  if (L.is_custom) {
+  if (L.is_custom && far_shortcut) {
     const float lb = aabb_lb(p, L.bmin, L.bmax);
     const float ex = L.bmax[0] - L.bmin[0], ey = L.bmax[1] - L.bmin[1], ez = L.bmax[2] - L.bmin[2];
     const float far_t = clampf(sqrtf(ex * ex + ey * ey + ez * ez) * 0.1f, 0.05f, 0.5f);
@@ -494,17 +496,29 @@ __shared__ int ft_bvh_stack[4][kMaxBvhStack];
 // one stack of BVH nodes: a node is entered when its lower bound does not exceed some lane's
 // best distance, each lane testing the reference's `node_lb > best` against its own best and
 // evaluating only the leaves that pass it; of two children the one most lanes find nearer is
-// visited first.  Every lane still gets the minimum over the leaves its own pruning leaves
-// open (the leaves are visited in an order common to the wave instead of each lane's
-// near-first order, which can only change which of two equally distant leaves names the
-// hit).  Nodes and leaves are fetched with scalar loads; no per-lane stack.  All lanes of the
-// wave that are active on entry must call it together.
+// visited first.  For a lane whose best is >= 0 the answer is the minimum over the leaves
+// its own pruning leaves open, whatever the visiting order (which can only change which of
+// two equally distant leaves names the hit).
+//
+// A negative best is order-dependent: every lower bound is >= 0, so the reference's
+// depth-first walk (nearer child first, lhs on ties) prunes all that follows the first leaf
+// with a negative distance, and returns that leaf.  Such a leaf's bounds contain p, and
+// between two children whose bounds both contain p the reference goes left, so its answer
+// is the leftmost such leaf of the tree: the least pre-order node index here (the tree is
+// stored parent first, lhs subtree before rhs).  The bricks grid's shell search (renderer.rs:
+// 1024-1129) visits p's cell first, its leaves in ascending leaf index, so there the answer is
+// the least leaf index.  A lane with a negative best therefore keeps open the nodes whose
+// bounds contain p and that can still hold an earlier leaf, and takes a negative leaf by
+// rank.  Nodes and leaves are fetched with scalar loads; no per-lane stack.  All lanes of
+// the wave that are active on entry must call it together.
 __device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
   const auto bvh = uniform_ptr(S.bvh);
   int* const stack = ft_bvh_stack[threadIdx.y];
+  const bool grid_order = S.accel == FT_ACCEL_BRICKS;
   int sp = 0;
   float best = __builtin_huge_valf();
+  int best_rank = 0x7fffffff;  // rank of the leaf holding a negative best
   Info bi;
   bool have = false;
   stack[sp++] = 0;
@@ -512,13 +526,24 @@ __device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
     const int ni = __builtin_amdgcn_readfirstlane(stack[--sp]);
     const DBvh nd = load_uniform(bvh + ni);
     if (S.flops) atomicAdd(S.flops, 3ull * kAabbFlops);
-    const bool need = !(aabb_lb(p, nd.bmin, nd.bmax) > best);
+    const float lb = aabb_lb(p, nd.bmin, nd.bmax);
+    const bool need = !(lb > best) || (best < 0.0f && lb == 0.0f && (grid_order || ni < best_rank));
     if (__builtin_amdgcn_ballot_w64(need) == 0ull) continue;
     if (nd.left < 0) {
       if (need) {
-        const Info info = leaf_info_uniform(S, slot, -nd.left - 1, p);
-        if (info.d < best) best = info.d;
-        if (!have || info.d < bi.d) { bi = info; have = true; }
+        const int li = -nd.left - 1;
+        const Info info = leaf_info_uniform(S, slot, li, p);
+        const int rank = grid_order ? li : ni;
+        const bool take = best < 0.0f ? (info.d < 0.0f && rank < best_rank) : info.d < best;
+        if (take) {
+          best = info.d;
+          best_rank = rank;
+          bi = info;
+          have = true;
+        } else if (!have) {
+          bi = info;
+          have = true;
+        }
       }
       continue;
     }
@@ -532,6 +557,50 @@ __device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
   if (!have) bi = eval_node(slot, S.root, p);
   return bi;
 }
+// The accel leaf that the reference's tree walks from the root reach at p
+// (resolve_material_from_node renderer.rs:6945-7059, resolve_surface_normal_from_node
+// renderer.rs:4462-4570).  Over a spine of hard unions each walk descends to the child of
+// smaller exact distance, lhs on ties, so it ends at the leaf of least distance, the least
+// leaf index among equals (leaves are numbered left to right).  That is not always the leaf
+// bvh_distance_info names: inside several primitives the accelerators stop at the first
+// negative leaf.  Here a node is entered while its bounds lie within max(best, 0), so every
+// leaf whose bounds contain p is evaluated once the best is negative; custom leaves are
+// evaluated exactly (no far shortcut).  Returns -1 only for an empty tree.  All lanes of the
+// wave that are active on entry must call it together.
+__device__ __noinline__ int bvh_walk_leaf(int slot, float3 p) {
+  const DevScene& S = scene_of(slot);
+  const auto bvh = uniform_ptr(S.bvh);
+  int* const stack = ft_bvh_stack[threadIdx.y];
+  int sp = 0;
+  float best = __builtin_huge_valf();
+  int leaf = -1;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const int ni = __builtin_amdgcn_readfirstlane(stack[--sp]);
+    const DBvh nd = load_uniform(bvh + ni);
+    const bool need = aabb_lb(p, nd.bmin, nd.bmax) <= fmaxf(best, 0.0f);
+    if (__builtin_amdgcn_ballot_w64(need) == 0ull) continue;
+    if (nd.left < 0) {
+      if (need) {
+        const int li = -nd.left - 1;
+        const float d = leaf_info_uniform(S, slot, li, p, false).d;
+        if (leaf < 0 || d < best || (d == best && li < leaf)) {
+          best = d;
+          leaf = li;
+        }
+      }
+      continue;
+    }
+    const DBvh L = load_uniform(bvh + nd.left);
+    const DBvh R = load_uniform(bvh + nd.right);
+    const float llb = aabb_lb(p, L.bmin, L.bmax), rlb = aabb_lb(p, R.bmin, R.bmax);
+    const bool left_first = wave_majority(!need || llb <= rlb);
+    stack[sp++] = left_first ? nd.right : nd.left;
+    stack[sp++] = left_first ? nd.left : nd.right;
+  }
+  return leaf;
+}
+
 // bvh_lower_bound (renderer.rs:788-822) for the wave's points, one node stack per wave.
 __device__ __noinline__ float bvh_lower_bound(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
@@ -604,7 +673,12 @@ namespace ftd {
 constexpr int kBrickSide = 9;                 // samples per brick edge (8 voxels)
 constexpr int kBrickSamples = kBrickSide * kBrickSide * kBrickSide;
 
-// Approximate |distance| from the atlas, or -1 where the lane must evaluate exactly.
+// A lower bound of |distance| from the atlas, or -1 where the lane must evaluate exactly.
+// The scene distance is 1-Lipschitz, so inside a voxel of size h the trilinear value of the
+// corner samples exceeds the true |distance| by at most the weighted corner distance,
+// sqrt(3) h / 2; FP16 corners add a relative rounding of 2^-11 each.  Subtracting both
+// makes every coarse step conservative: it never passes a surface the exact march would
+// stop at.
 __device__ __forceinline__ float atlas_distance(const DevScene& S, float3 p) {
   const float cell = 8.0f * S.atlas_voxel;
   const float qx = (p.x - S.atlas_origin[0]) / cell, qy = (p.y - S.atlas_origin[1]) / cell,
@@ -634,7 +708,9 @@ __device__ __forceinline__ float atlas_distance(const DevScene& S, float3 p) {
   const float x00 = c[0] + (c[1] - c[0]) * fx, x10 = c[2] + (c[3] - c[2]) * fx;
   const float x01 = c[4] + (c[5] - c[4]) * fx, x11 = c[6] + (c[7] - c[6]) * fx;
   const float y0 = x00 + (x10 - x00) * fy, y1 = x01 + (x11 - x01) * fy;
-  return fabsf(y0 + (y1 - y0) * fz);
+  const float v = fabsf(y0 + (y1 - y0) * fz);
+  const float rel = S.atlas_fp16 ? 1.0f / 1024.0f : 1.0f / 4194304.0f;
+  return fmaxf(v - 0.8660254f * S.atlas_voxel - v * rel, 0.0f);
 }
 
 struct AtlasCoarse {

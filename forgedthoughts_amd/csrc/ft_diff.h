@@ -22,6 +22,15 @@
 
 namespace ftd {
 
+// FLOPs per lane of one tree pass (FMA = 2; add, mul, min, max, abs, compare, sqrt = 1):
+// a sphere leaf translate 3 + length 6 + radius 1, a union_round join 13; a reverse pass
+// adds per sphere 19 (normalised offset, gradient) and per join 22 (partials, child
+// adjoints); one that accumulates parameter gradients adds per sphere 29 (seeded products
+// and four 64-lane sums of 6 adds) and per join 10.
+constexpr unsigned long long kDiffEvalSphere = 10, kDiffEvalJoin = 13;
+constexpr unsigned long long kDiffRevSphere = 19, kDiffRevJoin = 22;
+constexpr unsigned long long kDiffAccSphere = 29, kDiffAccJoin = 10;
+
 struct DiffArgs {
   const int4* nodes;      // per node: leaf sphere (or -1), join (or -1), lhs node, rhs node
   const float* th;        // parameters
@@ -70,16 +79,35 @@ struct DiffInfo {
 struct DiffEval {
   const DiffArgs* a;
   float* vals;
+  unsigned* evals;  // per-lane count of tree evaluations (work accounting)
   __device__ __forceinline__ DiffInfo operator()(float3 p) const {
     DiffInfo i;
+    ++*evals;
     i.d = diff_eval(*a, vals, p);
     i.obj = 0;
     i.mat = 0;
     i.leaf = 0;
     return i;
   }
-  __device__ __forceinline__ float dist(float3 p) const { return diff_eval(*a, vals, p); }
+  __device__ __forceinline__ float dist(float3 p) const {
+    ++*evals;
+    return diff_eval(*a, vals, p);
+  }
 };
+
+// Adds the FLOPs of `evals` plain passes, `revs` reverse passes and `accs` accumulating
+// reverse passes of this lane to *flops (one atomic per wave).
+__device__ __forceinline__ void diff_count(const DiffArgs& a, unsigned evals, unsigned revs, unsigned accs,
+                                          unsigned long long* flops) {
+  const unsigned long long np = (unsigned long long)a.n_prims, nj = np - 1;
+  const unsigned long long ev = kDiffEvalSphere * np + kDiffEvalJoin * nj;
+  const unsigned long long rv = ev + kDiffRevSphere * np + kDiffRevJoin * nj;
+  const unsigned long long ac = rv + kDiffAccSphere * np + kDiffAccJoin * nj;
+  unsigned long long f = ev * evals + rv * revs + ac * accs;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) f += __shfl_xor(f, m, 64);
+  if (threadIdx.x == 0) atomicAdd(flops, f);
+}
 
 __device__ __forceinline__ float wave_sum(float x) {
 #pragma unroll
@@ -155,8 +183,11 @@ __device__ __forceinline__ bool diff_item(const DiffArgs& a, int& view, int& til
   return true;
 }
 
-// Forward: t*, s per pixel (t* = -1, s = 0 on a miss).
-__device__ __forceinline__ void diff_forward_body(DiffArgs a, float* __restrict__ t_out, float* __restrict__ s_out) {
+// Forward: t*, s per pixel (t* = -1, s = 0 on a miss).  kCounted adds the work of the
+// image's lanes to *flops.
+template <bool kCounted>
+__device__ __forceinline__ void diff_forward_body(DiffArgs a, float* __restrict__ t_out, float* __restrict__ s_out,
+                                                  unsigned long long* flops) {
   int view, tile;
   CamArgs cam;
   if (!diff_item(a, view, tile, cam)) return;
@@ -165,7 +196,8 @@ __device__ __forceinline__ void diff_forward_body(DiffArgs a, float* __restrict_
   float* vals = diff_vals(a);
   float3 o, d;
   primary_ray(cam, inside ? x : 0, inside ? y : 0, 0, o, d);
-  const DiffEval ev{&a, vals};
+  unsigned n_evals = 0;
+  const DiffEval ev{&a, vals, &n_evals};
   Hit hit;
   const float bmin[3] = {a.bounds[0], a.bounds[1], a.bounds[2]};
   const float bmax[3] = {a.bounds[3], a.bounds[4], a.bounds[5]};
@@ -184,6 +216,7 @@ __device__ __forceinline__ void diff_forward_body(DiffArgs a, float* __restrict_
   const float3 G = mk(gx, gy, gz);
   const float gl = len(G);
   const float3 n = gl > 1.0e-12f ? mul(G, 1.0f / gl) : mk(0.0f, 1.0f, 0.0f);
+  if (kCounted) diff_count(a, inside ? n_evals + 6 : 0, inside ? 1 : 0, 0, flops);
   if (!inside) return;
   const size_t idx = (size_t)view * cam.width * cam.height + (size_t)y * cam.width + x;
   t_out[idx] = ts;
@@ -201,9 +234,11 @@ __device__ __forceinline__ bool interior_hit(const float* __restrict__ t, int w,
 }
 
 // Adjoint: accumulates sum over pixels of dL/dth into per-block partial sums (FP64).
+// kCounted adds the work of the lanes with a non-zero upstream gradient to *flops.
+template <bool kCounted>
 __device__ __forceinline__ void diff_adjoint_body(DiffArgs a, const float* __restrict__ t_img,
                                                   const float* __restrict__ gt_img, const float* __restrict__ gs_img,
-                                                  double* __restrict__ partials) {
+                                                  double* __restrict__ partials, unsigned long long* flops) {
   const int nth = blockDim.x * blockDim.y;
   const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   double* blk = diff_acc();
@@ -263,6 +298,7 @@ __device__ __forceinline__ void diff_adjoint_body(DiffArgs a, const float* __res
       const float gd = dot(g, d);
       const float s0 = (active && fabsf(gd) > 1.0e-6f) ? -(gt + c) / gd : 0.0f;
       diff_reverse(a, vals, xs, s0, true, g);
+      if (kCounted) diff_count(a, active ? 6 : 0, active ? 1 : 0, active ? 7 : 0, flops);
     }
   }
   __syncthreads();
@@ -354,11 +390,21 @@ __device__ __forceinline__ void diff_adam_body(float* __restrict__ th, const dou
 
 #define FT_DIFF_KERNEL extern "C" __global__ __launch_bounds__(128)
 FT_DIFF_KERNEL void ft_k_diff_forward(ftd::DiffArgs a, float* __restrict__ t, float* __restrict__ s) {
-  ftd::diff_forward_body(a, t, s);
+  ftd::diff_forward_body<false>(a, t, s, nullptr);
 }
 FT_DIFF_KERNEL void ft_k_diff_adjoint(ftd::DiffArgs a, const float* __restrict__ t, const float* __restrict__ gt,
                                       const float* __restrict__ gs, double* __restrict__ partials) {
-  ftd::diff_adjoint_body(a, t, gt, gs, partials);
+  ftd::diff_adjoint_body<false>(a, t, gt, gs, partials, nullptr);
+}
+// FLOP-counting variants (named apart so kernel traces keep them out of the timed runs)
+FT_DIFF_KERNEL void ft_k_diff_forward_counted(ftd::DiffArgs a, float* __restrict__ t, float* __restrict__ s,
+                                              unsigned long long* flops) {
+  ftd::diff_forward_body<true>(a, t, s, flops);
+}
+FT_DIFF_KERNEL void ft_k_diff_adjoint_counted(ftd::DiffArgs a, const float* __restrict__ t,
+                                              const float* __restrict__ gt, const float* __restrict__ gs,
+                                              double* __restrict__ partials, unsigned long long* flops) {
+  ftd::diff_adjoint_body<true>(a, t, gt, gs, partials, flops);
 }
 extern "C" __global__ __launch_bounds__(256) void ft_k_diff_loss(int width, int height, int n_views,
                                                                  const float* __restrict__ t, const float* __restrict__ s,

@@ -371,10 +371,11 @@ class Emitter {
     std::ostringstream o;
     o << "__device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w,\n"
       << "                                          const HookCtx* hc) {\n"
-      << "  MatPool pool;\n  pool.used = 0u;\n  pool.overflow = 0;\n  w.split = false;\n" << body_.str()
+      << "  MatPool& pool = w.pool;\n  pool.used = 0u;\n  pool.overflow = 0;\n  w.split = false;\n" << body_.str()
       << "  if (S.flops) atomicAdd(S.flops, " << (unsigned long long)(prim_cost_ + 0.5) << "ull);\n"
-      << "  w.normal = " << v.n << ";\n  w.mat_overflow = pool.overflow != 0;\n"
-      << "  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, " << v.m << ", p, w.a, hc);\n}\n";
+      << "  w.normal = " << v.n << ";\n  if (!w.split) w.a = " << v.m << ";\n"
+      << "  w.mat_overflow = pool.overflow != 0 || !finish_ref(slot, pool, w.a, p, hc) ||\n"
+      << "                   (w.split && !finish_ref(slot, pool, w.b, p, hc));\n}\n";
     return o.str();
   }
 
@@ -518,11 +519,13 @@ class Emitter {
     out.n = fresh("sn");
     body_ << "  const float3 " << out.n << " = len(" << bn << ") > 1.0e-6f ? normalize(" << bn << ") : (" << t
           << " < 0.5f ? " << a.n << " : " << rn << ");\n";
-    if (i == split_) {
-      body_ << "  ref_to_mat(slot, pool, " << a.m << ", p, w.a, hc);\n  ref_to_mat(slot, pool, " << b.m << ", p, w.b, hc);\n"
-            << "  w.split_t = " << t << ";\n  w.split = true;\n";
-    }
     out.m = fresh("sm");
+    if (i == split_) {
+      // the split node's children keep their references (only unary nodes lie above it)
+      body_ << "  w.a = " << a.m << ";\n  w.b = " << b.m << ";\n  w.split_t = " << t << ";\n  w.split = true;\n"
+            << "  const MRef " << out.m << " = " << a.m << ";\n";
+      return out;
+    }
     body_ << "  MRef " << out.m << " = " << a.m << ";\n  blend_refs(slot, pool, " << out.m << ", " << b.m << ", " << t
           << ", p, hc);\n";
     return out;

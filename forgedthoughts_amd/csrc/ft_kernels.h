@@ -16,6 +16,8 @@ namespace ftd {
 struct CamArgs {
   float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
   int width, height, aa, grid;
+  int tile_w;  // pixel tile of one wave: tile_w x (64 / tile_w), tile_w in {8, 16, 32}
+  int y0, rows;  // the launch covers image rows [y0, y0 + rows) (a band of a progressive render)
 };
 
 __device__ __forceinline__ void primary_ray(const CamArgs& c, int x, int y, int s, float3& o, float3& d) {
@@ -37,8 +39,18 @@ __device__ __forceinline__ float depth_preview_value(float t, float max_dist) { 
   return s * s;
 }
 
-// Pixels are visited in 8x8 tiles per wavefront (blockDim 64x4: four tiles per block) so
-// that the 64 lanes of a wave march spatially adjacent rays.
+// Pixels are visited in tiles of 64 per wavefront (8x8, 16x4 or 32x2, chosen per launch;
+// blockDim 64 x waves per block) so that the lanes of a wave march adjacent rays.
+__device__ __forceinline__ bool tile_xy(const CamArgs& c, int tile, int& x, int& y) {
+  const int lane = threadIdx.x;
+  const int tw = c.tile_w, th = 64 / tw;
+  const int tiles_x = (c.width + tw - 1) / tw;
+  const int tx = tile % tiles_x, ty = tile / tiles_x;
+  x = tx * tw + lane % tw;
+  y = c.y0 + ty * th + lane / tw;
+  return x < c.width && y < c.y0 + c.rows;
+}
+// 8x8 tiles (the differentiable kernels)
 __device__ __forceinline__ bool tile_xy(int width, int height, int tile, int& x, int& y) {
   const int lane = threadIdx.x;
   const int tiles_x = (width + 7) / 8;
@@ -52,7 +64,7 @@ __device__ __forceinline__ int wave_index() { return blockIdx.x * blockDim.y + t
 __device__ __forceinline__ void depth_tile(int slot, const CamArgs& cam, const TraceParams& tp, int tile,
                                            unsigned char* __restrict__ rgb, float* __restrict__ t_out) {
   int x, y;
-  if (!tile_xy(cam.width, cam.height, tile, x, y)) return;
+  if (!tile_xy(cam, tile, x, y)) return;
   float sum = 0.0f;
   float first_t = -1.0f;
   for (int s = 0; s < cam.aa; ++s) {
@@ -75,7 +87,7 @@ template <int kRayStack>
 __device__ __forceinline__ void whitted_tile(int slot, const CamArgs& cam, const TraceParams& tp, int aov, int tile,
                                              int s, float* __restrict__ samples, int* __restrict__ overflow) {
   int x, y;
-  if (!tile_xy(cam.width, cam.height, tile, x, y)) return;
+  if (!tile_xy(cam, tile, x, y)) return;
   float3 o, d;
   primary_ray(cam, x, y, s, o, d);
   float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow);
@@ -198,9 +210,9 @@ __device__ __forceinline__ void atlas_bricks_body(int slot, AtlasGrid g, const i
 // image k start at k times the per-image size).
 __device__ __forceinline__ void resolve_kernel_body(CamArgs cam, int aov, const float* __restrict__ samples,
                                                  unsigned char* __restrict__ rgb, float* __restrict__ hdr) {
-  const int pix = blockIdx.x * blockDim.x + threadIdx.x;
   const int npix = cam.width * cam.height;
-  if (pix >= npix) return;
+  const int pix = cam.y0 * cam.width + blockIdx.x * blockDim.x + threadIdx.x;
+  if (pix >= (cam.y0 + cam.rows) * cam.width) return;
   samples += (size_t)blockIdx.y * cam.aa * npix * 3;
   rgb += (size_t)blockIdx.y * npix * 3;
   if (hdr) hdr += (size_t)blockIdx.y * npix * 3;
@@ -227,8 +239,13 @@ __device__ __forceinline__ void clear_u64_body(unsigned long long* v, int n) {
 
 }  // namespace ftd
 
+// FT_WAVES_PER_EU: the occupancy the march kernels are compiled for (their VGPR budget,
+// 512 / waves); the launch-shape autotuner builds scene-specialised variants of it.
+#ifndef FT_WAVES_PER_EU
+#define FT_WAVES_PER_EU 4
+#endif
 #define FT_KERNEL extern "C" __global__ __launch_bounds__(256)
-#define FT_MARCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(4)))
+#define FT_MARCH_OCCUPANCY __attribute__((amdgpu_waves_per_eu(FT_WAVES_PER_EU)))
 
 // FT_RTC_KERNEL selects the one kernel a runtime-compiled (scene-specialised) code object
 // holds: 0 depth, 1/2/3 Whitted with a 16/48/128-entry ray stack.  The precompiled library

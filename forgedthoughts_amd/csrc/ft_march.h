@@ -41,6 +41,10 @@ __device__ __forceinline__ bool ray_aabb(float3 o, float3 d, const float* mn, co
 // and lanes in different phases still share every evaluation; the final distance_info at
 // the hit (which also names the primitive) is taken once per ray after the loop.
 // No coarse distance: every march step evaluates the scene.
+#ifndef FT_MARCH_UNROLL
+#define FT_MARCH_UNROLL 1
+#endif
+
 struct NoCoarse {
   __device__ __forceinline__ bool enabled() const { return false; }
   __device__ __forceinline__ float operator()(float3) const { return -1.0f; }
@@ -69,10 +73,11 @@ __device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin,
   float lo = 0.0f, hi = 0.0f, mid = 0.0f, dplus = 0.0f;
   float3 pos = mk(0, 0, 0), grad = mk(0, 0, 0);
   bool exact = !coarse.enabled();
-  for (;;) {
+  // One march step (or bisection step, or normal sample): 0 go on, 1 miss, 2 done.
+  auto step = [&]() -> int {
     float3 p;
     if (phase == 0) {
-      if (steps >= max_steps || traveled > max_t) return false;
+      if (steps >= max_steps || traveled > max_t) return 1;
       p = add(o, mul(d, traveled));
       if (!exact) {
         const float dc = coarse(p);
@@ -80,7 +85,7 @@ __device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin,
           prev = traveled;
           traveled += fmaxf(dc * sscale, min_step);
           ++steps;
-          continue;
+          return 0;
         }
         exact = true;
       }
@@ -119,8 +124,21 @@ __device__ __forceinline__ bool raymarch_core(const Eval& ev, const float* bmin,
         const int axis = k >> 1;
         grad = mk(axis == 0 ? g : grad.x, axis == 1 ? g : grad.y, axis == 2 ? g : grad.z);
       }
-      if (++k == 6) break;
+      if (++k == 6) return 2;
     }
+    return 0;
+  };
+  // FT_MARCH_UNROLL steps per loop iteration (a launch-shape autotuner parameter); the
+  // step sequence of every lane is the same for any value
+  for (;;) {
+    int r = 0;
+#pragma unroll
+    for (int u = 0; u < FT_MARCH_UNROLL; ++u) {
+      r = step();
+      if (r != 0) break;
+    }
+    if (r == 1) return false;
+    if (r == 2) break;
   }
   const auto inf = ev(pos);
   const float3 nrm = len(grad) > 1.0e-8f ? normalize(grad) : mk(0.0f, 1.0f, 0.0f);

@@ -21,7 +21,13 @@ __device__ __forceinline__ float dot(float3 a, float3 b) { return a.x * b.x + a.
 // correctly rounded sequence sqrtf expands to (about fifteen instructions, most of a
 // primitive's cost).  Distances differ from IEEE sqrt by at most an ulp; arguments here are
 // sums of squares far above the denormal range or exactly zero.
+#if defined(FT_STRICT_MATH)
+// strict arithmetic (FT_STRICT_MATH=1 at run time): correctly rounded square roots and no
+// multiply-add contraction, the oracle's arithmetic
+__device__ __forceinline__ float fsqrt(float x) { return sqrtf(x); }
+#else
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+#endif
 __device__ __forceinline__ float len(float3 a) { return fsqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
 __device__ __forceinline__ float3 cross(float3 a, float3 b) {
   return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);

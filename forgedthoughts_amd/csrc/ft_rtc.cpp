@@ -53,12 +53,29 @@ const std::vector<std::string>& options() {
 
 }  // namespace
 
-bool rtc_compile(const std::string& source, int kernel, bool counted, std::vector<char>& code, double& compile_ms,
-                 std::string& err) {
+bool strict_math() {
+  const char* v = std::getenv("FT_STRICT_MATH");
+  return v && v[0] == '1';
+}
+
+std::string cache_file(const std::string& name) {
+  const std::string dir = cache_dir();
+  return dir.empty() ? std::string() : dir + "/" + name;
+}
+
+bool rtc_compile(const std::string& source, int kernel, bool counted, int waves_per_eu, int unroll,
+                 int shade_inline, std::vector<char>& code, double& compile_ms, std::string& err) {
   compile_ms = 0.0;
   std::vector<std::string> opts = options();
   opts.push_back("-DFT_RTC_KERNEL=" + std::to_string(kernel));
   if (counted) opts.push_back("-DFT_COUNTED=1");
+  if (strict_math()) {
+    opts.push_back("-ffp-contract=off");
+    opts.push_back("-DFT_STRICT_MATH=1");
+  }
+  if (waves_per_eu != 4) opts.push_back("-DFT_WAVES_PER_EU=" + std::to_string(waves_per_eu));
+  if (unroll != 1) opts.push_back("-DFT_MARCH_UNROLL=" + std::to_string(unroll));
+  if (shade_inline) opts.push_back("-DFT_SHADE_INLINE=1");
   std::string key_text = source;
   for (const auto& o : opts) key_text += "\n" + o;
   for (int i = 0; i < kRtcHeaderCount; ++i) key_text += kRtcHeaderSources[i];

@@ -15,6 +15,14 @@
 
 #include "ft_device.h"
 
+// FT_SHADE_INLINE=1 inlines the shading functions (hit_material, shade_color) into the
+// Whitted loop (a code variant of the launch-shape autotuner); by default they are calls.
+#if defined(FT_SHADE_INLINE) && FT_SHADE_INLINE
+#define FT_SHADE_FN __forceinline__
+#else
+#define FT_SHADE_FN __noinline__
+#endif
+
 namespace ftd {
 
 enum { TAG_STANDARD = 0, TAG_LAMBERT = 1, TAG_METAL = 2, TAG_DIELECTRIC = 3 };
@@ -96,6 +104,13 @@ struct HookCtx {
   float front;
 };
 
+// the clamps resolve_leaf_material applies to every leaf set (renderer.rs:7472-7530)
+__device__ __forceinline__ void clamp_leaf(int model, DParams& a) {
+  if (model != TAG_LAMBERT) a.roughness = clampf(a.roughness, 0.0f, 1.0f);
+  if (model == TAG_STANDARD || model == TAG_DIELECTRIC) a.ior = clampf(a.ior, 1.0f, 3.0f);
+  a.emission_strength = fmaxf(a.emission_strength, 0.0f);
+}
+
 // resolve_leaf_material (renderer.rs:7164-7536): the parameter set of a leaf material at
 // `pos`; returns its model.  With a shading context, the material's per-hit hooks replace
 // the static colour, roughness, ior, thin_walled and emission before the clamps (the arms
@@ -159,9 +174,7 @@ __device__ inline int leaf_params(int slot, unsigned mat_id, unsigned obj, float
       set3(a.color, ((sx + sy + sz) & 1) == 0 ? v3(a.pattern_a) : v3(a.pattern_b));
     }
   }
-  if (model != TAG_LAMBERT) a.roughness = clampf(a.roughness, 0.0f, 1.0f);
-  if (model == TAG_STANDARD || model == TAG_DIELECTRIC) a.ior = clampf(a.ior, 1.0f, 3.0f);
-  a.emission_strength = fmaxf(a.emission_strength, 0.0f);
+  clamp_leaf(model, a);
   return model;
 }
 // resolve_dynamic_bump / resolve_dynamic_normal / orthonormal_tangent (renderer.rs:7749-7827):
@@ -489,14 +502,52 @@ __device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, con
   l = out;
 }
 
+// The result of one hit's walks: shading normal and material references.  Parameter sets
+// stay where they are (the scene's material table, or the pool for lerped and hooked sets)
+// and are read when a BSDF needs them, so no per-hit copy of the parameters is made.
 struct ShadeWalk {
   float3 normal;
-  Mat a;        // the walk material, or (split) the split node's lhs material
-  Mat b;        // (split) the split node's rhs material
+  MRef a;        // the walk material, or (split) the split node's lhs material
+  MRef b;        // (split) the split node's rhs material
   float split_t;
   bool split;
   bool mat_overflow;  // the material pool ran out: materials come from the per-node walks
+  MatPool pool;
 };
+
+// The parameter set of a handle after finish_refs: a pool entry, or a leaf material without
+// hooks or pattern read from the scene's table (with resolve_leaf_material's clamps).
+__device__ __forceinline__ void load_ref(int slot, const MatPool& pool, int h, DParams& out) {
+  if (h < 0) {
+    out = pool.e[-1 - h];
+    return;
+  }
+  const DevScene& S = scene_of(slot);
+  const unsigned mat_id = S.nodes[h & ~kLerped].mat;
+  out = S.mats[mat_id < (unsigned)S.n_mats ? mat_id : 0u];
+  clamp_leaf((int)out.model, out);
+  if (h & kLerped) out.hook = -1.0f;
+}
+// Leaf handles whose set depends on the hit (per-hit hooks, a pattern) are resolved once,
+// into pool entries; false when the pool ran out.
+__device__ __forceinline__ bool finish_handle(int slot, MatPool& pool, int& h, float3 p, const HookCtx* hc) {
+  if (h < 0) return true;
+  const DevScene& S = scene_of(slot);
+  const unsigned mat_id = S.nodes[h & ~kLerped].mat;
+  const DParams& m = S.mats[mat_id < (unsigned)S.n_mats ? mat_id : 0u];
+  if (m.hook < 0.0f && m.pattern == 0.0f) return true;
+  const int i = pool_take(pool);
+  if (i < 0) return false;
+  ref_params(slot, pool, h, p, pool.e[i], hc);
+  h = -1 - i;
+  return true;
+}
+__device__ __forceinline__ bool finish_ref(int slot, MatPool& pool, MRef& r, float3 p, const HookCtx* hc) {
+  return finish_handle(slot, pool, r.ha, p, hc) && (!r.blend || finish_handle(slot, pool, r.hb, p, hc));
+}
+// the dominant side of a reference (dominant_material_model / _params, renderer.rs:7653-7683)
+__device__ __forceinline__ int dom_model(const MRef& m) { return (!m.blend || m.t < 0.5f) ? m.a_model : m.b_model; }
+__device__ __forceinline__ int dom_handle(const MRef& m) { return (!m.blend || m.t < 0.5f) ? m.ha : m.hb; }
 
 // The sweep of node N (wave-uniform).  Stack entries: distance / lower bound in the LDS
 // value planes, normal and material per lane in registers / private memory.
@@ -513,7 +564,7 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
   float* __restrict__ slb = ft_lds + S.stack_depth * nth + tid;
   MRef ms[kMaxWalkMats];
   float3 ns[kMaxWalkMats];
-  MatPool pool;
+  MatPool& pool = w.pool;
   pool.used = 0u;
   pool.overflow = 0;
   int sp = 0;
@@ -624,17 +675,22 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
     const float3 right = blend_class(kind) == 2 ? mul(ns[ir], -1.0f) : ns[ir];
     const float3 bn = add(mul(left, 1.0f - tt), mul(right, tt));
     ns[il] = len(bn) > 1.0e-6f ? normalize(bn) : (tt < 0.5f ? left : right);
-    if (ni == split_node) {  // resolve_split_material_from_node (renderer.rs:7061-7120)
-      ref_to_mat(slot, pool, ms[il], p, w.a, hc);
-      ref_to_mat(slot, pool, ms[ir], p, w.b, hc);
+    if (ni == split_node) {
+      // resolve_split_material_from_node (renderer.rs:7061-7120): the split node's children
+      // keep their references (only unary nodes lie above it, so no later blend touches
+      // their pool entries, and the walk material above it is not used)
+      w.a = ms[il];
+      w.b = ms[ir];
       w.split_t = tt;
       w.split = true;
+      continue;
     }
     blend_refs(slot, pool, ms[il], ms[ir], tt, p, hc);
   }
   w.normal = ns[0];
-  w.mat_overflow = pool.overflow != 0;
-  if (!w.split && !w.mat_overflow) ref_to_mat(slot, pool, ms[0], p, w.a, hc);
+  if (!w.split) w.a = ms[0];
+  w.mat_overflow = pool.overflow != 0 || !finish_ref(slot, pool, w.a, p, hc) ||
+                   (w.split && !finish_ref(slot, pool, w.b, p, hc));
 }
 
 #if defined(FT_EMITTED) && FT_EMITTED_SWEEP
@@ -642,6 +698,30 @@ __device__ __forceinline__ void shade_sweep_uniform(const DevScene& S, int slot,
 __device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p, float eps, ShadeWalk& w,
                                           const HookCtx* hc);
 #endif
+
+// The node where the reference's root walks for the hit at p begin to differ per leaf: the
+// leaf they reach over a hard-union spine, or the root where the scene has no accel leaves
+// or its leaves cannot be trusted to hold the walk (root_walks).
+__device__ __forceinline__ int walk_start(int slot, const Hit& hit) {
+  const DevScene& S = scene_of(slot);
+  if (hit.leaf < 0 || S.root_walks) return S.root;
+  if (S.accel == FT_ACCEL_NAIVE || S.n_bvh == 0) return S.leaves[hit.leaf].node;
+  const int li = bvh_walk_leaf(slot, hit.pos);
+  return li >= 0 ? S.leaves[li].node : S.root;
+}
+
+// split_material / walk_material from `start` (the reference's per-node walks), their
+// parameter sets stored in the emptied pool (at most four entries).
+__device__ __noinline__ void walks_to_refs(int slot, int start, float3 p, ShadeWalk& w, const HookCtx* hc) {
+  Mat ma, mb;
+  w.split = split_material(slot, start, p, ma, mb, w.split_t, hc);
+  if (!w.split) walk_material(slot, start, p, ma, 0, hc);
+  w.pool.used = 0u;
+  w.pool.overflow = 0;
+  w.a = ref_of_mat(w.pool, ma);
+  if (w.split) w.b = ref_of_mat(w.pool, mb);
+  w.mat_overflow = false;
+}
 
 // The three walks from `start` for one hit: hard booleans first (per lane, the walks'
 // side choice), then the sweep of the node reached, lanes grouped by that node.
@@ -663,8 +743,7 @@ __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float ep
   }
   if (S.nodes[node].walk_split == -2 || !S.nodes[node].walk_ok) {
     w.normal = walk_normal(slot, start, p, eps);
-    w.split = split_material(slot, start, p, w.a, w.b, w.split_t, hc);
-    if (!w.split) walk_material(slot, start, p, w.a, 0, hc);
+    walks_to_refs(slot, start, p, w, hc);
     return;
   }
 #if defined(FT_EMITTED) && FT_EMITTED_SWEEP
@@ -676,10 +755,7 @@ __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float ep
     for_each_uniform(node, [&](int lead) { shade_sweep_uniform(S, slot, lead, p, eps, w, hc); });
   }
   w.normal = mul(w.normal, sign);
-  if (w.mat_overflow) {
-    w.split = split_material(slot, start, p, w.a, w.b, w.split_t, hc);
-    if (!w.split) walk_material(slot, start, p, w.a, 0, hc);
-  }
+  if (w.mat_overflow) walks_to_refs(slot, start, p, w, hc);
 }
 
 // ---------------------------------------------------------------- BSDFs
@@ -750,6 +826,23 @@ __device__ inline float3 mat_emission(const Mat& m) {
   if (!m.blend) return emission_of(m.a);
   return lerp_spec(emission_of(m.a), emission_of(m.b), m.t);
 }
+// the same on a material reference (materials/mod.rs:291-305), parameter sets read on use
+__device__ inline float3 ref_eval(int slot, const MatPool& pool, const MRef& m, float3 n, float3 wi, float3 wo) {
+  DParams a;
+  load_ref(slot, pool, m.ha, a);
+  const float3 fa = model_eval(m.a_model, a, n, wi, wo);
+  if (!m.blend) return fa;
+  load_ref(slot, pool, m.hb, a);
+  return lerp_spec(fa, model_eval(m.b_model, a, n, wi, wo), m.t);
+}
+__device__ inline float3 ref_emission(int slot, const MatPool& pool, const MRef& m) {
+  DParams a;
+  load_ref(slot, pool, m.ha, a);
+  const float3 ea = emission_of(a);
+  if (!m.blend) return ea;
+  load_ref(slot, pool, m.hb, a);
+  return lerp_spec(ea, emission_of(a), m.t);
+}
 
 // --------------------------------------------------------------- lights
 // offset_ray_origin (renderer.rs:5067-5078)
@@ -816,15 +909,15 @@ __device__ inline void sample_light(const DLight& L, float3 point, unsigned idx,
   rad = mul(v3(L.rad), 1.0f / (dist * dist));
 }
 
-// shade_color (renderer.rs:4217-4276)
-__device__ __noinline__ float3 shade_color(int slot, const Mat& mat, float3 hit_pos, float3 hit_n, bool front,
-                                           float3 nrm, float3 wo, float eps, float max_dist, unsigned obj,
-                                           unsigned mat_id, float cur_ior) {
+// shade_color (renderer.rs:4217-4276); `hk` is the hook entry of the material's dominant
+// set (-1: none)
+__device__ FT_SHADE_FN float3 shade_color(int slot, const MRef& mat, const MatPool& pool, int hk, float3 hit_pos,
+                                          float3 hit_n, bool front, float3 nrm, float3 wo, float eps,
+                                          float max_dist, unsigned obj, unsigned mat_id, float cur_ior) {
   const DevScene& S = scene_of(slot);
   if (S.n_lights == 0) return mk(1, 1, 1);
   // eval_bsdf (renderer.rs:4905-4920): while the material keeps its dynamic id, its eval hook
   // replaces the BSDF, with the context of bsdf_context_value (renderer.rs:5003-5030)
-  const int hk = (int)dom_params(mat).hook;
   const int ev = hk >= 0 ? S.hooks[hk].eval : -1;
   float3 args[FT_HOOK_PARAMS];
   if (ev >= 0) {
@@ -863,7 +956,7 @@ __device__ __noinline__ float3 shade_color(int slot, const Mat& mat, float3 hit_
         args[8] = wi;
         f = eval_fn(slot, ev, args);
       } else {
-        f = mat_eval(mat, nrm, wi, wo);
+        f = ref_eval(slot, pool, mat, nrm, wi, wo);
       }
       color = add(color, mul(cmul(f, rad), ndotl * sh * inv));
     }
@@ -966,14 +1059,14 @@ __device__ inline void push_lobes(RayStack<kRayStack>& st, float3 hp, float3 gn,
 // trace_hit_with_material (ray.rs:41-222) for one material of a hit; returns the local
 // term (already weighted by `thr`) and pushes the children.
 template <int kRayStack>
-__device__ __noinline__ float3 hit_material(int slot, const TraceParams& tp, const Hit& hit, float3 dir,
-                                            float3 nrm, const Medium& med, int depth, const Mat& mat, float3 thr,
-                                            RayStack<kRayStack>& st) {
-  const DevScene& S = scene_of(slot);
+__device__ FT_SHADE_FN float3 hit_material(int slot, const TraceParams& tp, const Hit& hit, float3 dir,
+                                           float3 nrm, const Medium& med, int depth, const MRef& mat,
+                                           const MatPool& pool, float3 thr, RayStack<kRayStack>& st) {
   const float3 hp = hit.pos;
   const float3 normal = hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f));
   const float3 wo = normalize(mul(dir, -1.0f));
-  const DParams& pr = dom_params(mat);
+  DParams pr;
+  load_ref(slot, pool, dom_handle(mat), pr);
   const float rough = clampf(pr.roughness, 0.0f, 1.0f);
   float transmission, ior, metallic, clearcoat, spec_strength;
   bool thin;
@@ -1015,9 +1108,10 @@ __device__ __noinline__ float3 hit_material(int slot, const TraceParams& tp, con
   const float local_weight = clampf((1.0f - transmission) * (1.0f - reflect_weight), 0.0f, 1.0f);
   float3 color = mk(0, 0, 0);
   if (local_weight != 0.0f)
-    color = mul(shade_color(slot, mat, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist, hit.obj, hit.mat, med.ior),
+    color = mul(shade_color(slot, mat, pool, (int)pr.hook, hp, hit.n, hit.front, nrm, wo, tp.eps, tp.max_dist,
+                            hit.obj, hit.mat, med.ior),
                 local_weight);
-  color = add(color, mat_emission(mat));
+  color = add(color, ref_emission(slot, pool, mat));
 
   if (reflect_weight > 1.0e-4f) {
     float3 tint = lerp_spec(spec_color, base_color, clampf(metallic, 0.0f, 1.0f));
@@ -1148,7 +1242,7 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
     if (!raymarch(slot, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
       c = cmul(cmul(environment(slot, rd), transmittance(med, tp.max_dist)), thr);
     } else {
-      const int start = hit.leaf >= 0 && !S.root_walks ? S.leaves[hit.leaf].node : S.root;
+      const int start = walk_start(slot, hit);
       // build_bsdf_context: tree-walk shading normal at eps 1e-4 (renderer.rs:4397-4449), and
       // the hit's material or split materials, from one shading sweep
       ShadeWalk w;
@@ -1159,13 +1253,14 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
                                            normalize(hit.front ? w.normal : mul(w.normal, -1.0f)));
       const float3 att = cmul(thr, transmittance(med, hit.t));
       c = mk(0, 0, 0);
-      if (w.split) {
-        const float tt = clampf(w.split_t, 0.0f, 1.0f);
-        if (1.0f - tt != 0.0f)
-          c = add(c, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, mul(att, 1.0f - tt), st));
-        if (tt != 0.0f) c = add(c, hit_material(slot, tp, hit, rd, sn, med, r.depth, w.b, mul(att, tt), st));
-      } else {
-        c = hit_material(slot, tp, hit, rd, sn, med, r.depth, w.a, att, st);
+      // the split's two materials, lerped by t (ray.rs:31-35; a side of weight 0 adds
+      // nothing), or the walk material
+      const float tt = w.split ? clampf(w.split_t, 0.0f, 1.0f) : 0.0f;
+      for (int side = 0; side < (w.split ? 2 : 1); ++side) {
+        const float wgt = !w.split ? 1.0f : (side == 0 ? 1.0f - tt : tt);
+        if (wgt == 0.0f) continue;
+        c = add(c, hit_material(slot, tp, hit, rd, sn, med, r.depth, side == 0 ? w.a : w.b, w.pool,
+                                w.split ? mul(att, wgt) : att, st));
       }
     }
     atomicAdd(&acc[r.owner * 3], c.x);
@@ -1181,7 +1276,7 @@ __device__ __noinline__ float3 trace_aov(int slot, const TraceParams& tp, float3
   const DevScene& S = scene_of(slot);
   Hit hit;
   if (!raymarch(slot, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, hit)) return mk(0, 0, 0);
-  const int start = hit.leaf >= 0 && !S.root_walks ? S.leaves[hit.leaf].node : S.root;
+  const int start = walk_start(slot, hit);
   Mat mat;
   const HookCtx hc{hit.pos, hit.front ? normalize(hit.n) : normalize(mul(hit.n, -1.0f)),
                    normalize(mul(d, -1.0f)), hit.front ? 1.0f : 0.0f};

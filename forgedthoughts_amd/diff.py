@@ -40,6 +40,7 @@ def _bind():
     L.ft_diff_loss_grad.argtypes = [vp, C.c_float, C.c_float, vp, vp]
     L.ft_diff_fit.argtypes = [vp, u32, C.c_float, C.c_float, C.c_float, vp]
     L.ft_diff_stats.argtypes = [vp, vp]
+    L.ft_diff_set_flop_counting.argtypes = [vp, C.c_int]
     _bound = True
 
 
@@ -131,6 +132,10 @@ class DiffRenderer:
         st = abi.RenderStatsC()
         _check(library().ft_diff_stats(self._h, C.addressof(st)))
         return st
+
+    def set_flop_counting(self, enable: bool) -> None:
+        """Later forwards / adjoints run the *_counted kernels; stats().sdf_flops sums them."""
+        _check(library().ft_diff_set_flop_counting(self._h, 1 if enable else 0))
 
 
 def _autograd():

@@ -30,14 +30,17 @@ ERROR_KINDS = {1: "MissingSceneRoot", 2: "UnsupportedObjectType", 3: "ExpectedOb
                9: "LaunchFailed", 10: "CallbackAbort"}
 
 EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_destroy", "ft_render_depth",
-           "ft_render_whitted", "ft_render_depth_device", "ft_render_whitted_device", "ft_scene_synchronize",
+           "ft_render_whitted", "ft_render_depth_device", "ft_render_whitted_device",
+           "ft_scene_synchronize",
            "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting",
            "ft_emit_scene_source", "ft_emit_compile", "ft_batch_create", "ft_batch_destroy",
            "ft_batch_render_device", "ft_batch_render", "ft_batch_synchronize", "ft_batch_stats",
            "ft_batch_set_flop_counting", "ft_diff_create", "ft_diff_destroy", "ft_diff_param_count",
            "ft_diff_set_params", "ft_diff_get_params", "ft_diff_forward", "ft_diff_forward_device",
            "ft_diff_synchronize", "ft_diff_backward_device", "ft_diff_set_target", "ft_diff_loss_grad",
-           "ft_diff_fit", "ft_diff_stats", "ft_scene_build_atlas"]
+           "ft_diff_fit", "ft_diff_stats", "ft_diff_set_flop_counting", "ft_scene_build_atlas", "ft_scene_set_launch_config",
+           "ft_scene_get_launch_config", "ft_scene_autotune", "ft_render_depth_progressive",
+           "ft_render_whitted_progressive"]
 
 
 class BackendUnavailable(RuntimeError):
@@ -73,6 +76,11 @@ def library():
         L.ft_abi_struct_size.argtypes = [C.c_char_p]
         L.ft_scene_set_flop_counting.argtypes = [vp, C.c_int]
         L.ft_scene_build_atlas.argtypes = [vp, C.c_float, u32, u32]
+        L.ft_scene_set_launch_config.argtypes = [vp, u32, vp]
+        L.ft_scene_get_launch_config.argtypes = [vp, u32, vp]
+        L.ft_scene_autotune.argtypes = [vp, u32, vp, vp, u32, vp]
+        L.ft_render_depth_progressive.argtypes = [vp, vp, u32, u32, abi.PROGRESS_FN, vp, vp]
+        L.ft_render_whitted_progressive.argtypes = [vp, vp, vp, abi.PROGRESS_FN, vp, vp]
         L.ft_emit_scene_source.argtypes = [vp, u32, C.c_char_p, C.c_uint64, C.POINTER(C.c_uint64)]
         L.ft_emit_compile.argtypes = [vp, u32, C.c_int, C.POINTER(C.c_double)]
         L.ft_batch_create.argtypes = [C.POINTER(vp), u32, u32, C.POINTER(vp)]
@@ -160,6 +168,39 @@ class GpuScene:
     def synchronize(self):
         _check(library().ft_scene_synchronize(self._h))
 
+    def _progressive(self, call, options: RenderOptions, on_tile) -> np.ndarray:
+        o = options.to_c()
+        rgb = np.zeros((o.height, o.width, 3), np.uint8)
+        raised = []
+
+        def trampoline(pg, user):
+            p = pg.contents
+            try:
+                img = np.ctypeslib.as_array(p.image, shape=(p.height, p.width, 3)).copy()
+                on_tile({"tiles_done": p.tiles_done, "tiles_total": p.tiles_total, "elapsed_ms": p.elapsed_ms,
+                         "tile": (p.tile_x, p.tile_y, p.tile_w, p.tile_h)}, img)
+            except BaseException as exc:  # stops the render; re-raised below
+                raised.append(exc)
+                return 1
+            return 0
+        cb = abi.PROGRESS_FN(trampoline) if on_tile is not None else abi.PROGRESS_FN()
+        rc = call(C.addressof(o), cb, rgb.ctypes.data)
+        if rc == 10 and raised:
+            raise raised[0]
+        _check(rc)
+        return rgb
+
+    def render_depth_progressive(self, options: RenderOptions, aa_samples: int, tile_size: int,
+                                 on_tile=None) -> np.ndarray:
+        """ft_render_depth_progressive: tiles of tile_size, on_tile(progress, image) per tile."""
+        return self._progressive(lambda o, cb, out: library().ft_render_depth_progressive(
+            self._h, o, int(aa_samples), int(tile_size), cb, None, out), options, on_tile)
+
+    def render_whitted_progressive(self, options: RenderOptions, settings: RaySettings, on_tile=None) -> np.ndarray:
+        s = settings.to_c()
+        return self._progressive(lambda o, cb, out: library().ft_render_whitted_progressive(
+            self._h, o, C.addressof(s), cb, None, out), options, on_tile)
+
     def stream(self) -> int:
         p = C.c_void_p()
         _check(library().ft_scene_stream(self._h, C.byref(p)))
@@ -176,6 +217,26 @@ class GpuScene:
         st = abi.RenderStatsC()
         _check(library().ft_scene_stats(self._h, C.addressof(st)))
         return st
+
+    def launch_config(self, kind: int) -> abi.LaunchConfigC:
+        c = abi.LaunchConfigC()
+        _check(library().ft_scene_get_launch_config(self._h, int(kind), C.addressof(c)))
+        return c
+
+    def set_launch_config(self, kind: int, block_waves=4, tile_width=8, waves_per_eu=4, march_unroll=1,
+                          shade_inline=0):
+        c = abi.LaunchConfigC(block_waves, tile_width, waves_per_eu, march_unroll, shade_inline)
+        _check(library().ft_scene_set_launch_config(self._h, int(kind), C.addressof(c)))
+
+    def autotune(self, kind: int, options: RenderOptions, settings: Optional[RaySettings] = None,
+                 flags: int = 0) -> abi.TuneReportC:
+        """Chooses the fastest launch shape for this scene's `kind` render (ft_scene_autotune)."""
+        o = options.to_c()
+        s = settings.to_c() if settings is not None else None
+        rep = abi.TuneReportC()
+        _check(library().ft_scene_autotune(self._h, int(kind), C.addressof(o), C.addressof(s) if s is not None else None,
+                                           int(flags), C.addressof(rep)))
+        return rep
 
 
 class GpuBatch:
@@ -270,45 +331,25 @@ def emit_compile(scene: CompiledScene, accel: AccelMode = AccelMode.Bvh, kernel:
 
 
 # ---------------------------------------------------------------- reference API
-def _tiles(width: int, height: int, tile: int):
-    tile = max(tile, 1)
-    for ty in range(0, height, tile):
-        for tx in range(0, width, tile):
-            yield tx, ty, min(tile, width - tx), min(tile, height - ty)
-
-
 def render_preview_progressive_with_accel(state: EvalState, options: RenderOptions, accel_mode: AccelMode,
                                           tile_size: int, aa_samples: int,
                                           on_tile: Optional[Callable[[dict, np.ndarray], None]] = None) -> np.ndarray:
-    """Depth preview of the whole frame; `on_tile(progress, image)` is called on the caller's
-    thread once per tile of `tile_size` in reference order (renderer.rs:3751-3834).  An
-    exception raised by the callback aborts the render, as an Err does in the reference."""
+    """Depth preview (renderer.rs:1238-1277, 3751-3834); `on_tile(progress, image)` runs on
+    the caller's thread once per finished tile of `tile_size`, in the reference's tile
+    order, with the image so far.  An exception raised by the callback stops the render
+    and propagates, as an Err from the reference's callback is returned."""
     scene = compile_scene(state)
     with GpuScene(scene, accel_mode) as gs:
-        rgb, _ = gs.render_depth(options, max(int(aa_samples), 1))
-    _progress(rgb, options, tile_size, on_tile)
-    return rgb
+        return gs.render_depth_progressive(options, max(int(aa_samples), 1), tile_size, on_tile)
 
 
 def render_ray_progressive_with_accel(state: EvalState, options: RenderOptions, accel_mode: AccelMode,
                                       settings: RaySettings,
                                       on_tile: Optional[Callable[[dict, np.ndarray], None]] = None) -> np.ndarray:
-    """Whitted trace (renderer.rs:1201-1236, 3635-3724)."""
+    """Whitted trace (renderer.rs:1201-1236, 3635-3724), tiles of settings.tile_size."""
     scene = compile_scene(state)
     with GpuScene(scene, accel_mode) as gs:
-        rgb, _ = gs.render_whitted(options, settings)
-    _progress(rgb, options, settings.tile_size, on_tile)
-    return rgb
-
-
-def _progress(rgb, options, tile_size, on_tile):
-    if on_tile is None:
-        return
-    tiles = list(_tiles(options.width, options.height, tile_size))
-    img = np.zeros_like(rgb)
-    for k, (tx, ty, tw, th) in enumerate(tiles):
-        img[ty:ty + th, tx:tx + tw] = rgb[ty:ty + th, tx:tx + tw]
-        on_tile({"tiles_done": k + 1, "tiles_total": len(tiles)}, img.copy())
+        return gs.render_whitted_progressive(options, settings, on_tile)
 
 
 def render_depth_png_with_accel(state: EvalState, output_path: str, options: RenderOptions,

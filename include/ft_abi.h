@@ -392,6 +392,30 @@ int ft_render_depth(ft_scene* scene, const ft_render_options* options, uint32_t 
 int ft_render_whitted(ft_scene* scene, const ft_render_options* options,
                       const ft_ray_settings* settings, uint8_t* rgb_out, float* hdr_out);
 
+/* Progressive renders with a progress callback (RayProgress and the on_tile callback of
+ * render_ray_progressive_with_accel renderer.rs:1201-1236, 3635-3724, and of
+ * render_preview_progressive_with_accel renderer.rs:1238-1277, 3751-3834).  The image is
+ * rendered in rows of tile_size x tile_size tiles (the reference's tile_jobs order,
+ * renderer.rs:3849-3862); the callback runs on the calling thread once per finished tile
+ * with the image so far (unfinished tiles black).  A non-zero return stops the render:
+ * the call returns FT_ERR_CALLBACK_ABORT (the reference returns the callback's error).
+ * rgb_out receives the finished image.  Synchronous. */
+typedef struct ft_progress {
+  uint32_t tiles_done;
+  uint32_t tiles_total;
+  uint64_t elapsed_ms;
+  uint32_t tile_x, tile_y, tile_w, tile_h;  /* the tile just finished */
+  uint32_t width, height;
+  const uint8_t* image;                     /* width * height * 3 bytes */
+} ft_progress;
+typedef int (*ft_progress_fn)(const ft_progress* progress, void* user);
+int ft_render_depth_progressive(ft_scene* scene, const ft_render_options* options, uint32_t aa_samples,
+                                uint32_t tile_size, ft_progress_fn on_tile, void* user, uint8_t* rgb_out);
+/* tile_size and aa_samples from the settings (RaySettings.tile_size / aa_samples). */
+int ft_render_whitted_progressive(ft_scene* scene, const ft_render_options* options,
+                                  const ft_ray_settings* settings, ft_progress_fn on_tile, void* user,
+                                  uint8_t* rgb_out);
+
 /* Device-resident variants: outputs are device pointers, the call enqueues on the
  * scene's stream and returns without synchronising (ft_scene_synchronize waits). */
 int ft_render_depth_device(ft_scene* scene, const ft_render_options* options,
@@ -493,7 +517,12 @@ int ft_diff_loss_grad(ft_diff* diff, float w_t, float w_s, double* loss_out, dou
 /* `steps` Adam steps on the device (beta1 0.9, beta2 0.999, eps 1e-8); losses_out
  * (nullable) receives L before each step.  Synchronous. */
 int ft_diff_fit(ft_diff* diff, uint32_t steps, float lr, float w_t, float w_s, double* losses_out);
+/* sdf_flops: FLOPs of the forward and adjoint tree passes since counting was turned on
+ * (FLOP counting on), else 0. */
 int ft_diff_stats(ft_diff* diff, ft_render_stats* out);
+/* Turns FLOP counting on (1, clears the counter) or off (0): later forwards and adjoints run
+ * the *_counted kernels, which add their per-lane tree-pass FLOPs with one atomic per wave. */
+int ft_diff_set_flop_counting(ft_diff* diff, int enable);
 
 /* Scene specialisation diagnostics (no GPU needed).  ft_emit_scene_source writes the
  * translation unit the emitter produces for `ir` under `accel_mode` (always emitted, the
@@ -516,6 +545,45 @@ int ft_emit_compile(const ft_scene_ir* ir, uint32_t accel_mode, int kernel, doub
  * so t agrees to about 2 * epsilon rather than bit for bit.  bits = 0 removes the atlas.
  * max_bricks bounds the memory (0: 1 << 20). */
 int ft_scene_build_atlas(ft_scene* scene, float voxel, uint32_t bits, uint32_t max_bricks);
+
+/* ----------------------------------------------------------- launch shapes
+ * How the march kernels are launched and compiled for one scene.  No counterpart in the
+ * reference, whose renderer is CPU loops over tiles and rows (renderer.rs:3659-3717). */
+typedef struct ft_launch_config {
+  uint32_t block_waves;   /* wavefronts per workgroup: 1, 2 or 4 */
+  uint32_t tile_width;    /* pixel tile of one wavefront: 8 (8x8), 16 (16x4) or 32 (32x2) */
+  uint32_t waves_per_eu;  /* occupancy the kernel is compiled for, 1..4 (VGPR budget 512 / n) */
+  uint32_t march_unroll;  /* march steps per loop iteration: 1, 2 or 4 */
+  uint32_t shade_inline;  /* 1: the Whitted shading functions are inlined into the trace loop */
+} ft_launch_config;
+#define FT_KIND_DEPTH 0u
+#define FT_KIND_WHITTED 1u
+/* The default is {4, 8, 4, 1, 0}, the shape of the precompiled kernels; any other occupancy,
+ * unroll or inlining runs from a runtime-compiled code object built on first use. */
+int ft_scene_set_launch_config(ft_scene* scene, uint32_t kind, const ft_launch_config* config);
+int ft_scene_get_launch_config(ft_scene* scene, uint32_t kind, ft_launch_config* config);
+
+typedef struct ft_tune_report {
+  ft_launch_config best;
+  double best_ms;            /* kernel time of the chosen shape on the tuning render */
+  double default_ms;         /* kernel time of the default shape on the same render */
+  double tune_ms;            /* wall time of the search, compilation included */
+  uint32_t candidates;       /* shapes timed */
+  uint32_t cached;           /* 1: the choice came from the tuning cache */
+  uint32_t tune_aa_samples;  /* samples per pixel of the tuning render */
+  uint32_t pad;
+} ft_tune_report;
+#define FT_TUNE_FULL 1u        /* time the request itself (default: Whitted at <= 4 samples) */
+#define FT_TUNE_SHAPE_ONLY 2u  /* search block and tile shapes only (no code variants to build) */
+#define FT_TUNE_NO_CACHE 4u    /* search even when the cache holds a choice */
+/* Autotuner: times the scene's `kind` render of this request under candidate shapes,
+ * first the code variants (occupancy 4/3/2, march unroll 1/2, shading calls or inlined) in
+ * the default launch shape,
+ * then block and tile shapes of the best code, and keeps the fastest as the scene's shape
+ * for that kind.  Choices are cached per (scene IR hash, kind, request) in the process and,
+ * when a cache directory is writable, on disk next to the code objects.  Synchronous. */
+int ft_scene_autotune(ft_scene* scene, uint32_t kind, const ft_render_options* options,
+                      const ft_ray_settings* settings, uint32_t flags, ft_tune_report* report);
 
 /* Turns FLOP counting on (1) or off (0) for later renders on this scene.  Counting adds
  * atomics to every SDF evaluation: use it for work accounting, never for timing. */

@@ -816,7 +816,7 @@ template <class T> bool bvh_distance_info(const Accel<T>& A, int ni, V3<T> p, T&
   T llb = point_aabb_lb(p, A.bvh[nd.lhs].bounds), rlb = point_aabb_lb(p, A.bvh[nd.rhs].bounds);
   int first = nd.lhs, second = nd.rhs;
   T flb = llb, slb = rlb;
-  if (!(llb <= rlb)) { first = nd.rhs; second = nd.lhs; flb = rlb; slb = llb; }
+  if (planted(42) ? !(llb < rlb) : !(llb <= rlb)) { first = nd.rhs; second = nd.lhs; flb = rlb; slb = llb; }
   bool have = false;
   DistInfo<T> bi{};
   if (flb <= best) have = bvh_distance_info(A, first, p, best, bi);
@@ -1079,7 +1079,8 @@ template <class T> void build_accel(const Scene<T>& S, uint32_t mode, Accel<T>& 
           for (size_t x = mn[0]; x <= mx[0]; ++x) A.cells[cell_index(A, x, y, z)].push_back(li);
     }
     for (auto& c : A.cells) {
-      std::sort(c.begin(), c.end());
+      if (planted(43)) std::sort(c.begin(), c.end(), [](uint32_t a, uint32_t b) { return a > b; });
+      else std::sort(c.begin(), c.end());
       c.erase(std::unique(c.begin(), c.end()), c.end());
     }
   }
@@ -1814,10 +1815,11 @@ template <class T> std::vector<V3<T>> lobe_dirs(V3<T> ideal_dir, T rough, uint32
   V3<T> seed = std::fabs(ideal.y) < T(0.99) ? V3<T>(0, 1, 0) : V3<T>(1, 0, 0);
   V3<T> tangent = ideal.cross(seed).normalize();
   V3<T> bitangent = tangent.cross(ideal).normalize();
-  T spread = rclamp(rough, T(0), T(0.5)) * T(0.35);
+  T spread = rclamp(rough, T(0), T(0.5)) * (planted(40) ? T(0.25) : T(0.35));
   std::vector<V3<T>> dirs{ideal};
   if (count <= 1 || spread <= T(1.0e-4)) return dirs;
-  const T off[6][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {T(0.707), T(0.707)}, {T(-0.707), T(0.707)}};
+  const T off[6][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {T(0.707), T(0.707)},
+                       {T(-0.707), planted(41) ? T(-0.707) : T(0.707)}};
   for (uint32_t k = 0; k + 1 < count && k < 6; ++k)
     dirs.push_back((ideal + tangent * (off[k][0] * spread) + bitangent * (off[k][1] * spread)).normalize());
   return dirs;
@@ -2143,6 +2145,47 @@ int ftref_distance_info(const ft_scene_ir* ir, uint32_t accel, int32_t precision
   return FT_OK;
 }
 
+// Brute-force view of the accelerator (test support, pins accel_distance_info): per point
+// and accel leaf the leaf's accel_leaf_distance_info (R:629-657) distance and object, and
+// per leaf its rank in the order the traversal meets leaves whose bounds contain a point:
+// the BVH's left-to-right leaf order (R:743-786 descends lhs first when both children's
+// bounds contain the point), the leaf index for the bricks grid (its cell lists ascend,
+// R:824-954).  With d_out null only *n_leaves is written.
+int ftref_leaf_table(const ft_scene_ir* ir, uint32_t accel, const float* p, int32_t n, float* d_out,
+                     uint32_t* obj_out, int32_t* rank_out, int32_t* n_leaves) {
+  if (!validate(ir) || !n_leaves) return FT_ERR_INVALID_ARGUMENT;
+  Scene<float> S;
+  load_scene(ir, S);
+  Accel<float> A;
+  build_accel(S, accel, A);
+  const int32_t L = (int32_t)A.leaves.size();
+  *n_leaves = L;
+  if (!d_out) return FT_OK;
+  if (rank_out) {
+    for (int32_t li = 0; li < L; ++li) rank_out[li] = li;
+    if (accel == FT_ACCEL_BVH && A.bvh_root >= 0) {
+      int32_t next = 0;
+      std::vector<int> stack{A.bvh_root};
+      while (!stack.empty()) {
+        const BvhNode<float>& nd = A.bvh[stack.back()];
+        stack.pop_back();
+        if (nd.leaf >= 0) { rank_out[nd.leaf] = next++; continue; }
+        stack.push_back(nd.rhs);
+        stack.push_back(nd.lhs);
+      }
+    }
+  }
+  for (int32_t k = 0; k < n; ++k) {
+    const V3<float> q(p[3 * k], p[3 * k + 1], p[3 * k + 2]);
+    for (int32_t li = 0; li < L; ++li) {
+      DistInfo<float> d = leaf_distance_info(S, A.leaves[li], q);
+      d_out[(size_t)k * L + li] = d.distance;
+      if (obj_out) obj_out[(size_t)k * L + li] = d.object_id;
+    }
+  }
+  return FT_OK;
+}
+
 int ftref_lower_bound(const ft_scene_ir* ir, uint32_t accel, int32_t precision, const float* p, int32_t n, float* lb) {
   if (!validate(ir)) return FT_ERR_INVALID_ARGUMENT;
   auto run = [&](auto tag) {
@@ -2221,6 +2264,15 @@ int ftref_refract(const float* args, int32_t n, float* out) {
     if (!refract(V3<float>(a[0], a[1], a[2]), V3<float>(a[3], a[4], a[5]), a[6], r)) r = V3<float>();
     out[3 * k] = r.x; out[3 * k + 1] = r.y; out[3 * k + 2] = r.z;
   }
+  return FT_OK;
+}
+
+// rough_lobe_directions (ray.rs:383-417) of `ideal` at roughness `rough` with `count`
+// lobes; out: up to 7 directions (3 floats each), *n_out of them.
+int ftref_lobe_dirs(const float* ideal, float rough, uint32_t count, float* out, int32_t* n_out) {
+  std::vector<V3<float>> d = lobe_dirs(V3<float>(ideal[0], ideal[1], ideal[2]), rough, count);
+  for (size_t k = 0; k < d.size(); ++k) { out[3 * k] = d[k].x; out[3 * k + 1] = d[k].y; out[3 * k + 2] = d[k].z; }
+  *n_out = (int32_t)d.size();
   return FT_OK;
 }
 

@@ -37,6 +37,10 @@ int ftref_distance_info(const ft_scene_ir* ir, uint32_t accel, int32_t precision
                         int32_t n, float* dist, uint32_t* object_id, uint32_t* material_id);
 
 /* accel.lower_bound(p) for n points. */
+/* Per point and accel leaf the leaf's distance and object; per leaf its traversal rank
+   (test support).  With d_out null only *n_leaves is written. */
+int ftref_leaf_table(const ft_scene_ir* ir, uint32_t accel, const float* p, int32_t n, float* d_out,
+                     uint32_t* obj_out, int32_t* rank_out, int32_t* n_leaves);
 int ftref_lower_bound(const ft_scene_ir* ir, uint32_t accel, int32_t precision, const float* p,
                       int32_t n, float* lb);
 
@@ -72,6 +76,8 @@ int ftref_light_sample(const ft_scene_ir* ir, uint32_t light, const float* point
 int ftref_material_eval(const ft_scene_ir* ir, uint32_t material, const float* nwiwo, int32_t n, float* out);
 int ftref_fresnel_dielectric(const float* args, int32_t n, float* out);
 int ftref_refract(const float* args, int32_t n, float* out);
+/* rough_lobe_directions: up to 7 directions into out (3 floats each), their number in *n_out. */
+int ftref_lobe_dirs(const float* ideal, float rough, uint32_t count, float* out, int32_t* n_out);
 
 const char* ftref_last_error(void);
 

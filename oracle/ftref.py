@@ -34,6 +34,8 @@ def _bind(path):
     L.ftref_material_eval.argtypes = [vp, u32, vp, i32, vp]
     L.ftref_fresnel_dielectric.argtypes = [vp, i32, vp]
     L.ftref_refract.argtypes = [vp, i32, vp]
+    L.ftref_lobe_dirs.argtypes = [vp, C.c_float, u32, vp, vp]
+    L.ftref_leaf_table.argtypes = [vp, u32, vp, i32, vp, vp, vp, vp]
     L.ftref_last_error.restype = C.c_char_p
     return L
 
@@ -114,6 +116,22 @@ def lower_bound(scene, accel, points, precision=0):
     ir = scene.ir()
     _check(lib().ftref_lower_bound(C.addressof(ir), int(accel), precision, p.ctypes.data, len(p), lb.ctypes.data))
     return lb
+
+
+def leaf_table(scene, accel, points):
+    """Brute force over the accelerator's leaves: (d[n, L], obj[n, L], rank[L]) - each leaf's
+    own distance and object at every point, and each leaf's traversal rank."""
+    p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    ir = scene.ir()
+    nl = C.c_int32(0)
+    _check(lib().ftref_leaf_table(C.addressof(ir), int(accel), p.ctypes.data, len(p), None, None, None,
+                                  C.byref(nl)))
+    d = np.zeros((len(p), nl.value), np.float32)
+    obj = np.zeros((len(p), nl.value), np.uint32)
+    rank = np.zeros(nl.value, np.int32)
+    _check(lib().ftref_leaf_table(C.addressof(ir), int(accel), p.ctypes.data, len(p), d.ctypes.data,
+                                  obj.ctypes.data, rank.ctypes.data, C.byref(nl)))
+    return d, obj, rank
 
 
 def eval_function(scene, fn, params):
@@ -203,3 +221,12 @@ def refract(incident, normal, eta):
     out = np.zeros((len(rec), 3), np.float32)
     _check(lib().ftref_refract(rec.ctypes.data, len(rec), out.ctypes.data))
     return out
+
+
+def lobe_dirs(ideal, rough, count):
+    """rough_lobe_directions: the lobe directions [n, 3]."""
+    i = np.ascontiguousarray(ideal, np.float32)
+    out = np.zeros((7, 3), np.float32)
+    n = np.zeros(1, np.int32)
+    _check(lib().ftref_lobe_dirs(i.ctypes.data, float(rough), int(count), out.ctypes.data, n.ctypes.data))
+    return out[:int(n[0])]

@@ -2,7 +2,8 @@
 
   * forward (t*, s) against the oracle's f32 forward of the same parametric scene;
   * adjoint gradient against central finite differences (h = 1e-4) of the oracle's FP64
-    forward, every parameter, relative error <= 1e-3;
+    forward, every parameter, relative error <= 1e-3 each (8 and 32 spheres), and the
+    directional derivative at the benchmark's full size;
   * one sphere: dt*/d(centre, radius) against the analytic derivative of the ray/sphere root;
   * Adam fit lowers the loss; the torch.autograd.Function gives the C-ABI gradient.
 """
@@ -50,34 +51,94 @@ def test_forward_matches_oracle(small):
     assert np.abs(s - sc)[both].max() <= 2e-3
 
 
-def test_gradient_against_fp64_finite_differences(small):
-    w = small
-    p32 = np.float32(w.params_true) + np.float32(0.02) * np.float32(np.sin(np.arange(len(w.params_true))))
+def _adjoint_vs_fd(w, p32, h=1e-4):
+    """The GPU adjoint g = J^T u for upstream gradients u = dL/d(t, s) of an L2 loss at the
+    GPU forward, against central differences (step h) of the FP64 oracle's forward
+    projected on the same u: fd_j = u . (F(p + h e_j) - F(p - h e_j)) / 2h, per parameter."""
+    import torch
     p = p32.astype(np.float64)
     tt, st = ftref.diff_forward(w.n_prims, w.params_true, w.views, w.options, w.light_dir, w.normal_step, precision=1)
-    import torch
     with DiffRenderer(w.n_prims, w.views, w.options, w.light_dir, w.normal_step) as dr:
         dr.set_params(p32)
         t, s = dr.forward()
         m = interior(t) & interior(tt)
-        gt = torch.from_numpy((2.0 * (t - tt) * m).astype(np.float32)).cuda()
-        gs = torch.from_numpy((2.0 * (s - st) * m).astype(np.float32)).cuda()
+        ut = (2.0 * (t - tt) * m).astype(np.float32)
+        us = (2.0 * (s - st) * m).astype(np.float32)
+        gt = torch.from_numpy(ut).cuda()
+        gs = torch.from_numpy(us).cuda()
         torch.cuda.synchronize()
         g = dr.backward_device(gt.data_ptr(), gs.data_ptr())
 
-    def loss(q):
+    def proj(q):
         a, b = ftref.diff_forward(w.n_prims, q, w.views, w.options, w.light_dir, w.normal_step, precision=1)
-        return float((((a - tt) ** 2) * m).sum() + (((b - st) ** 2) * m).sum())
+        return float((ut * np.where(m, a, 0.0)).sum() + (us * np.where(m, b, 0.0)).sum())
 
-    h = 1e-4
     fd = np.zeros_like(p)
     for j in range(p.size):
         e = np.zeros_like(p)
         e[j] = h
-        fd[j] = (loss(p + e) - loss(p - e)) / (2 * h)
-    err = np.abs(g - fd).max() / np.abs(fd).max()
-    print(f"[diff grad] max |g - fd| / max |fd| = {err:.2e}", flush=True)
-    assert err <= 1e-3, (g, fd)
+        fd[j] = (proj(p + e) - proj(p - e)) / (2 * h)
+    return g, fd
+
+
+def _perturbed(w):
+    return np.float32(w.params_true) + np.float32(0.02) * np.float32(np.sin(np.arange(len(w.params_true))))
+
+
+def _check_per_parameter(g, fd, label):
+    floor = 1e-5 * np.abs(fd).max()
+    bad = np.abs(g - fd) > 1e-3 * np.abs(fd) + floor
+    rel = np.abs(g - fd) / np.maximum(np.abs(fd), floor)
+    print(f"[diff grad {label}] worst per-parameter relative error {rel.max():.2e} (param {int(rel.argmax())}), "
+          f"floor {floor:.2e}", flush=True)
+    assert not bad.any(), [(int(j), float(g[j]), float(fd[j])) for j in np.nonzero(bad)[0]]
+
+
+def test_gradient_per_parameter_small(small):
+    w = small
+    g, fd = _adjoint_vs_fd(w, _perturbed(w))
+    _check_per_parameter(g, fd, "8 prims")
+
+
+def test_gradient_per_parameter_32_prims():
+    """All 159 parameters of the 32-sphere workload (centres, radii, every blend radius), 4
+    views at 128^2: |g_j - fd_j| <= 1e-3 |fd_j| + 1e-5 max|fd| for every j."""
+    w = synthetic.adjoint_fit(n_prims=32, n_views=4, size=128)
+    print(f"[diff grad] 32 prims, {len(w.views)} views at 128^2", flush=True)
+    g, fd = _adjoint_vs_fd(w, _perturbed(w))
+    _check_per_parameter(g, fd, "32 prims")
+
+
+def test_gradient_directional_full_size():
+    """The benchmark's size (32 spheres, 16 views at 512^2): the adjoint's directional
+    derivative g . v along a random unit direction against the FP64 oracle's central
+    difference of the same projected loss."""
+    import torch
+    w = synthetic.adjoint_fit()
+    p32 = _perturbed(w)
+    p = p32.astype(np.float64)
+    tt, st = ftref.diff_forward(w.n_prims, w.params_true, w.views, w.options, w.light_dir, w.normal_step, precision=1)
+    with DiffRenderer(w.n_prims, w.views, w.options, w.light_dir, w.normal_step) as dr:
+        dr.set_params(p32)
+        t, s = dr.forward()
+        m = interior(t) & interior(tt)
+        ut = (2.0 * (t - tt) * m).astype(np.float32)
+        us = (2.0 * (s - st) * m).astype(np.float32)
+        gt = torch.from_numpy(ut).cuda()
+        gs = torch.from_numpy(us).cuda()
+        torch.cuda.synchronize()
+        g = dr.backward_device(gt.data_ptr(), gs.data_ptr())
+    v = np.random.default_rng(7).standard_normal(p.size)
+    v /= np.linalg.norm(v)
+    h = 1e-4
+
+    def proj(q):
+        a, b = ftref.diff_forward(w.n_prims, q, w.views, w.options, w.light_dir, w.normal_step, precision=1)
+        return float((ut * np.where(m, a, 0.0)).sum() + (us * np.where(m, b, 0.0)).sum())
+    fd = (proj(p + h * v) - proj(p - h * v)) / (2 * h)
+    gv = float(np.dot(g, v))
+    print(f"[diff grad full] g.v {gv:.6e} fd {fd:.6e}", flush=True)
+    assert abs(gv - fd) <= 1e-3 * abs(fd), (gv, fd)
 
 
 def test_single_sphere_analytic_derivative():
@@ -147,3 +208,25 @@ def test_torch_autograd_function(small):
         torch.cuda.synchronize()
         ref = dr.backward_device(gt.data_ptr(), gs.data_ptr())
     np.testing.assert_allclose(params.grad.cpu().numpy(), np.float32(ref), rtol=1e-5, atol=1e-6)
+
+
+def test_counted_kernels_match_and_count(small):
+    """The FLOP-counting forward / adjoint give the same loss and gradient as the timed
+    kernels, and count at least the march's minimum work (every pixel of every view evaluates
+    the tree at least once; a hit adds 10 bisection steps and 6 normal samples)."""
+    w = small
+    tt, st = ftref.diff_forward(w.n_prims, w.params_true, w.views, w.options, w.light_dir, w.normal_step, precision=0)
+    with DiffRenderer(w.n_prims, w.views, w.options, w.light_dir, w.normal_step) as dr:
+        dr.set_params(w.params_init)
+        dr.set_target(tt.astype(np.float32), st.astype(np.float32))
+        l0, g0 = dr.loss_grad()
+        dr.set_flop_counting(True)
+        l1, g1 = dr.loss_grad()
+        flops = dr.stats().sdf_flops
+        dr.set_flop_counting(False)
+        t, _ = dr.forward()
+    assert l0 == l1
+    np.testing.assert_array_equal(g0, g1)
+    per_eval = 10 * w.n_prims + 13 * (w.n_prims - 1)
+    hits = int((t >= 0).sum())
+    assert flops >= per_eval * (t.size + 16 * hits)

@@ -78,6 +78,7 @@ def whitted_parity(scene, accel, opts, settings, label="", emit=None, min_frac=0
         check_path(gs, emit)
     d = np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2)
     frac = float((d <= 1).mean())
+    print(f"[whitted parity] {label}: {frac:.5f} of pixels within 1/255", flush=True)
     assert frac >= min_frac, f"{label}: only {frac:.5f} of pixels within 1/255 (max {d.max()})"
     return rgb_g, frac
 
@@ -126,11 +127,10 @@ def test_bricks_4096_scene():
     scene = compile_src(w.source)
     for accel in (AccelMode.Bvh, AccelMode.Bricks):
         depth_parity(scene, accel, w.options, 1, f"bricks_4096-depth-{accel.name}")
-    # Whitted through ~1300 overlapping glass primitives: up to bounce depth 3 the GPU
-    # matches the oracle to within one pixel; at depth 4 about 0.4% of pixels differ
-    # because a fourth-generation ray leaving a glass cluster escapes to the sky on one
-    # side and hits on the other (the oracle in f32 and in f64 differ on ~0.1%).  The
-    # bar for this scene: at least 99% of pixels within 1/255.
+    # Whitted through ~1300 overlapping glass primitives: rays inside several primitives
+    # take the distance of the first negative accel leaf in the reference's traversal order
+    # and the material of the root walk's leaf; the oracle in f32 and in f64 differ on
+    # ~0.1% of pixels here.  The bar: 99.9% of pixels within 1/255.
     rgb_c, _ = ftref.render_whitted(scene, AccelMode.Bvh, w.options, w.settings)
     rgb_64, _ = ftref.render_whitted(scene, AccelMode.Bvh, w.options, w.settings, precision=1)
     with GpuScene(scene, AccelMode.Bvh) as gs:
@@ -138,32 +138,80 @@ def test_bricks_4096_scene():
     off_gpu = int((np.abs(rgb_c.astype(int) - rgb_g.astype(int)).max(axis=2) > 1).sum())
     off_f64 = int((np.abs(rgb_c.astype(int) - rgb_64.astype(int)).max(axis=2) > 1).sum())
     print(f"[bricks_4096 whitted] gpu vs oracle: {off_gpu} px; oracle f32 vs f64: {off_f64} px", flush=True)
-    assert off_gpu <= 0.01 * rgb_g.shape[0] * rgb_g.shape[1]
+    assert off_gpu <= 0.001 * rgb_g.shape[0] * rgb_g.shape[1]
+
+
+SLICE_SMOOTH = """
+let camera = Camera { origin: vec3(0.0, 1.5, -6.0), target: vec3(0.0, 0.0, 0.0), fov_y: 45.0 };
+let key = PointLight { position: vec3(4.0, 6.0, -5.0), color: #ffffff, intensity: 60.0 };
+let m1 = Lambert { color: vec3(0.8, 0.4, 0.2) };
+let m2 = Metal { color: vec3(0.7, 0.7, 0.8), roughness: 0.2 };
+let m3 = Lambert { color: vec3(0.2, 0.6, 0.3) };
+var a = Sphere { radius: 1.2, material: m1 };
+var b = Box { size: vec3(1.6, 1.0, 1.6), material: m2 }; b.pos.x = 1.3;
+var c = Torus { major_radius: 0.9, minor_radius: 0.25, material: m3 }; c.pos.x = -1.6; c.rot.x = 70.0;
+var g = Box { size: vec3(20.0, 1.0, 20.0), material: m3 }; g.pos.y = -1.8;
+let scene = g + a.slice_y(-0.5, 0.7).union_round(b.smooth(1.5), 0.3) + c.smooth(0.6).slice_x(-2.2, -1.2);
+"""
+
+
+def test_slice_and_smooth_nodes(emit):
+    """Slice (max with an axis slab) and Smooth (surface offset 0.1 k) nodes, alone and
+    under a smooth union, in depth and Whitted against the oracle."""
+    scene = compile_src(SLICE_SMOOTH)
+    kinds = set(int(n["kind"]) for n in scene.nodes)
+    assert {abi.NODE["SLICE"], abi.NODE["SMOOTH"]} <= kinds
+    opts = RenderOptions(width=96, height=64, max_steps=256, max_dist=40.0, epsilon=1.0e-3)
+    depth_parity(scene, AccelMode.Bvh, opts, 1, "slice/smooth", emit)
+    whitted_parity(scene, AccelMode.Bvh, opts, RaySettings(3, 64, 1), "slice/smooth", emit)
+
+
+def test_bricks_mode_whitted():
+    """Whitted under AccelMode::Bricks, whose reference grid (renderer.rs:824-954) the
+    oracle builds and searches shell by shell, against the GPU's Bricks mode, on a
+    257-leaf hard-union scene at max depth 4."""
+    w = synthetic.bricks_4096(n_prims=256, width=96, height=54)
+    scene = compile_src(w.source)
+    # through ~80 overlapping glass primitives at depth 4 the same bar as test_bricks_4096_scene
+    whitted_parity(scene, AccelMode.Bricks, w.options, w.settings, "bricks-mode whitted", min_frac=0.999)
 
 
 @pytest.mark.parametrize("bits", [16, 32])
 def test_distance_atlas_march(bits):
     """Coarse steps on the FP16/FP32 distance atlas, exact evaluation within two voxels of
-    the surface: the same surface points as the exact march, t within 3 epsilon."""
+    the surface: hit masks as the exact march's (0.01%), |dt| <= 1e-4; Whitted against the
+    exact GPU render within 1/255 on 99.9% of pixels at bounce depth 2, 99.8% at depth 4."""
     w = synthetic.bricks_4096(width=160, height=90)
     scene = compile_src(w.source)
     opts = w.options
+    shallow = RaySettings(2, w.settings.tile_size, 1)
     print(f"[atlas] bits={bits} {opts.width}x{opts.height}", flush=True)
     _, t_c = ftref.render_depth(scene, AccelMode.Bvh, opts, 1)
     with GpuScene(scene, AccelMode.Bricks) as gs:
+        exact = [gs.render_whitted(opts, s)[0] for s in (shallow, w.settings)]
         gs.build_atlas(0.02, bits)
         st = gs.stats()
         assert st.atlas_bricks > 0 and st.atlas_bytes > 0
         _, t_g = gs.render_depth(opts, 1)
-        rgb_w, _ = gs.render_whitted(opts, w.settings)
+        coarse = [gs.render_whitted(opts, s)[0] for s in (shallow, w.settings)]
     hit_c, hit_g = t_c >= 0, t_g >= 0
-    assert (hit_c != hit_g).mean() <= 2e-3, f"mask mismatch {(hit_c != hit_g).sum()}"
     both = hit_c & hit_g
     dt = np.abs(t_c[both] - t_g[both])
-    assert np.quantile(dt, 0.995) <= 3 * opts.epsilon, f"dt q99.5 {np.quantile(dt, 0.995)}"
-    rgb_c, _ = ftref.render_whitted(scene, AccelMode.Bvh, opts, w.settings)
-    d = np.abs(rgb_c.astype(int) - rgb_w.astype(int)).max(axis=2)
-    assert float((d <= 2).mean()) >= 0.99, f"whitted: {(d > 2).mean():.4f} of pixels differ by more than 2"
+    # Whitted with the atlas against the same GPU render without it (the exact render's own
+    # parity with the oracle is test_bricks_4096_scene's)
+    within = [float((np.abs(a.astype(int) - b.astype(int)).max(axis=2) <= 1).mean()) for a, b in zip(exact, coarse)]
+    print(f"[atlas {bits}] mask mismatch {(hit_c != hit_g).sum()} / {hit_c.size}, max |dt| {dt.max():.3g}, "
+          f"whitted within 1/255: depth 2 {within[0]:.5f}, depth 4 {within[1]:.5f}", flush=True)
+    # the coarse steps are conservative (atlas_distance is a lower bound), so the exact march
+    # near the surface finds the oracle's hits
+    assert (hit_c != hit_g).sum() <= max(1, int(1e-4 * hit_c.size)), f"mask mismatch {(hit_c != hit_g).sum()}"
+    assert dt.max() <= 1e-4, f"max |dt| {dt.max()}"
+    # the exact steps start from where the coarse ones stopped, so hits move inside the
+    # epsilon band (by up to ~2e-5 here); through four glass bounces a few rays then leave a
+    # cluster on the other side (moving the oracle's camera by 1e-4 changes 0.3% of this
+    # scene's pixels at 96x54)
+    assert within[0] >= 0.999, f"whitted depth 2: {1 - within[0]:.4f} of pixels differ by more than 1"
+    assert within[1] >= 0.998, f"whitted depth 4: {1 - within[1]:.4f} of pixels differ by more than 1"
 
 
 def test_whitted_debug_aovs():
@@ -306,10 +354,10 @@ def test_non_numeric_hook_keeps_the_static_value():
     """A hook whose result is no colour (a string here) yields nothing per hit, so the
     material keeps its static colour (spectrum_from_value -> None, renderer.rs:7924-7935):
     the Whitted image matches the oracle and the same material without the hook."""
-    hooked = compile_src('material Odd { model: Lambert; color: vec3(0.8, 0.3, 0.2); '
+    hooked = compile_src('material Odd { model: Lambert; color = vec3(0.8, 0.3, 0.2); '
                          'fn color(ctx) { return "red"; } };\n'
                          "let scene = Sphere { radius: 1.0, material: Odd {} };")
-    plain = compile_src("material Odd { model: Lambert; color: vec3(0.8, 0.3, 0.2); };\n"
+    plain = compile_src("material Odd { model: Lambert; color = vec3(0.8, 0.3, 0.2); };\n"
                         "let scene = Sphere { radius: 1.0, material: Odd {} };")
     assert len(hooked.material_hooks) == 1
     opts = RenderOptions(width=32, height=32)

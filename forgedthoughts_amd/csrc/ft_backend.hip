@@ -396,6 +396,34 @@ int finish(ft_scene* s) {
 
 }  // namespace
 
+// Device memory of ft_render_whitted_device_noalloc: the overflow flag in the first 256
+// bytes, the per-sample radiance after it.  One copy per GPU, held until the process ends.
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+constexpr size_t kNoallocSamplesOffset = 256;
+static_assert((kNoallocScratchBytes - kNoallocSamplesOffset) / (3 * sizeof(float)) == FT_NOALLOC_MAX_SAMPLES,
+              "FT_NOALLOC_MAX_SAMPLES must match the scratch array");
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (the array's name on the host is only a
+// stand-in), looked up once per device with that device current.
+int noalloc_scratch(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(FT_ERR_INVALID_ARGUMENT, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    FT_HIP(hipSetDevice(device));
+    void* p = nullptr;
+    FT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = p;
+  }
+  *out = (unsigned char*)g_scratch_ptr[device];
+  return FT_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -800,6 +828,30 @@ int ft_render_whitted_device(ft_scene* s, const ft_render_options* o, const ft_r
                              void* hdr_dev) {
   if (!s || !rgb_dev) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
   return whitted_impl(s, o, rs, (unsigned char*)rgb_dev, (float*)hdr_dev);
+}
+
+int ft_render_whitted_device_noalloc(ft_scene* s, const ft_render_options* o, const ft_ray_settings* rs,
+                                     void* rgb_dev, void* hdr_dev) {
+  if (!s || !rgb_dev) return fail(FT_ERR_INVALID_ARGUMENT, "scene or output is null");
+  CamArgs cam;
+  TraceParams tp;
+  int kernel;
+  int rc = whitted_check(s, o, rs, cam, tp, kernel);
+  if (rc) return rc;
+  const uint64_t n_samples = (uint64_t)cam.aa * cam.width * cam.height;
+  if (n_samples > FT_NOALLOC_MAX_SAMPLES)
+    return fail(FT_ERR_INVALID_ARGUMENT, "aa * width * height = " + std::to_string(n_samples) +
+                                             " exceeds the allocation-free limit of " +
+                                             std::to_string(FT_NOALLOC_MAX_SAMPLES) + " samples");
+  if (s->dev.flops) return fail(FT_ERR_INVALID_ARGUMENT, "FLOP counting is on (its kernels are built on first use)");
+  if (needs_module(s, false, s->cfg[1]) && !module_function(s, kernel, false, s->cfg[1]))
+    return fail(FT_ERR_INVALID_ARGUMENT, "the scene's specialised Whitted kernel is not built yet: render once "
+                                         "with ft_render_whitted or ft_render_whitted_device first");
+  unsigned char* base = nullptr;
+  if ((rc = noalloc_scratch(s->device, &base))) return rc;
+  s->stats.sdf_flops = 0;
+  return whitted_enqueue(s, cam, tp, kernel, rs->debug_aov, (unsigned char*)rgb_dev, (float*)hdr_dev,
+                         (float*)(base + kNoallocSamplesOffset), (int*)base);
 }
 
 int ft_scene_synchronize(ft_scene* s) {

@@ -31,7 +31,7 @@ ERROR_KINDS = {1: "MissingSceneRoot", 2: "UnsupportedObjectType", 3: "ExpectedOb
 
 EXPORTS = ["ft_last_error", "ft_device_count", "ft_scene_create", "ft_scene_destroy", "ft_render_depth",
            "ft_render_whitted", "ft_render_depth_device", "ft_render_whitted_device",
-           "ft_scene_synchronize",
+           "ft_render_whitted_device_noalloc", "ft_scene_synchronize",
            "ft_scene_stream", "ft_scene_stats", "ft_abi_struct_size", "ft_scene_set_flop_counting",
            "ft_emit_scene_source", "ft_emit_compile", "ft_batch_create", "ft_batch_destroy",
            "ft_batch_render_device", "ft_batch_render", "ft_batch_synchronize", "ft_batch_stats",
@@ -70,6 +70,7 @@ def library():
         L.ft_render_whitted.argtypes = [vp, vp, vp, vp, vp]
         L.ft_render_depth_device.argtypes = [vp, vp, u32, vp, vp]
         L.ft_render_whitted_device.argtypes = [vp, vp, vp, vp, vp]
+        L.ft_render_whitted_device_noalloc.argtypes = [vp, vp, vp, vp, vp]
         L.ft_scene_synchronize.argtypes = [vp]
         L.ft_scene_stream.argtypes = [vp, C.POINTER(vp)]
         L.ft_scene_stats.argtypes = [vp, vp]

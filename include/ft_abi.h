@@ -422,6 +422,21 @@ int ft_render_depth_device(ft_scene* scene, const ft_render_options* options,
                            uint32_t aa_samples, void* rgb_dev, void* t_dev);
 int ft_render_whitted_device(ft_scene* scene, const ft_render_options* options,
                              const ft_ray_settings* settings, void* rgb_dev, void* hdr_dev);
+/* ft_render_whitted_device without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The per-sample radiance and the
+ * ray-stack overflow flag live in a fixed 64 MiB device array of the library (one per GPU,
+ * held until the process ends), so aa_samples * width * height must not exceed
+ * FT_NOALLOC_MAX_SAMPLES (e.g. 1920x1080 at 2 spp, 512x512 at 16 spp); larger requests fail
+ * with FT_ERR_INVALID_ARGUMENT.  The scene's Whitted kernel must already be built (render
+ * once with ft_render_whitted or ft_render_whitted_device first: building a specialised
+ * kernel loads a code object) and FLOP counting must be off.  Every call on one GPU uses
+ * that GPU's array: calls that overlap on one GPU (renders of any scenes in flight at the
+ * same time) race and must be serialised by the caller.  Enqueues and returns;
+ * ft_scene_synchronize waits and reports a ray-stack overflow. */
+#define FT_NOALLOC_MAX_SAMPLES 5592384u
+int ft_render_whitted_device_noalloc(ft_scene* scene, const ft_render_options* options,
+                                     const ft_ray_settings* settings, void* rgb_dev, void* hdr_dev);
 int ft_scene_synchronize(ft_scene* scene);
 
 /* The HIP stream (hipStream_t) the scene launches on, for callers that time it. */

@@ -147,25 +147,33 @@ def bench_batch(args, w):
     views = gb.views(w.views)
     both = abi.BATCH_DEPTH | abi.BATCH_WHITTED
 
-    def run(flags, steps):
-        for _ in range(args.warmup):
+    def run(flags, steps, label):
+        # a 256-scene step takes tens of seconds (several minutes with one launch per view):
+        # every warm-up and timed step reports when it ends
+        for k in range(args.warmup):
+            t1 = time.perf_counter()
             gb.render_device(views, opts, rs, flags, dd.data_ptr(), dw.data_ptr())
             gb.synchronize()
+            print(f"[bench] {label} warm-up {k + 1}/{args.warmup}: {(time.perf_counter() - t1) * 1e3:.1f} ms",
+                  file=sys.stderr, flush=True)
         ms, t1 = 0.0, time.perf_counter()
-        for _ in range(steps):
+        for k in range(steps):
             gb.render_device(views, opts, rs, flags, dd.data_ptr(), dw.data_ptr())
             gb.synchronize()
             ms += gb.stats().last_kernel_ms
+            print(f"[bench] {label} step {k + 1}/{steps} done", file=sys.stderr, flush=True)
         return (time.perf_counter() - t1) / steps, ms / steps
 
-    wall_b, kms_b = run(both | abi.BATCH_GRAPH, args.steps)
+    wall_b, kms_b = run(both | abi.BATCH_GRAPH, args.steps, "batched")
     print(f"[bench] batched: {wall_b * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
-    wall_s, kms_s = run(both | abi.BATCH_SEPARATE, max(1, args.steps // 2))
+    wall_s, kms_s = run(both | abi.BATCH_SEPARATE, max(1, args.steps // 2), "separate")
     print(f"[bench] separate: {wall_s * 1e3:.1f} ms/step", file=sys.stderr, flush=True)
     gb.set_flop_counting(True)
+    t1 = time.perf_counter()
     gb.render_device(views, opts, rs, both, dd.data_ptr(), dw.data_ptr())
     gb.synchronize()
     flops = gb.stats().sdf_flops
+    print(f"[bench] FLOP-counting frame: {(time.perf_counter() - t1) * 1e3:.1f} ms", file=sys.stderr, flush=True)
     gb.set_flop_counting(False)
     achieved = flops / (kms_b / 1e3) / 1e12
     line = {
@@ -420,6 +428,9 @@ def main():
     st = gs.stats()
     avg_kernel_s = kernel_ms / args.steps / 1e3
     achieved = flops / avg_kernel_s / 1e12 if avg_kernel_s > 0 else 0.0
+    # the dominant kernel actually launched (the Whitted kernel's ray-stack size follows the
+    # scene's materials), as named in kernel traces and in the counters profile
+    traffic_kernel = f"ft_k_whitted_{st.ray_stack}" if w.kind == "whitted" else "ft_k_depth"
 
     line = {
         "metric": "Mrays/s depth+Whitted 1080p",
@@ -443,7 +454,10 @@ def main():
                                                             "tuned": False}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic(w.name, f"ft_k_{w.kind}_16" if w.kind == "whitted" else "ft_k_depth"),
+                     "traffic": measured_traffic(w.name, traffic_kernel),
+                     "traffic_source": ({"profile": os.path.relpath(TRAFFIC_PROFILE, ROOT), "kernel": traffic_kernel,
+                                         "counters": "FETCH_SIZE, WRITE_SIZE of one frame, each scaled by a 1 GiB copy"}
+                                        if measured_traffic(w.name, traffic_kernel) is not None else None),
                      "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
                      "kernel_ms": round(kernel_ms / args.steps, 3), "stream_ms": round(stream_ms / args.steps, 3),
                      "sdf_flops_per_launch": int(flops)},

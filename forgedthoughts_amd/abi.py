@@ -113,7 +113,7 @@ class RenderStatsC(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("kernel_launches", C.c_uint64),
                 ("last_kernel_ms", C.c_float), ("flags", C.c_uint32), ("sdf_flops", C.c_uint64),
                 ("compile_ms", C.c_double), ("atlas_samples", C.c_uint64), ("atlas_bytes", C.c_uint64),
-                ("atlas_bricks", C.c_uint64)]
+                ("atlas_bricks", C.c_uint64), ("ray_stack", C.c_uint32)]
 
 
 STATS_EMITTED_SCENE, STATS_EMITTED_FUNCTIONS = 1, 2
@@ -124,7 +124,7 @@ class View(C.Structure):
                 ("target", C.c_float * 3)]
 
 
-BATCH_DEPTH, BATCH_WHITTED, BATCH_SEPARATE, BATCH_GRAPH = 1, 2, 4, 8
+BATCH_DEPTH, BATCH_WHITTED, BATCH_SEPARATE, BATCH_GRAPH, BATCH_TILED = 1, 2, 4, 8, 16
 
 
 class LaunchConfigC(C.Structure):

@@ -350,6 +350,7 @@ int whitted_enqueue(ft_scene* s, const CamArgs& cam_in, const TraceParams& tp_in
   s->timed = true;
   s->stats.primary_rays = (uint64_t)npix * cam.aa;
   s->stats.kernel_launches = 2;
+  s->stats.ray_stack = kernel == fth::kRtcWhitted16 ? 16u : (kernel == fth::kRtcWhitted48 ? 48u : 128u);
   return FT_OK;
 }
 
@@ -1106,7 +1107,8 @@ struct ft_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevBuf order, slots, cams, samples, overflow, flops;
+  DevBuf order, slots, cams, samples, overflow, flops, queue, tcache;
+  int cus = 0;  // compute units of the device (persistent launch size)
   bool counting = false;
   std::vector<int> h_order, h_slots;
   std::vector<CamArgs> h_cams;
@@ -1136,6 +1138,12 @@ void drop_graph(ft_batch* b) {
   b->exec = nullptr;
   b->graph = nullptr;
   b->graph_key = 0;
+}
+
+// The batch depth pass keeps its primary hits for the Whitted pass of the same call when
+// both run at one sample per pixel and the depth pass is the persistent kernel.
+bool primary_cache(uint32_t flags, const CamArgs& c0) {
+  return (flags & FT_BATCH_DEPTH) && (flags & FT_BATCH_WHITTED) && !(flags & FT_BATCH_TILED) && c0.aa == 1;
 }
 
 int ray_stack_kernel(long need) {
@@ -1196,17 +1204,32 @@ int batch_enqueue(ft_batch* b, uint32_t n_views, const TraceParams& tp, int aov,
   ba.tiles_per_view = tpv;
   const unsigned waves = (unsigned)((size_t)n_views * tpv);
   const dim3 grid((waves + 3) / 4);
+  float* const t_cache = primary_cache(flags, c0) ? (float*)b->tcache.ptr : nullptr;
   if (flags & FT_BATCH_DEPTH) {
-    hipLaunchKernelGGL(ft_k_batch_depth, grid, dim3(64, 4), lds, b->stream, ba, tp, depth_rgb);
-    ++launches;
+    if (c0.aa == 1 && !(flags & FT_BATCH_TILED)) {
+      // persistent: four 4-wave blocks per CU, each wave drawing chunks from the counter;
+      // with a Whitted pass in the same call its primary hits are kept for it (t_cache)
+      unsigned* queue = (unsigned*)b->queue.ptr;
+      const unsigned blocks = std::min((unsigned)std::max(b->cus, 1) * 4u, (waves + 3) / 4);
+      hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, b->stream, (int*)queue);
+      hipLaunchKernelGGL(ft_k_batch_depth_persistent, dim3(blocks), dim3(64, 4), lds, b->stream, ba, tp, depth_rgb,
+                         t_cache, queue);
+      launches += 2;
+    } else {
+      hipLaunchKernelGGL(ft_k_batch_depth, grid, dim3(64, 4), lds, b->stream, ba, tp, depth_rgb);
+      ++launches;
+    }
   }
   if (flags & FT_BATCH_WHITTED) {
     dim3 wg = grid;
     wg.y = (unsigned)c0.aa;
     switch (ray_stack_kernel(1 + (long)(tp.max_depth - 1) * (b->max_children - 1))) {
-      case 16: hipLaunchKernelGGL(ft_k_batch_whitted_16, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf); break;
-      case 48: hipLaunchKernelGGL(ft_k_batch_whitted_48, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf); break;
-      default: hipLaunchKernelGGL(ft_k_batch_whitted_128, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf); break;
+      case 16: hipLaunchKernelGGL(ft_k_batch_whitted_16, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf,
+                                  (const float*)t_cache); break;
+      case 48: hipLaunchKernelGGL(ft_k_batch_whitted_48, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf,
+                                  (const float*)t_cache); break;
+      default: hipLaunchKernelGGL(ft_k_batch_whitted_128, wg, dim3(64, 4), lds, b->stream, ba, tp, aov, samples, ovf,
+                                  (const float*)t_cache); break;
     }
     hipLaunchKernelGGL(ft_k_resolve, dim3((unsigned)((npix + 255) / 256), n_views), dim3(256), 0, b->stream, c0, aov,
                        samples, whitted_rgb, (float*)nullptr);
@@ -1269,6 +1292,9 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
     if ((rc = ensure(b->samples, (size_t)n_views * cams[0].aa * npix * 3 * sizeof(float)))) return rc;
     if ((rc = ensure(b->overflow, sizeof(int)))) return rc;
   }
+  if ((rc = ensure(b->queue, sizeof(unsigned)))) return rc;
+  if (primary_cache(flags, cams[0]) && (rc = ensure(b->tcache, (size_t)n_views * npix * sizeof(float)))) return rc;
+  if (b->cus == 0) FT_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device));
   // device tables (uploaded when they change; a captured graph reads them in place)
   std::vector<int> dev_slots(n_views);
   for (uint32_t v = 0; v < n_views; ++v) dev_slots[v] = b->scenes[slots[v]]->slot;
@@ -1302,6 +1328,8 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
     key = mix_bytes(key, &whitted_rgb, sizeof(whitted_rgb));
     key = mix_bytes(key, &b->samples.ptr, sizeof(void*));
     key = mix_bytes(key, &b->order.ptr, sizeof(void*));
+    key = mix_bytes(key, &b->queue.ptr, sizeof(void*));
+    key = mix_bytes(key, &b->tcache.ptr, sizeof(void*));
     if (!b->exec || key != b->graph_key || tables_changed) {
       drop_graph(b);
       FT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1391,7 +1419,7 @@ int ft_batch_destroy(ft_batch* b) {
   drop_graph(b);
   for (ft_scene* s : b->scenes) ft_scene_destroy(s);
   for (ft_scene* s : b->scenes) s->dev.flops = nullptr;  // the counter belongs to the batch
-  for (DevBuf* d : {&b->order, &b->slots, &b->cams, &b->samples, &b->overflow, &b->flops})
+  for (DevBuf* d : {&b->order, &b->slots, &b->cams, &b->samples, &b->overflow, &b->flops, &b->queue, &b->tcache})
     if (d->ptr) (void)hipFree(d->ptr);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);

@@ -391,6 +391,95 @@ __device__ __forceinline__ Info eval_uniform(const DevScene& S, int slot, int no
   return out;
 }
 
+// The same program walk for scenes without custom, modifier or noise nodes whose value
+// stack is at most kRegStack deep, with the stack in registers: the stack pointer is
+// wave-uniform, so the arrays are indexed through M0 (s_set_gpr_idx) and stay in VGPRs.
+// Without LDS accesses in the loop the scalar loads of the next record are the only
+// LGKM-counted operations, so their wait no longer waits for a stack read as well.
+// kInfo = false leaves out the plane of producing primitives (distance-only evaluations).
+constexpr int kRegStack = 16;
+template <bool kInfo>
+__device__ __forceinline__ Info eval_uniform_reg(const DevScene& S, int node, float3 p) {
+  const auto ops = uniform_ptr(S.ops);
+  const int pf = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_first);
+  const int pl = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_last);
+  float sd[kRegStack], slb[kRegStack];
+  int sid[kInfo ? kRegStack : 1];
+  int sp = 0;
+  DOp o = load_uniform(ops + pf);
+  for (int k = pf; k <= pl; ++k) {
+    const int code = o.code;
+    const int kind = code & kOpKindMask;
+    asm volatile("" ::"s"(code));
+    __builtin_amdgcn_sched_barrier(0);
+    const DOp next = load_uniform(ops + k + 1);
+    sp = __builtin_amdgcn_readfirstlane(sp);
+    if (is_prim(kind)) {
+      const int t = sp & (kRegStack - 1);
+      const float d = op_prim_distance(o, kind, p);
+      sd[t] = d;
+      slb[t] = d;
+      if (kInfo) sid[t] = o.node;
+      ++sp;
+    } else if (kind == FT_NODE_SLICE) {
+      const int t = (sp - 1) & (kRegStack - 1);
+      float c = comp(p, (int)o.p[0]);
+      sd[t] = fmaxf(sd[t], fmaxf(o.p[1] - c, c - o.p[2]));
+      if (code & kOpNeedsLb) slb[t] = op_aabb_lb(o, p);
+    } else if (kind == FT_NODE_SMOOTH) {
+      const int t = (sp - 1) & (kRegStack - 1);
+      sd[t] -= o.p[0] * 0.1f;
+      slb[t] -= o.p[0] * 0.1f;
+    } else {
+      const int r = (sp - 1) & (kRegStack - 1), l = (sp - 2) & (kRegStack - 1);
+      --sp;
+      const float a = sd[l], b = sd[r];
+      if (kind == FT_NODE_UNION) {
+        const float llb = slb[l], rlb = slb[r];
+        const bool take_l = llb <= rlb ? (a <= rlb || a <= b) : (!(b <= llb) && a <= b);
+        if (!take_l) {
+          sd[l] = b;
+          if (kInfo) sid[l] = sid[r];
+        }
+        slb[l] = fminf(llb, rlb);
+      } else if (kind == FT_NODE_INTERSECT) {
+        if (!(a >= b)) {
+          sd[l] = b;
+          if (kInfo) sid[l] = sid[r];
+        }
+        slb[l] = fmaxf(slb[l], slb[r]);
+      } else if (kind == FT_NODE_SUBTRACT) {
+        if (!(a >= -b)) {
+          sd[l] = -b;
+          if (kInfo) sid[l] = sid[r];
+        }
+      } else {
+        bool keep_l;
+        const float d = smooth_op(kind, a, b, o.p[0], o.p[1], keep_l);
+        if (kInfo && !keep_l) sid[l] = sid[r];
+        sd[l] = d;
+        if (code & kOpNeedsLb) slb[l] = op_aabb_lb(o, p);
+      }
+    }
+    o = next;
+  }
+  if (S.flops) atomicAdd(S.flops, (unsigned long long)S.nodes[S.prog[pl] & ~kBeginFlag].cost);
+  Info out;
+  out.d = sd[0];
+  out.lb = slb[0];
+  if (kInfo) {
+    const DNode& w = S.nodes[sid[0]];
+    out.obj = w.obj;
+    out.mat = w.mat;
+    out.leaf = w.leaf;
+  } else {
+    out.obj = 0u;
+    out.mat = 0u;
+    out.leaf = -1;
+  }
+  return out;
+}
+
 // True, for every active lane, when at least half of the active lanes hold v.
 __device__ __forceinline__ bool wave_majority(bool v) {
   const unsigned long long yes = __builtin_amdgcn_ballot_w64(v), all = __builtin_amdgcn_ballot_w64(true);
@@ -419,15 +508,33 @@ __device__ __noinline__ Info eval_node(int slot, int node, float3 p, int base = 
   const DevScene& S = scene_of(slot);
   const int b = __builtin_amdgcn_readfirstlane(base);
   Info out;
-  if (S.simple)
+  if (S.simple && S.stack_depth <= kRegStack)
+    for_each_uniform(node, [&](int lead) { out = eval_uniform_reg<true>(S, lead, p); });
+  else if (S.simple)
     for_each_uniform(node, [&](int lead) { out = eval_uniform<true>(S, slot, lead, p, b); });
   else
     for_each_uniform(node, [&](int lead) { out = eval_uniform<false>(S, slot, lead, p, b); });
   return out;
 }
+// The distance (and lower bound) of `node`'s subtree alone: Info.obj / mat / leaf are not set.
+// Inlined into its callers (the march loop, shadow rays): a call per distance evaluation
+// would save and restore the callee-saved VGPRs it uses through scratch memory every step.
+__device__ __forceinline__ Info eval_node_dist(int slot, int node, float3 p, int base = 0) {
+  const DevScene& S = scene_of(slot);
+  if (!(S.simple && S.stack_depth <= kRegStack)) return eval_node(slot, node, p, base);
+  Info out;
+  for_each_uniform(node, [&](int lead) { out = eval_uniform_reg<false>(S, lead, p); });
+  return out;
+}
 
 // sdf_lower_bound of the root (renderer.rs:5413-5543): post-order over the nodes reachable
 // through Union/Intersect/Subtract/Smooth; other nodes are terminals.
+// The same as a call, for the shading walks, which evaluate subtrees a few times per hit:
+// inlining the interpreter there would only raise the register pressure of the sweep.
+__device__ __noinline__ float node_dist(int slot, int node, float3 p, int base = 0) {
+  return eval_node_dist(slot, node, p, base).d;
+}
+
 __device__ __noinline__ float lower_bound_root(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
   const int nth = blockDim.x * blockDim.y;
@@ -458,6 +565,7 @@ __device__ __noinline__ float lower_bound_root(int slot, float3 p) {
 // accel_leaf_distance_info (renderer.rs:629-657) of leaf li, the same leaf for every lane
 // (li is wave-uniform): record and node fetches are scalar loads.  far_shortcut = false
 // evaluates a custom leaf exactly wherever p is (sdf_distance_info of the leaf's subtree).
+template <bool kInfo = true>
 __device__ __forceinline__ Info leaf_info_uniform(const DevScene& S, int slot, int li, float3 p,
                                                   bool far_shortcut = true) {
   const DLeaf L = load_uniform(uniform_ptr(S.leaves) + li);
@@ -484,7 +592,7 @@ __device__ __forceinline__ Info leaf_info_uniform(const DevScene& S, int slot, i
     if (S.flops) atomicAdd(S.flops, (unsigned long long)n.cost);
     return o;
   }
-  o = eval_node(slot, L.node, p);
+  o = kInfo ? eval_node(slot, L.node, p) : eval_node_dist(slot, L.node, p);
   o.leaf = li;
   return o;
 }
@@ -511,7 +619,10 @@ __shared__ int ft_bvh_stack[4][kMaxBvhStack];
 // bounds contain p and that can still hold an earlier leaf, and takes a negative leaf by
 // rank.  Nodes and leaves are fetched with scalar loads; no per-lane stack.  All lanes of
 // the wave that are active on entry must call it together.
-__device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
+// kInfo = false: the distance alone (Info.obj / mat are not set); that form is inlined
+// into the march loop (see eval_node_dist), the full one is a call (once per ray).
+template <bool kInfo>
+__device__ __forceinline__ Info bvh_distance_body(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
   const auto bvh = uniform_ptr(S.bvh);
   int* const stack = ft_bvh_stack[threadIdx.y];
@@ -532,7 +643,7 @@ __device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
     if (nd.left < 0) {
       if (need) {
         const int li = -nd.left - 1;
-        const Info info = leaf_info_uniform(S, slot, li, p);
+        const Info info = leaf_info_uniform<kInfo>(S, slot, li, p);
         const int rank = grid_order ? li : ni;
         const bool take = best < 0.0f ? (info.d < 0.0f && rank < best_rank) : info.d < best;
         if (take) {
@@ -554,9 +665,10 @@ __device__ __noinline__ Info bvh_distance_info(int slot, float3 p) {
     stack[sp++] = left_first ? nd.right : nd.left;
     stack[sp++] = left_first ? nd.left : nd.right;
   }
-  if (!have) bi = eval_node(slot, S.root, p);
+  if (!have) bi = kInfo ? eval_node(slot, S.root, p) : eval_node_dist(slot, S.root, p);
   return bi;
 }
+__device__ __noinline__ Info bvh_distance_info(int slot, float3 p) { return bvh_distance_body<true>(slot, p); }
 // The accel leaf that the reference's tree walks from the root reach at p
 // (resolve_material_from_node renderer.rs:6945-7059, resolve_surface_normal_from_node
 // renderer.rs:4462-4570).  Over a spine of hard unions each walk descends to the child of
@@ -642,12 +754,14 @@ __device__ __forceinline__ Info scene_info(int slot, float3 p) {
   if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_info(slot, p);
   return eval_node(slot, S.root, p);
 }
-// The distance of scene_info alone (emitted scenes skip the winning-primitive selects).
+// The distance of scene_info alone (the evaluations skip the winning-primitive selects).
 __device__ __forceinline__ float scene_dist(int slot, float3 p) {
 #if defined(FT_EMITTED) && FT_EMITTED_SCENE
   return ftx_scene_dist(scene_of(slot), p);
 #endif
-  return scene_info(slot, p).d;
+  const DevScene& S = scene_of(slot);
+  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_body<false>(slot, p).d;
+  return eval_node_dist(slot, S.root, p).d;
 }
 __device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
 #if defined(FT_EMITTED) && FT_EMITTED_SCENE
@@ -727,6 +841,12 @@ __device__ __noinline__ bool raymarch(int slot, float3 o, float3 d, int max_step
   const SceneEval ev{slot};
   const AtlasCoarse coarse{slot, S.atlas_on != 0, S.atlas_switch};
   return raymarch_core(ev, S.bmin, S.bmax, o, d, max_steps, eps, step_scale, min_t, max_t, hit, coarse);
+}
+// raymarch's result for a primary ray whose march ended at t_cached (-1: missed), taken
+// from the batch depth pass.
+__device__ __noinline__ bool raymarch_cached(int slot, float3 o, float3 d, float eps, float t_cached, Hit& hit) {
+  const SceneEval ev{slot};
+  return hit_from_t(ev, o, d, eps, t_cached, hit);
 }
 
 }  // namespace ftd

@@ -41,14 +41,16 @@ __device__ __forceinline__ float depth_preview_value(float t, float max_dist) { 
 
 // Pixels are visited in tiles of 64 per wavefront (8x8, 16x4 or 32x2, chosen per launch;
 // blockDim 64 x waves per block) so that the lanes of a wave march adjacent rays.
-__device__ __forceinline__ bool tile_xy(const CamArgs& c, int tile, int& x, int& y) {
-  const int lane = threadIdx.x;
+__device__ __forceinline__ bool tile_xy_of(const CamArgs& c, int tile, int lane, int& x, int& y) {
   const int tw = c.tile_w, th = 64 / tw;
   const int tiles_x = (c.width + tw - 1) / tw;
   const int tx = tile % tiles_x, ty = tile / tiles_x;
   x = tx * tw + lane % tw;
   y = c.y0 + ty * th + lane / tw;
   return x < c.width && y < c.y0 + c.rows;
+}
+__device__ __forceinline__ bool tile_xy(const CamArgs& c, int tile, int& x, int& y) {
+  return tile_xy_of(c, tile, (int)threadIdx.x, x, y);
 }
 // 8x8 tiles (the differentiable kernels)
 __device__ __forceinline__ bool tile_xy(int width, int height, int tile, int& x, int& y) {
@@ -85,12 +87,14 @@ __device__ __forceinline__ void depth_tile(int slot, const CamArgs& cam, const T
 
 template <int kRayStack>
 __device__ __forceinline__ void whitted_tile(int slot, const CamArgs& cam, const TraceParams& tp, int aov, int tile,
-                                             int s, float* __restrict__ samples, int* __restrict__ overflow) {
+                                             int s, float* __restrict__ samples, int* __restrict__ overflow,
+                                             const float* __restrict__ t_cache = nullptr) {
   int x, y;
   if (!tile_xy(cam, tile, x, y)) return;
   float3 o, d;
   primary_ray(cam, x, y, s, o, d);
-  float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow);
+  const float t0 = t_cache && s == 0 && cam.aa == 1 ? t_cache[(size_t)y * cam.width + x] : -2.0f;
+  float3 c = aov >= 0 ? trace_aov(slot, tp, o, d, aov) : trace_whitted<kRayStack>(slot, tp, o, d, overflow, t0);
   const size_t idx = ((size_t)s * cam.height + y) * cam.width + x;
   samples[idx * 3] = c.x;
   samples[idx * 3 + 1] = c.y;
@@ -137,14 +141,90 @@ __device__ __forceinline__ void batch_depth_body(BatchArgs b, TraceParams tp, un
   depth_tile(slot, cam, tp, tile, rgb + (size_t)view * cam.width * cam.height * 3, nullptr);
 }
 
+// The batch depth pass as a persistent kernel with per-lane refill (one AA sample per
+// pixel).  A wave takes chunks of kBatchChunkTiles tiles of one view from a global counter;
+// inside a chunk every lane whose ray has ended takes the chunk's next pixel at once, so the
+// wave keeps 64 rays in flight until the chunk runs dry instead of waiting for its slowest
+// ray per tile.  Each ray's march is MarchLane's step sequence, the one raymarch runs, so
+// depths and distances are those of ft_k_batch_depth bit for bit.  `queue` starts at zero;
+// every wave leaves once the counter passes the last chunk.  t_cache (optional) receives
+// each pixel's hit distance (-1: miss).
+constexpr int kBatchChunkTiles = 4;
+__device__ __forceinline__ void batch_depth_persistent_body(BatchArgs b, TraceParams tp, unsigned char* __restrict__ rgb,
+                                                           float* __restrict__ t_cache, unsigned* __restrict__ queue) {
+  const int chunks_per_view = (b.tiles_per_view + kBatchChunkTiles - 1) / kBatchChunkTiles;
+  const unsigned n_chunks = (unsigned)b.n_views * (unsigned)chunks_per_view;
+  for (;;) {
+    unsigned c = 0u;
+    if (threadIdx.x == 0) c = atomicAdd(queue, 1u);
+    c = __builtin_amdgcn_readlane(c, 0);
+    if (c >= n_chunks) return;
+    const int rank = (int)(c / (unsigned)chunks_per_view);
+    const int first = (int)(c % (unsigned)chunks_per_view) * kBatchChunkTiles;
+    const int n_items = min(kBatchChunkTiles, b.tiles_per_view - first) * 64;
+    const int view = __builtin_amdgcn_readfirstlane(uniform_ptr(b.order)[rank]);
+    const int slot = __builtin_amdgcn_readfirstlane(uniform_ptr(b.slots)[view]);
+    const CamArgs cam = load_uniform(uniform_ptr(b.cams) + view);
+    const DevScene& S = scene_of(slot);
+    const SceneEval ev{slot};
+    const AtlasCoarse coarse{slot, S.atlas_on != 0, S.atlas_switch};
+    const size_t npix = (size_t)cam.width * cam.height;
+    unsigned char* const out = rgb + (size_t)view * npix * 3;
+    float* const tout = t_cache ? t_cache + (size_t)view * npix : nullptr;
+    // depth_tile's arithmetic for one sample
+    auto write = [&](int p, bool h, float t) {
+      float sum = 0.0f;
+      sum = sum + (h ? depth_preview_value(t, tp.max_dist) : 0.0f);
+      const unsigned char v = to_u8(sum * (1.0f / (float)cam.aa));
+      out[(size_t)p * 3] = v;
+      out[(size_t)p * 3 + 1] = v;
+      out[(size_t)p * 3 + 2] = v;
+      if (tout) tout[p] = h ? t : -1.0f;
+    };
+    MarchLane m;
+    int next = 0;  // items of the chunk handed out so far (wave-uniform)
+    int pix = -1;  // this lane's pixel, -1: idle
+    for (;;) {
+      const unsigned long long idle = __builtin_amdgcn_ballot_w64(pix < 0);
+      if (idle != 0ull && next < n_items) {
+        if (pix < 0) {
+          const int it = next + lanes_below(idle);
+          int x, y;
+          if (it < n_items && tile_xy_of(cam, first + it / 64, it % 64, x, y)) {
+            float3 o, d;
+            primary_ray(cam, x, y, 0, o, d);
+            const int p = y * cam.width + x;
+            if (m.init(S.bmin, S.bmax, o, d, tp.max_steps, tp.eps, tp.step_scale, 0.0f, tp.max_dist, coarse)) pix = p;
+            else write(p, false, 0.0f);
+          }
+        }
+        next = min(next + (int)__popcll(idle), n_items);
+      }
+      if (__builtin_amdgcn_ballot_w64(pix >= 0) == 0ull) {
+        if (next >= n_items) break;
+        continue;
+      }
+      if (pix >= 0) {
+        const int r = m.step(ev, coarse);
+        if (r != 0) {
+          write(pix, r == 2, m.hi);
+          pix = -1;
+        }
+      }
+    }
+  }
+}
+
 template <int kRayStack>
 __device__ __forceinline__ void batch_whitted_body(BatchArgs b, TraceParams tp, int aov, float* __restrict__ samples,
-                                                   int* __restrict__ overflow) {
+                                                   int* __restrict__ overflow, const float* __restrict__ t_cache) {
   int view, tile, slot;
   CamArgs cam;
   if (!batch_item(b, view, tile, slot, cam)) return;
-  const size_t per_view = (size_t)cam.aa * cam.width * cam.height * 3;
-  whitted_tile<kRayStack>(slot, cam, tp, aov, tile, blockIdx.y, samples + (size_t)view * per_view, overflow);
+  const size_t npix = (size_t)cam.width * cam.height;
+  const size_t per_view = (size_t)cam.aa * npix * 3;
+  whitted_tile<kRayStack>(slot, cam, tp, aov, tile, blockIdx.y, samples + (size_t)view * per_view, overflow,
+                          t_cache ? t_cache + (size_t)view * npix : nullptr);
 }
 
 // Distance atlas construction (exact scene distances sampled on a grid).
@@ -284,27 +364,35 @@ FT_KERNEL FT_MARCH_OCCUPANCY void FT_KNAME(ft_k_whitted_128)(int slot, ftd::CamA
 FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_depth(ftd::BatchArgs b, ftd::TraceParams tp, unsigned char* __restrict__ rgb) {
   ftd::batch_depth_body(b, tp, rgb);
 }
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_depth_persistent(ftd::BatchArgs b, ftd::TraceParams tp,
+                                                              unsigned char* __restrict__ rgb,
+                                                              float* __restrict__ t_cache, unsigned* __restrict__ queue) {
+  ftd::batch_depth_persistent_body(b, tp, rgb, t_cache, queue);
+}
 FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_16(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
-                                                        float* __restrict__ samples, int* __restrict__ overflow) {
-  ftd::batch_whitted_body<16>(b, tp, aov, samples, overflow);
+                                                        float* __restrict__ samples, int* __restrict__ overflow,
+                                                        const float* __restrict__ t_cache) {
+  ftd::batch_whitted_body<16>(b, tp, aov, samples, overflow, t_cache);
 }
 FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_48(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
-                                                        float* __restrict__ samples, int* __restrict__ overflow) {
-  ftd::batch_whitted_body<48>(b, tp, aov, samples, overflow);
+                                                        float* __restrict__ samples, int* __restrict__ overflow,
+                                                        const float* __restrict__ t_cache) {
+  ftd::batch_whitted_body<48>(b, tp, aov, samples, overflow, t_cache);
 }
 FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_128(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
-                                                         float* __restrict__ samples, int* __restrict__ overflow) {
-  ftd::batch_whitted_body<128>(b, tp, aov, samples, overflow);
+                                                        float* __restrict__ samples, int* __restrict__ overflow,
+                                                        const float* __restrict__ t_cache) {
+  ftd::batch_whitted_body<128>(b, tp, aov, samples, overflow, t_cache);
 }
 FT_KERNEL void ft_k_resolve(ftd::CamArgs cam, int aov, const float* __restrict__ samples, unsigned char* __restrict__ rgb,
                             float* __restrict__ hdr) {
   ftd::resolve_kernel_body(cam, aov, samples, rgb, hdr);
 }
-FT_KERNEL void ft_k_atlas_cells(int slot, ftd::AtlasGrid g, float* __restrict__ cell_d, int* __restrict__ cells,
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_atlas_cells(int slot, ftd::AtlasGrid g, float* __restrict__ cell_d, int* __restrict__ cells,
                                 int* __restrict__ brick_cell, unsigned* __restrict__ counter) {
   ftd::atlas_cells_body(slot, g, cell_d, cells, brick_cell, counter);
 }
-FT_KERNEL void ft_k_atlas_bricks(int slot, ftd::AtlasGrid g, const int* __restrict__ brick_cell, int n_bricks, int fp16,
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_atlas_bricks(int slot, ftd::AtlasGrid g, const int* __restrict__ brick_cell, int n_bricks, int fp16,
                                  void* __restrict__ bricks) {
   ftd::atlas_bricks_body(slot, g, brick_cell, n_bricks, fp16, bricks);
 }

@@ -268,10 +268,10 @@ __device__ __forceinline__ bool is_hard(int k) { return k == FT_NODE_UNION || k 
 __device__ inline float3 node_normal(int slot, int node, float3 p, float eps) {
   const DevScene& S = scene_of(slot);
   float e = fmaxf(eps, 1.0e-5f);
-  float d1 = eval_node(slot, node, mk(p.x + e, p.y - e, p.z - e)).d;
-  float d2 = eval_node(slot, node, mk(p.x - e, p.y - e, p.z + e)).d;
-  float d3 = eval_node(slot, node, mk(p.x - e, p.y + e, p.z - e)).d;
-  float d4 = eval_node(slot, node, mk(p.x + e, p.y + e, p.z + e)).d;
+  float d1 = node_dist(slot, node, mk(p.x + e, p.y - e, p.z - e));
+  float d2 = node_dist(slot, node, mk(p.x - e, p.y - e, p.z + e));
+  float d3 = node_dist(slot, node, mk(p.x - e, p.y + e, p.z - e));
+  float d4 = node_dist(slot, node, mk(p.x + e, p.y + e, p.z + e));
   float3 n = add(add(add(mk(d1, -d1, -d1), mk(-d2, -d2, d2)), mk(-d3, d3, -d3)), mk(d4, d4, d4));
   if (len(n) > 1.0e-6f) return normalize(n);
   return mk(0.0f, 1.0f, 0.0f);
@@ -293,7 +293,7 @@ __device__ __noinline__ float3 walk_normal(int slot, int start, float3 p, float 
     if (n.kind == FT_NODE_SLICE || n.kind == FT_NODE_SMOOTH) { fr[fsp++] = {n.lhs, 0}; continue; }
     if (is_hard(n.kind)) {
       if (f.phase == 1) { res[rsp - 1] = mul(res[rsp - 1], -1.0f); continue; }
-      float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+      float dl = node_dist(slot, n.lhs, p), dr = node_dist(slot, n.rhs, p);
       if (hard_left(n.kind, dl, dr)) fr[fsp++] = {n.lhs, 0};
       else {
         if (n.kind == FT_NODE_SUBTRACT) fr[fsp++] = {f.node, 1};
@@ -304,7 +304,7 @@ __device__ __noinline__ float3 walk_normal(int slot, int start, float3 p, float 
     if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
     if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
     float3 right = res[--rsp], left = res[--rsp];
-    float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+    float dl = node_dist(slot, n.lhs, p), dr = node_dist(slot, n.rhs, p);
     if (blend_class(n.kind) == 2) right = mul(right, -1.0f);
     float t = blend_t(n, dl, dr);
     float3 nn = add(mul(left, 1.0f - t), mul(right, t));
@@ -330,14 +330,14 @@ __device__ __noinline__ void walk_material(int slot, int start, float3 p, Mat& o
       continue;
     }
     if (is_hard(n.kind)) {
-      float dl = eval_node(slot, n.lhs, p, base).d, dr = eval_node(slot, n.rhs, p, base).d;
+      float dl = node_dist(slot, n.lhs, p, base), dr = node_dist(slot, n.rhs, p, base);
       fr[fsp++] = {hard_left(n.kind, dl, dr) ? n.lhs : n.rhs, 0};
       continue;
     }
     if (f.phase == 0) { fr[fsp++] = {f.node, 1}; fr[fsp++] = {n.lhs, 0}; continue; }
     if (f.phase == 1) { fr[fsp++] = {f.node, 2}; fr[fsp++] = {n.rhs, 0}; continue; }
     --rsp;
-    float dl = eval_node(slot, n.lhs, p, base).d, dr = eval_node(slot, n.rhs, p, base).d;
+    float dl = node_dist(slot, n.lhs, p, base), dr = node_dist(slot, n.rhs, p, base);
     blend_materials(res[rsp - 1], res[rsp], blend_t(n, dl, dr));
   }
   out = res[0];
@@ -351,13 +351,13 @@ __device__ __noinline__ bool split_material(int slot, int start, float3 p, Mat& 
   for (;;) {
     const DNode& n = S.nodes[node];
     if (is_hard(n.kind)) {
-      float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+      float dl = node_dist(slot, n.lhs, p), dr = node_dist(slot, n.rhs, p);
       node = hard_left(n.kind, dl, dr) ? n.lhs : n.rhs;
       continue;
     }
     if (n.kind == FT_NODE_SMOOTH || n.kind == FT_NODE_NOISE) { node = n.lhs; continue; }
     if (blend_class(n.kind) == 0) return false;
-    float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+    float dl = node_dist(slot, n.lhs, p), dr = node_dist(slot, n.rhs, p);
     walk_material(slot, n.lhs, p, a, 0, hc);
     walk_material(slot, n.rhs, p, b, 0, hc);
     t = blend_t(n, dl, dr);
@@ -466,17 +466,19 @@ __device__ inline MRef ref_of_mat(MatPool& pool, const Mat& m) {
   return r;
 }
 // the lerp of two single parameter sets of one model, 0 < t < 1, into a pool entry
-__device__ __noinline__ void blend_refs_lerp(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p,
-                                            const HookCtx* hc) {
-  const int dst = l.ha < 0 ? -1 - l.ha : (r.ha < 0 ? -1 - r.ha : pool_take(pool));
-  if (dst < 0) return;
+// The handles go in and out by value: a reference would give the caller's MRef variables
+// an address, and the emitted sweep's hundreds of them would live in scratch memory.
+__device__ __noinline__ int blend_refs_lerp(int slot, MatPool& pool, int lha, int rha, float t, float3 p,
+                                           const HookCtx* hc) {
+  const int dst = lha < 0 ? -1 - lha : (rha < 0 ? -1 - rha : pool_take(pool));
+  if (dst < 0) return lha;
   DParams a, b;
-  ref_params(slot, pool, l.ha, p, a, hc);
-  ref_params(slot, pool, r.ha, p, b, hc);
+  ref_params(slot, pool, lha, p, a, hc);
+  ref_params(slot, pool, rha, p, b, hc);
   blend_params(a, b, t);
   pool.e[dst] = a;
-  if (l.ha < 0 && r.ha < 0) pool_drop(pool, r.ha);  // dst is l's entry
-  l.ha = -1 - dst;
+  if (lha < 0 && rha < 0) pool_drop(pool, rha);  // dst is l's entry
+  return -1 - dst;
 }
 // blend_materials on references; r is consumed
 __device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, const MRef& r, float t, float3 p,
@@ -486,7 +488,7 @@ __device__ __forceinline__ void blend_refs(int slot, MatPool& pool, MRef& l, con
     // a lerp result, which has no dynamic id
     if (t == 0.0f) { pool_drop(pool, r.ha); mark_lerped(pool, l.ha); return; }
     if (t == 1.0f) { pool_drop(pool, l.ha); l = r; mark_lerped(pool, l.ha); return; }
-    blend_refs_lerp(slot, pool, l, r, t, p, hc);
+    l.ha = blend_refs_lerp(slot, pool, l.ha, r.ha, t, p, hc);
     return;
   }
   const bool la = !l.blend || l.t < 0.5f, ra = !r.blend || r.t < 0.5f;
@@ -733,7 +735,7 @@ __device__ __noinline__ void shade_walks(int slot, int start, float3 p, float ep
   for (;;) {
     const DNode& n = S.nodes[node];
     if (!is_hard(n.kind)) break;
-    const float dl = eval_node(slot, n.lhs, p).d, dr = eval_node(slot, n.rhs, p).d;
+    const float dl = node_dist(slot, n.lhs, p), dr = node_dist(slot, n.rhs, p);
     if (hard_left(n.kind, dl, dr)) {
       node = n.lhs;
     } else {
@@ -1194,7 +1196,10 @@ __shared__ float ft_whitted_acc[4][64 * 3];
 // the radiance of this lane's sample.  All lanes of the wave that are active on entry
 // must call it together.
 template <int kRayStack>
-__device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, float3 o, float3 d, int* overflow) {
+// t_cached >= -1: the primary ray's march is known to end there (-1: a miss), from the batch
+// depth pass; the trace then takes raymarch_cached's hit instead of marching it again.
+__device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, float3 o, float3 d, int* overflow,
+                                             float t_cached = -2.0f) {
   const DevScene& S = scene_of(slot);
   const int me = (int)threadIdx.x;
   float* const acc = ft_whitted_acc[threadIdx.y];
@@ -1239,7 +1244,10 @@ __device__ __noinline__ float3 trace_whitted(int slot, const TraceParams& tp, fl
     const float min_t = r.depth == 0 ? 0.0f : tp.eps * 2.0f;
     float3 c;
     Hit hit;
-    if (!raymarch(slot, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit)) {
+    const bool h = r.depth == 0 && r.owner == me && t_cached >= -1.0f
+                       ? raymarch_cached(slot, ro, rd, tp.eps, t_cached, hit)
+                       : raymarch(slot, ro, rd, tp.max_steps, tp.eps, tp.step_scale, min_t, tp.max_dist, hit);
+    if (!h) {
       c = cmul(cmul(environment(slot, rd), transmittance(med, tp.max_dist)), thr);
     } else {
       const int start = walk_start(slot, hit);

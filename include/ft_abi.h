@@ -453,6 +453,7 @@ typedef struct ft_render_stats {
   uint64_t atlas_samples; /* distance-atlas samples taken by the last render (counting on) */
   uint64_t atlas_bytes;   /* device bytes of the scene's distance atlas (0: none) */
   uint64_t atlas_bricks;  /* bricks in the atlas */
+  uint32_t ray_stack;     /* ray-stack entries of the last Whitted kernel (ft_k_whitted_N), 0: none yet */
 } ft_render_stats;
 #define FT_STATS_EMITTED_SCENE 1u     /* scene distance runs as specialised straight-line code */
 #define FT_STATS_EMITTED_FUNCTIONS 2u /* Forge functions run as specialised code */
@@ -480,6 +481,8 @@ typedef struct ft_view {
                                  the batched launch (for comparison) */
 #define FT_BATCH_GRAPH 8u     /* capture the launch sequence in a HIP graph; a later call with the
                                  same views, options, flags and outputs replays it */
+#define FT_BATCH_TILED 16u    /* depth pass as one wave per 8x8 tile instead of the persistent
+                                 kernel with per-lane refill (same images; for comparison) */
 
 int ft_batch_create(const ft_scene_ir* const* scenes, uint32_t n_scenes, uint32_t accel_mode, ft_batch** out);
 int ft_batch_destroy(ft_batch* batch);

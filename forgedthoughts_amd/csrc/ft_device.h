@@ -404,7 +404,12 @@ __device__ __forceinline__ Info eval_uniform_reg(const DevScene& S, int node, fl
   const int pf = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_first);
   const int pl = __builtin_amdgcn_readfirstlane(S.nodes[node].prog_last);
   float sd[kRegStack], slb[kRegStack];
-  int sid[kInfo ? kRegStack : 1];
+  int sid[kRegStack];
+  for (int i = 0; i < kRegStack; ++i) {
+    sd[i] = 0.0f;
+    slb[i] = 0.0f;
+    if (kInfo) sid[i] = 0;
+  }
   int sp = 0;
   DOp o = load_uniform(ops + pf);
   for (int k = pf; k <= pl; ++k) {
@@ -414,53 +419,63 @@ __device__ __forceinline__ Info eval_uniform_reg(const DevScene& S, int node, fl
     __builtin_amdgcn_sched_barrier(0);
     const DOp next = load_uniform(ops + k + 1);
     sp = __builtin_amdgcn_readfirstlane(sp);
+    // Every node reads the top two entries and writes one entry (a push, or the new top):
+    // one indexed read per operand and one indexed write per plane, after the kind's
+    // branch, so the planes are never copied to merge the branches.
+    const int i1 = (sp - 1) & (kRegStack - 1), i0 = (sp - 2) & (kRegStack - 1);
+    const float d1 = sd[i1], d0 = sd[i0], lb1 = slb[i1], lb0 = slb[i0];
+    const int id1 = kInfo ? sid[i1] : 0, id0 = kInfo ? sid[i0] : 0;
+    int wt;
+    float vd, vlb;
+    int vid;
     if (is_prim(kind)) {
-      const int t = sp & (kRegStack - 1);
-      const float d = op_prim_distance(o, kind, p);
-      sd[t] = d;
-      slb[t] = d;
-      if (kInfo) sid[t] = o.node;
+      wt = sp & (kRegStack - 1);
+      vd = op_prim_distance(o, kind, p);
+      vlb = vd;
+      vid = o.node;
       ++sp;
     } else if (kind == FT_NODE_SLICE) {
-      const int t = (sp - 1) & (kRegStack - 1);
-      float c = comp(p, (int)o.p[0]);
-      sd[t] = fmaxf(sd[t], fmaxf(o.p[1] - c, c - o.p[2]));
-      if (code & kOpNeedsLb) slb[t] = op_aabb_lb(o, p);
+      wt = i1;
+      const float c = comp(p, (int)o.p[0]);
+      vd = fmaxf(d1, fmaxf(o.p[1] - c, c - o.p[2]));
+      vlb = (code & kOpNeedsLb) ? op_aabb_lb(o, p) : lb1;
+      vid = id1;
     } else if (kind == FT_NODE_SMOOTH) {
-      const int t = (sp - 1) & (kRegStack - 1);
-      sd[t] -= o.p[0] * 0.1f;
-      slb[t] -= o.p[0] * 0.1f;
+      wt = i1;
+      vd = d1 - o.p[0] * 0.1f;
+      vlb = lb1 - o.p[0] * 0.1f;
+      vid = id1;
     } else {
-      const int r = (sp - 1) & (kRegStack - 1), l = (sp - 2) & (kRegStack - 1);
+      // binary: lhs at i0, rhs at i1
+      wt = i0;
       --sp;
-      const float a = sd[l], b = sd[r];
+      const float a = d0, b = d1;
       if (kind == FT_NODE_UNION) {
-        const float llb = slb[l], rlb = slb[r];
-        const bool take_l = llb <= rlb ? (a <= rlb || a <= b) : (!(b <= llb) && a <= b);
-        if (!take_l) {
-          sd[l] = b;
-          if (kInfo) sid[l] = sid[r];
-        }
-        slb[l] = fminf(llb, rlb);
+        // lower-bound short circuit (renderer.rs:5711-5731) evaluated eagerly
+        const bool take_l = lb0 <= lb1 ? (a <= lb1 || a <= b) : (!(b <= lb0) && a <= b);
+        vd = take_l ? a : b;
+        vid = take_l ? id0 : id1;
+        vlb = fminf(lb0, lb1);
       } else if (kind == FT_NODE_INTERSECT) {
-        if (!(a >= b)) {
-          sd[l] = b;
-          if (kInfo) sid[l] = sid[r];
-        }
-        slb[l] = fmaxf(slb[l], slb[r]);
+        const bool keep_l = a >= b;
+        vd = keep_l ? a : b;
+        vid = keep_l ? id0 : id1;
+        vlb = fmaxf(lb0, lb1);
       } else if (kind == FT_NODE_SUBTRACT) {
-        if (!(a >= -b)) {
-          sd[l] = -b;
-          if (kInfo) sid[l] = sid[r];
-        }
+        const bool keep_l = a >= -b;
+        vd = keep_l ? a : -b;
+        vid = keep_l ? id0 : id1;
+        vlb = lb0;
       } else {
         bool keep_l;
-        const float d = smooth_op(kind, a, b, o.p[0], o.p[1], keep_l);
-        if (kInfo && !keep_l) sid[l] = sid[r];
-        sd[l] = d;
-        if (code & kOpNeedsLb) slb[l] = op_aabb_lb(o, p);
+        vd = smooth_op(kind, a, b, o.p[0], o.p[1], keep_l);
+        vid = keep_l ? id0 : id1;
+        vlb = (code & kOpNeedsLb) ? op_aabb_lb(o, p) : lb0;
       }
     }
+    sd[wt] = vd;
+    slb[wt] = vlb;
+    if (kInfo) sid[wt] = vid;
     o = next;
   }
   if (S.flops) atomicAdd(S.flops, (unsigned long long)S.nodes[S.prog[pl] & ~kBeginFlag].cost);

@@ -707,7 +707,7 @@ __device__ __forceinline__ void ftx_sweep(const DevScene& S, int slot, float3 p,
 __device__ __forceinline__ int walk_start(int slot, const Hit& hit) {
   const DevScene& S = scene_of(slot);
   if (hit.leaf < 0 || S.root_walks) return S.root;
-  if (S.accel == FT_ACCEL_NAIVE || S.n_bvh == 0) return S.leaves[hit.leaf].node;
+  if (S.accel == FT_ACCEL_NAIVE || S.n_bvh == 0 || S.n_leaves == 1) return S.leaves[hit.leaf].node;
   const int li = bvh_walk_leaf(slot, hit.pos);
   return li >= 0 ? S.leaves[li].node : S.root;
 }

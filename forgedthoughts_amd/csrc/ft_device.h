@@ -729,7 +729,9 @@ __device__ __noinline__ int bvh_walk_leaf(int slot, float3 p) {
 }
 
 // bvh_lower_bound (renderer.rs:788-822) for the wave's points, one node stack per wave.
-__device__ __noinline__ float bvh_lower_bound(int slot, float3 p) {
+// Inlined into its one caller, the shadow march (a call per step would save and restore
+// callee-saved VGPRs through scratch memory).
+__device__ __forceinline__ float bvh_lower_bound(int slot, float3 p) {
   const DevScene& S = scene_of(slot);
   const auto bvh = uniform_ptr(S.bvh);
   int* const stack = ft_bvh_stack[threadIdx.y];

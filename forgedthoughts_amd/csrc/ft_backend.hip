@@ -1209,9 +1209,10 @@ int batch_enqueue(ft_batch* b, uint32_t n_views, const TraceParams& tp, int aov,
     if (c0.aa == 1 && !(flags & FT_BATCH_TILED)) {
       // persistent: four 4-wave blocks per CU, each wave drawing chunks from the counter;
       // with a Whitted pass in the same call its primary hits are kept for it (t_cache)
-      unsigned* queue = (unsigned*)b->queue.ptr;
+      unsigned* queue = (unsigned*)b->queue.ptr;  // one chunk counter per XCD
       const unsigned blocks = std::min((unsigned)std::max(b->cus, 1) * 4u, (waves + 3) / 4);
-      hipLaunchKernelGGL(ft_k_clear_flag, dim3(1), dim3(64), 0, b->stream, (int*)queue);
+      hipLaunchKernelGGL(ft_k_clear_u64, dim3(1), dim3(64), 0, b->stream, (unsigned long long*)queue,
+                         (int)(kXcds / 2));
       hipLaunchKernelGGL(ft_k_batch_depth_persistent, dim3(blocks), dim3(64, 4), lds, b->stream, ba, tp, depth_rgb,
                          t_cache, queue);
       launches += 2;
@@ -1292,7 +1293,7 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
     if ((rc = ensure(b->samples, (size_t)n_views * cams[0].aa * npix * 3 * sizeof(float)))) return rc;
     if ((rc = ensure(b->overflow, sizeof(int)))) return rc;
   }
-  if ((rc = ensure(b->queue, sizeof(unsigned)))) return rc;
+  if ((rc = ensure(b->queue, kXcds * sizeof(unsigned)))) return rc;
   if (primary_cache(flags, cams[0]) && (rc = ensure(b->tcache, (size_t)n_views * npix * sizeof(float)))) return rc;
   if (b->cus == 0) FT_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device));
   // device tables (uploaded when they change; a captured graph reads them in place)

@@ -1107,7 +1107,7 @@ struct ft_batch {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  DevBuf order, slots, cams, samples, overflow, flops, queue, tcache;
+  DevBuf order, slots, cams, samples, overflow, flops, queue, tcache, plist;
   int cus = 0;  // compute units of the device (persistent launch size)
   bool counting = false;
   std::vector<int> h_order, h_slots;
@@ -1221,7 +1221,23 @@ int batch_enqueue(ft_batch* b, uint32_t n_views, const TraceParams& tp, int aov,
       ++launches;
     }
   }
-  if (flags & FT_BATCH_WHITTED) {
+  if ((flags & FT_BATCH_WHITTED) && t_cache && aov < 0) {
+    // hit-first pixel order per view, then the Whitted pass over it from the primary hits
+    unsigned* plist = (unsigned*)b->plist.ptr;
+    hipLaunchKernelGGL(ft_k_batch_partition, dim3(n_views), dim3(256), 0, b->stream, (int)npix,
+                       (const float*)t_cache, plist);
+    switch (ray_stack_kernel(1 + (long)(tp.max_depth - 1) * (b->max_children - 1))) {
+      case 16: hipLaunchKernelGGL(ft_k_batch_whitted_list_16, grid, dim3(64, 4), lds, b->stream, ba, tp, samples, ovf,
+                                  (const float*)t_cache, (const unsigned*)plist); break;
+      case 48: hipLaunchKernelGGL(ft_k_batch_whitted_list_48, grid, dim3(64, 4), lds, b->stream, ba, tp, samples, ovf,
+                                  (const float*)t_cache, (const unsigned*)plist); break;
+      default: hipLaunchKernelGGL(ft_k_batch_whitted_list_128, grid, dim3(64, 4), lds, b->stream, ba, tp, samples, ovf,
+                                  (const float*)t_cache, (const unsigned*)plist); break;
+    }
+    hipLaunchKernelGGL(ft_k_resolve, dim3((unsigned)((npix + 255) / 256), n_views), dim3(256), 0, b->stream, c0, aov,
+                       samples, whitted_rgb, (float*)nullptr);
+    launches += 3;
+  } else if (flags & FT_BATCH_WHITTED) {
     dim3 wg = grid;
     wg.y = (unsigned)c0.aa;
     switch (ray_stack_kernel(1 + (long)(tp.max_depth - 1) * (b->max_children - 1))) {
@@ -1294,7 +1310,9 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
     if ((rc = ensure(b->overflow, sizeof(int)))) return rc;
   }
   if ((rc = ensure(b->queue, kXcds * sizeof(unsigned)))) return rc;
-  if (primary_cache(flags, cams[0]) && (rc = ensure(b->tcache, (size_t)n_views * npix * sizeof(float)))) return rc;
+  if (primary_cache(flags, cams[0]) && ((rc = ensure(b->tcache, (size_t)n_views * npix * sizeof(float))) ||
+                                        (rc = ensure(b->plist, (size_t)n_views * npix * sizeof(unsigned)))))
+    return rc;
   if (b->cus == 0) FT_HIP(hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, b->device));
   // device tables (uploaded when they change; a captured graph reads them in place)
   std::vector<int> dev_slots(n_views);
@@ -1331,6 +1349,7 @@ int batch_render_impl(ft_batch* b, const ft_view* views, uint32_t n_views, const
     key = mix_bytes(key, &b->order.ptr, sizeof(void*));
     key = mix_bytes(key, &b->queue.ptr, sizeof(void*));
     key = mix_bytes(key, &b->tcache.ptr, sizeof(void*));
+    key = mix_bytes(key, &b->plist.ptr, sizeof(void*));
     if (!b->exec || key != b->graph_key || tables_changed) {
       drop_graph(b);
       FT_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -1420,7 +1439,8 @@ int ft_batch_destroy(ft_batch* b) {
   drop_graph(b);
   for (ft_scene* s : b->scenes) ft_scene_destroy(s);
   for (ft_scene* s : b->scenes) s->dev.flops = nullptr;  // the counter belongs to the batch
-  for (DevBuf* d : {&b->order, &b->slots, &b->cams, &b->samples, &b->overflow, &b->flops, &b->queue, &b->tcache})
+  for (DevBuf* d : {&b->order, &b->slots, &b->cams, &b->samples, &b->overflow, &b->flops, &b->queue, &b->tcache,
+                     &b->plist})
     if (d->ptr) (void)hipFree(d->ptr);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);

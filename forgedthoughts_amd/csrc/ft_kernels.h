@@ -246,6 +246,64 @@ __device__ __forceinline__ void batch_whitted_body(BatchArgs b, TraceParams tp, 
                           t_cache ? t_cache + (size_t)view * npix : nullptr);
 }
 
+// Hit-first pixel order for the batch Whitted pass.  After the depth pass, one 256-thread
+// block per view writes the view's pixel indices, those whose primary ray hit (t_cache >= 0)
+// first, then the misses, each group in ascending pixel order (a stable partition: the
+// order is fixed by the image, not by timing).  Shading waves then hold 64 hits, instead of
+// the few hits of a tile on a silhouette next to lanes that finished with the sky.
+__device__ __forceinline__ void batch_partition_body(int npix, const float* __restrict__ t_cache,
+                                                    unsigned* __restrict__ list) {
+  __shared__ int cnt[256];
+  const int view = (int)blockIdx.x, t = (int)threadIdx.x;
+  const float* tv = t_cache + (size_t)view * npix;
+  unsigned* lv = list + (size_t)view * npix;
+  const int seg = (npix + 255) / 256;
+  const int p0 = min(t * seg, npix), p1 = min(p0 + seg, npix);
+  int hits = 0;
+  for (int p = p0; p < p1; ++p) hits += tv[p] >= 0.0f ? 1 : 0;
+  cnt[t] = hits;
+  __syncthreads();
+  // exclusive scan of the per-thread hit counts (256 entries, read by one thread each)
+  int before = 0, total = 0;
+  for (int i = 0; i < 256; ++i) {
+    const int c = cnt[i];
+    if (i < t) before += c;
+    total += c;
+  }
+  int h = before, m = total + (p0 - before);  // misses before p0 = p0 - hits before p0
+  for (int p = p0; p < p1; ++p) {
+    if (tv[p] >= 0.0f) lv[h++] = (unsigned)p;
+    else lv[m++] = (unsigned)p;
+  }
+}
+
+// The batch Whitted pass over the hit-first order: wave k of a view traces list entries
+// 64k .. 64k + 63, each from the depth pass's primary hit (one sample per pixel).
+template <int kRayStack>
+__device__ __forceinline__ void batch_whitted_list_body(BatchArgs b, TraceParams tp, float* __restrict__ samples,
+                                                       int* __restrict__ overflow, const float* __restrict__ t_cache,
+                                                       const unsigned* __restrict__ list) {
+  const int w = __builtin_amdgcn_readfirstlane(wave_index());
+  const int rank = w / b.tiles_per_view;
+  if (rank >= b.n_views) return;
+  const int view = __builtin_amdgcn_readfirstlane(uniform_ptr(b.order)[rank]);
+  const int k = w - rank * b.tiles_per_view;
+  const int slot = __builtin_amdgcn_readfirstlane(uniform_ptr(b.slots)[view]);
+  const CamArgs cam = load_uniform(uniform_ptr(b.cams) + view);
+  const int npix = cam.width * cam.height;
+  const int e = k * 64 + (int)threadIdx.x;
+  if (e >= npix) return;
+  const int p = (int)list[(size_t)view * npix + e];
+  const int x = p % cam.width, y = p / cam.width;
+  float3 o, d;
+  primary_ray(cam, x, y, 0, o, d);
+  const float3 c = trace_whitted<kRayStack>(slot, tp, o, d, overflow, t_cache[(size_t)view * npix + p]);
+  float* const out = samples + ((size_t)view * npix + p) * 3;
+  out[0] = c.x;
+  out[1] = c.y;
+  out[2] = c.z;
+}
+
 // Distance atlas construction (exact scene distances sampled on a grid).
 struct AtlasGrid {
   float ox, oy, oz, voxel;
@@ -387,6 +445,27 @@ FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_depth_persistent(ftd::BatchArgs b, 
                                                               unsigned char* __restrict__ rgb,
                                                               float* __restrict__ t_cache, unsigned* __restrict__ queue) {
   ftd::batch_depth_persistent_body(b, tp, rgb, t_cache, queue);
+}
+FT_KERNEL void ft_k_batch_partition(int npix, const float* __restrict__ t_cache, unsigned* __restrict__ list) {
+  ftd::batch_partition_body(npix, t_cache, list);
+}
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_list_16(ftd::BatchArgs b, ftd::TraceParams tp,
+                                                             float* __restrict__ samples, int* __restrict__ overflow,
+                                                             const float* __restrict__ t_cache,
+                                                             const unsigned* __restrict__ list) {
+  ftd::batch_whitted_list_body<16>(b, tp, samples, overflow, t_cache, list);
+}
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_list_48(ftd::BatchArgs b, ftd::TraceParams tp,
+                                                             float* __restrict__ samples, int* __restrict__ overflow,
+                                                             const float* __restrict__ t_cache,
+                                                             const unsigned* __restrict__ list) {
+  ftd::batch_whitted_list_body<48>(b, tp, samples, overflow, t_cache, list);
+}
+FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_list_128(ftd::BatchArgs b, ftd::TraceParams tp,
+                                                              float* __restrict__ samples, int* __restrict__ overflow,
+                                                              const float* __restrict__ t_cache,
+                                                              const unsigned* __restrict__ list) {
+  ftd::batch_whitted_list_body<128>(b, tp, samples, overflow, t_cache, list);
 }
 FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_16(ftd::BatchArgs b, ftd::TraceParams tp, int aov,
                                                         float* __restrict__ samples, int* __restrict__ overflow,

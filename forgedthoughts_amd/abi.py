@@ -132,7 +132,8 @@ class LaunchConfigC(C.Structure):
                 ("march_unroll", C.c_uint32), ("shade_inline", C.c_uint32)]
 
     def as_dict(self):
-        return {"block_waves": self.block_waves, "tile": f"{self.tile_width}x{64 // self.tile_width}",
+        tile = f"{self.tile_width}x{64 // self.tile_width}" if self.tile_width else "samples of a pixel block"
+        return {"block_waves": self.block_waves, "tile": tile,
                 "waves_per_eu": self.waves_per_eu, "march_unroll": self.march_unroll,
                 "shade_inline": bool(self.shade_inline)}
 

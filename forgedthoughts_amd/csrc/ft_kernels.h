@@ -16,7 +16,7 @@ namespace ftd {
 struct CamArgs {
   float ox, oy, oz, fx, fy, fz, rx, ry, rz, ux, uy, uz, tan_fov, aspect;
   int width, height, aa, grid;
-  int tile_w;  // pixel tile of one wave: tile_w x (64 / tile_w), tile_w in {8, 16, 32}
+  int tile_w;  // pixel tile of one wave: tile_w x (64 / tile_w), tile_w in {8, 16, 32}; 0: sample-major
   int y0, rows;  // the launch covers image rows [y0, y0 + rows) (a band of a progressive render)
 };
 
@@ -41,7 +41,18 @@ __device__ __forceinline__ float depth_preview_value(float t, float max_dist) { 
 
 // Pixels are visited in tiles of 64 per wavefront (8x8, 16x4 or 32x2, chosen per launch;
 // blockDim 64 x waves per block) so that the lanes of a wave march adjacent rays.
+// tile_w == 0 (Whitted, aa in {4, 16, 64}): a wave holds every AA sample of a square block
+// of 64 / aa pixels (sample = lane % aa), so its rays start from neighbouring points of
+// the same few pixels and tend to hit the same surfaces.
+__device__ __forceinline__ int sample_block_w(int aa) { return aa == 4 ? 4 : (aa == 16 ? 2 : 1); }
 __device__ __forceinline__ bool tile_xy_of(const CamArgs& c, int tile, int lane, int& x, int& y) {
+  if (c.tile_w == 0) {
+    const int bw = sample_block_w(c.aa), pi = lane / c.aa;
+    const int blocks_x = (c.width + bw - 1) / bw;
+    x = (tile % blocks_x) * bw + pi % bw;
+    y = c.y0 + (tile / blocks_x) * bw + pi / bw;
+    return x < c.width && y < c.y0 + c.rows;
+  }
   const int tw = c.tile_w, th = 64 / tw;
   const int tiles_x = (c.width + tw - 1) / tw;
   const int tx = tile % tiles_x, ty = tile / tiles_x;
@@ -109,7 +120,8 @@ __device__ __forceinline__ void depth_kernel_body(int slot, CamArgs cam, TracePa
 template <int kRayStack>
 __device__ __forceinline__ void whitted_kernel_body(int slot, CamArgs cam, TraceParams tp, int aov,
                                                     float* __restrict__ samples, int* __restrict__ overflow) {
-  whitted_tile<kRayStack>(slot, cam, tp, aov, wave_index(), blockIdx.y, samples, overflow);
+  const int s = cam.tile_w == 0 ? (int)threadIdx.x % cam.aa : (int)blockIdx.y;
+  whitted_tile<kRayStack>(slot, cam, tp, aov, wave_index(), s, samples, overflow);
 }
 
 // Batches: many (scene, view) images in one launch.  Wave w renders tile w % tiles_per_view

@@ -569,15 +569,18 @@ int ft_scene_build_atlas(ft_scene* scene, float voxel, uint32_t bits, uint32_t m
  * reference, whose renderer is CPU loops over tiles and rows (renderer.rs:3659-3717). */
 typedef struct ft_launch_config {
   uint32_t block_waves;   /* wavefronts per workgroup: 1, 2 or 4 */
-  uint32_t tile_width;    /* pixel tile of one wavefront: 8 (8x8), 16 (16x4) or 32 (32x2) */
+  uint32_t tile_width;    /* pixel tile of one wavefront: 8 (8x8), 16 (16x4) or 32 (32x2); 0 (Whitted
+                             with 4, 16 or 64 samples): every sample of a 4x4, 2x2 or 1x1 pixel
+                             block (other sample counts use 8x8) */
   uint32_t waves_per_eu;  /* occupancy the kernel is compiled for, 1..4 (VGPR budget 512 / n) */
   uint32_t march_unroll;  /* march steps per loop iteration: 1, 2 or 4 */
   uint32_t shade_inline;  /* 1: the Whitted shading functions are inlined into the trace loop */
 } ft_launch_config;
 #define FT_KIND_DEPTH 0u
 #define FT_KIND_WHITTED 1u
-/* The default is {4, 8, 4, 1, 0}, the shape of the precompiled kernels; any other occupancy,
- * unroll or inlining runs from a runtime-compiled code object built on first use. */
+/* The defaults are {4, 8, 4, 1, 0} (depth) and {1, 0, 4, 1, 0} (Whitted), both with the code of
+ * the precompiled kernels; any other occupancy, unroll or inlining runs from a runtime-compiled
+ * code object built on first use. */
 int ft_scene_set_launch_config(ft_scene* scene, uint32_t kind, const ft_launch_config* config);
 int ft_scene_get_launch_config(ft_scene* scene, uint32_t kind, ft_launch_config* config);
 

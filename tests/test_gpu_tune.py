@@ -14,6 +14,8 @@ pytestmark = pytest.mark.gpu
 
 SHAPES = [(4, 8, 4, 1, 0), (1, 16, 4, 1, 0), (2, 32, 4, 1, 0), (4, 8, 2, 1, 0), (4, 16, 3, 2, 0),
           (1, 8, 4, 4, 0), (4, 8, 4, 1, 1), (2, 16, 2, 1, 1)]
+# sample-major Whitted shapes (tile width 0) and the sample counts they apply to
+SAMPLE_MAJOR = [(1, 0, 4, 1, 0), (4, 0, 2, 2, 1), (2, 0, 3, 1, 0)]
 
 
 @pytest.fixture(scope="module")
@@ -42,6 +44,36 @@ def test_every_launch_shape_renders_the_oracle_image(c2_small, shape):
     assert int((hit != hit_c).sum()) <= max(1, hit.size // 10000)
     both = hit & hit_c
     assert float(np.abs(t[both] - t_c[both]).max()) <= 1e-4
+
+
+def test_default_whitted_shape_is_sample_major():
+    scene = compile_src("let scene = Sphere { radius: 1.0 };")
+    with GpuScene(scene, abi.ACCEL_BVH) as gs:
+        w = gs.launch_config(abi.KIND_WHITTED)
+        d = gs.launch_config(abi.KIND_DEPTH)
+    assert (w.block_waves, w.tile_width, w.waves_per_eu, w.march_unroll, w.shade_inline) == (1, 0, 4, 1, 0)
+    assert (d.block_waves, d.tile_width, d.waves_per_eu, d.march_unroll, d.shade_inline) == (4, 8, 4, 1, 0)
+
+
+@pytest.mark.parametrize("spp", [4, 16, 64, 9])
+@pytest.mark.parametrize("shape", SAMPLE_MAJOR, ids=[f"bw{a}_t{b}_o{c}_u{d}_i{e}" for a, b, c, d, e in SAMPLE_MAJOR])
+def test_sample_major_waves_render_the_pixel_tile_image(shape, spp):
+    """A sample-major wave traces the rays an 8x8-tile wave traces, so the image is the same byte
+    for byte (spp 9 has no sample-major form and falls back to 8x8 tiles); ragged 61x37 image
+    edges leave partly filled waves."""
+    w = synthetic.whitted_1080p(width=61, height=37, spp=spp)
+    scene = compile_src(w.source)
+    print(f"[sample-major] {shape} spp {spp}", flush=True)
+    with GpuScene(scene, w.accel) as gs:
+        gs.set_launch_config(abi.KIND_WHITTED, 4, 8, shape[2], shape[3], shape[4])
+        rgb_t, _ = gs.render_whitted(w.options, w.settings)
+        gs.set_launch_config(abi.KIND_WHITTED, *shape)
+        rgb_s, _ = gs.render_whitted(w.options, w.settings)
+    assert np.array_equal(rgb_s, rgb_t)
+    if spp == 4:
+        rgb_c, _ = ftref.render_whitted(scene, w.accel, w.options, w.settings)
+        d = np.abs(rgb_s.astype(int) - rgb_c.astype(int)).max(axis=2)
+        assert float((d <= 1).mean()) >= 0.999
 
 
 def test_autotune_chooses_a_shape_and_caches_it(c2_small):
@@ -73,3 +105,5 @@ def test_invalid_launch_shape_is_refused():
             gs.set_launch_config(abi.KIND_DEPTH, 3, 8, 4, 1, 0)
         with pytest.raises(RenderError):
             gs.set_launch_config(abi.KIND_DEPTH, 4, 12, 4, 1, 0)
+        with pytest.raises(RenderError):
+            gs.set_launch_config(abi.KIND_DEPTH, 1, 0, 4, 1, 0)  # sample-major is Whitted only

@@ -299,19 +299,37 @@ def copy_calibration(mib: int):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "whitted_1080p_counters.md")
 
 
-def measured_traffic(workload: str, kernel: str):
+def measured_traffic(workload: str, kernel: str, shape: dict, kernel_ms: float):
     """HBM bytes per launch of `kernel` on `workload` as measured with the FETCH_SIZE /
     WRITE_SIZE counters (scaled by a plain copy of known size) and recorded in the counters
-    profile, or None when that workload has no counter profile."""
+    profile, with where it came from; (None, reason) when that workload has no counter
+    profile or the profile no longer describes this run: it was taken with another launch
+    shape, or its frame took more than 15% longer or shorter than this run's, so the kernel
+    has changed since and its traffic must be measured again."""
+    bytes_, meta = None, {}
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 fields = line.split()
-                if line.startswith("traffic_bytes_per_launch") and fields[1:3] == [workload, kernel]:
-                    return int(line.split("=")[-1].strip())
+                if fields[1:3] != [workload, kernel]:
+                    continue
+                if fields[0] == "traffic_bytes_per_launch":
+                    bytes_ = int(line.split("=")[-1].strip())
+                elif fields[0] == "traffic_launch":
+                    meta = dict(kv.split("=", 1) for kv in fields[3:])
     except OSError:
         pass
-    return None
+    if bytes_ is None:
+        return None, {"stale": f"no counter profile for {workload} / {kernel}"}
+    src = {"profile": os.path.relpath(TRAFFIC_PROFILE, ROOT), "kernel": kernel,
+           "counters": "FETCH_SIZE, WRITE_SIZE of one frame, each scaled by a 1 GiB copy"}
+    want = ",".join(str(shape[k]) for k in ("block_waves", "tile", "waves_per_eu", "march_unroll", "shade_inline"))
+    if meta.get("shape") != want:
+        return None, dict(src, stale=f"profiled launch shape {meta.get('shape')}, this run {want}")
+    prof_ms = float(meta.get("kernel_ms", "0"))
+    if not (0.85 * prof_ms <= kernel_ms <= 1.15 * prof_ms):
+        return None, dict(src, stale=f"profiled frame {prof_ms} ms, this run {kernel_ms:.3f} ms")
+    return bytes_, dict(src, profiled_kernel_ms=prof_ms)
 
 
 def main():
@@ -431,6 +449,11 @@ def main():
     # the dominant kernel actually launched (the Whitted kernel's ray-stack size follows the
     # scene's materials), as named in kernel traces and in the counters profile
     traffic_kernel = f"ft_k_whitted_{st.ray_stack}" if w.kind == "whitted" else "ft_k_depth"
+    cfg = gs.launch_config(kind)
+    traffic, traffic_src = measured_traffic(
+        w.name, traffic_kernel,
+        {"block_waves": cfg.block_waves, "tile": cfg.tile_width, "waves_per_eu": cfg.waves_per_eu,
+         "march_unroll": cfg.march_unroll, "shade_inline": cfg.shade_inline}, kernel_ms / args.steps)
 
     line = {
         "metric": "Mrays/s depth+Whitted 1080p",
@@ -454,10 +477,7 @@ def main():
                                                             "tuned": False}},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                     "traffic": measured_traffic(w.name, traffic_kernel),
-                     "traffic_source": ({"profile": os.path.relpath(TRAFFIC_PROFILE, ROOT), "kernel": traffic_kernel,
-                                         "counters": "FETCH_SIZE, WRITE_SIZE of one frame, each scaled by a 1 GiB copy"}
-                                        if measured_traffic(w.name, traffic_kernel) is not None else None),
+                     "traffic": traffic, "traffic_source": traffic_src,
                      "peak_without_packed_fp32": PEAK_FP32_TFLOPS / 2,
                      "kernel_ms": round(kernel_ms / args.steps, 3), "stream_ms": round(stream_ms / args.steps, 3),
                      "sdf_flops_per_launch": int(flops)},

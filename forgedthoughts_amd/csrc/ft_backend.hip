@@ -1264,7 +1264,7 @@ int batch_enqueue(ft_batch* b, uint32_t n_views, const TraceParams& tp, int aov,
   if ((flags & FT_BATCH_WHITTED) && t_cache && aov < 0) {
     // hit-first pixel order per view, then the Whitted pass over it from the primary hits
     unsigned* plist = (unsigned*)b->plist.ptr;
-    hipLaunchKernelGGL(ft_k_batch_partition, dim3(n_views), dim3(256), 0, b->stream, (int)npix,
+    hipLaunchKernelGGL(ft_k_batch_partition, dim3(n_views), dim3(256), 0, b->stream, c0.width, c0.height,
                        (const float*)t_cache, plist);
     switch (ray_stack_kernel(1 + (long)(tp.max_depth - 1) * (b->max_children - 1))) {
       case 16: hipLaunchKernelGGL(ft_k_batch_whitted_list_16, grid, dim3(64, 4), lds, b->stream, ba, tp, samples, ovf,

@@ -258,21 +258,35 @@ __device__ __forceinline__ void batch_whitted_body(BatchArgs b, TraceParams tp, 
                           t_cache ? t_cache + (size_t)view * npix : nullptr);
 }
 
+// Tile order of a width x height image: bands of 8 rows top to bottom, each band's 8-column
+// tiles left to right, each tile's pixels row by row (the last band and the last tile of a
+// band may be narrower).  A bijection of [0, width * height) for any image size.
+__device__ __forceinline__ int tile_order_pixel(int q, int width, int height) {
+  const int band = q / (8 * width), r = q - band * 8 * width;
+  const int hb = min(8, height - 8 * band);
+  const int tx = r / (8 * hb), r2 = r - tx * 8 * hb;
+  const int wt = min(8, width - 8 * tx);
+  const int ly = r2 / wt, lx = r2 - ly * wt;
+  return (8 * band + ly) * width + 8 * tx + lx;
+}
+
 // Hit-first pixel order for the batch Whitted pass.  After the depth pass, one 256-thread
 // block per view writes the view's pixel indices, those whose primary ray hit (t_cache >= 0)
-// first, then the misses, each group in ascending pixel order (a stable partition: the
-// order is fixed by the image, not by timing).  Shading waves then hold 64 hits, instead of
-// the few hits of a tile on a silhouette next to lanes that finished with the sky.
-__device__ __forceinline__ void batch_partition_body(int npix, const float* __restrict__ t_cache,
+// first, then the misses, each group in tile order (a stable partition: the order is fixed
+// by the image, not by timing).  Shading waves then hold 64 hits, instead of the few hits of
+// a tile on a silhouette next to lanes that finished with the sky, and the 64 hits of a wave
+// lie in one or two neighbouring 8x8 tiles, so their bounce and shadow rays stay coherent.
+__device__ __forceinline__ void batch_partition_body(int width, int height, const float* __restrict__ t_cache,
                                                     unsigned* __restrict__ list) {
   __shared__ int cnt[256];
+  const int npix = width * height;
   const int view = (int)blockIdx.x, t = (int)threadIdx.x;
   const float* tv = t_cache + (size_t)view * npix;
   unsigned* lv = list + (size_t)view * npix;
   const int seg = (npix + 255) / 256;
-  const int p0 = min(t * seg, npix), p1 = min(p0 + seg, npix);
+  const int q0 = min(t * seg, npix), q1 = min(q0 + seg, npix);
   int hits = 0;
-  for (int p = p0; p < p1; ++p) hits += tv[p] >= 0.0f ? 1 : 0;
+  for (int q = q0; q < q1; ++q) hits += tv[tile_order_pixel(q, width, height)] >= 0.0f ? 1 : 0;
   cnt[t] = hits;
   __syncthreads();
   // exclusive scan of the per-thread hit counts (256 entries, read by one thread each)
@@ -282,8 +296,9 @@ __device__ __forceinline__ void batch_partition_body(int npix, const float* __re
     if (i < t) before += c;
     total += c;
   }
-  int h = before, m = total + (p0 - before);  // misses before p0 = p0 - hits before p0
-  for (int p = p0; p < p1; ++p) {
+  int h = before, m = total + (q0 - before);  // misses before q0 = q0 - hits before q0
+  for (int q = q0; q < q1; ++q) {
+    const int p = tile_order_pixel(q, width, height);
     if (tv[p] >= 0.0f) lv[h++] = (unsigned)p;
     else lv[m++] = (unsigned)p;
   }
@@ -458,8 +473,9 @@ FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_depth_persistent(ftd::BatchArgs b, 
                                                               float* __restrict__ t_cache, unsigned* __restrict__ queue) {
   ftd::batch_depth_persistent_body(b, tp, rgb, t_cache, queue);
 }
-FT_KERNEL void ft_k_batch_partition(int npix, const float* __restrict__ t_cache, unsigned* __restrict__ list) {
-  ftd::batch_partition_body(npix, t_cache, list);
+FT_KERNEL void ft_k_batch_partition(int width, int height, const float* __restrict__ t_cache,
+                                    unsigned* __restrict__ list) {
+  ftd::batch_partition_body(width, height, t_cache, list);
 }
 FT_KERNEL FT_MARCH_OCCUPANCY void ft_k_batch_whitted_list_16(ftd::BatchArgs b, ftd::TraceParams tp,
                                                              float* __restrict__ samples, int* __restrict__ overflow,

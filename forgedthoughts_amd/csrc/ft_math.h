@@ -55,11 +55,21 @@ __device__ __forceinline__ float3 rot_z(float3 v, float s, float c) { return mk(
 
 constexpr unsigned kAabbFlops = kAabbFlopsHost;
 
-// point_aabb_lower_bound (renderer.rs:5388-5411)
+// point_aabb_lower_bound (renderer.rs:5388-5411): per axis, lo - p below the box, p - hi
+// above it, else 0.  Written as one select per axis: p > hi exactly when p - hi > 0 (a
+// difference of distinct finite floats is never zero; NaN compares false and fmaxf drops
+// it), so max(p - hi, 0) is the reference's inner branch up to the sign of a zero, which the
+// square removes.  Every operand is computed, so the compiler emits v_cndmask rather than
+// the exec-mask branches it made of the nested conditionals (eight scalar instructions per
+// axis, which made the interpreted BVH walks scalar-issue bound).
+__device__ __forceinline__ float aabb_gap(float v, float lo, float hi) {
+  const float below = lo - v, above = fmaxf(v - hi, 0.0f);
+  return v < lo ? below : above;
+}
 __device__ __forceinline__ float aabb_lb(float3 p, const float* mn, const float* mx) {
-  float dx = p.x < mn[0] ? mn[0] - p.x : (p.x > mx[0] ? p.x - mx[0] : 0.0f);
-  float dy = p.y < mn[1] ? mn[1] - p.y : (p.y > mx[1] ? p.y - mx[1] : 0.0f);
-  float dz = p.z < mn[2] ? mn[2] - p.z : (p.z > mx[2] ? p.z - mx[2] : 0.0f);
+  const float dx = aabb_gap(p.x, mn[0], mx[0]);
+  const float dy = aabb_gap(p.y, mn[1], mx[1]);
+  const float dz = aabb_gap(p.z, mn[2], mx[2]);
   return fsqrt(dx * dx + dy * dy + dz * dz);
 }
 

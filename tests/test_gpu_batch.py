@@ -86,19 +86,23 @@ def test_batch_errors(small_batch):
             gb.render(gb.views([(0, (0, 0, 10), (0, 0, 0), 45.0)]), w.options, w.settings, 0)
 
 
-def test_persistent_depth_and_primary_hit_reuse():
+@pytest.mark.parametrize("shape", [(50, 50), (61, 37)], ids=["50x50", "61x37"])
+def test_persistent_depth_and_primary_hit_reuse(shape):
     """The persistent depth pass (per-lane refill from chunks of tiles) and the Whitted pass
-    that resumes from its primary hits give the bytes of the tiled launches, on an image
-    size that leaves partial tiles and a partial last chunk."""
+    that resumes from its primary hits (over each view's hits in tile order) give the bytes
+    of the tiled launches, on image sizes that leave partial tiles in both directions and a
+    partial last chunk."""
+    import dataclasses
     w = synthetic.batch_256(n_scenes=5, views_per_scene=3, size=50, lo=8, hi=128, seed=21)
+    opts = dataclasses.replace(w.options, width=shape[0], height=shape[1])
     scenes = [compile_src(s) for s in w.sources]
-    print(f"[batch persistent] {len(scenes)} scenes x {len(w.views)} views {w.options.width}^2", flush=True)
+    print(f"[batch persistent] {len(scenes)} scenes x {len(w.views)} views {shape}", flush=True)
     with GpuBatch(scenes, w.accel) as gb:
         views = gb.views(w.views)
-        d1, w1 = gb.render(views, w.options, w.settings, BOTH)
-        d2, w2 = gb.render(views, w.options, w.settings, BOTH | abi.BATCH_TILED)
-        _, w3 = gb.render(views, w.options, w.settings, abi.BATCH_WHITTED)
-        d4, _ = gb.render(views, w.options, w.settings, abi.BATCH_DEPTH)
+        d1, w1 = gb.render(views, opts, w.settings, BOTH)
+        d2, w2 = gb.render(views, opts, w.settings, BOTH | abi.BATCH_TILED)
+        _, w3 = gb.render(views, opts, w.settings, abi.BATCH_WHITTED)
+        d4, _ = gb.render(views, opts, w.settings, abi.BATCH_DEPTH)
     np.testing.assert_array_equal(d1, d2)
     np.testing.assert_array_equal(d1, d4)
     np.testing.assert_array_equal(w1, w2)

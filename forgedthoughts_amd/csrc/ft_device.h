@@ -728,10 +728,16 @@ __device__ __noinline__ int bvh_walk_leaf(int slot, float3 p) {
   return leaf;
 }
 
-// bvh_lower_bound (renderer.rs:788-822) for the wave's points, one node stack per wave.
-// Inlined into its one caller, the shadow march (a call per step would save and restore
-// callee-saved VGPRs through scratch memory).
-__device__ __forceinline__ float bvh_lower_bound(int slot, float3 p) {
+// bvh_lower_bound (renderer.rs:788-822) for the wave's points, one node stack per wave:
+// the least AABB lower bound over the accel leaves.  A node is skipped when its bound
+// exceeds the lane's best, which no leaf below it can beat, so the minimum does not depend
+// on the visiting order, and the child most lanes find nearer goes first (the best falls
+// early and prunes more).  Its one caller, the shadow march, takes the exact distance
+// wherever the bound is at or below `stop`, so a lane whose best reaches `stop` is done:
+// the result is the exact minimum where that exceeds `stop`, else some value <= `stop`.
+// Inlined into the shadow march (a call per step would save and restore callee-saved VGPRs
+// through scratch memory).
+__device__ __forceinline__ float bvh_lower_bound(int slot, float3 p, float stop) {
   const DevScene& S = scene_of(slot);
   const auto bvh = uniform_ptr(S.bvh);
   int* const stack = ft_bvh_stack[threadIdx.y];
@@ -741,14 +747,21 @@ __device__ __forceinline__ float bvh_lower_bound(int slot, float3 p) {
   while (sp > 0) {
     const DBvh nd = load_uniform(bvh + __builtin_amdgcn_readfirstlane(stack[--sp]));
     const float lb = aabb_lb(p, nd.bmin, nd.bmax);
-    const bool need = !(lb > best);
-    if (__builtin_amdgcn_ballot_w64(need) == 0ull) continue;
+    const bool need = !(lb > best) && best > stop;
+    if (__builtin_amdgcn_ballot_w64(need) == 0ull) {
+      if (__builtin_amdgcn_ballot_w64(best > stop) == 0ull) break;
+      continue;
+    }
     if (nd.left < 0) {
       if (need) best = fminf(best, lb);
       continue;
     }
-    stack[sp++] = nd.right;
-    stack[sp++] = nd.left;
+    const DBvh L = load_uniform(bvh + nd.left);
+    const DBvh R = load_uniform(bvh + nd.right);
+    const float llb = aabb_lb(p, L.bmin, L.bmax), rlb = aabb_lb(p, R.bmin, R.bmax);
+    const bool left_first = wave_majority(!need || llb <= rlb);
+    stack[sp++] = left_first ? nd.right : nd.left;
+    stack[sp++] = left_first ? nd.left : nd.right;
   }
   return best;
 }
@@ -780,12 +793,13 @@ __device__ __forceinline__ float scene_dist(int slot, float3 p) {
   if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_distance_body<false>(slot, p).d;
   return eval_node_dist(slot, S.root, p).d;
 }
-__device__ __forceinline__ float scene_lower_bound(int slot, float3 p) {
+// `stop`: the caller only needs the bound where it exceeds stop (bvh_lower_bound).
+__device__ __forceinline__ float scene_lower_bound(int slot, float3 p, float stop) {
 #if defined(FT_EMITTED) && FT_EMITTED_SCENE
   return ftx_lower_bound(scene_of(slot), p);
 #endif
   const DevScene& S = scene_of(slot);
-  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_lower_bound(slot, p);
+  if (S.accel != FT_ACCEL_NAIVE && S.n_bvh > 0) return bvh_lower_bound(slot, p, stop);
   return lower_bound_root(slot, p);
 }
 

@@ -863,7 +863,7 @@ __device__ __noinline__ float shadow_visibility(int slot, float3 o, float3 d, fl
   for (int k = 0; k < 80; ++k) {
     if (t >= max_t) return clampf(vis, 0.0f, 1.0f);
     float3 p = add(o, mul(d, t));
-    float lower = fabsf(scene_lower_bound(slot, p));
+    float lower = fabsf(scene_lower_bound(slot, p, lb_gate));
     float h = lower > lb_gate ? lower : fabsf(scene_dist(slot, p));
     if (h < eps * 4.0f) return 0.0f;
     vis = fminf(vis, clampf(10.0f * h / t, 0.0f, 1.0f));

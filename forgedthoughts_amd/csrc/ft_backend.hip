@@ -91,17 +91,16 @@ struct LaunchCfg {
   }
 };
 
-// Default shapes.  Whitted: one-wave blocks of sample-major waves; all samples of a pixel
-// start at the same pixel, so the wave's rays take near-identical paths through the scene
-// and its BVH (1080p at 16 spp: 59 ms against 103 ms for four-wave blocks of 8x8 pixel
-// tiles, the same image byte for byte), and a one-wave block frees its CU slot and LDS
-// as soon as its own rays end.
+// Default shapes: one-wave blocks, since a block holds its CU slot and LDS until its last
+// wave ends and the waves of a frame differ widely in cost (depth previews 4-21% faster
+// than with four-wave blocks).  Whitted: sample-major waves; all samples of a pixel start
+// at the same pixel, so the wave's rays take near-identical paths through the scene and its
+// BVH (1080p at 16 spp: 59 ms against 103 ms for four-wave blocks of 8x8 pixel tiles, the
+// same image byte for byte).
 LaunchCfg default_cfg(uint32_t kind) {
   LaunchCfg c;
-  if (kind == FT_KIND_WHITTED) {
-    c.block_waves = 1;
-    c.tile_w = 0;
-  }
+  c.block_waves = 1;
+  if (kind == FT_KIND_WHITTED) c.tile_w = 0;
   return c;
 }
 

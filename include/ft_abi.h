@@ -578,7 +578,7 @@ typedef struct ft_launch_config {
 } ft_launch_config;
 #define FT_KIND_DEPTH 0u
 #define FT_KIND_WHITTED 1u
-/* The defaults are {4, 8, 4, 1, 0} (depth) and {1, 0, 4, 1, 0} (Whitted), both with the code of
+/* The defaults are {1, 8, 4, 1, 0} (depth) and {1, 0, 4, 1, 0} (Whitted), both with the code of
  * the precompiled kernels; any other occupancy, unroll or inlining runs from a runtime-compiled
  * code object built on first use. */
 int ft_scene_set_launch_config(ft_scene* scene, uint32_t kind, const ft_launch_config* config);

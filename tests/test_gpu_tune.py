@@ -46,13 +46,13 @@ def test_every_launch_shape_renders_the_oracle_image(c2_small, shape):
     assert float(np.abs(t[both] - t_c[both]).max()) <= 1e-4
 
 
-def test_default_whitted_shape_is_sample_major():
+def test_default_shapes_one_wave_blocks_and_sample_major_whitted():
     scene = compile_src("let scene = Sphere { radius: 1.0 };")
     with GpuScene(scene, abi.ACCEL_BVH) as gs:
         w = gs.launch_config(abi.KIND_WHITTED)
         d = gs.launch_config(abi.KIND_DEPTH)
     assert (w.block_waves, w.tile_width, w.waves_per_eu, w.march_unroll, w.shade_inline) == (1, 0, 4, 1, 0)
-    assert (d.block_waves, d.tile_width, d.waves_per_eu, d.march_unroll, d.shade_inline) == (4, 8, 4, 1, 0)
+    assert (d.block_waves, d.tile_width, d.waves_per_eu, d.march_unroll, d.shade_inline) == (1, 8, 4, 1, 0)
 
 
 @pytest.mark.parametrize("spp", [4, 16, 64, 9])

@@ -224,8 +224,13 @@ class GpuScene:
         _check(library().ft_scene_get_launch_config(self._h, int(kind), C.addressof(c)))
         return c
 
-    def set_launch_config(self, kind: int, block_waves=4, tile_width=8, waves_per_eu=4, march_unroll=1,
+    def set_launch_config(self, kind: int, block_waves=1, tile_width=None, waves_per_eu=4, march_unroll=1,
                           shade_inline=0):
+        """Sets the launch shape of the scene's `kind` kernel (ft_scene_set_launch_config); the
+        defaults are the library's: one wave per workgroup, 8x8 pixel tiles for depth and
+        sample-major waves (tile width 0) for Whitted."""
+        if tile_width is None:
+            tile_width = 0 if kind == abi.KIND_WHITTED else 8
         c = abi.LaunchConfigC(block_waves, tile_width, waves_per_eu, march_unroll, shade_inline)
         _check(library().ft_scene_set_launch_config(self._h, int(kind), C.addressof(c)))
 

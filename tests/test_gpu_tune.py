@@ -49,6 +49,10 @@ def test_every_launch_shape_renders_the_oracle_image(c2_small, shape):
 def test_default_shapes_one_wave_blocks_and_sample_major_whitted():
     scene = compile_src("let scene = Sphere { radius: 1.0 };")
     with GpuScene(scene, abi.ACCEL_BVH) as gs:
+        gs.set_launch_config(abi.KIND_WHITTED, 4, 16)
+        gs.set_launch_config(abi.KIND_DEPTH, 4, 16)
+        gs.set_launch_config(abi.KIND_WHITTED)  # the Python defaults restore the library's
+        gs.set_launch_config(abi.KIND_DEPTH)
         w = gs.launch_config(abi.KIND_WHITTED)
         d = gs.launch_config(abi.KIND_DEPTH)
     assert (w.block_waves, w.tile_width, w.waves_per_eu, w.march_unroll, w.shade_inline) == (1, 0, 4, 1, 0)

@@ -58,16 +58,20 @@ def test_preview_progressive_calls_back_per_tile():
     assert not seen[0][3][y:y + 40, x:x + 40].any() or not ref[y:y + 40, x:x + 40].any()
 
 
-def test_ray_progressive_matches_the_whole_frame_render():
+@pytest.mark.parametrize("spp,tile", [(2, 32), (4, 30)])
+def test_ray_progressive_matches_the_whole_frame_render(spp, tile):
+    """Bands of rows against the whole frame: at 2 spp the default Whitted shape falls back to
+    8x8 pixel tiles, at 4 spp its sample-major waves cover 4x4-pixel blocks, which 30-row
+    bands cut."""
     name, _, opts, settings = REGRESSION_CASES[2]
     state = _state(name)
-    s = RaySettings(settings.max_depth, 32, 2)
+    s = RaySettings(settings.max_depth, tile, spp)
     count = []
     rgb = render.render_ray_progressive_with_accel(state, opts, AccelMode.Bvh, s, lambda p, im: count.append(p))
     with render.GpuScene(compile_scene(state), AccelMode.Bvh) as gs:
         whole, _ = gs.render_whitted(opts, s)
     np.testing.assert_array_equal(rgb, whole)
-    assert len(count) == len(_tile_order(opts.width, opts.height, 32))
+    assert len(count) == len(_tile_order(opts.width, opts.height, tile))
     ref, _ = ftref.render_whitted(compile_scene(state), AccelMode.Bvh, opts, s)
     d = np.abs(rgb.astype(int) - ref.astype(int)).max(axis=2)
     assert float((d <= 1).mean()) >= 0.999

@@ -87,7 +87,7 @@ struct LaunchCfg {
   bool valid(uint32_t kind) const {
     return (block_waves == 1 || block_waves == 2 || block_waves == 4) &&
            (tile_w == 8 || tile_w == 16 || tile_w == 32 || (tile_w == 0 && kind == FT_KIND_WHITTED)) &&
-           wpe >= 1 && wpe <= 4 && (unroll == 1 || unroll == 2 || unroll == 4) && shade_inline <= 1;
+           wpe >= 1 && wpe <= 8 && (unroll == 1 || unroll == 2 || unroll == 4) && shade_inline <= 1;
   }
 };
 
@@ -577,7 +577,7 @@ bool tune_cache_get(uint64_t key, uint32_t kind, LaunchCfg& c) {
     return true;
   }
   char name[48];
-  std::snprintf(name, sizeof(name), "tune3_%016llx.txt", (unsigned long long)key);
+  std::snprintf(name, sizeof(name), "tune5_%016llx.txt", (unsigned long long)key);
   const std::string path = fth::cache_file(name);
   if (path.empty()) return false;
   FILE* f = std::fopen(path.c_str(), "r");
@@ -597,7 +597,7 @@ void tune_cache_put(uint64_t key, const LaunchCfg& c) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = c;
   char name[48];
-  std::snprintf(name, sizeof(name), "tune3_%016llx.txt", (unsigned long long)key);
+  std::snprintf(name, sizeof(name), "tune5_%016llx.txt", (unsigned long long)key);
   const std::string path = fth::cache_file(name);
   if (path.empty()) return;
   const std::string tmp = path + ".tmp";
@@ -662,8 +662,11 @@ int autotune_impl(ft_scene* s, uint32_t kind, const ft_render_options* o, const 
   ft_ray_settings rs{};
   if (rs_in) rs = *rs_in;
   else rs.aa_samples = 1;
-  // the tuning render: the request itself, or (Whitted) the same frame at <= 4 samples
-  if (kind == FT_KIND_WHITTED && !(flags & FT_TUNE_FULL) && rs.aa_samples > 4) rs.aa_samples = 4;
+  // the tuning render: the request itself, or (Whitted) the same frame at <= 16 samples.  A
+  // sample-major wave covers a pixel block whose size follows the sample count (4x4 pixels at
+  // 4 spp, 2x2 at 16), so a render at fewer samples than the request can rank the shapes
+  // differently (at 4 spp two-wave workgroups won, at 16 spp they took 68.7 against 59.5 ms).
+  if (kind == FT_KIND_WHITTED && !(flags & FT_TUNE_FULL) && rs.aa_samples > 16) rs.aa_samples = 16;
   if ((rc = ensure(s->rgb, (size_t)o->width * o->height * 3))) return rc;
   int k = fth::kRtcDepth;
   if (kind == FT_KIND_WHITTED) {
@@ -687,7 +690,8 @@ int autotune_impl(ft_scene* s, uint32_t kind, const ft_render_options* o, const 
     std::vector<LaunchCfg> code = {dflt};
     if (!(flags & FT_TUNE_SHAPE_ONLY)) {
       // (waves per EU, march unroll, shade inline) of each code variant
-      static const uint32_t variants[6][3] = {{3, 1, 0}, {2, 1, 0}, {4, 2, 0}, {2, 2, 0}, {4, 1, 1}, {2, 1, 1}};
+      static const uint32_t variants[7][3] = {{5, 1, 0}, {3, 1, 0}, {2, 1, 0}, {4, 2, 0}, {2, 2, 0}, {4, 1, 1},
+                                              {2, 1, 1}};
       for (const auto& v : variants) {
         LaunchCfg c = dflt;
         c.wpe = v[0];

@@ -572,7 +572,7 @@ typedef struct ft_launch_config {
   uint32_t tile_width;    /* pixel tile of one wavefront: 8 (8x8), 16 (16x4) or 32 (32x2); 0 (Whitted
                              with 4, 16 or 64 samples): every sample of a 4x4, 2x2 or 1x1 pixel
                              block (other sample counts use 8x8) */
-  uint32_t waves_per_eu;  /* occupancy the kernel is compiled for, 1..4 (VGPR budget 512 / n) */
+  uint32_t waves_per_eu;  /* occupancy the kernel is compiled for, 1..8 (VGPR budget 512 / n) */
   uint32_t march_unroll;  /* march steps per loop iteration: 1, 2 or 4 */
   uint32_t shade_inline;  /* 1: the Whitted shading functions are inlined into the trace loop */
 } ft_launch_config;
@@ -594,11 +594,11 @@ typedef struct ft_tune_report {
   uint32_t tune_aa_samples;  /* samples per pixel of the tuning render */
   uint32_t pad;
 } ft_tune_report;
-#define FT_TUNE_FULL 1u        /* time the request itself (default: Whitted at <= 4 samples) */
+#define FT_TUNE_FULL 1u        /* time the request itself (default: Whitted at <= 16 samples) */
 #define FT_TUNE_SHAPE_ONLY 2u  /* search block and tile shapes only (no code variants to build) */
 #define FT_TUNE_NO_CACHE 4u    /* search even when the cache holds a choice */
 /* Autotuner: times the scene's `kind` render of this request under candidate shapes,
- * first the code variants (occupancy 4/3/2, march unroll 1/2, shading calls or inlined) in
+ * first the code variants (occupancy 5/4/3/2, march unroll 1/2, shading calls or inlined) in
  * the default launch shape,
  * then block and tile shapes of the best code, and keeps the fastest as the scene's shape
  * for that kind.  Choices are cached per (scene IR hash, kind, request) in the process and,

@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 SHAPES = [(4, 8, 4, 1, 0), (1, 16, 4, 1, 0), (2, 32, 4, 1, 0), (4, 8, 2, 1, 0), (4, 16, 3, 2, 0),
           (1, 8, 4, 4, 0), (4, 8, 4, 1, 1), (2, 16, 2, 1, 1)]
 # sample-major Whitted shapes (tile width 0) and the sample counts they apply to
-SAMPLE_MAJOR = [(1, 0, 4, 1, 0), (4, 0, 2, 2, 1), (2, 0, 3, 1, 0)]
+SAMPLE_MAJOR = [(1, 0, 4, 1, 0), (4, 0, 2, 2, 1), (2, 0, 3, 1, 0), (1, 0, 5, 1, 0)]
 
 
 @pytest.fixture(scope="module")
@@ -111,3 +111,5 @@ def test_invalid_launch_shape_is_refused():
             gs.set_launch_config(abi.KIND_DEPTH, 4, 12, 4, 1, 0)
         with pytest.raises(RenderError):
             gs.set_launch_config(abi.KIND_DEPTH, 1, 0, 4, 1, 0)  # sample-major is Whitted only
+        with pytest.raises(RenderError):
+            gs.set_launch_config(abi.KIND_WHITTED, 1, 0, 9, 1, 0)  # occupancy 1..8

@@ -302,30 +302,32 @@ TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "whitted_1080p_counters.md")
 def measured_traffic(workload: str, kernel: str, shape: dict, kernel_ms: float):
     """HBM bytes per launch of `kernel` on `workload` as measured with the FETCH_SIZE /
     WRITE_SIZE counters (scaled by a plain copy of known size) and recorded in the counters
-    profile, with where it came from; (None, reason) when that workload has no counter
-    profile or the profile no longer describes this run: it was taken with another launch
-    shape, or its frame took more than 15% longer or shorter than this run's, so the kernel
-    has changed since and its traffic must be measured again."""
-    bytes_, meta = None, {}
+    profile, one line per profiled launch shape
+    (`traffic_bytes_per_launch WORKLOAD KERNEL shape=BW,TILE,WPE,UNROLL,INLINE kernel_ms=MS = BYTES`),
+    with where it came from; (None, reason) when the profile holds no line for this workload,
+    kernel and launch shape, or the profiled frame took more than 15% longer or shorter than
+    this run's, so the kernel has changed since and its traffic must be measured again."""
+    entries = []
     try:
         with open(TRAFFIC_PROFILE) as f:
             for line in f:
                 fields = line.split()
-                if fields[1:3] != [workload, kernel]:
+                if len(fields) < 4 or fields[0] != "traffic_bytes_per_launch" or fields[1:3] != [workload, kernel]:
                     continue
-                if fields[0] == "traffic_bytes_per_launch":
-                    bytes_ = int(line.split("=")[-1].strip())
-                elif fields[0] == "traffic_launch":
-                    meta = dict(kv.split("=", 1) for kv in fields[3:])
+                meta = dict(kv.split("=", 1) for kv in fields[3:] if "=" in kv and not kv.startswith("="))
+                entries.append((meta, int(line.split("=")[-1].strip())))
     except OSError:
         pass
-    if bytes_ is None:
+    if not entries:
         return None, {"stale": f"no counter profile for {workload} / {kernel}"}
     src = {"profile": os.path.relpath(TRAFFIC_PROFILE, ROOT), "kernel": kernel,
            "counters": "FETCH_SIZE, WRITE_SIZE of one frame, each scaled by a 1 GiB copy"}
     want = ",".join(str(shape[k]) for k in ("block_waves", "tile", "waves_per_eu", "march_unroll", "shade_inline"))
-    if meta.get("shape") != want:
-        return None, dict(src, stale=f"profiled launch shape {meta.get('shape')}, this run {want}")
+    match = [(m, b) for m, b in entries if m.get("shape") == want]
+    if not match:
+        return None, dict(src, stale=f"profiled launch shapes {[m.get('shape') for m, _ in entries]}, this run {want}")
+    meta, bytes_ = match[0]
+    src["shape"] = want
     prof_ms = float(meta.get("kernel_ms", "0"))
     if not (0.85 * prof_ms <= kernel_ms <= 1.15 * prof_ms):
         return None, dict(src, stale=f"profiled frame {prof_ms} ms, this run {kernel_ms:.3f} ms")
@@ -341,6 +343,9 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
+    ap.add_argument("--launch", default="", metavar="BW,TILE,WPE,UNROLL,INLINE",
+                    help="single-scene workloads: use this launch shape instead of autotuning (block waves, "
+                         "tile width (0: sample-major), waves per EU, march unroll, shading inlined)")
     ap.add_argument("--fp64-rows", type=int, default=1,
                     help="rows of the timed frame also checked against the oracle in FP64")
     ap.add_argument("--scenes", type=int, default=0, help="batch workloads: number of scenes (default: the workload's)")
@@ -382,7 +387,14 @@ def main():
     from forgedthoughts_amd import abi
     kind = abi.KIND_WHITTED if w.kind == "whitted" else abi.KIND_DEPTH
     tune = None
-    if not args.no_tune:
+    if args.launch:
+        shape = [int(v) for v in args.launch.split(",")]
+        if len(shape) != 5:
+            print("--launch takes five comma-separated integers", file=sys.stderr)
+            sys.exit(2)
+        gs.set_launch_config(kind, *shape)
+        tune = {"chosen": gs.launch_config(kind).as_dict(), "tuned": False, "forced": True}
+    elif not args.no_tune:
         t_tune = time.perf_counter()
         rep = gs.autotune(kind, opts, w.settings if w.kind == "whitted" else RaySettings(1, 64, w.aa))
         tune = {"chosen": rep.best.as_dict(), "tuned": True, "cached": bool(rep.cached),

@@ -13,30 +13,33 @@ def _bench():
     return mod
 
 
-def _profile_line(bench, prefix):
+def _profile_lines(bench):
     with open(bench.TRAFFIC_PROFILE) as f:
-        for line in f:
-            if line.startswith(prefix):
-                return line.split()
-    raise AssertionError(f"no {prefix} line")
+        return [line.split() for line in f if line.startswith("traffic_bytes_per_launch")]
 
 
 def test_profile_traffic_is_used_only_for_the_profiled_run():
     bench = _bench()
-    meta = _profile_line(bench, "traffic_launch")
-    workload, kernel = meta[1], meta[2]
-    kv = dict(x.split("=", 1) for x in meta[3:])
-    bw, tw, wpe, un, inl = (int(v) for v in kv["shape"].split(","))
-    shape = {"block_waves": bw, "tile": tw, "waves_per_eu": wpe, "march_unroll": un, "shade_inline": inl}
-    ms = float(kv["kernel_ms"])
-    want = int(_profile_line(bench, "traffic_bytes_per_launch")[-1])
-
-    got, src = bench.measured_traffic(workload, kernel, shape, ms * 1.05)
-    assert got == want and "stale" not in src and src["kernel"] == kernel
-
-    got, src = bench.measured_traffic(workload, kernel, dict(shape, block_waves=4 if bw != 4 else 2), ms)
+    lines = _profile_lines(bench)
+    assert len(lines) >= 2
+    shapes = set()
+    for fields in lines:
+        workload, kernel = fields[1], fields[2]
+        kv = dict(x.split("=", 1) for x in fields[3:5])
+        bw, tw, wpe, un, inl = (int(v) for v in kv["shape"].split(","))
+        shapes.add((bw, tw, wpe, un, inl))
+        shape = {"block_waves": bw, "tile": tw, "waves_per_eu": wpe, "march_unroll": un, "shade_inline": inl}
+        ms = float(kv["kernel_ms"])
+        want = int(fields[-1])
+        # each profiled shape finds its own line
+        got, src = bench.measured_traffic(workload, kernel, shape, ms * 1.05)
+        assert got == want and "stale" not in src and src["kernel"] == kernel and src["shape"] == kv["shape"]
+        got, src = bench.measured_traffic(workload, kernel, shape, ms * 1.3)
+        assert got is None and "frame" in src["stale"]
+        got, src = bench.measured_traffic(workload, "ft_k_no_such_kernel", shape, ms)
+        assert got is None and "no counter profile" in src["stale"]
+    # a shape nobody profiled has no traffic figure
+    other = {"block_waves": 2, "tile": 32, "waves_per_eu": 1, "march_unroll": 4, "shade_inline": 1}
+    assert (2, 32, 1, 4, 1) not in shapes
+    got, src = bench.measured_traffic(lines[0][1], lines[0][2], other, 60.0)
     assert got is None and "launch shape" in src["stale"]
-    got, src = bench.measured_traffic(workload, kernel, shape, ms * 1.3)
-    assert got is None and "frame" in src["stale"]
-    got, src = bench.measured_traffic(workload, "ft_k_no_such_kernel", shape, ms)
-    assert got is None and "no counter profile" in src["stale"]

@@ -434,6 +434,14 @@ __device__ __forceinline__ Info eval_uniform_reg(const DevScene& S, int node, fl
       vlb = vd;
       vid = o.node;
       ++sp;
+    } else if (kind >= FT_NODE_UNION_ROUND && kind <= FT_NODE_TONGUE) {
+      // the blend joins first: with the primitives they are nearly every node of a program
+      wt = i0;
+      --sp;
+      bool keep_l;
+      vd = smooth_op(kind, d0, d1, o.p[0], o.p[1], keep_l);
+      vid = keep_l ? id0 : id1;
+      vlb = (code & kOpNeedsLb) ? op_aabb_lb(o, p) : lb0;
     } else if (kind == FT_NODE_SLICE) {
       wt = i1;
       const float c = comp(p, (int)o.p[0]);
@@ -446,7 +454,7 @@ __device__ __forceinline__ Info eval_uniform_reg(const DevScene& S, int node, fl
       vlb = lb1 - o.p[0] * 0.1f;
       vid = id1;
     } else {
-      // binary: lhs at i0, rhs at i1
+      // hard booleans: lhs at i0, rhs at i1
       wt = i0;
       --sp;
       const float a = d0, b = d1;
@@ -461,16 +469,11 @@ __device__ __forceinline__ Info eval_uniform_reg(const DevScene& S, int node, fl
         vd = keep_l ? a : b;
         vid = keep_l ? id0 : id1;
         vlb = fmaxf(lb0, lb1);
-      } else if (kind == FT_NODE_SUBTRACT) {
+      } else {  // FT_NODE_SUBTRACT
         const bool keep_l = a >= -b;
         vd = keep_l ? a : -b;
         vid = keep_l ? id0 : id1;
         vlb = lb0;
-      } else {
-        bool keep_l;
-        vd = smooth_op(kind, a, b, o.p[0], o.p[1], keep_l);
-        vid = keep_l ? id0 : id1;
-        vlb = (code & kOpNeedsLb) ? op_aabb_lb(o, p) : lb0;
       }
     }
     sd[wt] = vd;

@@ -1165,7 +1165,7 @@ template <class T> bool raymarch_hit(const Accel<T>& A, V3<T> o, V3<T> d, const 
       V3<T> pos = o + d * hi;
       V3<T> nrm = estimate_normal(A, pos, rmax(op.epsilon * T(4), T(1.0e-5)));
       DistInfo<T> fin = ev_info(A, pos);
-      hit = {hi, pos, nrm, nrm.dot(d) < T(0), fin.object_id, fin.material_id};
+      hit = {hi, pos, nrm, nrm.dot(d) < T(0), fin.object_id, fin.material_id, V3<T>()};
       return true;
     }
     prev = traveled;

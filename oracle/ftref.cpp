@@ -9,6 +9,12 @@
 //
 // Only tests/, bench.py's CPU-baseline leg and the package smoke check load this file's
 // library; the HIP backend never does.
+//
+// Pinning (tests/test_oracle_pins.py, the reference fixtures, closed forms): every function
+// is checked except these, which are parity unpinned: the lattice-sphere noise of the noise
+// modifier (sd_base_noise and the octave rotation; the modifier is pinned only where the
+// noise drops out), the fmin / fmax NaN behaviour of the reference's JIT, and the order of
+// lights and materials the reference takes from a HashMap (this file keeps source order).
 
 #include "ftref.h"
 
@@ -240,12 +246,12 @@ template <class T> T apply_noise_modifier(V3<T> p, T d, uint32_t octaves, T scal
   uint32_t n = octaves < 1 ? 1 : octaves;
   for (uint32_t k = 0; k < n; ++k) {
     T nn = s * sd_base_noise(q);
-    nn = op_smax(nn, d - T(0.1) * s, T(0.3) * s);
+    nn = op_smax(nn, d - (planted(44) ? T(0.2) : T(0.1)) * s, T(0.3) * s);
     d = op_smin(nn, d, T(0.3) * s);
     t += d;
     q = V3<T>(m[0][0] * q.x + m[0][1] * q.y + m[0][2] * q.z, m[1][0] * q.x + m[1][1] * q.y + m[1][2] * q.z,
               m[2][0] * q.x + m[2][1] * q.y + m[2][2] * q.z) * os;
-    s *= T(0.415);
+    s *= planted(45) ? T(0.5) : T(0.415);
     q.z += T(-4.33) * t * s;
   }
   return d;

@@ -362,6 +362,30 @@ def pin_value_noise_and_fbm():
     assert np.abs(noise_fn("f3", p) - expect).max() < 3e-6
 
 
+@pin(44, 45)
+def pin_noise_modifier_far_and_deep():
+    """The fbm noise modifier (R:5957-6009, Quilez's fbm of lattice spheres) where the noise
+    drops out.  Octave k (scale s = 0.415^k) takes nn = smax(s * noise, d - s/10, 3s/10), then
+    d = smin(nn, d, 3s/10), with the polynomial smin / smax (k - |a - b|)^2 / (4k).  A lattice
+    sphere has radius below 0.7 and a cell point lies within sqrt(3)/2 of a corner, so the
+    noise term is in (-0.7 s, 0.87 s).  Far outside (d >= 2) smax yields d - s/10 exactly and
+    smin then d - s/10 - (s/5)^2 / (4 * 3s/10) = d - 2s/15, so n octaves subtract
+    (2/15) * sum 0.415^k; deep inside (d <= -2) smax yields the noise term and smin keeps d,
+    so the distance is unchanged.  Octave counts below one run one octave."""
+    rng = np.random.default_rng(44)
+    dirs = rng.normal(size=(64, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    far = dirs * 13.0            # sphere of radius 10: d = 3
+    deep = dirs * 2.0            # d = -8
+    for octaves, expect_n in ((0, 1), (1, 1), (2, 2), (3, 3), (7, 7)):
+        src = f"let scene = Sphere {{ radius: 10.0 }}.noise({octaves}.0, 1.3, 1.7);"
+        drop = (2.0 / 15.0) * sum(0.415 ** k for k in range(expect_n))
+        df = dist(src, far)
+        assert np.abs(df - (3.0 - drop)).max() < 3e-5, (octaves, df[:3], 3.0 - drop)
+        dd = dist(src, deep)
+        assert np.abs(dd - (-8.0)).max() < 3e-5, (octaves, dd[:3])
+
+
 # ------------------------------------------------------------------ optics
 def fresnel_textbook(ci, n1, n2):
     st2 = (n1 / n2) ** 2 * (1 - ci * ci)
